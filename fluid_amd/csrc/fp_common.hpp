@@ -1,0 +1,136 @@
+// fp_common.hpp -- device helpers shared by the projection kernels (gfx950).
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#define FP_BLOCK 256
+#define FP_WAVES (FP_BLOCK / 64)
+
+namespace fp {
+
+// Ghost-fluid fraction, lib/core/cm12-numerics.ts:37-43
+// (min(1, max(0.05, |phi_l| / max(|phi_l| + |phi_a|, eps)))), eps as the
+// multigrid coefficient bake uses (webgpu-uniform-pressure-multigrid.wgsl.ts:90-92).
+__device__ __forceinline__ float ghost_theta(float phi_l, float phi_a) {
+  const float den = fmaxf(fabsf(phi_l) + fabsf(phi_a), 1e-9f);
+  return fminf(1.0f, fmaxf(0.05f, fabsf(phi_l) / den));
+}
+__device__ __forceinline__ double ghost_theta_d(double phi_l, double phi_a) {
+  const double den = fmax(fabs(phi_l) + fabs(phi_a), 1e-9);
+  return fmin(1.0, fmax(0.05, fabs(phi_l) / den));
+}
+
+// h^2-scaled face coefficient between cells with level sets pc, pn:
+// g between two liquid cells, g/theta across the free surface, 0 between two
+// air cells (mgCoefficientRaw, webgpu-uniform-pressure-multigrid.wgsl.ts:93-115).
+__device__ __forceinline__ float face_weight(float pc, float pn, float g) {
+  const bool lc = pc < 0.0f, ln = pn < 0.0f;
+  if (lc && ln) return g;
+  if (!(lc || ln)) return 0.0f;
+  return lc ? g / ghost_theta(pc, pn) : g / ghost_theta(pn, pc);
+}
+__device__ __forceinline__ double face_weight_d(float pc, float pn) {
+  const bool lc = pc < 0.0f, ln = pn < 0.0f;
+  if (lc && ln) return 1.0;
+  if (!(lc || ln)) return 0.0;
+  return lc ? 1.0 / ghost_theta_d(pc, pn) : 1.0 / ghost_theta_d(pn, pc);
+}
+
+// Storage conversions of multigrid vectors.
+template <typename T> struct Store;
+template <> struct Store<float> {
+  static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+};
+template <> struct Store<__half> {
+  static __device__ __forceinline__ float ld(const __half* p, size_t i) { return __half2float(p[i]); }
+  static __device__ __forceinline__ void st(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
+};
+template <> struct Store<__hip_bfloat16> {
+  static __device__ __forceinline__ float ld(const __hip_bfloat16* p, size_t i) { return __bfloat162float(p[i]); }
+  static __device__ __forceinline__ void st(__hip_bfloat16* p, size_t i, float v) { p[i] = __float2bfloat16(v); }
+};
+
+// Wave (64 lanes) and block sums of doubles.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block-reduce NV doubles; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][FP_WAVES];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < FP_WAVES; ++w) s += sh[k][w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double atomic_load_d(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Deterministic grid reduction: every block stores its partials, the last
+// block to arrive (ticket counter) sums them in a fixed order.  Returns true
+// in the last block; `out` is valid in its thread 0.  The counter is reset by
+// the last block, so the kernel can be replayed (graphs) without a memset.
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter, double (&out)[NV]) {
+  __shared__ bool last;
+  block_sum<NV>(v);
+  const unsigned G = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + blockIdx.x] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    last = (t == G - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    s[k] = 0.0;
+    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s[k] += atomic_load_d(&partials[(size_t)k * G + b]);
+  }
+  block_sum<NV>(s);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = s[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
+}
+
+// Max of non-negative floats through their bit patterns (order preserving).
+__device__ __forceinline__ void atomic_max_nonneg(unsigned* p, float v) {
+  if (v > 0.0f) atomicMax(p, __float_as_uint(v));
+}
+
+}  // namespace fp

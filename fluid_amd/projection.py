@@ -1,0 +1,333 @@
+"""Host-side mirror of the reference's projection interface over the C ABI.
+
+``UniformPressureProjection`` follows the surface of the reference's
+``WebGPUUniformPressureMultigrid`` (lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:228-607):
+construction plans the hierarchy and raises ``ValueError`` (the reference's
+``RangeError``, :251-256) for an unplannable lattice; ``project`` encodes one
+projection (``encode`` :327-364 followed by the ``project`` kernel,
+webgpu-uniform-reference.wgsl.ts:1052-1099); ``diagnostics`` holds the decoded
+report (``readStats``, webgpu-uniform-reference.ts:1358-1421); ``destroy`` is
+idempotent (:595-606).
+
+The GPU library must be present: there is no CPU fallback.  Arrays may be
+numpy arrays (host) or torch CUDA tensors (device, zero-copy).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libfluid_projection.so")
+
+FP_OK = 0
+FLAG_NAMES = {
+    0x1: "NONFINITE",
+    0x2: "NONPOSITIVE_CURVATURE",
+    0x4: "NONPOSITIVE_PRECONDITIONER",
+    0x8: "NONCONVERGENCE",
+    0x10: "UNANCHORED_COMPONENT",
+    0x20: "PRECISION_FALLBACK",
+    0x40: "COARSE_SINGULAR",
+}
+LAYOUT_UNIFORM, LAYOUT_TALL_CELL = 0, 1
+MEMORY_HOST, MEMORY_DEVICE = 0, 1
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "cg_update": 4,
+           "true_residual": 5, "prolong_fine": 6, "restrict_fine": 7}
+
+EXPORTED_SYMBOLS = [
+    "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
+    "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
+    "fp_grid_face_counts", "fp_project", "fp_probe_apply", "fp_probe_rhs", "fp_probe_vcycle",
+    "fp_probe_level_phi",
+]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("layout", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("band", C.c_int32), ("reserved", C.c_int32), ("h", C.c_double),
+                ("column_bases", C.c_void_p)]
+
+
+class Inputs(C.Structure):
+    _fields_ = [("memory", C.c_int32), ("reserved", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p),
+                ("v", C.c_void_p), ("w", C.c_void_p), ("target_divergence", C.c_void_p),
+                ("dt", C.c_double), ("rho", C.c_double)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("pressure", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_double), ("max_iterations", C.c_int32), ("max_outer", C.c_int32),
+                ("precision", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32),
+                ("warm_start", C.c_int32), ("phi_preserve_levels", C.c_int32), ("check_every", C.c_int32),
+                ("timed_kernel", C.c_int32)]
+
+
+class Report(C.Structure):
+    _fields_ = [("flags", C.c_int32), ("iterations", C.c_int32), ("outer_iterations", C.c_int32),
+                ("levels", C.c_int32), ("relative_residual", C.c_double), ("rhs_norm", C.c_double),
+                ("max_div_before", C.c_double), ("max_div_after", C.c_double), ("device_ms", C.c_double),
+                ("liquid_samples", C.c_int64), ("samples", C.c_int64), ("timed_kernel_ms", C.c_double),
+                ("timed_kernel_launches", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["flag_names"] = [n for b, n in FLAG_NAMES.items() if self.flags & b]
+        return d
+
+
+class FluidProjectionError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build()")
+    # One HIP runtime per process: when PyTorch is present its libamdhip64
+    # (SONAME libamdhip64.so.7) must be the one the library binds to, so torch
+    # is loaded first and the dynamic loader reuses its runtime for ours.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.fp_last_error.restype = C.c_char_p
+    L.fp_abi_version.restype = C.c_int
+    L.fp_ctx_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(vp)]
+    L.fp_ctx_destroy.argtypes = [vp]
+    L.fp_ctx_bytes_in_use.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.fp_default_options.argtypes = [C.POINTER(Options)]
+    L.fp_grid_bytes.argtypes = [C.POINTER(GridDesc), C.c_int32, C.POINTER(C.c_uint64)]
+    L.fp_grid_create.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(vp)]
+    L.fp_grid_destroy.argtypes = [vp]
+    L.fp_grid_levels.argtypes = [vp, C.c_int32, vp, vp, C.POINTER(C.c_int32)]
+    L.fp_grid_face_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.fp_project.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(Report)]
+    L.fp_probe_apply.argtypes = [vp, vp, vp, vp]
+    L.fp_probe_rhs.argtypes = [vp, C.POINTER(Inputs), vp]
+    L.fp_probe_vcycle.argtypes = [vp, vp, C.POINTER(Options), vp, vp]
+    L.fp_probe_level_phi.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("fp_last_error", "fp_abi_version"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != FP_OK:
+        raise FluidProjectionError(rc, _lib.fp_last_error().decode(errors="replace"))
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    _check(load_library().fp_default_options(C.byref(o)))
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        if k == "timed_kernel" and isinstance(v, str):
+            v = KERNELS[v]
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One device, one stream, an HBM budget (fp_ctx_*)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        L = load_library()
+        self._h = C.c_void_p()
+        _check(L.fp_ctx_create(device, hbm_budget_bytes, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bytes_in_use(self) -> int:
+        b = C.c_uint64()
+        _check(_lib.fp_ctx_bytes_in_use(self._h, C.byref(b)))
+        return b.value
+
+    def destroy(self):
+        if self._h:
+            _check(_lib.fp_ctx_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _ptr(a, dtype):
+    """(pointer, memory kind, keepalive) for a numpy array or a torch CUDA tensor."""
+    if a is None:
+        return None, None, None
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        import torch
+        want = torch.float32 if dtype == np.float32 else torch.float64
+        if a.dtype != want or not a.is_contiguous():
+            raise TypeError(f"tensor must be contiguous {want}")
+        return a.data_ptr(), (MEMORY_DEVICE if a.is_cuda else MEMORY_HOST), a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr.ctypes.data, MEMORY_HOST, arr
+
+
+@dataclass
+class ProjectionResult:
+    pressure: object
+    u: object
+    v: object
+    w: object
+    report: dict
+
+
+class UniformPressureProjection:
+    """One resident uniform MAC grid (nx, ny, nz) of cubic cells of size h."""
+
+    def __init__(self, dims, spacing, context: Context | None = None):
+        nx, ny, nz = (int(x) for x in dims)
+        h = float(spacing if np.isscalar(spacing) else spacing[0])
+        if not np.isscalar(spacing) and not all(abs(float(s) - h) <= 1e-12 * h for s in spacing):
+            raise ValueError("cells must be cubic (equal spacing on every axis)")
+        self.ctx = context or default_context()
+        self.dims = (nx, ny, nz)
+        self.h = h
+        desc = GridDesc(LAYOUT_UNIFORM, nx, ny, nz, 0, 0, h, None)
+        self._g = C.c_void_p()
+        rc = load_library().fp_grid_create(self.ctx.handle, C.byref(desc), C.byref(self._g))
+        if rc == -1 or rc == -4:
+            raise ValueError(_lib.fp_last_error().decode())
+        _check(rc)
+        self.diagnostics = None
+
+    @property
+    def handle(self):
+        return self._g
+
+    @property
+    def level_count(self) -> int:
+        return len(self.levels()[0])
+
+    def levels(self):
+        dims = np.zeros(48, np.int32)
+        strides = np.zeros(48, np.int32)
+        L = C.c_int32()
+        _check(_lib.fp_grid_levels(self._g, 16, dims.ctypes.data, strides.ctypes.data, C.byref(L)))
+        return dims[: 3 * L.value].reshape(-1, 3), strides[: 3 * L.value].reshape(-1, 3)
+
+    def project(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, options: Options | None = None,
+                **opts) -> ProjectionResult:
+        """One projection.  Host arrays in -> host arrays out; CUDA tensors in ->
+        results written into `out` tensors (or fresh tensors) on the device."""
+        nx, ny, nz = self.dims
+        o = options or default_options(**opts)
+        parts = [_ptr(phi, np.float32), _ptr(u, np.float32), _ptr(v, np.float32), _ptr(w, np.float32),
+                 _ptr(target_divergence, np.float32)]
+        kinds = {p[1] for p in parts if p[1] is not None}
+        if len(kinds) != 1:
+            raise TypeError("all arrays must live in the same memory (host or device)")
+        mem = kinds.pop()
+        shapes = [(nx, ny, nz), (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+        for arr, shp in zip((phi, u, v, w), shapes):
+            if tuple(arr.shape) != shp:
+                raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
+        ins = Inputs(mem, 0, parts[0][0], parts[1][0], parts[2][0], parts[3][0], parts[4][0], float(dt), float(rho))
+        if mem == MEMORY_HOST:
+            p = np.zeros((nx, ny, nz), np.float32)
+            uo = np.zeros(shapes[1], np.float32)
+            vo = np.zeros(shapes[2], np.float32)
+            wo = np.zeros(shapes[3], np.float32)
+            outs = Outputs(p.ctypes.data, uo.ctypes.data, vo.ctypes.data, wo.ctypes.data)
+        else:
+            import torch
+            dev = phi.device
+            if out is None:
+                out = (torch.empty(shapes[0], device=dev), torch.empty(shapes[1], device=dev),
+                       torch.empty(shapes[2], device=dev), torch.empty(shapes[3], device=dev))
+            p, uo, vo, wo = out
+            outs = Outputs(p.data_ptr() if p is not None else None, uo.data_ptr(), vo.data_ptr(), wo.data_ptr())
+        rep = Report()
+        _check(_lib.fp_project(self._g, C.byref(ins), C.byref(outs), C.byref(o), C.byref(rep)))
+        self.diagnostics = rep.as_dict()
+        return ProjectionResult(p, uo, vo, wo, self.diagnostics)
+
+    # ---- parity probes --------------------------------------------------------
+    def probe_apply(self, phi, p):
+        phi = np.ascontiguousarray(phi, np.float32)
+        p = np.ascontiguousarray(p, np.float64)
+        out = np.zeros(self.dims, np.float64)
+        _check(_lib.fp_probe_apply(self._g, phi.ctypes.data, p.ctypes.data, out.ctypes.data))
+        return out
+
+    def probe_rhs(self, phi, u, v, w, dt, rho, target_divergence=None):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (phi, u, v, w)]
+        q = np.ascontiguousarray(target_divergence, np.float32) if target_divergence is not None else None
+        ins = Inputs(MEMORY_HOST, 0, arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data,
+                     arrs[3].ctypes.data, q.ctypes.data if q is not None else None, float(dt), float(rho))
+        b = np.zeros(self.dims, np.float64)
+        _check(_lib.fp_probe_rhs(self._g, C.byref(ins), b.ctypes.data))
+        return b
+
+    def probe_vcycle(self, phi, r, options: Options | None = None, **opts):
+        o = options or default_options(**opts)
+        phi = np.ascontiguousarray(phi, np.float32)
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros(self.dims, np.float32)
+        _check(_lib.fp_probe_vcycle(self._g, phi.ctypes.data, C.byref(o), r.ctypes.data, z.ctypes.data))
+        return z
+
+    def probe_level_phi(self, phi, level, preserve=2):
+        dims, _ = self.levels()
+        out = np.zeros(tuple(int(x) for x in dims[level]), np.float32)
+        phi = np.ascontiguousarray(phi, np.float32)
+        _check(_lib.fp_probe_level_phi(self._g, phi.ctypes.data, level, preserve, out.ctypes.data))
+        return out
+
+    def destroy(self):
+        if self._g:
+            _check(_lib.fp_grid_destroy(self._g))
+            self._g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def project_scene(scene, **opts) -> ProjectionResult:
+    """Convenience: one projection of a fluid_amd.scenes.UniformScene."""
+    proj = UniformPressureProjection(scene.shape, scene.h)
+    try:
+        return proj.project(scene.phi, scene.u, scene.v, scene.w, scene.dt, scene.rho, **opts)
+    finally:
+        proj.destroy()

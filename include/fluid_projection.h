@@ -1,0 +1,193 @@
+/*
+ * fluid_projection.h -- C ABI of the MI355X MAC-grid pressure projection.
+ *
+ * This is the drop-in boundary a host binds (N-API addon, ctypes, C++): plain
+ * pointers and sizes, C linkage, no exceptions, no torch types.  It replaces,
+ * for one GPU, the projection stage that the reference encodes in WebGPU:
+ *
+ *   fp_grid_create   <- new WebGPUUniformPressureMultigrid(device, dims, spacing)
+ *                       lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:228-309
+ *                       (throws RangeError on an unplannable lattice, :251-256)
+ *                       and the restricted tall-cell layout of lib/core/tall-cell-grid.ts:309-436
+ *   fp_project       <- WebGPUUniformPressureMultigrid.encode (…multigrid.ts:327-364)
+ *                       followed by the `project` kernel
+ *                       (lib/methods/uniform/webgpu-uniform-reference.ts:1280-1282,
+ *                        webgpu-uniform-reference.wgsl.ts:1052-1099)
+ *   fp_report        <- the 60-byte multigrid diagnostics buffer decoded by readStats
+ *                       (webgpu-uniform-reference.ts:1358-1421) and the MGPCG
+ *                       error bits of webgpu-octree-pipelined-mgpcg.ts:66-73
+ *   fp_grid_destroy  <- WebGPUUniformPressureMultigrid.destroy (…multigrid.ts:595-606)
+ *   fp_submit/fp_poll   no reference counterpart (the reference solves one grid per
+ *                       frame); they serve many independent projections.
+ *
+ * Conventions
+ *   Arrays are C order with z fastest.  Uniform grid (nx,ny,nz):
+ *     cells  phi, pressure, target_divergence : (nx, ny, nz)
+ *     u (nx+1, ny, nz), v (nx, ny+1, nz), w (nx, ny, nz+1); face u(i,j,k) lies
+ *     between cells i-1 and i.  Domain-boundary faces are closed walls: their
+ *     input values are ignored and they are written as 0.
+ *   Tall-cell grid (nx, fine_ny, nz, band B): packed samples (nx, B+2, nz);
+ *     packed row 0 is the tall-cell bottom sample, 1 its top sample, 2+k the
+ *     cubic band cell base+k (lib/core/tall-cell-grid.ts:206-249 indexing,
+ *     transposed to z-fastest).  Face layouts are described at
+ *     fp_tall_face_counts below.
+ *   Liquid iff phi < 0.  Pressure is in Pa, velocity in m/s, h in m, dt in s,
+ *   rho in kg/m^3.
+ *
+ * Errors: every entry point returns FP_OK (0) or a negative FP_ERR_* code and
+ * then fp_last_error() describes it (thread-local).  Numerical outcomes are
+ * never errors: they are FP_FLAG_* bits in fp_report.flags.
+ * Threading: a context is used by one host thread at a time.
+ */
+#ifndef FLUID_PROJECTION_H
+#define FLUID_PROJECTION_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_ABI_VERSION 1
+
+/* return codes */
+#define FP_OK 0
+#define FP_ERR_INVALID_ARGUMENT (-1)
+#define FP_ERR_DEVICE (-2)
+#define FP_ERR_OUT_OF_MEMORY (-3)
+#define FP_ERR_LAYOUT (-4)
+#define FP_ERR_BUDGET (-5)
+#define FP_ERR_STATE (-6)
+#define FP_ERR_UNSUPPORTED (-7)
+
+/* fp_report.flags (after webgpu-octree-pipelined-mgpcg.ts:66-73) */
+#define FP_FLAG_NONFINITE 0x1
+#define FP_FLAG_NONPOSITIVE_CURVATURE 0x2
+#define FP_FLAG_NONPOSITIVE_PRECONDITIONER 0x4
+#define FP_FLAG_NONCONVERGENCE 0x8
+#define FP_FLAG_UNANCHORED_COMPONENT 0x10
+#define FP_FLAG_PRECISION_FALLBACK 0x20
+#define FP_FLAG_COARSE_SINGULAR 0x40
+
+#define FP_LAYOUT_UNIFORM 0
+#define FP_LAYOUT_TALL_CELL 1
+
+#define FP_MEMORY_HOST 0
+#define FP_MEMORY_DEVICE 1
+
+#define FP_PRECISION_FP32 0        /* FP32 multigrid and CG, FP64 refinement */
+#define FP_PRECISION_MIXED_FP16 1  /* FP16 multigrid (per-level scaling), FP32 CG, FP64 refinement */
+#define FP_PRECISION_MIXED_BF16 2  /* BF16 multigrid, FP32 CG, FP64 refinement */
+
+typedef struct fp_ctx fp_ctx;
+typedef struct fp_grid fp_grid;
+
+typedef struct {
+  int32_t layout;   /* FP_LAYOUT_* */
+  int32_t nx;       /* cells along x */
+  int32_t ny;       /* uniform: cells along y; tall-cell: fine rows (fineNy) */
+  int32_t nz;       /* cells along z */
+  int32_t band;     /* tall-cell: cubic band rows B (regularLayers); else 0 */
+  int32_t reserved;
+  double h;         /* cubic cell size (m) */
+  const int32_t* column_bases; /* tall-cell: (nx, nz) host array, tall height per column (>= 2) */
+} fp_grid_desc;
+
+typedef struct {
+  int32_t memory;   /* FP_MEMORY_HOST or FP_MEMORY_DEVICE for every array below */
+  int32_t reserved;
+  const float* phi; /* level set at samples, metres, negative in liquid */
+  const float* u;
+  const float* v;
+  const float* w;
+  const float* target_divergence; /* optional desired divergence (1/s) per sample, or NULL */
+  double dt;
+  double rho;
+} fp_inputs;
+
+typedef struct {
+  float* pressure;  /* optional (NULL): pressure per sample (Pa) */
+  float* u;         /* projected face velocities; may alias fp_inputs u/v/w */
+  float* v;
+  float* w;
+} fp_outputs;
+
+typedef struct {
+  double relative_tolerance; /* FP64 ||b - A p||_2 / ||b||_2 target (default 1e-6) */
+  int32_t max_iterations;    /* inner MGPCG iterations per refinement pass (default 100) */
+  int32_t max_outer;         /* FP64 refinement passes (default 4) */
+  int32_t precision;         /* FP_PRECISION_* (default MIXED_FP16) */
+  int32_t pre_sweeps;        /* red-black GS sweeps before restriction (default 2) */
+  int32_t post_sweeps;       /* ... after prolongation (default 2) */
+  int32_t warm_start;        /* 1: start from the grid's resident pressure of the last solve */
+  int32_t phi_preserve_levels; /* CM11a C (default 2) */
+  int32_t check_every;       /* host convergence poll period in iterations (default 1) */
+  int32_t timed_kernel;      /* FP_KERNEL_* whose launches are timed with HIP events (0: none) */
+} fp_options;
+
+/* kernel classes for fp_options.timed_kernel */
+#define FP_KERNEL_NONE 0
+#define FP_KERNEL_SMOOTH_FINE 1    /* red-black colour pass on the finest level */
+#define FP_KERNEL_RESIDUAL_FINE 2  /* r - A e on the finest level */
+#define FP_KERNEL_CG_DIRECTION 3   /* d = z + beta d, q = A d, d.q */
+#define FP_KERNEL_CG_UPDATE 4      /* x += alpha d, r -= alpha q, r.r */
+#define FP_KERNEL_TRUE_RESIDUAL 5  /* FP64 b - A x */
+#define FP_KERNEL_PROLONG_FINE 6   /* trilinear prolongation into the finest level */
+#define FP_KERNEL_RESTRICT_FINE 7  /* restriction out of the finest level */
+
+typedef struct {
+  int32_t flags;             /* FP_FLAG_* */
+  int32_t iterations;        /* inner MGPCG iterations, all passes */
+  int32_t outer_iterations;  /* FP64 refinement passes */
+  int32_t levels;            /* multigrid levels */
+  double relative_residual;  /* final FP64 ||b - A p|| / ||b|| */
+  double rhs_norm;           /* ||b||_2 (h^2-scaled units) */
+  double max_div_before;     /* max |div u| over liquid samples before projection (1/s) */
+  double max_div_after;      /* ... after, from the FP32 output velocities */
+  double device_ms;          /* device time of the whole projection */
+  int64_t liquid_samples;
+  int64_t samples;
+  double timed_kernel_ms;        /* summed device time of the timed kernel's launches */
+  int32_t timed_kernel_launches; /* how many launches were timed */
+  int32_t reserved;
+} fp_report;
+
+/* ---- context ------------------------------------------------------------ */
+const char* fp_last_error(void);
+int fp_abi_version(void);
+int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out);
+int fp_ctx_destroy(fp_ctx* ctx);
+int fp_ctx_bytes_in_use(const fp_ctx* ctx, uint64_t* bytes);
+
+/* ---- grids --------------------------------------------------------------- */
+int fp_default_options(fp_options* options);
+/* Bytes a grid of this description would hold on the device. */
+int fp_grid_bytes(const fp_grid_desc* desc, int32_t precision, uint64_t* bytes);
+int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out);
+int fp_grid_destroy(fp_grid* grid);
+/* Multigrid plan: writes levels*3 dims (and strides) into the arrays of
+ * capacity 3*max_levels; returns the level count through *levels. */
+int fp_grid_levels(const fp_grid* grid, int32_t max_levels, int32_t* dims, int32_t* strides, int32_t* levels);
+/* Face counts of the grid's three velocity arrays. */
+int fp_grid_face_counts(const fp_grid* grid, int64_t* nu, int64_t* nv, int64_t* nw);
+
+/* ---- projection ------------------------------------------------------------ */
+/* One complete projection: labels + theta assembly -> divergence/rhs ->
+ * MGPCG to the FP64 relative tolerance -> gradient subtract.  Synchronous. */
+int fp_project(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_options* options, fp_report* report);
+
+/* ---- parity probes (device work on host arrays; used by the test suite) ---- */
+/* out = A p (FP64) on the finest level, h^2-scaled. */
+int fp_probe_apply(fp_grid* grid, const float* phi, const double* p, double* out);
+/* b (FP64, h^2-scaled) from host inputs. */
+int fp_probe_rhs(fp_grid* grid, const fp_inputs* in, double* b);
+/* z = M^-1 r, one V-cycle of the preconditioner in the given precision. */
+int fp_probe_vcycle(fp_grid* grid, const float* phi, const fp_options* options, const float* r, float* z);
+/* The level-l level set of the multigrid hierarchy (FP32). */
+int fp_probe_level_phi(fp_grid* grid, const float* phi, int32_t level, int32_t phi_preserve_levels, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

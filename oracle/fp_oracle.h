@@ -1,0 +1,148 @@
+/*
+ * fp_oracle.h -- FP64 CPU oracle for the MAC-grid pressure projection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker.
+ *
+ * The oracle is a plain-C restatement of the reference's algebra:
+ *   - ghost-fluid theta ........ lib/core/cm12-numerics.ts:37-43
+ *   - A = G^T W G / theta ....... lib/methods/adaptive-mass/sparse-atlas-composite-projection.ts:1078-1082,1171-1192
+ *   - rhs = G^T W u ............. sparse-atlas-composite-projection.ts:1238-1261
+ *   - Jacobi-PCG + nullspace .... sparse-atlas-composite-projection.ts:1194-1211,1346-1457
+ *   - Neumaier/Kahan dots ........ sparse-atlas-composite-projection.ts:1213-1225,
+ *                                  lib/methods/octree-shared/octree-pipelined-pcg.ts:146-157
+ *   - generic direct-curvature PCG lib/methods/octree-shared/octree-pipelined-pcg.ts:305-594
+ *   - level planning ............ lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:99-169
+ *   - coarse phi (CM11a Eq.15-16) lib/methods/uniform/webgpu-uniform-pressure-multigrid.wgsl.ts:169-195
+ *   - gradient subtract ......... lib/methods/uniform/webgpu-uniform-reference.wgsl.ts:1052-1099
+ *   - tall-cell layout .......... lib/core/tall-cell-grid.ts:61-106,309-436
+ *
+ * Storage order everywhere is C order with z fastest:
+ *   cell (i,j,k)            -> (i*ny + j)*nz + k
+ *   x-face u(i,j,k), i<=nx  -> (i*ny + j)*nz + k          (between cells i-1 and i)
+ *   y-face v(i,j,k), j<=ny  -> (i*(ny+1) + j)*nz + k
+ *   z-face w(i,j,k), k<=nz  -> (i*ny + j)*(nz+1) + k
+ */
+#ifndef FP_ORACLE_H
+#define FP_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_THETA_MIN 0.05
+#define ORC_THETA_EPS 1e-9
+#define ORC_MAX_LEVELS 16
+
+/* ---- shared numerics ---------------------------------------------------- */
+double orc_ghost_theta(double liquid_phi, double air_phi, double eps);
+double orc_dot_kahan(const double* a, const double* b, size_t n);     /* sparse-atlas dot() */
+double orc_dot_neumaier(const double* a, const double* b, size_t n);  /* octree-pipelined-pcg dot() */
+
+/* ---- multigrid level planning ------------------------------------------- */
+/* Returns the number of levels (>=1) or -1.  dims[l*3+a], strides[l*3+a]
+ * (stride of level l relative to l-1; level 0 strides are 1).  `extended`
+ * (out, may be NULL) is 1 when the reference planner rejects the lattice and
+ * the ceil-halving extension produced the plan. */
+int orc_plan_levels(int nx, int ny, int nz, int* dims, int* strides, int* extended);
+/* The reference planner alone: returns levels or -1 on rejection. */
+int orc_plan_levels_reference(int nx, int ny, int nz, int* dims);
+
+/* ---- uniform MAC grid --------------------------------------------------- */
+/* Face weights (h^2-scaled coefficients): 0 on walls and air-air faces,
+ * 1 between two liquid cells, 1/theta across a liquid-air face. */
+void orc_face_weights(int nx, int ny, int nz, const float* phi,
+                      double* wx, double* wy, double* wz);
+/* Level-l variant used by the multigrid: phi in double, per-axis geometric
+ * factor g[a] multiplies every weight of axis a. */
+void orc_face_weights_d(int nx, int ny, int nz, const double* phi, const double g[3],
+                        double* wx, double* wy, double* wz);
+/* b = -(rho h/dt) * sum(u_out - u_in) + (rho h^2/dt) q  on liquid cells. */
+void orc_rhs(int nx, int ny, int nz, double h, double dt, double rho,
+             const float* phi, const float* u, const float* v, const float* w,
+             const float* q, double* b);
+/* out = A p on liquid cells (0 elsewhere). */
+void orc_apply(int nx, int ny, int nz, const float* phi,
+               const double* wx, const double* wy, const double* wz,
+               const double* p, double* out);
+void orc_apply_d(int nx, int ny, int nz, const double* phi,
+                 const double* wx, const double* wy, const double* wz,
+                 const double* p, double* out);
+void orc_diagonal(int nx, int ny, int nz, const float* phi,
+                  const double* wx, const double* wy, const double* wz, double* diag);
+/* Liquid components through liquid-liquid faces.  comp[c] = component id or
+ * -1; anchored[id] = 1 when the component has a liquid-air face.  Returns the
+ * component count. */
+int orc_components(int nx, int ny, int nz, const float* phi,
+                   const double* wx, const double* wy, const double* wz,
+                   int32_t* comp, uint8_t* anchored);
+/* Velocity update u -= dt/(rho h) * w * (p+ - p-), air p = 0; faces with no
+ * liquid side and wall faces become 0.  Outputs may alias the inputs. */
+void orc_project(int nx, int ny, int nz, double h, double dt, double rho,
+                 const float* phi, const double* wx, const double* wy, const double* wz,
+                 const double* p, const float* u, const float* v, const float* w,
+                 double* uo, double* vo, double* wo);
+/* max over liquid cells of |(1/h) sum(u_out - u_in) - q|. */
+double orc_max_divergence(int nx, int ny, int nz, double h, const float* phi,
+                          const double* u, const double* v, const double* w, const float* q);
+
+/* ---- solvers ------------------------------------------------------------ */
+typedef struct {
+  int32_t iterations;
+  int32_t converged;
+  int32_t components;
+  int32_t unanchored;
+  double  rhs_norm;
+  double  relative_residual;   /* ||b - A p|| / ||b||, recomputed, after nullspace removal */
+  double  max_residual;
+} orc_solve_report;
+
+/* Sparse-atlas Jacobi-PCG (sparse-atlas-composite-projection.ts:1346-1457). */
+int orc_jacobi_pcg(int nx, int ny, int nz, const float* phi, const double* b,
+                   double rel_tol, double abs_tol, int max_iter,
+                   double* p, orc_solve_report* rep);
+
+typedef struct {
+  int32_t pre_sweeps;
+  int32_t post_sweeps;
+  int32_t phi_preserve_levels;  /* C of CM11a (2) */
+} orc_mg_options;
+
+/* FP64 MGPCG with the multigrid V-cycle the GPU path uses. */
+int orc_mgpcg(int nx, int ny, int nz, const float* phi, const double* b,
+              const orc_mg_options* opt, double rel_tol, int max_iter,
+              double* p, orc_solve_report* rep);
+/* One application of the V-cycle preconditioner z = M^-1 r (for parity). */
+int orc_vcycle_apply(int nx, int ny, int nz, const float* phi, const orc_mg_options* opt,
+                     const double* r, double* z);
+/* Coarse-level phi and weights of the hierarchy, for parity of the
+ * coarsening maps: writes the level-l phi into out (size of level l). */
+int orc_level_phi(int nx, int ny, int nz, const float* phi, int phi_preserve_levels,
+                  int level, double* out);
+
+/* Generic direct-curvature PCG on a dense row-major SPD matrix with an
+ * optional diagonal (Jacobi) preconditioner -- octree-pipelined-pcg.ts:305-594.
+ * Returns 0 on acceptance, otherwise a failure code:
+ *   11/12/13 operator not linear/symmetric/positive (audit),
+ *   21/22/23 preconditioner not linear/symmetric/positive,
+ *   3 non-positive preconditioned residual, 4 non-positive curvature,
+ *   5 maximum iterations, 6 true-residual validation, 7 non-finite. */
+int orc_dense_pcg(int n, const double* A, const double* b, const double* x0,
+                  int use_jacobi, int max_iter, double rel_tol, double abs_tol,
+                  int validate, double* x, int* iterations, int* reductions,
+                  double* residual_norm);
+
+/* ---- restricted tall-cell layout ---------------------------------------- */
+/* chooseTallCellBase (tall-cell-grid.ts:61-82). */
+int orc_tall_choose_base(int lowest_surface, int highest_surface, int fine_ny,
+                         int regular_layers, int liquid_halo, int air_halo);
+/* limitNeighboringTallCellBases (tall-cell-grid.ts:84-106), in place. */
+void orc_tall_limit_bases(int32_t* bases, int nx, int nz, int max_delta, int passes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,281 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/fp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  It is the checker, never the measured or
+shipped path.  Arrays are numpy, C order, z fastest:
+cells (nx, ny, nz); u (nx+1, ny, nz); v (nx, ny+1, nz); w (nx, ny, nz+1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libfp_oracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class SolveReport(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("components", C.c_int32),
+        ("unanchored", C.c_int32),
+        ("rhs_norm", C.c_double),
+        ("relative_residual", C.c_double),
+        ("max_residual", C.c_double),
+    ]
+
+
+class MGOptions(C.Structure):
+    _fields_ = [("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("phi_preserve_levels", C.c_int32)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc) into oracle/_build/."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        i = C.c_int
+        d = C.c_double
+        L.orc_ghost_theta.restype = d
+        L.orc_ghost_theta.argtypes = [d, d, d]
+        L.orc_dot_kahan.restype = d
+        L.orc_dot_kahan.argtypes = [_dp, _dp, C.c_size_t]
+        L.orc_dot_neumaier.restype = d
+        L.orc_dot_neumaier.argtypes = [_dp, _dp, C.c_size_t]
+        L.orc_plan_levels.restype = i
+        L.orc_plan_levels.argtypes = [i, i, i, _ip, _ip, C.POINTER(i)]
+        L.orc_plan_levels_reference.restype = i
+        L.orc_plan_levels_reference.argtypes = [i, i, i, _ip]
+        L.orc_face_weights.argtypes = [i, i, i, _fp, _dp, _dp, _dp]
+        L.orc_rhs.argtypes = [i, i, i, d, d, d, _fp, _fp, _fp, _fp, C.c_void_p, _dp]
+        L.orc_apply.argtypes = [i, i, i, _fp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_diagonal.argtypes = [i, i, i, _fp, _dp, _dp, _dp, _dp]
+        L.orc_components.restype = i
+        L.orc_components.argtypes = [i, i, i, _fp, _dp, _dp, _dp, _ip, _u8]
+        L.orc_project.argtypes = [i, i, i, d, d, d, _fp, _dp, _dp, _dp, _dp, _fp, _fp, _fp, _dp, _dp, _dp]
+        L.orc_max_divergence.restype = d
+        L.orc_max_divergence.argtypes = [i, i, i, d, _fp, _dp, _dp, _dp, C.c_void_p]
+        L.orc_jacobi_pcg.restype = i
+        L.orc_jacobi_pcg.argtypes = [i, i, i, _fp, _dp, d, d, i, _dp, C.POINTER(SolveReport)]
+        L.orc_mgpcg.restype = i
+        L.orc_mgpcg.argtypes = [i, i, i, _fp, _dp, C.POINTER(MGOptions), d, i, _dp, C.POINTER(SolveReport)]
+        L.orc_vcycle_apply.restype = i
+        L.orc_vcycle_apply.argtypes = [i, i, i, _fp, C.POINTER(MGOptions), _dp, _dp]
+        L.orc_level_phi.restype = i
+        L.orc_level_phi.argtypes = [i, i, i, _fp, i, i, _dp]
+        L.orc_dense_pcg.restype = i
+        L.orc_dense_pcg.argtypes = [i, _dp, _dp, C.c_void_p, i, i, d, d, i, _dp,
+                                    C.POINTER(i), C.POINTER(i), C.POINTER(d)]
+        L.orc_tall_choose_base.restype = i
+        L.orc_tall_choose_base.argtypes = [i, i, i, i, i, i]
+        L.orc_tall_limit_bases.argtypes = [_ip, i, i, i, i]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def ghost_theta(liquid_phi: float, air_phi: float, eps: float = 1e-9) -> float:
+    return lib().orc_ghost_theta(float(liquid_phi), float(air_phi), float(eps))
+
+
+def plan_levels(nx, ny, nz):
+    """(dims[L,3], strides[L,3], extended) or None when no plan exists."""
+    dims = np.zeros(16 * 3, np.int32)
+    strides = np.zeros(16 * 3, np.int32)
+    ext = C.c_int(0)
+    L = lib().orc_plan_levels(nx, ny, nz, dims, strides, C.byref(ext))
+    if L < 0:
+        return None
+    return dims[: 3 * L].reshape(L, 3), strides[: 3 * L].reshape(L, 3), bool(ext.value)
+
+
+def plan_levels_reference(nx, ny, nz):
+    dims = np.zeros(16 * 3, np.int32)
+    L = lib().orc_plan_levels_reference(nx, ny, nz, dims)
+    if L < 0:
+        return None
+    return dims[: 3 * L].reshape(L, 3)
+
+
+def face_weights(phi):
+    nx, ny, nz = phi.shape
+    wx = np.zeros((nx + 1, ny, nz))
+    wy = np.zeros((nx, ny + 1, nz))
+    wz = np.zeros((nx, ny, nz + 1))
+    lib().orc_face_weights(nx, ny, nz, _f32(phi), wx, wy, wz)
+    return wx, wy, wz
+
+
+def rhs(phi, u, v, w, h, dt, rho, q=None):
+    nx, ny, nz = phi.shape
+    b = np.zeros((nx, ny, nz))
+    qa = _f32(q) if q is not None else None
+    lib().orc_rhs(nx, ny, nz, h, dt, rho, _f32(phi), _f32(u), _f32(v), _f32(w),
+                  qa.ctypes.data if qa is not None else None, b)
+    return b
+
+
+def apply(phi, p, weights=None):
+    nx, ny, nz = phi.shape
+    wx, wy, wz = weights if weights is not None else face_weights(phi)
+    out = np.zeros((nx, ny, nz))
+    lib().orc_apply(nx, ny, nz, _f32(phi), wx, wy, wz, _f64(p), out)
+    return out
+
+
+def diagonal(phi, weights=None):
+    nx, ny, nz = phi.shape
+    wx, wy, wz = weights if weights is not None else face_weights(phi)
+    out = np.zeros((nx, ny, nz))
+    lib().orc_diagonal(nx, ny, nz, _f32(phi), wx, wy, wz, out)
+    return out
+
+
+def components(phi, weights=None):
+    nx, ny, nz = phi.shape
+    wx, wy, wz = weights if weights is not None else face_weights(phi)
+    comp = np.zeros((nx, ny, nz), np.int32)
+    anchored = np.zeros(nx * ny * nz + 1, np.uint8)
+    n = lib().orc_components(nx, ny, nz, _f32(phi), wx, wy, wz, comp, anchored)
+    return comp, anchored[:n].astype(bool)
+
+
+def project(phi, p, u, v, w, h, dt, rho, weights=None):
+    nx, ny, nz = phi.shape
+    wx, wy, wz = weights if weights is not None else face_weights(phi)
+    uo = np.zeros((nx + 1, ny, nz))
+    vo = np.zeros((nx, ny + 1, nz))
+    wo = np.zeros((nx, ny, nz + 1))
+    lib().orc_project(nx, ny, nz, h, dt, rho, _f32(phi), wx, wy, wz, _f64(p),
+                      _f32(u), _f32(v), _f32(w), uo, vo, wo)
+    return uo, vo, wo
+
+
+def max_divergence(phi, u, v, w, h, q=None):
+    nx, ny, nz = phi.shape
+    qa = _f32(q) if q is not None else None
+    return lib().orc_max_divergence(nx, ny, nz, h, _f32(phi), _f64(u), _f64(v), _f64(w),
+                                    qa.ctypes.data if qa is not None else None)
+
+
+def _report(rep):
+    return {f: getattr(rep, f) for f, _ in SolveReport._fields_}
+
+
+def jacobi_pcg(phi, b, rel_tol=1e-10, abs_tol=1e-12, max_iter=None, p0=None):
+    nx, ny, nz = phi.shape
+    p = _f64(p0).copy() if p0 is not None else np.zeros((nx, ny, nz))
+    rep = SolveReport()
+    if max_iter is None:
+        max_iter = max(64, 4 * nx * ny * nz)
+    status = lib().orc_jacobi_pcg(nx, ny, nz, _f32(phi), _f64(b), rel_tol, abs_tol, int(max_iter), p,
+                                  C.byref(rep))
+    out = _report(rep)
+    out["status"] = status
+    return p, out
+
+
+def mgpcg(phi, b, rel_tol=1e-10, max_iter=200, pre=2, post=2, preserve=2, p0=None):
+    nx, ny, nz = phi.shape
+    p = _f64(p0).copy() if p0 is not None else np.zeros((nx, ny, nz))
+    rep = SolveReport()
+    opt = MGOptions(pre, post, preserve)
+    status = lib().orc_mgpcg(nx, ny, nz, _f32(phi), _f64(b), C.byref(opt), rel_tol, int(max_iter), p,
+                             C.byref(rep))
+    out = _report(rep)
+    out["status"] = status
+    return p, out
+
+
+def vcycle(phi, r, pre=2, post=2, preserve=2):
+    nx, ny, nz = phi.shape
+    z = np.zeros((nx, ny, nz))
+    opt = MGOptions(pre, post, preserve)
+    if lib().orc_vcycle_apply(nx, ny, nz, _f32(phi), C.byref(opt), _f64(r), z) != 0:
+        raise ValueError("no multigrid plan")
+    return z
+
+
+def level_phi(phi, level, preserve=2):
+    nx, ny, nz = phi.shape
+    plan = plan_levels(nx, ny, nz)
+    dims = plan[0][level]
+    out = np.zeros(tuple(int(x) for x in dims))
+    if lib().orc_level_phi(nx, ny, nz, _f32(phi), preserve, level, out) != 0:
+        raise ValueError("bad level")
+    return out
+
+
+def dense_pcg(A, b, x0=None, jacobi=True, max_iter=None, rel_tol=1e-10, abs_tol=0.0, validate=True):
+    A = _f64(A)
+    n = A.shape[0]
+    x = np.zeros(n)
+    it, red, rn = C.c_int(0), C.c_int(0), C.c_double(0)
+    x0a = _f64(x0) if x0 is not None else None
+    status = lib().orc_dense_pcg(n, A, _f64(b), x0a.ctypes.data if x0a is not None else None,
+                                 int(jacobi), int(max_iter if max_iter is not None else n), rel_tol, abs_tol,
+                                 int(validate), x, C.byref(it), C.byref(red), C.byref(rn))
+    return x, {"status": status, "iterations": it.value, "reductions": red.value, "residual_norm": rn.value}
+
+
+def tall_choose_base(lowest, highest, fine_ny, regular_layers=32, liquid_halo=16, air_halo=8):
+    return lib().orc_tall_choose_base(lowest, highest, fine_ny, regular_layers, liquid_halo, air_halo)
+
+
+def tall_limit_bases(bases, max_delta=4, passes=None):
+    b = np.ascontiguousarray(bases, dtype=np.int32).copy()
+    nx, nz = b.shape
+    if passes is None:
+        passes = nx + nz
+    lib().orc_tall_limit_bases(b, nx, nz, max_delta, passes)
+    return b
+
+
+def uniform_projection(phi, u, v, w, h, dt, rho, rel_tol=1e-10, solver="mgpcg", q=None, p0=None):
+    """Full FP64 projection: rhs -> solve -> gradient subtract -> diagnostics."""
+    weights = face_weights(phi)
+    b = rhs(phi, u, v, w, h, dt, rho, q)
+    if solver == "mgpcg":
+        p, rep = mgpcg(phi, b, rel_tol=rel_tol, p0=p0)
+    else:
+        p, rep = jacobi_pcg(phi, b, rel_tol=rel_tol, abs_tol=0.0, p0=p0)
+    uo, vo, wo = project(phi, p, u, v, w, h, dt, rho, weights)
+    rep["max_div_before"] = max_divergence(phi, u.astype(np.float64) * _wall_mask(u, 0),
+                                           v.astype(np.float64) * _wall_mask(v, 1),
+                                           w.astype(np.float64) * _wall_mask(w, 2), h, q)
+    rep["max_div_after"] = max_divergence(phi, uo, vo, wo, h, q)
+    return p, (uo, vo, wo), b, rep
+
+
+def _wall_mask(a, axis):
+    m = np.ones(a.shape)
+    sl = [slice(None)] * 3
+    sl[axis] = 0
+    m[tuple(sl)] = 0
+    sl[axis] = -1
+    m[tuple(sl)] = 0
+    return m

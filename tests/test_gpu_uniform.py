@@ -1,0 +1,237 @@
+"""GPU parity of the uniform projection (HIP library through the C ABI)
+against the FP64 CPU oracle on identical seeded inputs.
+
+Bars (stated per test): integer maps bit-exact; FP64 rhs/operator to 1e-12
+relative; pressure and projected face velocities within 1e-5 relative L2 of
+an FP64 oracle solve; FP64 relative residual <= the requested tolerance; max
+|div u| after projection no larger than the oracle's at the same tolerance
+(both measured on FP32-rounded velocities)."""
+import numpy as np
+import pytest
+
+from fluid_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["fp32", "fp16", "bf16"]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return scenes.dam_break_scene(64, 0)
+
+
+@pytest.fixture(scope="module")
+def c0_oracle(oracle, c0):
+    s = c0
+    p, vel, b, rep = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-12)
+    return p, vel, b, rep
+
+
+def oracle_maxdiv_fp32(oracle, s, tol):
+    p, (uo, vo, wo), b, rep = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=tol)
+    return oracle.max_divergence(s.phi, uo.astype(np.float32), vo.astype(np.float32), wo.astype(np.float32), s.h)
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 64), (384, 384, 384), (37, 37, 37), (128, 128, 8), (48, 32, 40),
+                                  (100, 100, 100), (2, 2, 2)])
+def test_level_plan_bit_exact(fpl, oracle, dims):
+    g = fpl.UniformPressureProjection(dims, 1.0 / dims[0])
+    d, s = g.levels()
+    od, os_, _ = oracle.plan_levels(*dims)
+    assert d.tolist() == od.tolist()
+    assert s.tolist() == os_.tolist()
+    g.destroy()
+
+
+def test_coarse_level_sets_and_labels(fpl, oracle, c0):
+    g = fpl.UniformPressureProjection(c0.shape, c0.h)
+    dims, _ = g.levels()
+    for level in range(1, len(dims)):
+        gp = g.probe_level_phi(c0.phi, level)
+        op = oracle.level_phi(c0.phi, level)
+        assert gp.shape == op.shape
+        np.testing.assert_allclose(gp, op, rtol=1e-5, atol=1e-7 * (1 + np.abs(op).max()))
+        # labels bit-exact wherever the FP64 value is not within rounding of 0
+        sure = np.abs(op) > 1e-6 * np.abs(op).max()
+        assert np.array_equal((gp < 0)[sure], (op < 0)[sure])
+    g.destroy()
+
+
+def test_rhs_matches_oracle(fpl, oracle, c0):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    b = g.probe_rhs(s.phi, s.u, s.v, s.w, s.dt, s.rho)
+    ob = oracle.rhs(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho)
+    assert np.abs(b - ob).max() <= 1e-12 * np.abs(ob).max()
+    q = np.random.default_rng(3).uniform(-1, 1, s.shape).astype(np.float32)
+    bq = g.probe_rhs(s.phi, s.u, s.v, s.w, s.dt, s.rho, target_divergence=q)
+    obq = oracle.rhs(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, q)
+    assert np.abs(bq - obq).max() <= 1e-12 * np.abs(obq).max()
+    g.destroy()
+
+
+def test_operator_matches_oracle(fpl, oracle, c0):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    p = np.random.default_rng(1).standard_normal(s.shape)
+    ap = g.probe_apply(s.phi, p)
+    oap = oracle.apply(s.phi, p)
+    assert np.abs(ap - oap).max() <= 1e-12 * np.abs(oap).max()
+    g.destroy()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16", 3e-2), ("fp16", 1e-2)])
+def test_vcycle_matches_oracle(fpl, oracle, c0, precision, tol):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    r = oracle.rhs(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho).astype(np.float32)
+    z = g.probe_vcycle(s.phi, r, precision=precision)
+    oz = oracle.vcycle(s.phi, r.astype(np.float64))
+    assert rel_l2(z, oz) < tol, rel_l2(z, oz)
+    g.destroy()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_projection_c0_parity(fpl, oracle, c0, c0_oracle, precision):
+    s = c0
+    op, (ou, ov, ow), ob, orep = c0_oracle
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    res = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision=precision, relative_tolerance=1e-6)
+    rep = res.report
+    print(precision, rep)
+    assert rep["flags"] == 0, rep["flag_names"]
+    assert rep["relative_residual"] <= 1e-6
+    assert rel_l2(res.pressure, op) < 1e-5
+    for a, b in ((res.u, ou), (res.v, ov), (res.w, ow)):
+        assert rel_l2(a, b) < 1e-5
+    # divergence: the GPU's FP32 output against the oracle's at the same tolerance
+    gdiv = oracle.max_divergence(s.phi, res.u, res.v, res.w, s.h)
+    assert gdiv == pytest.approx(rep["max_div_after"], rel=1e-3)
+    odiv = oracle_maxdiv_fp32(oracle, s, 1e-6)
+    print("max div gpu", gdiv, "oracle at the same tolerance", odiv)
+    assert gdiv <= odiv
+    assert rep["max_div_before"] == pytest.approx(orep["max_div_before"], rel=1e-5)
+    assert rep["liquid_samples"] == int((s.phi < 0).sum())
+    g.destroy()
+
+
+def test_tight_tolerance_reaches_fp64_accuracy(fpl, c0, c0_oracle):
+    s = c0
+    op = c0_oracle[0]
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    res = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp16", relative_tolerance=1e-10)
+    assert res.report["relative_residual"] <= 1e-10
+    assert rel_l2(res.pressure, op) < 1e-7  # FP32 output rounding
+    g.destroy()
+
+
+def test_hydrostatic_column_gpu(fpl):
+    n, H = 32, 0.53
+    h = 1.0 / n
+    y = (np.arange(n) + 0.5) * h
+    phi = np.broadcast_to((y - H)[None, :, None], (n, n, n)).astype(np.float32).copy()
+    dt, rho, gacc = 1 / 60, 998.2, 9.81
+    mu, mv, mw = scenes.fluid_face_masks(phi)
+    u = np.zeros((n + 1, n, n), np.float32)
+    v = np.where(mv, np.float32(-gacc * dt), 0).astype(np.float32)
+    w = np.zeros((n, n, n + 1), np.float32)
+    g = fpl.UniformPressureProjection((n, n, n), h)
+    res = g.project(phi, u, v, w, dt, rho, relative_tolerance=1e-9)
+    exact = np.where(phi < 0, rho * gacc * (H - y)[None, :, None], 0.0)
+    assert np.abs(res.pressure - exact).max() < 2e-6 * exact.max()
+    assert np.abs(res.v).max() < 1e-5
+    g.destroy()
+
+
+def test_empty_and_full_domains(fpl):
+    n = 16
+    g = fpl.UniformPressureProjection((n, n, n), 1 / n)
+    rng = np.random.default_rng(0)
+    u = rng.uniform(-1, 1, (n + 1, n, n)).astype(np.float32)
+    v = rng.uniform(-1, 1, (n, n + 1, n)).astype(np.float32)
+    w = rng.uniform(-1, 1, (n, n, n + 1)).astype(np.float32)
+    air = np.ones((n, n, n), np.float32)
+    res = g.project(air, u, v, w, 0.01, 1000.0)
+    assert res.report["flags"] == 0
+    assert np.abs(res.pressure).max() == 0 and np.abs(res.u).max() == 0
+    g.destroy()
+
+
+@pytest.mark.parametrize("dims", [(37, 37, 37), (48, 32, 40), (5, 7, 3)])
+def test_uneven_grids(fpl, oracle, dims):
+    n = dims[0]
+    rng = np.random.default_rng(sum(dims))
+    nx, ny, nz = dims
+    h = 1.0 / n
+    c = lambda m: (np.arange(m) + 0.5) * h
+    phi = scenes.box_sdf(c(nx)[:, None, None], c(ny)[None, :, None], c(nz)[None, None, :],
+                         (0, 0, 0), (0.55, 0.45 * ny * h * 1.3, 0.6)).astype(np.float32)
+    mu, mv, mw = scenes.fluid_face_masks(phi)
+    u = np.where(mu, rng.uniform(-0.3, 0.3, mu.shape), 0).astype(np.float32)
+    v = np.where(mv, rng.uniform(-0.3, 0.3, mv.shape) - 0.08, 0).astype(np.float32)
+    w = np.where(mw, rng.uniform(-0.3, 0.3, mw.shape), 0).astype(np.float32)
+    op, (ou, ov, ow), ob, orep = oracle.uniform_projection(phi, u, v, w, h, 1 / 120, 998.2, rel_tol=1e-12)
+    g = fpl.UniformPressureProjection(dims, h)
+    res = g.project(phi, u, v, w, 1 / 120, 998.2, relative_tolerance=1e-7)
+    assert res.report["flags"] == 0, res.report
+    assert res.report["relative_residual"] <= 1e-7
+    assert rel_l2(res.pressure, op) < 1e-5
+    assert rel_l2(res.v, ov) < 1e-5
+    g.destroy()
+
+
+def test_warm_start_is_same_solution_fewer_iterations(fpl, c0, c0_oracle):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    cold = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, relative_tolerance=1e-7)
+    warm = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, relative_tolerance=1e-7, warm_start=1)
+    assert warm.report["iterations"] < cold.report["iterations"]
+    assert rel_l2(warm.pressure, c0_oracle[0]) < 1e-5
+    g.destroy()
+
+
+def test_device_memory_path_matches_host(fpl, c0):
+    import torch
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    host = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho)
+    t = [torch.from_numpy(a).cuda() for a in (s.phi, s.u, s.v, s.w)]
+    dev = g.project(*t, s.dt, s.rho)
+    torch.cuda.synchronize()
+    assert np.array_equal(dev.pressure.cpu().numpy(), host.pressure)
+    assert np.array_equal(dev.u.cpu().numpy(), host.u)
+    g.destroy()
+
+
+def test_repeatable_bitwise(fpl, c0):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    a = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho)
+    b = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho)
+    assert np.array_equal(a.pressure, b.pressure) and np.array_equal(a.w, b.w)
+    g.destroy()
+
+
+def test_iteration_cap_reports_nonconvergence(fpl, c0):
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    res = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, max_iterations=1, max_outer=1, relative_tolerance=1e-9)
+    assert "NONCONVERGENCE" in res.report["flag_names"]
+    assert np.all(np.isfinite(res.pressure))
+    g.destroy()
+
+
+def test_budget_is_enforced(fpl):
+    ctx = fpl.Context(0, hbm_budget_bytes=1 << 20)
+    with pytest.raises(fpl.FluidProjectionError) as e:
+        fpl.UniformPressureProjection((64, 64, 64), 1 / 64, context=ctx)
+    assert e.value.code == -5
+    ctx.destroy()
