@@ -39,7 +39,27 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(res.stderr)
     os.replace(tmp, LIB)
+    build_node_addon()
     return LIB
+
+
+NODE_ADDON = os.path.join(LIBDIR, "fluid_projection.node")
+
+
+def build_node_addon() -> str | None:
+    """The N-API addon (plain g++ against the node headers) over the C ABI."""
+    inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(inc, "node_api.h")) or not shutil.which("g++"):
+        return None
+    tmp = NODE_ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-DNODE_GYP_MODULE_NAME=fluid_projection",
+           "-I", inc, os.path.join(HERE, "node", "fp_addon.cc"), "-L", LIBDIR, "-lfluid_projection",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("N-API addon build failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, NODE_ADDON)
+    return NODE_ADDON
 
 
 if __name__ == "__main__":

@@ -108,6 +108,7 @@ int fp_default_options(fp_options* o) {
   o->phi_preserve_levels = 2;
   o->check_every = 1;
   o->timed_kernel = FP_KERNEL_NONE;
+  o->graphs = 1;
   return FP_OK;
 }
 

@@ -94,38 +94,89 @@ __device__ __forceinline__ double atomic_load_d(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deterministic grid reduction: every block stores its partials, the last
-// block to arrive (ticket counter) sums them in a fixed order.  Returns true
-// in the last block; `out` is valid in its thread 0.  The counter is reset by
-// the last block, so the kernel can be replayed (graphs) without a memset.
+// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS
+// operations and meets the other waves, without draining outstanding global
+// loads (so a prefetch issued before it stays in flight).  The memory clobber
+// keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Two-kernel deterministic reduction.  Producers store one partial per
+// workgroup (no fence: the kernel boundary publishes them); a one-workgroup
+// finisher sums them in a fixed order.  This keeps device-scope fences (an L2
+// write-back on CDNA's per-XCD caches) out of the bandwidth-bound kernels.
+__device__ __forceinline__ unsigned grid_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
+__device__ __forceinline__ unsigned block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter, double (&out)[NV]) {
-  __shared__ bool last;
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
   block_sum<NV>(v);
-  const unsigned G = gridDim.x;
   if (threadIdx.x == 0) {
+    const unsigned G = grid_blocks(), b = block_linear();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + blockIdx.x] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    last = (t == G - 1);
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + b] = v[k];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* partials, unsigned G, double (&out)[NV]) {
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     s[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s[k] += atomic_load_d(&partials[(size_t)k * G + b]);
+    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s[k] += partials[(size_t)k * G + b];
   }
   block_sum<NV>(s);
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = s[k];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = 0; k < NV; ++k) out[k] = s[k];
+}
+
+// Block max of non-negative floats; valid in thread 0.
+__device__ __forceinline__ float block_max(float m) {
+  __shared__ float shm[FP_WAVES];
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  float r = 0.0f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < FP_WAVES; ++w) r = fmaxf(r, shm[w]);
+  __syncthreads();
+  return r;
+}
+
+// As block_partials, plus a max stored after the NV sums (slot NV).  Maxima go
+// through partials too: one device-scope atomic per wave on a single address
+// serialises across the XCDs.
+template <int NV>
+__device__ __forceinline__ void block_partials_max(double (&v)[NV], float m, double* partials) {
+  const float bm = block_max(m);
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+    const unsigned G = grid_blocks(), b = block_linear();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + b] = v[k];
+    partials[(size_t)NV * G + b] = (double)bm;
   }
-  return true;
+}
+
+template <int NV>
+__device__ __forceinline__ void sum_partials_max(const double* partials, unsigned G, double (&out)[NV], float& mx) {
+  sum_partials<NV>(partials, G, out);
+  float m = 0.0f;
+  for (unsigned b = threadIdx.x; b < G; b += blockDim.x) m = fmaxf(m, (float)partials[(size_t)NV * G + b]);
+  mx = block_max(m);
+}
+
+// Fast h^2-scaled face coefficient for the FP32 solve and the multigrid:
+// 1/theta = clamp(max(|l| + |a|, eps) / |l|, 1, 20) with a hardware reciprocal
+// (one ulp), selected without branches.  The FP64 operator keeps the exact
+// quotient of face_weight_d.
+__device__ __forceinline__ float face_weight_fast(float pc, float pn, float g) {
+  const bool lc = pc < 0.0f, ln = pn < 0.0f;
+  const float l = lc ? pc : pn, a = lc ? pn : pc;
+  const float al = fabsf(l);
+  const float inv = fminf(20.0f, fmaxf(1.0f, fmaxf(al + fabsf(a), 1e-9f) * __builtin_amdgcn_rcpf(al)));
+  const float w = (lc && ln) ? g : g * inv;
+  return (lc || ln) ? w : 0.0f;
 }
 
 // Max of non-negative floats through their bit patterns (order preserving).

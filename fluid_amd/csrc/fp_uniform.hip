@@ -19,6 +19,8 @@
 //                    device-side alpha/beta, merged reductions, early exit
 //   subtract ....... project (webgpu-uniform-reference.wgsl.ts:1052-1099)
 #include "fp_uniform.hpp"
+#include "fp_sweep.hpp"
+#include "fp_march.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -49,12 +51,40 @@ struct LevelArgs {
 
 #define GRID_STRIDE(idx, N) for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (N); idx += (size_t)gridDim.x * blockDim.x)
 
+// 4-wide loads of the storage types (arrays are 256-byte aligned).
+template <typename T> struct V4;
+template <> struct V4<float> {
+  static __device__ __forceinline__ float4 ld(const float* p, size_t i4) { return reinterpret_cast<const float4*>(p)[i4]; }
+};
+template <> struct V4<__half> {
+  static __device__ __forceinline__ float4 ld(const __half* p, size_t i4) {
+    const uint2 u = reinterpret_cast<const uint2*>(p)[i4];
+    __half2 a, b;
+    __builtin_memcpy(&a, &u.x, 4);
+    __builtin_memcpy(&b, &u.y, 4);
+    const float2 fa = __half22float2(a), fb = __half22float2(b);
+    return make_float4(fa.x, fa.y, fb.x, fb.y);
+  }
+};
+template <> struct V4<__hip_bfloat16> {
+  static __device__ __forceinline__ float4 ld(const __hip_bfloat16* p, size_t i4) {
+    const uint2 u = reinterpret_cast<const uint2*>(p)[i4];
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u), __uint_as_float(u.y << 16),
+                       __uint_as_float(u.y & 0xffff0000u));
+  }
+};
+
 // --------------------------------------------------------------------------
 // generic fills
 // --------------------------------------------------------------------------
 template <typename T>
 __global__ void k_fill(T* p, size_t n, T v) {
   GRID_STRIDE(i, n) p[i] = v;
+}
+__global__ void k_fill4(float* p, size_t n, float v) {
+  const size_t n4 = n / 4;
+  GRID_STRIDE(i, n4) reinterpret_cast<float4*>(p)[i] = make_float4(v, v, v, v);
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) p[n4 * 4 + threadIdx.x] = v;
 }
 
 // --------------------------------------------------------------------------
@@ -88,10 +118,15 @@ __global__ void k_rhs(Grid3 g, const float* __restrict__ phi, const float* __res
     }
     b[idx] = val;
   }
-  dmax = wave_max(dmax);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&st->divmax_before, dmax);
+  block_partials_max<2>(acc, dmax, partials);
+}
+
+__global__ void k_fin_rhs(const double* __restrict__ partials, unsigned G, double tol2, State* st) {
   double out[2];
-  if (grid_sum<2>(acc, partials, &st->counters[0], out) && threadIdx.x == 0) {
+  float mx;
+  sum_partials_max<2>(partials, G, out, mx);
+  if (threadIdx.x == 0) {
+    st->divmax_before = __float_as_uint(mx);
     st->bb = out[0];
     // The refinement stops at a quarter of the requested relative residual:
     // the max-norm divergence left in the FP32 output then stays below that
@@ -101,68 +136,39 @@ __global__ void k_rhs(Grid3 g, const float* __restrict__ phi, const float* __res
   }
 }
 
-// --------------------------------------------------------------------------
-// FP64 true residual r = b - A x (x = FP64 refinement iterate), written to the
-// FP32 CG residual; decides convergence of the refinement and the target of
-// the next inner solve.
-// --------------------------------------------------------------------------
-__global__ void k_true_residual(Grid3 g, const float* __restrict__ phi, const double* __restrict__ b,
-                                const double* __restrict__ x, float* __restrict__ r, State* st,
-                                double* partials) {
-  double acc[1] = {0.0};
-  float rmax = 0.0f;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  GRID_STRIDE(idx, g.n()) {
-    int i, j, k;
-    g.ijk(idx, i, j, k);
-    const float pc = phi[idx];
-    double res = 0.0;
-    if (pc < 0.0f) {
-      const double xc = x[idx];
-      double ap = 0.0;
-#define NBR(cond, off)                                         \
-  if (cond) {                                                  \
-    const size_t m = idx + (off);                              \
-    const float pn = phi[m];                                   \
-    const double wf = face_weight_d(pc, pn);                   \
-    ap += wf * (xc - (pn < 0.0f ? x[m] : 0.0));                \
-  }
-      NBR(i > 0, -(ptrdiff_t)((size_t)ny * nz))
-      NBR(i + 1 < nx, (ptrdiff_t)((size_t)ny * nz))
-      NBR(j > 0, -(ptrdiff_t)nz)
-      NBR(j + 1 < ny, (ptrdiff_t)nz)
-      NBR(k > 0, -1)
-      NBR(k + 1 < nz, 1)
-#undef NBR
-      res = b[idx] - ap;
-      acc[0] += res * res;
-      rmax = fmaxf(rmax, (float)fabs(res));
-    }
-    r[idx] = (float)res;
-  }
-  rmax = wave_max(rmax);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&st->rmax_bits, rmax);
+// Refinement control after the FP64 true residual (OpTrueResidual): decides
+// convergence of the refinement and the target of the next inner solve.
+__global__ void k_fin_true(const double* __restrict__ partials, unsigned G, State* st) {
   double out[1];
-  if (grid_sum<1>(acc, partials, &st->counters[1], out) && threadIdx.x == 0) {
+  float m;
+  sum_partials_max<1>(partials, G, out, m);
+  if (threadIdx.x == 0) {
     const double rr = out[0];
     st->rr64 = rr;
     const bool finite = isfinite(rr);
     st->outer_done = (!finite) || rr <= st->tol2_bb;
     if (!finite) st->flags |= 0x1;  // FP_FLAG_NONFINITE
-    // inner solve: a tenth of the requested tolerance (margin for the max-norm
+    // inner solve: a tenth of the refinement target (margin for the max-norm
     // divergence), at most what FP32 CG can be trusted to reduce in one pass.
     st->inner_target2 = fmax(0.01 * st->tol2_bb, 1e-12 * rr);
     st->iter = 0;
     st->done = st->outer_done;
-    const float m = __uint_as_float(__hip_atomic_load(&st->rmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
-    __hip_atomic_store(&st->rmax_bits, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
 // x += e; e = 0 (folds one inner solve into the FP64 iterate).
 __global__ void k_accumulate(size_t n, double* __restrict__ x, float* __restrict__ e) {
-  GRID_STRIDE(i, n) {
+  const size_t n4 = n / 4;
+  GRID_STRIDE(i, n4) {
+    const float4 ev = reinterpret_cast<const float4*>(e)[i];
+    double4 xv = reinterpret_cast<const double4*>(x)[i];
+    xv.x += ev.x; xv.y += ev.y; xv.z += ev.z; xv.w += ev.w;
+    reinterpret_cast<double4*>(x)[i] = xv;
+    reinterpret_cast<float4*>(e)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const size_t i = n4 * 4 + threadIdx.x;
     x[i] += (double)e[i];
     e[i] = 0.0f;
   }
@@ -495,9 +501,27 @@ __global__ void k_cg_rz(size_t n, const float* __restrict__ r, const T* __restri
   if (st->done) return;
   const float zm = zmul_static / st->rscale;
   double acc[1] = {0.0};
-  GRID_STRIDE(i, n) acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
+  {
+    const size_t n4 = n / 4;
+    GRID_STRIDE(i, n4) {
+      const float4 rv = reinterpret_cast<const float4*>(r)[i];
+      const float4 zv = V4<T>::ld(z, i);
+      acc[0] += ((double)rv.x * (double)(zv.x * zm) + (double)rv.y * (double)(zv.y * zm)) +
+                ((double)rv.z * (double)(zv.z * zm) + (double)rv.w * (double)(zv.w * zm));
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+      const size_t i = n4 * 4 + threadIdx.x;
+      acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
+    }
+  }
+  block_partials<1>(acc, partials);
+}
+
+__global__ void k_fin_rz(const double* __restrict__ partials, unsigned G, State* st) {
+  if (st->done) return;
   double out[1];
-  if (grid_sum<1>(acc, partials, &st->counters[2], out) && threadIdx.x == 0) {
+  sum_partials<1>(partials, G, out);
+  if (threadIdx.x == 0) {
     const double rz = out[0];
     if (!(rz > 0.0) || !isfinite(rz)) {
       st->flags |= isfinite(rz) ? 0x4 : 0x1;  // NONPOSITIVE_PRECONDITIONER / NONFINITE
@@ -509,46 +533,11 @@ __global__ void k_cg_rz(size_t n, const float* __restrict__ r, const T* __restri
   }
 }
 
-// d = z + beta d_old ; q = A d ; delta = d . q  (FP32 operator)
-template <typename T>
-__global__ void k_cg_direction(LevelArgs L, const T* __restrict__ z, float zmul_static,
-                               const float* __restrict__ dold, float* __restrict__ dnew, float* __restrict__ q,
-                               State* st, double* partials) {
+__global__ void k_fin_dq(const double* __restrict__ partials, unsigned G, State* st) {
   if (st->done) return;
-  const float zm = zmul_static / st->rscale;
-  const float beta = (float)st->beta;
-  const Grid3 g = L.g;
-  const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
-  double acc[1] = {0.0};
-  GRID_STRIDE(idx, g.n()) {
-    int i, j, k;
-    g.ijk(idx, i, j, k);
-    const float pc = L.phi[idx];
-    float dc = 0.0f, qc = 0.0f;
-    if (pc < 0.0f) {
-      dc = ldT(z, idx) * zm + beta * dold[idx];
-#define DNB(cond, off, gg)                                                    \
-  if (cond) {                                                                 \
-    const size_t m = idx + (off);                                             \
-    const float pn = L.phi[m];                                                \
-    const float wf = face_weight(pc, pn, gg);                                 \
-    const float dn = (pn < 0.0f) ? (ldT(z, m) * zm + beta * dold[m]) : 0.0f;  \
-    qc += wf * (dc - dn);                                                     \
-  }
-      DNB(i > 0, -(ptrdiff_t)sx, L.gx)
-      DNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
-      DNB(j > 0, -(ptrdiff_t)sy, L.gy)
-      DNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
-      DNB(k > 0, -1, L.gz)
-      DNB(k + 1 < g.nz, 1, L.gz)
-#undef DNB
-      acc[0] += (double)dc * (double)qc;
-    }
-    dnew[idx] = dc;
-    q[idx] = qc;
-  }
   double out[1];
-  if (grid_sum<1>(acc, partials, &st->counters[3], out) && threadIdx.x == 0) {
+  sum_partials<1>(partials, G, out);
+  if (threadIdx.x == 0) {
     const double dq = out[0];
     if (!(dq > 0.0) || !isfinite(dq)) {
       st->flags |= isfinite(dq) ? 0x2 : 0x1;  // NONPOSITIVE_CURVATURE / NONFINITE
@@ -561,23 +550,41 @@ __global__ void k_cg_direction(LevelArgs L, const T* __restrict__ z, float zmul_
 
 // e += alpha d ; r -= alpha q ; ||r||^2, max|r| -> convergence and next rscale.
 __global__ void k_cg_update(size_t n, const float* __restrict__ d, const float* __restrict__ q,
-                            float* __restrict__ e, float* __restrict__ r, int max_iter, State* st,
-                            double* partials) {
+                            float* __restrict__ e, float* __restrict__ r, State* st, double* partials) {
   if (st->done) return;
   const float alpha = (float)st->alpha;
   double acc[1] = {0.0};
   float rmax = 0.0f;
-  GRID_STRIDE(i, n) {
+  const size_t n4 = n / 4;
+  GRID_STRIDE(i, n4) {
+    const float4 dv = reinterpret_cast<const float4*>(d)[i];
+    const float4 qv = reinterpret_cast<const float4*>(q)[i];
+    float4 ev = reinterpret_cast<const float4*>(e)[i];
+    float4 rv = reinterpret_cast<const float4*>(r)[i];
+    ev.x += alpha * dv.x; ev.y += alpha * dv.y; ev.z += alpha * dv.z; ev.w += alpha * dv.w;
+    rv.x -= alpha * qv.x; rv.y -= alpha * qv.y; rv.z -= alpha * qv.z; rv.w -= alpha * qv.w;
+    reinterpret_cast<float4*>(e)[i] = ev;
+    reinterpret_cast<float4*>(r)[i] = rv;
+    acc[0] += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
+    rmax = fmaxf(rmax, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const size_t i = n4 * 4 + threadIdx.x;
     e[i] += alpha * d[i];
     const float rn = r[i] - alpha * q[i];
     r[i] = rn;
     acc[0] += (double)rn * (double)rn;
     rmax = fmaxf(rmax, fabsf(rn));
   }
-  rmax = wave_max(rmax);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&st->rmax_bits, rmax);
+  block_partials_max<1>(acc, rmax, partials);
+}
+
+__global__ void k_fin_update(const double* __restrict__ partials, unsigned G, int max_iter, State* st) {
+  if (st->done) return;
   double out[1];
-  if (grid_sum<1>(acc, partials, &st->counters[4], out) && threadIdx.x == 0) {
+  float m;
+  sum_partials_max<1>(partials, G, out, m);
+  if (threadIdx.x == 0) {
     const double rr = out[0];
     st->rr = rr;
     st->iter += 1;
@@ -588,9 +595,7 @@ __global__ void k_cg_update(size_t n, const float* __restrict__ d, const float* 
     } else if (rr <= st->inner_target2 || st->iter >= max_iter) {
       st->done = 1;
     }
-    const float m = __uint_as_float(__hip_atomic_load(&st->rmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
-    __hip_atomic_store(&st->rmax_bits, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -632,7 +637,7 @@ __global__ void k_subtract(Grid3 g, int axis, const float* __restrict__ phi, con
 
 __global__ void k_div_after(Grid3 g, const float* __restrict__ phi, const float* __restrict__ u,
                             const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ q,
-                            double inv_h, State* st) {
+                            double inv_h, double* partials) {
   float dmax = 0.0f;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   GRID_STRIDE(idx, g.n()) {
@@ -644,8 +649,15 @@ __global__ void k_div_after(Grid3 g, const float* __restrict__ phi, const float*
                         ((double)w[((size_t)i * ny + j) * (nz + 1) + k + 1] - (double)w[((size_t)i * ny + j) * (nz + 1) + k]);
     dmax = fmaxf(dmax, (float)fabs(flux * inv_h - (q ? (double)q[idx] : 0.0)));
   }
-  dmax = wave_max(dmax);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&st->divmax_after, dmax);
+  double none[1] = {0.0};
+  block_partials_max<1>(none, dmax, partials);
+}
+
+__global__ void k_fin_divafter(const double* __restrict__ partials, unsigned G, State* st) {
+  double out[1];
+  float mx;
+  sum_partials_max<1>(partials, G, out, mx);
+  if (threadIdx.x == 0) st->divmax_after = __float_as_uint(mx);
 }
 
 // FP64 operator probe: out = A p.
@@ -797,6 +809,8 @@ void UniformSolver::timer_end(hipStream_t s) {
 }
 
 void UniformSolver::release() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  graph_exec_ = nullptr;
   for (void* p : allocations_) (void)hipFree(p);
   allocations_.clear();
   for (void* p : pinned_) (void)hipHostFree(p);
@@ -832,7 +846,8 @@ size_t UniformSolver::estimate_bytes(int nx, int ny, int nz) {
   for (int l = 0; l < L; ++l) {
     const size_t nl = (size_t)dims[l][0] * dims[l][1] * dims[l][2];
     add(nl * 4);      // phi
-    add(nl * 4 * 3);  // r e t
+    add(nl * 4 * 4);  // r e e2 t
+    add(nl * 16);     // packed coefficients
   }
   add(512 * 512 * 8 + 4096);
   add(sizeof(State) + 8 * kMaxBlocks * 8);
@@ -872,6 +887,8 @@ int UniformSolver::create(int nx, int ny, int nz, double h) {
     if ((rc = alloc((void**)&lv.phi, lv.n * 4))) return rc;
     if ((rc = alloc(&lv.r, lv.n * 4))) return rc;
     if ((rc = alloc(&lv.e, lv.n * 4))) return rc;
+    if ((rc = alloc(&lv.e2, lv.n * 4))) return rc;
+    if ((rc = alloc(&lv.w, lv.n * 16))) return rc;
     if ((rc = alloc(&lv.t, lv.n * 4))) return rc;
   }
   const UniformLevel& c = lv_[L_ - 1];
@@ -912,9 +929,52 @@ void UniformSolver::set_level_scales(int precision) {
   }
 }
 
+// Fused sweeps run on levels large enough to fill a 256-lane column tile.
+static bool fused_level(const UniformLevel& l) { return l.d[2] >= 16 && l.d[1] >= 4 && l.d[0] >= 4 && l.n >= 4096; }
+
+template <typename TR, typename T>
+static void launch_sweep(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein, T* eout,
+                         int first, int zero_init, const ProlongArgs& P, const State* st) {
+  const W4<T>* w = (const W4<T>*)f.w;
+  const SweepGeom G{f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2]};
+  const int TZ = f.d[2] >= 64 ? 64 : 32;
+  const int TY = 256 / TZ;
+  const int tz = (f.d[2] + TZ - 1) / TZ, ty = (f.d[1] + TY - 1) / TY;
+  const long tiles = (long)tz * ty;
+  int chunk = (int)std::max<long>(8, ((long)f.d[0] * tiles + 4095) / 4096);
+  chunk = std::min(chunk, f.d[0]);
+  const int cx = (f.d[0] + chunk - 1) / chunk;
+  const dim3 grid(tz, ty, cx);
+  if (TZ == 64)
+    hipLaunchKernelGGL((k_sweep2<64, TR, T>), grid, dim3(256), 0, s, G, w, r, rmul, dyn, ein, eout, first, zero_init,
+                       P, chunk, st);
+  else
+    hipLaunchKernelGGL((k_sweep2<32, TR, T>), grid, dim3(256), 0, s, G, w, r, rmul, dyn, ein, eout, first, zero_init,
+                       P, chunk, st);
+}
+
+void UniformSolver::build_weights(hipStream_t s, int precision) {
+  for (int l = 0; l < L_ - 1; ++l) {
+    const UniformLevel& f = lv_[l];
+    if (!fused_level(f)) continue;
+    if (precision == FP_PRECISION_MIXED_FP16)
+      LAUNCH(k_build_weights<__half>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2], (const float*)f.phi,
+             (W4<__half>*)f.w);
+    else if (precision == FP_PRECISION_MIXED_BF16)
+      LAUNCH(k_build_weights<__hip_bfloat16>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2],
+             (const float*)f.phi, (W4<__hip_bfloat16>*)f.w);
+    else
+      LAUNCH(k_build_weights<float>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2], (const float*)f.phi,
+             (W4<float>*)f.w);
+  }
+}
+
 template <typename T>
 int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
-  // level 0's right-hand side is the FP32 CG residual times the device rscale
+  // level 0's right-hand side is the FP32 CG residual times the device rscale.
+  // cur[l] tracks which of the level's two correction buffers holds e.
+  int cur[kMaxLevels] = {0};
+  const ProlongArgs noP{nullptr, nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
   for (int l = 0; l < L_ - 1; ++l) {
     UniformLevel& f = lv_[l];
     UniformLevel& c = lv_[l + 1];
@@ -922,20 +982,49 @@ int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
     const bool top = (l == 0);
     const void* rr = top ? (const void*)r_ : f.r;
     const float rmul = (top ? 1.0f : f.Sr) / f.Se;  // r_true/Se in stored-e units
-    T* e = (T*)f.e;
-    if (pre == 0) LAUNCH(k_fill<T>, f.n, s, e, f.n, T(0.0f));
+    const bool fused = fused_level(f);
+    if (pre == 0) LAUNCH(k_fill<T>, f.n, s, (T*)f.e, f.n, T(0.0f));
     for (int sw = 0; sw < pre; ++sw) {
-      for (int colour = 0; colour < 2; ++colour) {
-        const int zi = (sw == 0 && colour == 0) ? 1 : 0;
-        if (top) TLAUNCH(FP_KERNEL_SMOOTH_FINE, (k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, zi, (const State*)st_);
-        else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, zi, (const State*)st_);
+      if (fused) {
+        T* ein = (T*)(cur[l] ? f.e2 : f.e);
+        T* eout = (T*)(cur[l] ? f.e : f.e2);
+        const bool t = top && timed_kernel_ == FP_KERNEL_SMOOTH_FINE;
+        if (t) timer_begin(s);
+        if (top) launch_sweep<float, T>(s, f, (const float*)rr, rmul, 1, ein, eout, 0, sw == 0, noP, st_);
+        else launch_sweep<T, T>(s, f, (const T*)rr, rmul, 0, ein, eout, 0, sw == 0, noP, st_);
+        if (t) timer_end(s);
+        cur[l] ^= 1;
+      } else {
+        T* e = (T*)(cur[l] ? f.e2 : f.e);
+        for (int colour = 0; colour < 2; ++colour) {
+          const int zi = (sw == 0 && colour == 0) ? 1 : 0;
+          if (top) LAUNCH((k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, zi, (const State*)st_);
+          else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, zi, (const State*)st_);
+        }
       }
     }
+    T* e = (T*)(cur[l] ? f.e2 : f.e);
     // t = r - A e, stored with scale St
     const float r_t = (top ? 1.0f : f.Sr) / f.St;
     const float e_t = f.Se / f.St;
-    if (top) TLAUNCH(FP_KERNEL_RESIDUAL_FINE, (k_residual<float, T>), f.n, s, fa, (const float*)rr, r_t, 1, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
-    else LAUNCH((k_residual<T, T>), f.n, s, fa, (const T*)rr, r_t, 0, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
+    if (fused) {
+      const bool t = top && timed_kernel_ == FP_KERNEL_RESIDUAL_FINE;
+      if (t) timer_begin(s);
+      if (top) {
+        OpResidualW<float, T> op{f.d[1], f.d[2], (const W4<T>*)f.w, (const float*)rr, (const T*)e, (T*)f.t, r_t, e_t,
+                                 1.0f, 1, 0.0f, st_, nullptr};
+        launch_march(s, f.d[0], f.d[1], f.d[2], op, st_, 1);
+      } else {
+        OpResidualW<T, T> op{f.d[1], f.d[2], (const W4<T>*)f.w, (const T*)rr, (const T*)e, (T*)f.t, r_t, e_t, 1.0f, 0,
+                             0.0f, st_, nullptr};
+        launch_march(s, f.d[0], f.d[1], f.d[2], op, st_, 1);
+      }
+      if (t) timer_end(s);
+    } else if (top) {
+      TLAUNCH(FP_KERNEL_RESIDUAL_FINE, (k_residual<float, T>), f.n, s, fa, (const float*)rr, r_t, 1, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
+    } else {
+      LAUNCH((k_residual<T, T>), f.n, s, fa, (const T*)rr, r_t, 0, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
+    }
     const int k2 = (c.s[0] == 2) + (c.s[1] == 2) + (c.s[2] == 2);
     const float fac = (1.0f / (float)(1 << k2)) * f.St / c.Sr;
     TLAUNCH(top ? FP_KERNEL_RESTRICT_FINE : -1, k_restrict<T>, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const T*)f.t,
@@ -945,6 +1034,7 @@ int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
     UniformLevel& c = lv_[L_ - 1];
     const bool top = (L_ == 1);
     const float rmul = top ? 1.0f : c.Sr;
+    cur[L_ - 1] = 0;
     if (top)
       hipLaunchKernelGGL((k_coarse_solve<T, float>), dim3(1), dim3(FP_BLOCK), 0, s, Grid3{c.d[0], c.d[1], c.d[2]},
                          (const int*)cmap_, (const int*)ncoarse_, (const double*)minv_, (const float*)r_, rmul, 1,
@@ -959,19 +1049,36 @@ int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
     UniformLevel& c = lv_[l + 1];
     const LevelArgs fa = largs(f);
     const bool top = (l == 0);
-    T* e = (T*)f.e;
-    TLAUNCH(top ? FP_KERNEL_PROLONG_FINE : -1, k_prolong_add<T>, f.n, s, fa, e, Grid3{c.d[0], c.d[1], c.d[2]},
-            (const float*)c.phi, (const T*)c.e, c.s[0], c.s[1], c.s[2], c.Se / f.Se, (const State*)st_);
     const void* rr = top ? (const void*)r_ : f.r;
     const float rmul = (top ? 1.0f : f.Sr) / f.Se;
-    for (int sw = 0; sw < post; ++sw) {
-      for (int cc = 0; cc < 2; ++cc) {
-        const int colour = 1 - cc;
-        if (top) TLAUNCH(FP_KERNEL_SMOOTH_FINE, (k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, 0, (const State*)st_);
-        else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, 0, (const State*)st_);
+    const bool fused = fused_level(f);
+    const T* ec = (const T*)(cur[l + 1] ? c.e2 : c.e);
+    if (!fused || post == 0) {
+      T* e = (T*)(cur[l] ? f.e2 : f.e);
+      TLAUNCH(top ? FP_KERNEL_PROLONG_FINE : -1, k_prolong_add<T>, f.n, s, fa, e, Grid3{c.d[0], c.d[1], c.d[2]},
+              (const float*)c.phi, ec, c.s[0], c.s[1], c.s[2], c.Se / f.Se, (const State*)st_);
+      for (int sw = 0; sw < post; ++sw)
+        for (int cc = 0; cc < 2; ++cc) {
+          const int colour = 1 - cc;
+          if (top) LAUNCH((k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, 0, (const State*)st_);
+          else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, 0, (const State*)st_);
+        }
+    } else {
+      for (int sw = 0; sw < post; ++sw) {
+        ProlongArgs P = noP;
+        if (sw == 0) P = ProlongArgs{(const void*)ec, (const float*)c.phi, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / f.Se};
+        T* ein = (T*)(cur[l] ? f.e2 : f.e);
+        T* eout = (T*)(cur[l] ? f.e : f.e2);
+        const bool t = top && timed_kernel_ == FP_KERNEL_SMOOTH_FINE;
+        if (t) timer_begin(s);
+        if (top) launch_sweep<float, T>(s, f, (const float*)rr, rmul, 1, ein, eout, 1, 0, P, st_);
+        else launch_sweep<T, T>(s, f, (const T*)rr, rmul, 0, ein, eout, 1, 0, P, st_);
+        if (t) timer_end(s);
+        cur[l] ^= 1;
       }
     }
   }
+  zbuf_ = cur[0] ? lv_[0].e2 : lv_[0].e;
   return FP_OK;
 }
 
@@ -980,11 +1087,55 @@ int UniformSolver::iteration(hipStream_t s, int pre, int post, int max_iter, int
   vcycle<T>(s, pre, post);
   const UniformLevel& f = lv_[0];
   const float zmul = f.Se;  // z_true * rscale = e_stored * Se
-  LAUNCH(k_cg_rz<T>, n_, s, n_, (const float*)r_, (const T*)f.e, zmul, st_, partials_);
-  TLAUNCH(FP_KERNEL_CG_DIRECTION, k_cg_direction<T>, n_, s, largs(f), (const T*)f.e, zmul, (const float*)d_[parity],
-          d_[parity ^ 1], q_, st_, partials_);
-  TLAUNCH(FP_KERNEL_CG_UPDATE, k_cg_update, n_, s, n_, (const float*)d_[parity ^ 1], (const float*)q_, e_, r_, max_iter,
+  const T* z = (const T*)zbuf_;
+  LAUNCH(k_cg_rz<T>, n_ / 4 + 1, s, n_, (const float*)r_, z, zmul, st_, partials_);
+  hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_ / 4 + 1), st_);
+  {
+    const bool t = timed_kernel_ == FP_KERNEL_CG_DIRECTION;
+    if (t) timer_begin(s);
+    OpDirection<T> op{ny_, nz_, f.phi, z, (const float*)d_[parity], d_[parity ^ 1], q_, zmul, 0.0f, 0.0f,
+                      f.g[0], f.g[1], f.g[2], st_, partials_};
+    const unsigned G = launch_march(s, nx_, ny_, nz_, op, st_, 1);
+    if (t) timer_end(s);
+    hipLaunchKernelGGL(k_fin_dq, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, G, st_);
+  }
+  TLAUNCH(FP_KERNEL_CG_UPDATE, k_cg_update, n_ / 4 + 1, s, n_, (const float*)d_[parity ^ 1], (const float*)q_, e_, r_,
           st_, partials_);
+  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_ / 4 + 1),
+                     max_iter, st_);
+  return FP_OK;
+}
+
+void UniformSolver::run_iteration(hipStream_t s, int precision, int pre, int post, int max_iter, int parity) {
+  if (precision == FP_PRECISION_MIXED_FP16) iteration<__half>(s, pre, post, max_iter, parity);
+  else if (precision == FP_PRECISION_MIXED_BF16) iteration<__hip_bfloat16>(s, pre, post, max_iter, parity);
+  else iteration<float>(s, pre, post, max_iter, parity);
+}
+
+int UniformSolver::ensure_graph(hipStream_t s, int precision, int pre, int post, int max_iter) {
+  if (graph_exec_ && graph_key_[0] == precision && graph_key_[1] == pre && graph_key_[2] == post &&
+      graph_key_[3] == max_iter)
+    return FP_OK;
+  if (graph_exec_) {
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  run_iteration(s, precision, pre, post, max_iter, 0);
+  run_iteration(s, precision, pre, post, max_iter, 1);
+  HIPCHK(hipStreamEndCapture(s, &g));
+  hipError_t e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    graph_exec_ = nullptr;
+    set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  graph_key_[0] = precision;
+  graph_key_[1] = pre;
+  graph_key_[2] = post;
+  graph_key_[3] = max_iter;
   return FP_OK;
 }
 
@@ -1049,18 +1200,26 @@ int UniformSolver::project(hipStream_t s, const fp_inputs* in, fp_outputs* out, 
   const double rho = in->rho, dt = in->dt;
   LAUNCH(k_rhs, n_, s, g, phi_l0, u, v, w, q, rho * h_ / dt, rho * h_ * h_ / dt, 1.0 / h_, tol * tol, b_, st_,
          partials_);
+  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), tol * tol, st_);
   if (!opt->warm_start) LAUNCH(k_fill<double>, n_, s, x_, n_, 0.0);
-  LAUNCH(k_fill<float>, n_, s, e_, n_, 0.0f);
-  LAUNCH(k_fill<float>, n_, s, d_[0], n_, 0.0f);
+  LAUNCH(k_fill4, n_ / 4 + 1, s, e_, n_, 0.0f);
+  LAUNCH(k_fill4, n_ / 4 + 1, s, d_[0], n_, 0.0f);
   set_level_scales(opt->precision);
+  build_weights(s, opt->precision);
   int outer = 0, total = 0;
   int flags = 0;
   const int max_iter = std::max(1, opt->max_iterations);
   const int check = std::max(1, opt->check_every);
   bool converged = false;
   for (outer = 0; outer < std::max(1, opt->max_outer) + 1; ++outer) {
-    TLAUNCH(FP_KERNEL_TRUE_RESIDUAL, k_true_residual, n_, s, g, phi_l0, (const double*)b_, (const double*)x_, r_, st_,
-            partials_);
+    {
+      const bool t = timed_kernel_ == FP_KERNEL_TRUE_RESIDUAL;
+      if (t) timer_begin(s);
+      OpTrueResidual op{ny_, nz_, phi_l0, (const double*)b_, (const double*)x_, r_, st_, partials_};
+      const unsigned G = launch_march(s, nx_, ny_, nz_, op, st_, 0);
+      if (t) timer_end(s);
+      hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, G, st_);
+    }
     HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (host_state_->outer_done) {
@@ -1068,27 +1227,40 @@ int UniformSolver::project(hipStream_t s, const fp_inputs* in, fp_outputs* out, 
       break;
     }
     if (outer == std::max(1, opt->max_outer)) break;
-    int parity = 0;
-    for (int it = 0; it < max_iter; ++it) {
-      if (opt->precision == FP_PRECISION_MIXED_FP16) iteration<__half>(s, opt->pre_sweeps, opt->post_sweeps, max_iter, parity);
-      else if (opt->precision == FP_PRECISION_MIXED_BF16) iteration<__hip_bfloat16>(s, opt->pre_sweeps, opt->post_sweeps, max_iter, parity);
-      else iteration<float>(s, opt->pre_sweeps, opt->post_sweeps, max_iter, parity);
-      parity ^= 1;
-      if ((it + 1) % check == 0 || it + 1 == max_iter) {
-        HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (host_state_->done) break;
+    const bool graphs = opt->graphs && timed_kernel_ == 0;
+    if (graphs) {
+      // one hipGraph holds two CG + V-cycle iterations (the d ping-pong);
+      // kernels of finished iterations return at once, so replays past
+      // convergence cost only their launch
+      int rc = ensure_graph(s, opt->precision, opt->pre_sweeps, opt->post_sweeps, max_iter);
+      if (rc) return rc;
+      for (int it = 0; it < max_iter; it += 2) {
+        HIPCHK(hipGraphLaunch(graph_exec_, s));
+        if ((it + 2) % std::max(2, check + (check & 1)) == 0 || it + 2 >= max_iter) {
+          HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
+          HIPCHK(hipStreamSynchronize(s));
+          if (host_state_->done) break;
+        }
+      }
+    } else {
+      int parity = 0;
+      for (int it = 0; it < max_iter; ++it) {
+        run_iteration(s, opt->precision, opt->pre_sweeps, opt->post_sweeps, max_iter, parity);
+        parity ^= 1;
+        if ((it + 1) % check == 0 || it + 1 == max_iter) {
+          HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
+          HIPCHK(hipStreamSynchronize(s));
+          if (host_state_->done) break;
+        }
       }
     }
-    // the last direction buffer written is d_[parity]; make it d_[0] for the next pass
-    if (parity == 1) std::swap(d_[0], d_[1]);
     if (host_state_->flags & 0x7) {
       // a broken recurrence: keep what the FP64 iterate has (fail closed)
       flags |= host_state_->flags;
-      LAUNCH(k_fill<float>, n_, s, e_, n_, 0.0f);
+      LAUNCH(k_fill4, n_ / 4 + 1, s, e_, n_, 0.0f);
     }
-    LAUNCH(k_accumulate, n_, s, n_, x_, e_);
-    LAUNCH(k_fill<float>, n_, s, d_[0], n_, 0.0f);
+    LAUNCH(k_accumulate, n_ / 4 + 1, s, n_, x_, e_);
+    LAUNCH(k_fill4, n_ / 4 + 1, s, d_[0], n_, 0.0f);
   }
   total = host_state_->total_iter;
   flags |= host_state_->flags;
@@ -1109,7 +1281,10 @@ int UniformSolver::project(hipStream_t s, const fp_inputs* in, fp_outputs* out, 
   if (dev_u) LAUNCH(k_subtract, nu, s, g, 0, phi_l0, (const double*)x_, u, dev_u, sc);
   if (dev_v) LAUNCH(k_subtract, nv, s, g, 1, phi_l0, (const double*)x_, v, dev_v, sc);
   if (dev_w) LAUNCH(k_subtract, nw, s, g, 2, phi_l0, (const double*)x_, w, dev_w, sc);
-  if (dev_u && dev_v && dev_w) LAUNCH(k_div_after, n_, s, g, phi_l0, dev_u, dev_v, dev_w, q, 1.0 / h_, st_);
+  if (dev_u && dev_v && dev_w) {
+    LAUNCH(k_div_after, n_, s, g, phi_l0, dev_u, dev_v, dev_w, q, 1.0 / h_, partials_);
+    hipLaunchKernelGGL(k_fin_divafter, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), st_);
+  }
   if (pout && dev_p) LAUNCH(k_pressure_out, n_, s, n_, phi_l0, (const double*)x_, dev_p);
   HIPCHK(hipEventRecord(ev1, s));
   if (host_out) {
@@ -1189,6 +1364,7 @@ int UniformSolver::probe_rhs(hipStream_t s, const fp_inputs* in, double* b_host)
   const double rho = in->rho, dt = in->dt;
   LAUNCH(k_rhs, n_, s, Grid3{nx_, ny_, nz_}, (const float*)lv_[0].phi, (const float*)in_u_, (const float*)in_v_,
          (const float*)in_w_, q, rho * h_ / dt, rho * h_ * h_ / dt, 1.0 / h_, 0.0, b_, st_, partials_);
+  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), 0.0, st_);
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipMemcpy(b_host, b_, n_ * 8, hipMemcpyDeviceToHost));
   return FP_OK;
@@ -1214,6 +1390,7 @@ int UniformSolver::probe_vcycle(hipStream_t s, const float* phi_host, const fp_o
                        st_);
   }
   set_level_scales(opt->precision);
+  build_weights(s, opt->precision);
   // normalise like the CG does: rscale = 2^-ceil(log2 max|r|)
   float m = 0.0f;
   for (size_t i = 0; i < n_; ++i) m = std::max(m, std::fabs(r_host[i]));
@@ -1225,18 +1402,18 @@ int UniformSolver::probe_vcycle(hipStream_t s, const float* phi_host, const fp_o
     vcycle<__half>(s, opt->pre_sweeps, opt->post_sweeps);
     std::vector<__half> hz(n_);
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(hz.data(), f.e, n_ * 2, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hz.data(), zbuf_, n_ * 2, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n_; ++i) z_host[i] = __half2float(hz[i]) * f.Se / rs;
   } else if (opt->precision == FP_PRECISION_MIXED_BF16) {
     vcycle<__hip_bfloat16>(s, opt->pre_sweeps, opt->post_sweeps);
     std::vector<__hip_bfloat16> hz(n_);
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(hz.data(), f.e, n_ * 2, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hz.data(), zbuf_, n_ * 2, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n_; ++i) z_host[i] = __bfloat162float(hz[i]) * f.Se / rs;
   } else {
     vcycle<float>(s, opt->pre_sweeps, opt->post_sweeps);
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(tmp.data(), f.e, n_ * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), zbuf_, n_ * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n_; ++i) z_host[i] = tmp[i] * f.Se / rs;
   }
   return FP_OK;

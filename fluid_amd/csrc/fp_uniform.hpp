@@ -9,7 +9,7 @@
 namespace fp {
 
 constexpr int kMaxLevels = 16;
-constexpr unsigned kMaxBlocks = 4096;
+constexpr unsigned kMaxBlocks = 16384;
 
 void set_error(const char* fmt, ...);
 
@@ -43,6 +43,8 @@ struct UniformLevel {
   float* phi = nullptr;
   void* r = nullptr;  // restricted right-hand side (storage T)
   void* e = nullptr;  // correction (storage T); level 0: the preconditioned residual z
+  void* e2 = nullptr; // second correction buffer (fused sweeps ping-pong)
+  void* w = nullptr;  // packed coefficients W4<T> (fused levels)
   void* t = nullptr;  // residual scratch (storage T)
   float Sr = 1, Se = 1, St = 1;  // stored = true / S
 };
@@ -69,9 +71,14 @@ class UniformSolver {
   int alloc(void** p, size_t bytes);
   void release();
   void set_level_scales(int precision);
+  void build_weights(hipStream_t s, int precision);
   int setup_levels(hipStream_t s, const float* phi0, int preserve);
   template <typename T> int vcycle(hipStream_t s, int pre, int post);
   template <typename T> int iteration(hipStream_t s, int pre, int post, int max_iter, int parity);
+  void run_iteration(hipStream_t s, int precision, int pre, int post, int max_iter, int parity);
+  int ensure_graph(hipStream_t s, int precision, int pre, int post, int max_iter);
+  hipGraphExec_t graph_exec_ = nullptr;
+  int graph_key_[4] = {-1, -1, -1, -1};
 
   int nx_ = 0, ny_ = 0, nz_ = 0;
   double h_ = 0;
@@ -95,6 +102,7 @@ class UniformSolver {
   State* host_upload_ = nullptr;
   void timer_begin(hipStream_t s);
   void timer_end(hipStream_t s);
+  void* zbuf_ = nullptr;  // level-0 buffer holding z after the last V-cycle
   int timed_kernel_ = 0;
   int timer_used_ = 0;
   std::vector<hipEvent_t> timer_events_;

@@ -1,0 +1,345 @@
+// fp_addon.cc -- thin N-API binding of the C ABI (include/fluid_projection.h).
+//
+// What the reference's TypeScript host would load in place of its WebGPU
+// projection stage (WebGPUUniformPressureMultigrid.encode + `project`,
+// lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:327-364).  Every
+// call maps one-to-one onto a C entry point; `project` runs as napi_async_work
+// and resolves a Promise, so the JS thread never blocks on the GPU (the
+// reference's readStats is asynchronous the same way,
+// lib/methods/uniform/webgpu-uniform-reference.ts:1358-1421).
+#include <node_api.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fluid_projection.h"
+
+namespace {
+
+#define NAPI_OK(call)                                                     \
+  do {                                                                    \
+    napi_status _s = (call);                                              \
+    if (_s != napi_ok) {                                                  \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call);        \
+      return nullptr;                                                     \
+    }                                                                     \
+  } while (0)
+
+napi_value throw_fp(napi_env env, int rc) {
+  char code[32];
+  snprintf(code, sizeof(code), "FP_ERR_%d", -rc);
+  std::string msg = fp_last_error();
+  // contract violations throw RangeError like the reference's constructor
+  if (rc == FP_ERR_INVALID_ARGUMENT || rc == FP_ERR_LAYOUT) napi_throw_range_error(env, code, msg.c_str());
+  else napi_throw_error(env, code, msg.c_str());
+  return nullptr;
+}
+
+bool get_named(napi_env env, napi_value obj, const char* name, napi_value* out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return false;
+  if (napi_get_named_property(env, obj, name, out) != napi_ok) return false;
+  napi_valuetype t;
+  napi_typeof(env, *out, &t);
+  return t != napi_undefined && t != napi_null;
+}
+
+double get_number(napi_env env, napi_value obj, const char* name, double dflt) {
+  napi_value v;
+  if (!get_named(env, obj, name, &v)) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+// Float32Array -> pointer + element count (borrowed; the JS object is kept alive by a reference)
+bool get_f32(napi_env env, napi_value v, float** data, size_t* len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type type;
+  napi_value ab;
+  size_t off;
+  void* p;
+  if (napi_get_typedarray_info(env, v, &type, len, &p, &ab, &off) != napi_ok) return false;
+  if (type != napi_float32_array) return false;
+  *data = (float*)p;
+  return true;
+}
+
+void* unwrap_external(napi_env env, napi_value v) {
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return p;
+}
+
+// ---- context ----------------------------------------------------------------
+napi_value create_context(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t device = 0;
+  double budget = 0;
+  if (argc > 0) napi_get_value_int32(env, argv[0], &device);
+  if (argc > 1) napi_get_value_double(env, argv[1], &budget);
+  fp_ctx* ctx = nullptr;
+  int rc = fp_ctx_create(device, (uint64_t)budget, &ctx);
+  if (rc) return throw_fp(env, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, ctx, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value destroy_context(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) return nullptr;
+  int rc = fp_ctx_destroy((fp_ctx*)unwrap_external(env, argv[0]));
+  if (rc) return throw_fp(env, rc);
+  return nullptr;
+}
+
+// ---- grid -----------------------------------------------------------------------
+napi_value create_grid(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) {
+    napi_throw_type_error(env, nullptr, "createGrid(context, {nx, ny, nz, h, layout?, band?, columnBases?})");
+    return nullptr;
+  }
+  fp_ctx* ctx = (fp_ctx*)unwrap_external(env, argv[0]);
+  fp_grid_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.layout = (int32_t)get_number(env, argv[1], "layout", FP_LAYOUT_UNIFORM);
+  d.nx = (int32_t)get_number(env, argv[1], "nx", 0);
+  d.ny = (int32_t)get_number(env, argv[1], "ny", 0);
+  d.nz = (int32_t)get_number(env, argv[1], "nz", 0);
+  d.band = (int32_t)get_number(env, argv[1], "band", 0);
+  d.h = get_number(env, argv[1], "h", 0.0);
+  std::vector<int32_t> bases;
+  napi_value cb;
+  if (get_named(env, argv[1], "columnBases", &cb)) {
+    uint32_t n = 0;
+    napi_get_array_length(env, cb, &n);
+    bool is_ta = false;
+    napi_is_typedarray(env, cb, &is_ta);
+    if (is_ta) {
+      napi_typedarray_type type;
+      size_t len;
+      void* p;
+      napi_value ab;
+      size_t off;
+      napi_get_typedarray_info(env, cb, &type, &len, &p, &ab, &off);
+      bases.resize(len);
+      for (size_t i = 0; i < len; ++i)
+        bases[i] = type == napi_int32_array ? ((int32_t*)p)[i] : (int32_t)((float*)p)[i];
+    }
+    d.column_bases = bases.data();
+  }
+  fp_grid* g = nullptr;
+  int rc = fp_grid_create(ctx, &d, &g);
+  if (rc) return throw_fp(env, rc);
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, g, nullptr, nullptr, &ext));
+  return ext;
+}
+
+napi_value destroy_grid(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 1) return nullptr;
+  int rc = fp_grid_destroy((fp_grid*)unwrap_external(env, argv[0]));
+  if (rc) return throw_fp(env, rc);
+  return nullptr;
+}
+
+napi_value grid_levels(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  fp_grid* g = (fp_grid*)unwrap_external(env, argv[0]);
+  int32_t dims[48], strides[48], L = 0;
+  int rc = fp_grid_levels(g, 16, dims, strides, &L);
+  if (rc) return throw_fp(env, rc);
+  napi_value arr;
+  NAPI_OK(napi_create_array_with_length(env, L, &arr));
+  for (int l = 0; l < L; ++l) {
+    napi_value row;
+    NAPI_OK(napi_create_array_with_length(env, 3, &row));
+    for (int a = 0; a < 3; ++a) {
+      napi_value v;
+      NAPI_OK(napi_create_int32(env, dims[l * 3 + a], &v));
+      NAPI_OK(napi_set_element(env, row, a, v));
+    }
+    NAPI_OK(napi_set_element(env, arr, l, row));
+  }
+  return arr;
+}
+
+// ---- project (async) ---------------------------------------------------------------
+struct ProjectJob {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  std::vector<napi_ref> keep;  // input/output typed arrays stay alive
+  fp_grid* grid = nullptr;
+  fp_inputs in{};
+  fp_outputs out{};
+  fp_options opt{};
+  fp_report rep{};
+  int rc = 0;
+  std::string error;
+  napi_ref result = nullptr;   // {pressure, u, v, w}
+};
+
+void project_execute(napi_env, void* data) {
+  ProjectJob* j = (ProjectJob*)data;
+  j->rc = fp_project(j->grid, &j->in, &j->out, &j->opt, &j->rep);
+  if (j->rc) j->error = fp_last_error();
+}
+
+napi_value report_object(napi_env env, const fp_report& r) {
+  napi_value o, v;
+  napi_create_object(env, &o);
+  auto num = [&](const char* k, double x) {
+    napi_create_double(env, x, &v);
+    napi_set_named_property(env, o, k, v);
+  };
+  num("flags", r.flags);
+  num("iterations", r.iterations);
+  num("outerIterations", r.outer_iterations);
+  num("levels", r.levels);
+  num("relativeResidual", r.relative_residual);
+  num("rhsNorm", r.rhs_norm);
+  num("maxDivergenceBefore", r.max_div_before);
+  num("maxDivergenceAfter", r.max_div_after);
+  num("deviceMs", r.device_ms);
+  num("liquidSamples", (double)r.liquid_samples);
+  num("samples", (double)r.samples);
+  return o;
+}
+
+void project_complete(napi_env env, napi_status, void* data) {
+  ProjectJob* j = (ProjectJob*)data;
+  if (j->rc) {
+    napi_value err, msg, code;
+    napi_create_string_utf8(env, j->error.c_str(), NAPI_AUTO_LENGTH, &msg);
+    char c[32];
+    snprintf(c, sizeof(c), "FP_ERR_%d", -j->rc);
+    napi_create_string_utf8(env, c, NAPI_AUTO_LENGTH, &code);
+    napi_create_error(env, code, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  } else {
+    napi_value res;
+    napi_get_reference_value(env, j->result, &res);
+    napi_set_named_property(env, res, "report", report_object(env, j->rep));
+    napi_resolve_deferred(env, j->deferred, res);
+  }
+  for (napi_ref r : j->keep) napi_delete_reference(env, r);
+  napi_delete_reference(env, j->result);
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+
+// project(grid, {phi, u, v, w, targetDivergence?, dt, rho}, {pressure, u, v, w}, options?) -> Promise
+napi_value project(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "project(grid, inputs, outputs, options?)");
+    return nullptr;
+  }
+  ProjectJob* j = new ProjectJob();
+  j->grid = (fp_grid*)unwrap_external(env, argv[0]);
+  fp_default_options(&j->opt);
+  j->in.memory = FP_MEMORY_HOST;
+  const char* in_names[5] = {"phi", "u", "v", "w", "targetDivergence"};
+  const float** in_ptrs[5] = {&j->in.phi, &j->in.u, &j->in.v, &j->in.w, &j->in.target_divergence};
+  for (int k = 0; k < 5; ++k) {
+    napi_value v;
+    if (!get_named(env, argv[1], in_names[k], &v)) continue;
+    float* p;
+    size_t len;
+    if (!get_f32(env, v, &p, &len)) {
+      delete j;
+      napi_throw_type_error(env, nullptr, "inputs must be Float32Array");
+      return nullptr;
+    }
+    *in_ptrs[k] = p;
+    napi_ref r;
+    napi_create_reference(env, v, 1, &r);
+    j->keep.push_back(r);
+  }
+  j->in.dt = get_number(env, argv[1], "dt", 0.0);
+  j->in.rho = get_number(env, argv[1], "rho", 998.2);
+  const char* out_names[4] = {"pressure", "u", "v", "w"};
+  float** out_ptrs[4] = {&j->out.pressure, &j->out.u, &j->out.v, &j->out.w};
+  for (int k = 0; k < 4; ++k) {
+    napi_value v;
+    if (!get_named(env, argv[2], out_names[k], &v)) continue;
+    float* p;
+    size_t len;
+    if (!get_f32(env, v, &p, &len)) {
+      delete j;
+      napi_throw_type_error(env, nullptr, "outputs must be Float32Array");
+      return nullptr;
+    }
+    *out_ptrs[k] = p;
+  }
+  napi_create_reference(env, argv[2], 1, &j->result);
+  if (argc > 3) {
+    napi_valuetype t;
+    napi_typeof(env, argv[3], &t);
+    if (t == napi_object) {
+      j->opt.relative_tolerance = get_number(env, argv[3], "relativeTolerance", j->opt.relative_tolerance);
+      j->opt.max_iterations = (int32_t)get_number(env, argv[3], "maxIterations", j->opt.max_iterations);
+      j->opt.max_outer = (int32_t)get_number(env, argv[3], "maxOuter", j->opt.max_outer);
+      j->opt.precision = (int32_t)get_number(env, argv[3], "precision", j->opt.precision);
+      j->opt.pre_sweeps = (int32_t)get_number(env, argv[3], "preSweeps", j->opt.pre_sweeps);
+      j->opt.post_sweeps = (int32_t)get_number(env, argv[3], "postSweeps", j->opt.post_sweeps);
+      j->opt.warm_start = (int32_t)get_number(env, argv[3], "warmStart", j->opt.warm_start);
+    }
+  }
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &j->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "fluidProjection", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, project_execute, project_complete, j, &j->work));
+  NAPI_OK(napi_queue_async_work(env, j->work));
+  return promise;
+}
+
+napi_value last_error(napi_env env, napi_callback_info) {
+  napi_value s;
+  NAPI_OK(napi_create_string_utf8(env, fp_last_error(), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+napi_value abi_version(napi_env env, napi_callback_info) {
+  napi_value v;
+  NAPI_OK(napi_create_int32(env, fp_abi_version(), &v));
+  return v;
+}
+
+napi_value init(napi_env env, napi_value exports) {
+  struct {
+    const char* name;
+    napi_callback cb;
+  } fns[] = {{"createContext", create_context}, {"destroyContext", destroy_context}, {"createGrid", create_grid},
+             {"destroyGrid", destroy_grid},     {"gridLevels", grid_levels},         {"project", project},
+             {"lastError", last_error},         {"abiVersion", abi_version}};
+  for (auto& f : fns) {
+    napi_value fn;
+    NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));
+    NAPI_OK(napi_set_named_property(env, exports, f.name, fn));
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -67,7 +67,7 @@ class Options(C.Structure):
     _fields_ = [("relative_tolerance", C.c_double), ("max_iterations", C.c_int32), ("max_outer", C.c_int32),
                 ("precision", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32),
                 ("warm_start", C.c_int32), ("phi_preserve_levels", C.c_int32), ("check_every", C.c_int32),
-                ("timed_kernel", C.c_int32)]
+                ("timed_kernel", C.c_int32), ("graphs", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Report(C.Structure):

@@ -124,6 +124,8 @@ typedef struct {
   int32_t phi_preserve_levels; /* CM11a C (default 2) */
   int32_t check_every;       /* host convergence poll period in iterations (default 1) */
   int32_t timed_kernel;      /* FP_KERNEL_* whose launches are timed with HIP events (0: none) */
+  int32_t graphs;            /* 1: replay each pair of CG + V-cycle iterations as one hipGraph (default) */
+  int32_t reserved;
 } fp_options;
 
 /* kernel classes for fp_options.timed_kernel */

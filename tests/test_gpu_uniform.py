@@ -235,3 +235,18 @@ def test_budget_is_enforced(fpl):
         fpl.UniformPressureProjection((64, 64, 64), 1 / 64, context=ctx)
     assert e.value.code == -5
     ctx.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_graph_replay_matches_plain_launches(fpl, c0, precision):
+    """Each pair of iterations replayed as a hipGraph gives the same bits as
+    plain launches, also across several replays of the same graph."""
+    s = c0
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    plain = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision=precision, graphs=0)
+    for _ in range(3):
+        gr = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision=precision, graphs=1)
+        assert gr.report["iterations"] == plain.report["iterations"]
+        assert np.array_equal(gr.pressure, plain.pressure)
+        assert np.array_equal(gr.u, plain.u) and np.array_equal(gr.w, plain.w)
+    g.destroy()
