@@ -932,25 +932,38 @@ void UniformSolver::set_level_scales(int precision) {
 // Fused sweeps run on levels large enough to fill a 256-lane column tile.
 static bool fused_level(const UniformLevel& l) { return l.d[2] >= 16 && l.d[1] >= 4 && l.d[0] >= 4 && l.n >= 4096; }
 
+template <int TZ, typename TR, typename T>
+static void launch_sweep_tz(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein,
+                            T* eout, int first, int zero_init, const ProlongArgs& P, const State* st) {
+  constexpr int TY = 256 / TZ;
+  const W4<T>* w = (const W4<T>*)f.w;
+  const SweepGeom G{f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2]};
+  const int tz = (f.d[2] + TZ - 1) / TZ, ty = (f.d[1] + TY - 1) / TY;
+  const long tiles = (long)tz * ty;
+  // about two rounds of 8 resident workgroups per CU on 256 CUs, at least 8
+  // planes per workgroup (the march carries 4 planes of start-up)
+  const long cx_target = std::max<long>(1, 4096 / tiles);
+  int chunk = (int)std::max<long>(8, (f.d[0] + cx_target - 1) / cx_target);
+  chunk = std::min(chunk, f.d[0]);
+  const int cx = (f.d[0] + chunk - 1) / chunk;
+  const int ng = (int)(tiles * cx);
+  const dim3 grid(ng), block(256);
+  if (zero_init)
+    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, false, true>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+                       chunk, tz, ty, ng, st);
+  else if (P.ec)
+    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, true, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+                       chunk, tz, ty, ng, st);
+  else
+    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, false, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+                       chunk, tz, ty, ng, st);
+}
+
 template <typename TR, typename T>
 static void launch_sweep(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein, T* eout,
                          int first, int zero_init, const ProlongArgs& P, const State* st) {
-  const W4<T>* w = (const W4<T>*)f.w;
-  const SweepGeom G{f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2]};
-  const int TZ = f.d[2] >= 64 ? 64 : 32;
-  const int TY = 256 / TZ;
-  const int tz = (f.d[2] + TZ - 1) / TZ, ty = (f.d[1] + TY - 1) / TY;
-  const long tiles = (long)tz * ty;
-  int chunk = (int)std::max<long>(8, ((long)f.d[0] * tiles + 4095) / 4096);
-  chunk = std::min(chunk, f.d[0]);
-  const int cx = (f.d[0] + chunk - 1) / chunk;
-  const dim3 grid(tz, ty, cx);
-  if (TZ == 64)
-    hipLaunchKernelGGL((k_sweep2<64, TR, T>), grid, dim3(256), 0, s, G, w, r, rmul, dyn, ein, eout, first, zero_init,
-                       P, chunk, st);
-  else
-    hipLaunchKernelGGL((k_sweep2<32, TR, T>), grid, dim3(256), 0, s, G, w, r, rmul, dyn, ein, eout, first, zero_init,
-                       P, chunk, st);
+  if (f.d[2] >= 32) launch_sweep_tz<32, TR, T>(s, f, r, rmul, dyn, ein, eout, first, zero_init, P, st);
+  else launch_sweep_tz<16, TR, T>(s, f, r, rmul, dyn, ein, eout, first, zero_init, P, st);
 }
 
 void UniformSolver::build_weights(hipStream_t s, int precision) {
@@ -974,7 +987,7 @@ int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
   // level 0's right-hand side is the FP32 CG residual times the device rscale.
   // cur[l] tracks which of the level's two correction buffers holds e.
   int cur[kMaxLevels] = {0};
-  const ProlongArgs noP{nullptr, nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
+  const ProlongArgs noP{nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
   for (int l = 0; l < L_ - 1; ++l) {
     UniformLevel& f = lv_[l];
     UniformLevel& c = lv_[l + 1];
@@ -1066,7 +1079,7 @@ int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
     } else {
       for (int sw = 0; sw < post; ++sw) {
         ProlongArgs P = noP;
-        if (sw == 0) P = ProlongArgs{(const void*)ec, (const float*)c.phi, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / f.Se};
+        if (sw == 0) P = ProlongArgs{(const void*)ec, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / f.Se};
         T* ein = (T*)(cur[l] ? f.e2 : f.e);
         T* eout = (T*)(cur[l] ? f.e : f.e2);
         const bool t = top && timed_kernel_ == FP_KERNEL_SMOOTH_FINE;
