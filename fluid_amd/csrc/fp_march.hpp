@@ -25,8 +25,8 @@
 namespace fp {
 
 template <int TZ, class Op>
-__global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in, int chunk, const State* st,
-                                               int honour_done) {
+__global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in, int chunk, int tiles_z,
+                                               int tiles_y, int ngroups, const State* st, int honour_done) {
   if (honour_done && st->done) return;
   constexpr int TY = 256 / TZ;
   constexpr int RZ = TZ + 2, RY = TY + 2;
@@ -35,8 +35,10 @@ __global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in,
   Op op = op_in;
   op.prepare();
   float mx = 0.0f;
-  const int z0 = blockIdx.x * TZ, y0 = blockIdx.y * TY;
-  const int x0 = blockIdx.z * chunk;
+  int tzi, tyi, cxi;
+  tile_of_group(blockIdx.x, tiles_z, tiles_y, ngroups, tzi, tyi, cxi);
+  const int z0 = tzi * TZ, y0 = tyi * TY;
+  const int x0 = cxi * chunk;
   const int x1 = min(x0 + chunk, nx);
   const int tid = threadIdx.x;
   const int ly = tid / TZ, lz = tid - ly * TZ;
@@ -89,25 +91,33 @@ __global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in,
 }
 
 struct MarchGrid {
-  dim3 grid;
+  int ngroups;
   int chunk;
-  int tz;
+  int tz, ty;     // tiles along z and y
+  int TZ;         // tile width along z (64 or 32)
 };
 
+#ifndef FP_MARCH_TZ
+#define FP_MARCH_TZ 32
+#endif
+
+// Tiles of TZ x 256/TZ columns, x chunks of at least 8 planes, about
+// target_blocks workgroups (at most the 16384 partials of the reductions).
 inline MarchGrid march_grid(int nx, int ny, int nz, long target_blocks = 4096) {
-  const int TZ = nz >= 64 ? 64 : 32;
+  const int TZ = nz >= FP_MARCH_TZ ? FP_MARCH_TZ : 32;
   const int TY = 256 / TZ;
   const int tz = (nz + TZ - 1) / TZ, ty = (ny + TY - 1) / TY;
   const long tiles = (long)tz * ty;
   long c = ((long)nx * tiles + target_blocks - 1) / target_blocks;
   if (c < 8) c = 8;
   if (c > nx) c = nx;
-  // the grid-reduction scratch holds at most kMaxBlocks partials
   while ((long)tiles * ((nx + c - 1) / c) > 16384) ++c;
   MarchGrid m;
   m.chunk = (int)c;
-  m.grid = dim3(tz, ty, (nx + m.chunk - 1) / m.chunk);
-  m.tz = TZ;
+  m.tz = tz;
+  m.ty = ty;
+  m.TZ = TZ;
+  m.ngroups = (int)(tiles * ((nx + c - 1) / c));
   return m;
 }
 
@@ -297,11 +307,13 @@ struct OpTrueResidual {
 template <class Op>
 inline unsigned launch_march(hipStream_t s, int nx, int ny, int nz, const Op& op, const State* st, int honour_done) {
   const MarchGrid m = march_grid(nx, ny, nz);
-  if (m.tz == 64)
-    hipLaunchKernelGGL((k_march<64, Op>), m.grid, dim3(256), 0, s, nx, ny, nz, op, m.chunk, st, honour_done);
+  if (m.TZ == 64)
+    hipLaunchKernelGGL((k_march<64, Op>), dim3(m.ngroups), dim3(256), 0, s, nx, ny, nz, op, m.chunk, m.tz, m.ty,
+                       m.ngroups, st, honour_done);
   else
-    hipLaunchKernelGGL((k_march<32, Op>), m.grid, dim3(256), 0, s, nx, ny, nz, op, m.chunk, st, honour_done);
-  return m.grid.x * m.grid.y * m.grid.z;
+    hipLaunchKernelGGL((k_march<32, Op>), dim3(m.ngroups), dim3(256), 0, s, nx, ny, nz, op, m.chunk, m.tz, m.ty,
+                       m.ngroups, st, honour_done);
+  return (unsigned)m.ngroups;
 }
 
 }  // namespace fp

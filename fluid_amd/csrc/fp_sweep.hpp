@@ -71,13 +71,28 @@ __global__ void k_build_weights(int nx, int ny, int nz, float gx, float gy, floa
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the device, each
-// with its own L2: group g runs on XCD g % 8.  Remap so that every XCD walks a
-// contiguous run of tiles (z fastest, then y, then the x chunk): neighbouring
-// tiles, which read each other's halo rows, then share one L2.
+// with its own L2: group g runs on XCD g % 8.  Logical tiles run z fastest,
+// then y, then the x chunk.  Mode 2 keeps the x chunks in lock step across
+// the device and gives every XCD one y slab of each chunk (tiles % 8 == 0):
+// neighbouring tiles, which read each other's halo rows, share an L2 while the
+// device streams the same few x planes.  Mode 1 gives every XCD a contiguous
+// run of tiles; mode 0 is the identity.
+#ifndef FP_XCD_REMAP
+#define FP_XCD_REMAP 2
+#endif
 __device__ __forceinline__ void tile_of_group(int g, int tiles_z, int tiles_y, int ngroups, int& tzi, int& tyi,
                                               int& cxi) {
-  const int per = ngroups >> 3;
-  const int l = (g < (per << 3)) ? (g & 7) * per + (g >> 3) : g;
+  const int tiles = tiles_z * tiles_y;
+  int l = g;
+  if (FP_XCD_REMAP == 2 && (tiles & 7) == 0) {
+    const int S = tiles >> 3;
+    const int i = g >> 3;
+    const int chunk = i / S;
+    l = chunk * tiles + (g & 7) * S + (i - chunk * S);
+  } else if (FP_XCD_REMAP == 1) {
+    const int per = ngroups >> 3;
+    if (g < (per << 3)) l = (g & 7) * per + (g >> 3);
+  }
   tzi = l % tiles_z;
   const int t = l / tiles_z;
   tyi = t % tiles_y;
@@ -135,41 +150,48 @@ __device__ __forceinline__ void window_taps(int f, int s, int nc, int c0, int& a
 }
 
 // One red-black sweep.  TZ x TY = 256 lanes; TR: storage of the right-hand
-// side, T: of e, of the packed coefficients and of the LDS staging.  LDS holds
-// four planes of coefficients (rows/columns -2..T of the tile; -2..T+1 with
-// PROLONG), three of the old correction (-2..T+1), four of the scaled
-// right-hand side and three of the first-colour values (-1..T).  Per plane
-// step L two barriers order the work:
-//   commit plane L | A | prefetch L+1, colour 1 at R = L-1 | B | colour 2 at L-2
+// side, T: of e, of the packed coefficients and of the LDS staging.
+//
+// LDS per plane slot: coefficients on rows/columns -2..T of the tile (-2..T+1
+// with PROLONG), the old correction on -2..T+1, the scaled right-hand side and
+// the first-colour values on -1..T.  Four slots of coefficients, r and
+// first-colour values, three of the old correction.  Per plane step L:
+//   commit plane L | A | prefetch L+D, colour 1 at R = L-1 | B | colour 2 at L-2
 // (PROLONG adds the coarse correction to plane L's second-colour liquid cells
-// between A and a third barrier).  The prefetch holds raw loaded bits: masks,
-// scaling and conversions happen at the commit, so the loads of plane L+1
-// stay in flight across the colour passes of the current planes.  Colour-1
-// lanes cover the first-colour cells of the ring-extended tile (CY rows of
-// CZ/2), colour-2 lanes the second-colour cells of the tile (TY rows of TZ/2).
-template <int TZ, typename TR, typename T, bool PROLONG, bool ZERO>
+// between A and a third barrier).
+//
+// The plane loop is unrolled by 4 from x0-2, so the four-slot indices and the
+// register set of a plane (D sets of raw prefetched bits, D | 4) are
+// compile-time constants; the old-correction slots rotate through three scalar
+// offsets.  Masks, scaling and conversions happen at the commit, so a plane's
+// loads stay in flight for D plane steps, and planes outside the grid are not
+// loaded at all (the commit stages zeros for them).  Colour-1 lanes cover the
+// first-colour cells of the ring-extended tile (CY rows of CZ/2), colour-2
+// lanes the second-colour cells of the tile (TY rows of TZ/2).
+template <int TZ, int D, typename TR, typename T, bool PROLONG, bool ZERO>
 __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __restrict__ wts,
                                                 const TR* __restrict__ r, float rmul, int dyn,
                                                 const T* __restrict__ ein, T* __restrict__ eout, int first,
                                                 ProlongArgs P, int chunk, int tiles_z, int tiles_y, int ngroups,
                                                 const State* st) {
+  static_assert(D == 1 || D == 2 || D == 4, "the prefetch depth must divide the unroll of 4");
   if (st->done) return;
   constexpr int TY = 256 / TZ;
-  constexpr int EY = TY + 4, EZ = TZ + 4;                    // old correction: -2 .. T+1
-  constexpr int WY = PROLONG ? EY : TY + 3;                  // coefficients: -2 .. T (T+1)
+  constexpr int EY = TY + 4, EZ = TZ + 4;    // old correction: -2 .. T+1
+  constexpr int WY = PROLONG ? EY : TY + 3;  // coefficients: -2 .. T (T+1)
   constexpr int WZ = PROLONG ? EZ : TZ + 3;
-  constexpr int CY = TY + 2, CZ = TZ + 2;                    // r, first-colour values: -1 .. T
+  constexpr int CY = TY + 2, CZ = TZ + 2;    // r, first-colour values: -1 .. T
   constexpr int WN = WY * WZ, EN = EY * EZ, CN = CY * CZ;
-  constexpr int PN = PROLONG ? 2 * EN : 0;                   // two coarse planes, window <= EY x EZ
+  constexpr int PN = PROLONG ? 2 * EN : 0;   // two coarse planes, window <= EY x EZ
   constexpr int NW = (WN + 255) / 256, NE = ZERO ? 0 : (EN + 255) / 256, NR = (CN + 255) / 256;
   constexpr int NP = (PN + 255) / 256;
   constexpr int HF = CZ / 2, HS = TZ / 2;
   static_assert(CY * HF <= 256 && TY * HS <= 256, "lane maps exceed the workgroup");
   using WB = typename W4Bits<T>::type;
   __shared__ WB s_w[4][WN];
-  __shared__ T s_e[3][EN];
+  __shared__ T s_e[ZERO ? 1 : 3 * EN];
   __shared__ T s_r[4][CN];
-  __shared__ T s_c[3][CN];
+  __shared__ T s_c[4][CN];
   __shared__ T s_p[PROLONG ? PN : 1];
 
   const float rm = dyn ? rmul * st->rscale : rmul;
@@ -179,7 +201,7 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
   const int z0 = tzi * TZ, y0 = tyi * TY, x0 = cxi * chunk;
   const int x1 = min(x0 + chunk, nx);
   const int tid = threadIdx.x;
-  const size_t plane = (size_t)ny * nz;
+  const long long plane = (long long)ny * nz;
   const int second = 1 - first;
 
   // in-plane offsets of the staged cells (-1: outside the grid)
@@ -205,7 +227,7 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
   // coarse window (PROLONG): rows/columns from cy0/cz0, planes pa(L), pa(L)+1
   const int cy0 = PROLONG ? (P.sy == 2 ? (y0 >> 1) - 2 : y0 - 2) : 0;
   const int cz0 = PROLONG ? (P.sz == 2 ? (z0 >> 1) - 2 : z0 - 2) : 0;
-  const size_t cplane = PROLONG ? (size_t)P.cny * P.cnz : 0;
+  const long long cplane = PROLONG ? (long long)P.cny * P.cnz : 0;
   int op[NP > 0 ? NP : 1], opl[NP > 0 ? NP : 1];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
@@ -217,61 +239,103 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
   }
   auto coarse_plane0 = [&](int L) { return P.sx == 2 ? ((L - 1) >> 1) : L; };
 
-  WB pw[NW];
-  T pe[NE > 0 ? NE : 1];
-  TR pr[NR];
-  T pp[NP > 0 ? NP : 1];
-  auto fetch = [&](int L) {
-    const size_t base = (size_t)min(max(L, 0), nx - 1) * plane;
+  // D register sets of raw prefetched planes
+  WB pw[D][NW];
+  T pe[D][NE > 0 ? NE : 1];
+  TR pr[D][NR];
+  T pp[D][NP > 0 ? NP : 1];
+  int lf = x0 - 2;                 // next plane to fetch
+  long long fo = (long long)lf * plane;
+  auto fetch = [&](int d) {
+    if (lf >= 0 && lf < nx) {
+      const W4<T>* wp = wts + fo;
+      const TR* rp = r + fo;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) pw[k] = *reinterpret_cast<const WB*>(wts + base + max(ow[k], 0));
+      for (int k = 0; k < NW; ++k) pw[d][k] = *reinterpret_cast<const WB*>(wp + max(ow[k], 0));
+      if constexpr (!ZERO) {
+        const T* ep = ein + fo;
 #pragma unroll
-    for (int k = 0; k < NE; ++k) pe[k] = ein[base + max(oe[k], 0)];
+        for (int k = 0; k < NE; ++k) pe[d][k] = ep[max(oe[k], 0)];
+      }
 #pragma unroll
-    for (int k = 0; k < NR; ++k) pr[k] = r[base + max(orr[k], 0)];
-    if constexpr (PROLONG) {
-      const int pa = coarse_plane0(L);
-      const T* ec = (const T*)P.ec;
+      for (int k = 0; k < NR; ++k) pr[d][k] = rp[max(orr[k], 0)];
+      if constexpr (PROLONG) {
+        const int pa = coarse_plane0(lf);
+        const T* ec = (const T*)P.ec;
 #pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const int c = min(max(pa + opl[k], 0), P.cnx - 1);
-        pp[k] = ec[(size_t)c * cplane + max(op[k], 0)];
+        for (int k = 0; k < NP; ++k) {
+          const int c = min(max(pa + opl[k], 0), P.cnx - 1);
+          pp[d][k] = ec[(long long)c * cplane + max(op[k], 0)];
+        }
       }
     }
+    ++lf;
+    fo += plane;
   };
 
-  // colour-1 lane: a first-colour cell of the ring-extended tile
+  // colour-1 lane: a first-colour cell of the ring-extended tile; its z
+  // column alternates with the plane parity (index 0: even steps from x0-2)
   const bool c1 = tid < CY * HF;
   const int c1y = tid / HF, c1k = tid - (tid / HF) * HF;
   const int c1gy = y0 + c1y - 1;
+  int c1lz[2];
+  bool c1st[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int R = x0 - 3 + h;  // parity of R = L - 1 on steps of parity h
+    c1lz[h] = 2 * c1k + ((first + R + c1gy + 1) & 1);
+    const int gz = z0 + c1lz[h] - 1;
+    c1st[h] = c1 && c1y >= 1 && c1y <= TY && c1lz[h] >= 1 && c1lz[h] <= TZ && c1gy < ny && gz < nz;
+  }
   // colour-2 lane: a second-colour cell of the tile
   const bool c2 = tid < TY * HS;
   const int c2y = tid / HS, c2k = tid - (tid / HS) * HS;
   const int c2gy = y0 + c2y;
+  int c2lz[2];
+  bool c2st[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int B = x0 - 4 + h;  // parity of B = L - 2 on steps of parity h
+    c2lz[h] = 2 * c2k + ((second + B + c2gy) & 1);
+    c2st[h] = c2 && c2gy < ny && z0 + c2lz[h] < nz;
+  }
 
-  fetch(x0 - 2);
-  for (int L = x0 - 2; L <= x1 + 1; ++L) {
-    const int m4 = (L + 4) & 3, m3 = (L + 3) % 3;
+#pragma unroll
+  for (int d = 0; d < D; ++d) fetch(d);
+
+  int e0 = 2 * EN, e1 = EN, e2 = 0;   // old-correction slots of planes L, L-1, L-2
+  long long oR = (long long)(x0 - 3) * plane;  // offset of plane R = L - 1
+  float wxm2 = 0.0f;                   // colour 2: -x coefficient of plane B-1, read one step early
+
+  auto step = [&](int L, int d) {
+    {
+      const int t = e2;
+      e2 = e1;
+      e1 = e0;
+      e0 = t;
+    }
+    const int h = d & 1;
+    const int s0 = d, s1 = (d + 3) & 3, s2 = (d + 2) & 3, s3 = (d + 1) & 3;  // slots of L, L-1, L-2, L-3
+    const int rs = d % D;
     const bool pin = (L >= 0 && L < nx);
     // ---- commit plane L ----
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       const int q = tid + 256 * k;
-      if (k + 1 < NW || q < WN) s_w[m4][q] = bits_unless(pw[k], pin && ow[k] >= 0);
+      if (k + 1 < NW || q < WN) s_w[s0][q] = bits_unless(pw[rs][k], pin && ow[k] >= 0);
     }
+    if constexpr (!ZERO) {
 #pragma unroll
-    for (int k = 0; k < (EN + 255) / 256; ++k) {
-      const int q = tid + 256 * k;
-      if (k + 1 < (EN + 255) / 256 || q < EN) {
-        if constexpr (ZERO) s_e[m3][q] = to_store<T>(0.0f);
-        else s_e[m3][q] = value_unless(pe[k], pin && oe[k] >= 0);
+      for (int k = 0; k < NE; ++k) {
+        const int q = tid + 256 * k;
+        if (k + 1 < NE || q < EN) s_e[e0 + q] = value_unless(pe[rs][k], pin && oe[k] >= 0);
       }
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int q = tid + 256 * k;
-      const float v = Store<TR>::ld(&pr[k], 0) * rm;
-      if (k + 1 < NR || q < CN) s_r[m4][q] = to_store<T>((pin && orr[k] >= 0) ? v : 0.0f);
+      const float v = Store<TR>::ld(&pr[rs][k], 0) * rm;
+      if (k + 1 < NR || q < CN) s_r[s0][q] = to_store<T>((pin && orr[k] >= 0) ? v : 0.0f);
     }
     if constexpr (PROLONG) {
       const int pa = coarse_plane0(L);
@@ -279,11 +343,11 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
       for (int k = 0; k < NP; ++k) {
         const int q = tid + 256 * k;
         const int c = pa + opl[k];
-        if (k + 1 < NP || q < PN) s_p[q] = value_unless(pp[k], c >= 0 && c < P.cnx && op[k] >= 0);
+        if (k + 1 < NP || q < PN) s_p[q] = value_unless(pp[rs][k], c >= 0 && c < P.cnx && op[k] >= 0);
       }
     }
     lds_barrier();  // A
-    if (L + 1 <= x1 + 1) fetch(L + 1);
+    if (L + D <= x1 + 1) fetch(rs);
 
     if constexpr (PROLONG) {
       // e(L) += fac * P ec at second-colour liquid cells (coarse air holds e = 0)
@@ -294,7 +358,7 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
         for (int q = tid; q < EN; q += 256) {
           const int ey = q / EZ, ez = q - (q / EZ) * EZ;
           const int gy = y0 + ey - 2, gz = z0 + ez - 2;
-          const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[m4][q]);
+          const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[s0][q]);
           if (gy >= 0 && gy < ny && gz >= 0 && gz < nz && ((L + gy + gz) & 1) == second && wf(wc.id) > 0.0f) {
             int ya, yb, za, zb;
             float wya, wyb, wza, wzb;
@@ -307,72 +371,73 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
                        wyb * (wza * wf(pa_[yb * EZ + za]) + wzb * wf(pa_[yb * EZ + zb]))) +
                 wxb * (wya * (wza * wf(pb_[ya * EZ + za]) + wzb * wf(pb_[ya * EZ + zb])) +
                        wyb * (wza * wf(pb_[yb * EZ + za]) + wzb * wf(pb_[yb * EZ + zb])));
-            s_e[m3][q] = to_store<T>(wf(s_e[m3][q]) + P.fac * acc);
+            s_e[e0 + q] = to_store<T>(wf(s_e[e0 + q]) + P.fac * acc);
           }
         }
       }
       lds_barrier();  // A2
     }
 
-    const int B = L - 2;
-    float wxm2 = 0.0f;
-    int wi2 = 0, ci2 = 0, gz2 = 0;
+    const int R = L - 1, B = L - 2;
+    float wxm = 0.0f;
     if (L >= x0) {
-      // ---- colour 1 at plane R = L - 1 ----
-      const int R = L - 1;
-      const int sR = (R + 4) & 3, sRm = (R + 3) & 3;
-      const int eR = (R + 3) % 3, eRm = (R + 2) % 3, eRp = m3;
-      const W4<T>* w0 = reinterpret_cast<const W4<T>*>(s_w[sR]);
-      const W4<T>* wm = reinterpret_cast<const W4<T>*>(s_w[sRm]);
+      // ---- colour 1 at plane R ----
+      const W4<T>* w0 = reinterpret_cast<const W4<T>*>(s_w[s1]);
+      const W4<T>* wm = reinterpret_cast<const W4<T>*>(s_w[s2]);
       if (c1) {
-        const int lz = 2 * c1k + ((first + R + c1gy + 1) & 1);
-        const int gz = z0 + lz - 1;
+        const int lz = c1lz[h];
         const int wi = (c1y + 1) * WZ + lz + 1;
-        const int ei = (c1y + 1) * EZ + lz + 1;
+        const int ci = c1y * CZ + lz;
         const W4<T> wc = w0[wi];
         const float idg = wf(wc.id);
-        const T* e0 = s_e[eR];
-        float acc = wf(s_r[sR][c1y * CZ + lz]);
-        acc += wf(wc.wx) * wf(s_e[eRp][ei]);
-        acc += wf(wm[wi].wx) * wf(s_e[eRm][ei]);
-        acc += wf(wc.wy) * wf(e0[ei + EZ]);
-        acc += wf(w0[wi - WZ].wy) * wf(e0[ei - EZ]);
-        acc += wf(wc.wz) * wf(e0[ei + 1]);
-        acc += wf(w0[wi - 1].wz) * wf(e0[ei - 1]);
+        float acc = wf(s_r[s1][ci]);
+        if constexpr (!ZERO) {
+          const int ei = (c1y + 1) * EZ + lz + 1;
+          const T* eo = s_e + e1;
+          acc += wf(wc.wx) * wf(s_e[e0 + ei]);
+          acc += wf(wm[wi].wx) * wf(s_e[e2 + ei]);
+          acc += wf(wc.wy) * wf(eo[ei + EZ]);
+          acc += wf(w0[wi - WZ].wy) * wf(eo[ei - EZ]);
+          acc += wf(wc.wz) * wf(eo[ei + 1]);
+          acc += wf(w0[wi - 1].wz) * wf(eo[ei - 1]);
+        }
         const float val = idg > 0.0f ? acc * idg : 0.0f;
-        s_c[eR][c1y * CZ + lz] = to_store<T>(val);
-        if (R >= x0 && R < x1 && c1y >= 1 && c1y <= TY && lz >= 1 && lz <= TZ && c1gy < ny && gz < nz)
-          Store<T>::st(eout, (size_t)R * plane + (size_t)c1gy * nz + gz, val);
+        s_c[s1][ci] = to_store<T>(val);
+        if (R >= x0 && R < x1 && c1st[h])
+          Store<T>::st(eout + oR, (size_t)(c1gy * nz + z0 + lz - 1), val);
       }
-      // colour-2 lanes read the -x coefficient of plane B - 1 now: its slot
-      // is refilled at the next commit, which no barrier separates from colour 2
-      if (c2) {
-        const int lz = 2 * c2k + ((second + B + c2gy) & 1);
-        gz2 = z0 + lz;
-        wi2 = (c2y + 2) * WZ + lz + 2;
-        ci2 = (c2y + 1) * CZ + lz + 1;
-        wxm2 = wf(reinterpret_cast<const W4<T>*>(s_w[(B + 3) & 3])[wi2].wx);
-      }
+      // colour-2 lanes read the -x coefficient of the next step's plane B - 1
+      // (plane L - 2) now: its slot is refilled at the next commit
+      if (c2) wxm = wf(wm[(c2y + 2) * WZ + c2lz[h ^ 1] + 2].wx);
     }
     lds_barrier();  // B
     if (L >= x0 + 2 && B < x1 && c2) {
-      // ---- colour 2 at plane B = L - 2 ----
-      const int sB = (B + 4) & 3;
-      const int cB = (B + 3) % 3, cBm = (B + 2) % 3, cBp = (B + 4) % 3;
-      const W4<T>* w0 = reinterpret_cast<const W4<T>*>(s_w[sB]);
-      const W4<T> wc = w0[wi2];
+      // ---- colour 2 at plane B ----
+      const int lz = c2lz[h];
+      const int wi = (c2y + 2) * WZ + lz + 2;
+      const int ci = (c2y + 1) * CZ + lz + 1;
+      const W4<T>* w0 = reinterpret_cast<const W4<T>*>(s_w[s2]);
+      const W4<T> wc = w0[wi];
       const float idg = wf(wc.id);
-      const T* c0 = s_c[cB];
-      float acc = wf(s_r[sB][ci2]);
-      acc += wf(wc.wx) * wf(s_c[cBp][ci2]);
-      acc += wxm2 * wf(s_c[cBm][ci2]);
-      acc += wf(wc.wy) * wf(c0[ci2 + CZ]);
-      acc += wf(w0[wi2 - WZ].wy) * wf(c0[ci2 - CZ]);
-      acc += wf(wc.wz) * wf(c0[ci2 + 1]);
-      acc += wf(w0[wi2 - 1].wz) * wf(c0[ci2 - 1]);
+      const T* cc = s_c[s2];
+      float acc = wf(s_r[s2][ci]);
+      acc += wf(wc.wx) * wf(s_c[s1][ci]);
+      acc += wxm2 * wf(s_c[s3][ci]);
+      acc += wf(wc.wy) * wf(cc[ci + CZ]);
+      acc += wf(w0[wi - WZ].wy) * wf(cc[ci - CZ]);
+      acc += wf(wc.wz) * wf(cc[ci + 1]);
+      acc += wf(w0[wi - 1].wz) * wf(cc[ci - 1]);
       const float val = idg > 0.0f ? acc * idg : 0.0f;
-      if (c2gy < ny && gz2 < nz) Store<T>::st(eout, (size_t)B * plane + (size_t)c2gy * nz + gz2, val);
+      if (c2st[h]) Store<T>::st(eout + (oR - plane), (size_t)(c2gy * nz + z0 + lz), val);
     }
+    wxm2 = wxm;
+    oR += plane;
+  };
+
+  for (int L0 = x0 - 2; L0 <= x1 + 1; L0 += 4) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+      if (L0 + d <= x1 + 1) step(L0 + d, d);
   }
 }
 

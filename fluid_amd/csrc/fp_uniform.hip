@@ -932,6 +932,11 @@ void UniformSolver::set_level_scales(int precision) {
 // Fused sweeps run on levels large enough to fill a 256-lane column tile.
 static bool fused_level(const UniformLevel& l) { return l.d[2] >= 16 && l.d[1] >= 4 && l.d[0] >= 4 && l.n >= 4096; }
 
+#ifndef FP_SWEEP_DEPTH
+#define FP_SWEEP_DEPTH 1
+#endif
+constexpr int kSweepDepth = FP_SWEEP_DEPTH;  // planes of loads in flight per workgroup
+
 template <int TZ, typename TR, typename T>
 static void launch_sweep_tz(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein,
                             T* eout, int first, int zero_init, const ProlongArgs& P, const State* st) {
@@ -949,13 +954,13 @@ static void launch_sweep_tz(hipStream_t s, const UniformLevel& f, const TR* r, f
   const int ng = (int)(tiles * cx);
   const dim3 grid(ng), block(256);
   if (zero_init)
-    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, false, true>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, false, true>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
                        chunk, tz, ty, ng, st);
   else if (P.ec)
-    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, true, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, true, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
                        chunk, tz, ty, ng, st);
   else
-    hipLaunchKernelGGL((k_sweep3<TZ, TR, T, false, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
+    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, false, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
                        chunk, tz, ty, ng, st);
 }
 
