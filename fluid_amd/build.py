@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfluid_projection.so")
-SOURCES = ["fp_uniform.hip", "fp_tall.hip", "fp_batch.hip", "fp_api.hip"]
+SOURCES = ["fp_uniform.hip", "fp_serve.hip", "fp_tall.hip", "fp_api.hip"]
 ARCH = os.environ.get("FP_OFFLOAD_ARCH", "gfx950")
 
 
@@ -31,7 +31,7 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-result", "-Wno-unused-variable",
+           "-Wall", "-Wno-unused-result", "-Wno-unused-variable", "-pthread",
            "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

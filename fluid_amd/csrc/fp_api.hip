@@ -1,10 +1,14 @@
 // fp_api.hip -- the C ABI (include/fluid_projection.h) over the HIP solvers.
 #include "fp_uniform.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <mutex>
+#include <vector>
 
 namespace fp {
 static thread_local char g_error[1024] = "";
@@ -24,12 +28,15 @@ struct fp_ctx {
   uint64_t in_use = 0;
   hipStream_t stream = nullptr;
   int grids = 0;
+  fp::Projector* proj = nullptr;
+  fp::Server* server = nullptr;  // created by the first fp_submit
+  std::mutex gpu;                // serialises synchronous calls with the serving worker
 };
 
 struct fp_grid {
   fp_ctx* ctx = nullptr;
   fp_grid_desc desc{};
-  fp::UniformSolver* uniform = nullptr;
+  fp::UniformGrid* uniform = nullptr;
   uint64_t bytes = 0;
 };
 
@@ -66,10 +73,16 @@ int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out) {
   if (!ctx) return FP_ERR_OUT_OF_MEMORY;
   ctx->device = device;
   ctx->budget = hbm_budget_bytes;
+  ctx->proj = new (std::nothrow) fp::Projector();
+  if (!ctx->proj) {
+    delete ctx;
+    return FP_ERR_OUT_OF_MEMORY;
+  }
   // default priority stream (no priority streams on this pool)
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    delete ctx->proj;
     delete ctx;
     return FP_ERR_DEVICE;
   }
@@ -84,6 +97,10 @@ int fp_ctx_destroy(fp_ctx* ctx) {
     return FP_ERR_STATE;
   }
   (void)hipSetDevice(ctx->device);
+  delete ctx->server;
+  ctx->server = nullptr;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx->proj;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return FP_OK;
@@ -130,7 +147,7 @@ int fp_grid_bytes(const fp_grid_desc* desc, int32_t precision, uint64_t* bytes) 
   int rc = validate_desc(desc);
   if (rc) return rc;
   CHECK_ARG(bytes != nullptr, "bytes is NULL");
-  const size_t b = fp::UniformSolver::estimate_bytes(desc->nx, desc->ny, desc->nz);
+  const size_t b = fp::UniformGrid::estimate_bytes(desc->nx, desc->ny, desc->nz);
   if (b == 0) {
     set_error("no multigrid hierarchy for %d x %d x %d", desc->nx, desc->ny, desc->nz);
     return FP_ERR_LAYOUT;
@@ -158,7 +175,7 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
   g->ctx = ctx;
   g->desc = *desc;
   g->desc.column_bases = nullptr;
-  g->uniform = new (std::nothrow) fp::UniformSolver();
+  g->uniform = new (std::nothrow) fp::UniformGrid();
   if (!g->uniform) { delete g; return FP_ERR_OUT_OF_MEMORY; }
   rc = g->uniform->create(desc->nx, desc->ny, desc->nz, desc->h);
   if (rc) {
@@ -175,10 +192,16 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
 
 int fp_grid_destroy(fp_grid* g) {
   if (!g) return FP_OK;
+  if (g->ctx->server && g->ctx->server->jobs_on(g->uniform) > 0) {
+    set_error("fp_grid_destroy: the grid has submitted projections still running (fp_poll them first)");
+    return FP_ERR_STATE;
+  }
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
   g->ctx->in_use -= g->bytes;
   g->ctx->grids -= 1;
+  g->ctx->proj->forget(g->uniform->id());
   delete g->uniform;
   delete g;
   return FP_OK;
@@ -221,35 +244,130 @@ static int validate_inputs(const fp_inputs* in, const fp_outputs* out, const fp_
 
 int fp_project(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options* o, fp_report* rep) {
   CHECK_ARG(g != nullptr, "fp_project: grid is NULL");
+  return fp_project_batch(&g, 1, in, out, o, rep);
+}
+
+int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, fp_outputs* out,
+                     const fp_options* o, fp_report* reps) {
+  CHECK_ARG(grids != nullptr && count >= 1, "fp_project_batch: no grids");
+  CHECK_ARG(in != nullptr && out != nullptr, "fp_project_batch: NULL inputs/outputs");
+  fp_ctx* ctx = grids[0] ? grids[0]->ctx : nullptr;
+  CHECK_ARG(ctx != nullptr, "fp_project_batch: grid 0 is NULL");
+  std::vector<fp::UniformGrid*> list;
+  for (int i = 0; i < count; ++i) {
+    CHECK_ARG(grids[i] != nullptr, "fp_project_batch: grid %d is NULL", i);
+    CHECK_ARG(grids[i]->ctx == ctx, "fp_project_batch: grid %d belongs to another context", i);
+    for (int j = 0; j < i; ++j) CHECK_ARG(grids[j] != grids[i], "fp_project_batch: grid %d repeated", i);
+    int rc = validate_inputs(&in[i], &out[i], o);
+    if (rc) return rc;
+    list.push_back(grids[i]->uniform);
+  }
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  for (int i = 0; i < count; ++i) {
+    int rc = list[i]->bind(ctx->stream, &in[i], &out[i], o);
+    if (rc) return rc;
+  }
+  int rc = ctx->proj->run(ctx->stream, list, o, reps);
+  if (rc) return rc;
+  for (int i = 0; i < count; ++i) {
+    rc = list[i]->finish_outputs(ctx->stream, &in[i], &out[i]);
+    if (rc) return rc;
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    set_error("fp_project: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  return FP_OK;
+}
+
+int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options* o, uint64_t* ticket) {
+  CHECK_ARG(g != nullptr && ticket != nullptr, "fp_submit: NULL argument");
   int rc = validate_inputs(in, out, o);
   if (rc) return rc;
-  (void)hipSetDevice(g->ctx->device);
-  return g->uniform->project(g->ctx->stream, in, out, o, rep);
+  fp_ctx* ctx = g->ctx;
+  if (!ctx->server) {
+    ctx->server = new (std::nothrow) fp::Server(ctx->stream, ctx->device, ctx->proj, &ctx->gpu);
+    if (!ctx->server) return FP_ERR_OUT_OF_MEMORY;
+  }
+  return ctx->server->submit(g->uniform, in, out, o, ticket);
+}
+
+int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report, double* latency_ms) {
+  CHECK_ARG(ctx != nullptr, "fp_poll: context is NULL");
+  if (!ctx->server) {
+    set_error("fp_poll: nothing was submitted on this context");
+    return FP_ERR_INVALID_ARGUMENT;
+  }
+  return ctx->server->poll(ticket, timeout_ms, report, latency_ms);
 }
 
 int fp_probe_apply(fp_grid* g, const float* phi, const double* p, double* out) {
   CHECK_ARG(g && phi && p && out, "fp_probe_apply: NULL argument");
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
-  return g->uniform->probe_apply(g->ctx->stream, phi, p, out);
+  return fp::probe_apply(g->ctx->stream, g->uniform, phi, p, out);
 }
 
 int fp_probe_rhs(fp_grid* g, const fp_inputs* in, double* b) {
   CHECK_ARG(g && in && b && in->phi && in->u && in->v && in->w, "fp_probe_rhs: NULL argument");
   CHECK_ARG(in->memory == FP_MEMORY_HOST, "fp_probe_rhs takes host arrays");
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
-  return g->uniform->probe_rhs(g->ctx->stream, in, b);
+  int rc = g->uniform->bind(g->ctx->stream, in, nullptr, nullptr);
+  if (rc) return rc;
+  return g->ctx->proj->probe_rhs(g->ctx->stream, g->uniform, b);
 }
 
 int fp_probe_vcycle(fp_grid* g, const float* phi, const fp_options* o, const float* r, float* z) {
   CHECK_ARG(g && phi && o && r && z, "fp_probe_vcycle: NULL argument");
+  CHECK_ARG(o->precision >= 0 && o->precision <= 2, "unknown precision %d", o->precision);
+  CHECK_ARG(o->pre_sweeps == o->post_sweeps && o->pre_sweeps >= 0 && o->pre_sweeps <= 16,
+            "pre and post sweeps must match and lie in [0,16]");
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
-  return g->uniform->probe_vcycle(g->ctx->stream, phi, o, r, z);
+  fp::GridRec& R = *g->uniform->host_rec();
+  const size_t n = (size_t)R.n;
+  hipError_t e = hipMemcpy(R.lv[0].phi, phi, n * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(R.r, r, n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("fp_probe_vcycle: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  // normalise like the CG does: rscale = 2^-ceil(log2 max|r|)
+  float m = 0.0f;
+  for (size_t i = 0; i < n; ++i) m = std::max(m, std::fabs(r[i]));
+  const float rs = m > 0.0f ? std::exp2(-std::ceil(std::log2(m))) : 1.0f;
+  std::vector<uint32_t> raw(n);
+  int rc = g->ctx->proj->probe_vcycle(g->ctx->stream, g->uniform, o, rs, raw.data());
+  if (rc) return rc;
+  const float se = R.lv[0].Se;
+  if (o->precision == FP_PRECISION_MIXED_FP16) {
+    const __half* h = reinterpret_cast<const __half*>(raw.data());
+    for (size_t i = 0; i < n; ++i) z[i] = __half2float(h[i]) * se / rs;
+  } else if (o->precision == FP_PRECISION_MIXED_BF16) {
+    const __hip_bfloat16* h = reinterpret_cast<const __hip_bfloat16*>(raw.data());
+    for (size_t i = 0; i < n; ++i) z[i] = __bfloat162float(h[i]) * se / rs;
+  } else {
+    const float* f = reinterpret_cast<const float*>(raw.data());
+    for (size_t i = 0; i < n; ++i) z[i] = f[i] * se / rs;
+  }
+  return FP_OK;
 }
 
 int fp_probe_level_phi(fp_grid* g, const float* phi, int32_t level, int32_t preserve, float* out) {
   CHECK_ARG(g && phi && out, "fp_probe_level_phi: NULL argument");
+  CHECK_ARG(level >= 0 && level < g->uniform->levels(), "level %d out of range [0,%d)", level, g->uniform->levels());
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
-  return g->uniform->probe_level_phi(g->ctx->stream, phi, level, preserve, out);
+  const fp::GridRec& R = *g->uniform->host_rec();
+  hipError_t e = hipMemcpy(R.lv[0].phi, phi, (size_t)R.n * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("fp_probe_level_phi: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  return g->ctx->proj->probe_level_phi(g->ctx->stream, g->uniform, preserve, level, out);
 }
 
 }  // extern "C"

@@ -101,19 +101,17 @@ __device__ __forceinline__ double atomic_load_d(const double* p) {
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Two-kernel deterministic reduction.  Producers store one partial per
-// workgroup (no fence: the kernel boundary publishes them); a one-workgroup
-// finisher sums them in a fixed order.  This keeps device-scope fences (an L2
-// write-back on CDNA's per-XCD caches) out of the bandwidth-bound kernels.
-__device__ __forceinline__ unsigned grid_blocks() { return gridDim.x * gridDim.y * gridDim.z; }
-__device__ __forceinline__ unsigned block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-
+// workgroup of their grid (no fence: the kernel boundary publishes them); a
+// one-workgroup-per-grid finisher sums them in a fixed order.  This keeps
+// device-scope fences (an L2 write-back on CDNA's per-XCD caches) out of the
+// bandwidth-bound kernels.  blk / nblk: the workgroup's index and count within
+// its grid (a batched launch covers several grids).
 template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials, int blk, int nblk) {
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
-    const unsigned G = grid_blocks(), b = block_linear();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + b] = v[k];
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * nblk + blk] = v[k];
   }
 }
 
@@ -147,14 +145,13 @@ __device__ __forceinline__ float block_max(float m) {
 // through partials too: one device-scope atomic per wave on a single address
 // serialises across the XCDs.
 template <int NV>
-__device__ __forceinline__ void block_partials_max(double (&v)[NV], float m, double* partials) {
+__device__ __forceinline__ void block_partials_max(double (&v)[NV], float m, double* partials, int blk, int nblk) {
   const float bm = block_max(m);
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
-    const unsigned G = grid_blocks(), b = block_linear();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + b] = v[k];
-    partials[(size_t)NV * G + b] = (double)bm;
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * nblk + blk] = v[k];
+    partials[(size_t)NV * nblk + blk] = (double)bm;
   }
 }
 

@@ -1,7 +1,7 @@
 // fp_march.hpp -- 7-point stencil passes as column-tile x-marches (gfx950).
 //
-// A 256-lane workgroup owns a (TY x TZ) tile of (y, z) columns (z fastest: one
-// 64-lane wave per row when TZ = 64) and walks a chunk of x planes.  Each plane
+// A 256-lane workgroup owns a (TY x TZ) = 8 x 32 tile of (y, z) columns (z
+// fastest) and walks a chunk of x planes.  Each plane
 // is loaded once into LDS with a one-cell in-plane ring, three planes rotate,
 // and every lane evaluates the stencil of its own column at the middle plane.
 // No integer division, coalesced row loads, and the +-x neighbours come from
@@ -9,34 +9,50 @@
 //
 // An operation `Op` supplies
 //   using Cell = ...;                                   LDS record per cell
+//   static Op make(const GridRec&, const MarchLaunch&); binds the grid's arrays
 //   void prepare();                                     reads device scalars once per workgroup
 //   Cell load(int x, int y, int z, bool inside) const;  staging (inside: in the domain)
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
 //              double* acc, float* mx) const;           nb/has: -x,+x,-y,+y,-z,+z
 //   static constexpr int NV;                            sum-reduction width
 //   static constexpr bool WITH_MAX;                     also reduce a max (stored after the sums)
-//   double* partials;                                   per-workgroup partials (a finisher
-//                                                       kernel completes them)
+// Reductions go to the grid's partials (a finisher kernel completes them).
 #pragma once
 
 #include "fp_common.hpp"
 #include "fp_sweep.hpp"
+#include "fp_uniform.hpp"
 
 namespace fp {
 
-template <int TZ, class Op>
-__global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in, int chunk, int tiles_z,
-                                               int tiles_y, int ngroups, const State* st, int honour_done) {
-  if (honour_done && st->done) return;
+// Launch-uniform parameters of a march (the same for every grid of a batch).
+struct MarchLaunch {
+  int level;    // multigrid level (V-cycle residual)
+  int which_e;  // correction buffer (0: e, 1: e2) the residual reads
+};
+
+constexpr int kMarchTZ = 32;
+
+// honour: 0 always runs, 1 skips grids whose inner solve is done, 2 skips
+// grids whose refinement has converged.
+template <class Op>
+__global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int honour, MarchLaunch la) {
+  FP_PART(parts, np);
+  if (honour == 1 && G.st->done) return;
+  if (honour == 2 && G.st->outer_done) return;
+  constexpr int TZ = kMarchTZ;
   constexpr int TY = 256 / TZ;
   constexpr int RZ = TZ + 2, RY = TY + 2;
   using Cell = typename Op::Cell;
   __shared__ Cell s[3][RY][RZ];
-  Op op = op_in;
+  const LevelRec& lv = G.lv[la.level];
+  const int nx = lv.d[0], ny = lv.d[1], nz = lv.d[2];
+  const int chunk = lv.mg.chunk;
+  Op op = Op::make(G, la);
   op.prepare();
   float mx = 0.0f;
   int tzi, tyi, cxi;
-  tile_of_group(blockIdx.x, tiles_z, tiles_y, ngroups, tzi, tyi, cxi);
+  tile_of_group(blk, lv.mg.tz, lv.mg.ty, nblk, tzi, tyi, cxi);
   const int z0 = tzi * TZ, y0 = tyi * TY;
   const int x0 = cxi * chunk;
   const int x1 = min(x0 + chunk, nx);
@@ -85,38 +101,25 @@ __global__ __launch_bounds__(256) void k_march(int nx, int ny, int nz, Op op_in,
     lds_barrier();
   }
   if constexpr (Op::NV > 0) {
-    if constexpr (Op::WITH_MAX) block_partials_max<Op::NV>(acc, mx, op.partials);
-    else block_partials<Op::NV>(acc, op.partials);
+    if constexpr (Op::WITH_MAX) block_partials_max<Op::NV>(acc, mx, G.partials, blk, nblk);
+    else block_partials<Op::NV>(acc, G.partials, blk, nblk);
   }
 }
 
-struct MarchGrid {
-  int ngroups;
-  int chunk;
-  int tz, ty;     // tiles along z and y
-  int TZ;         // tile width along z (64 or 32)
-};
-
-#ifndef FP_MARCH_TZ
-#define FP_MARCH_TZ 32
-#endif
-
-// Tiles of TZ x 256/TZ columns, x chunks of at least 8 planes, about
-// target_blocks workgroups (at most the 16384 partials of the reductions).
-inline MarchGrid march_grid(int nx, int ny, int nz, long target_blocks = 4096) {
-  const int TZ = nz >= FP_MARCH_TZ ? FP_MARCH_TZ : 32;
-  const int TY = 256 / TZ;
+// Tiles of 32 x 8 columns, x chunks of at least 8 planes, about target_blocks
+// workgroups (at most the 16384 partials of the reductions).
+inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = 4096) {
+  const int TZ = kMarchTZ, TY = 256 / TZ;
   const int tz = (nz + TZ - 1) / TZ, ty = (ny + TY - 1) / TY;
   const long tiles = (long)tz * ty;
   long c = ((long)nx * tiles + target_blocks - 1) / target_blocks;
   if (c < 8) c = 8;
   if (c > nx) c = nx;
-  while ((long)tiles * ((nx + c - 1) / c) > 16384) ++c;
-  MarchGrid m;
+  while ((long)tiles * ((nx + c - 1) / c) > (long)kMaxBlocks) ++c;
+  TileGeo m;
   m.chunk = (int)c;
   m.tz = tz;
   m.ty = ty;
-  m.TZ = TZ;
   m.ngroups = (int)(tiles * ((nx + c - 1) / c));
   return m;
 }
@@ -138,8 +141,26 @@ struct OpDirection {
   float zmul;  // static part of the z scale; divided by the device rscale
   float zm, beta;
   float gx, gy, gz;
-  State* st;
-  double* partials;
+  const State* st;
+  static __device__ __forceinline__ OpDirection make(const GridRec& G, const MarchLaunch& la) {
+    const LevelRec& l0 = G.lv[0];
+    OpDirection o;
+    o.ny = l0.d[1];
+    o.nz = l0.d[2];
+    o.phi = l0.phi;
+    o.z = (const T*)(l0.efinal ? l0.e2 : l0.e);
+    // CG direction buffers alternate with the grid's own iteration count
+    const int parity = G.st->iter & 1;
+    o.dold = G.d[parity];
+    o.dnew = G.d[parity ^ 1];
+    o.q = G.q;
+    o.zmul = l0.Se;
+    o.gx = l0.g[0];
+    o.gy = l0.g[1];
+    o.gz = l0.g[2];
+    o.st = G.st;
+    return o;
+  }
   __device__ __forceinline__ void prepare() {
     zm = zmul / st->rscale;
     beta = (float)st->beta;
@@ -170,51 +191,6 @@ struct OpDirection {
 };
 
 // ---------------------------------------------------------------------------
-// V-cycle residual: t = (r*rm - emul * A e) * tinv on liquid cells.
-// ---------------------------------------------------------------------------
-template <typename TR, typename T>
-struct OpResidual {
-  struct Cell { float e; float p; };
-  static constexpr int NV = 0;
-  static constexpr bool WITH_MAX = false;
-  int ny, nz;
-  const float* phi;
-  const TR* r;
-  const T* e;
-  T* t;
-  float rmul, emul, tinv;
-  int dyn;  // right-hand side scaled by the device rscale (finest level)
-  float rm;
-  float gx, gy, gz;
-  const State* st;
-  double* partials;
-  __device__ __forceinline__ void prepare() { rm = dyn ? rmul * st->rscale : rmul; }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
-    Cell c{0.0f, 1.0f};
-    if (inside) {
-      const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.p = phi[i];
-      if (c.p < 0.0f) c.e = Store<T>::ld(e, i);
-    }
-    return c;
-  }
-  __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool* has,
-                                        double*, float*) const {
-    const size_t i = ((size_t)x * ny + y) * nz + zz;
-    float res = 0.0f;
-    if (c.p < 0.0f) {
-      const float g[6] = {gx, gx, gy, gy, gz, gz};
-      float ap = 0.0f;
-#pragma unroll
-      for (int f = 0; f < 6; ++f)
-        if (has[f]) ap += face_weight_fast(c.p, nb[f].p, g[f]) * (c.e - nb[f].e);
-      res = (Store<TR>::ld(r, i) * rm - emul * ap) * tinv;
-    }
-    Store<T>::st(t, i, res);
-  }
-};
-
-// ---------------------------------------------------------------------------
 // V-cycle residual on the packed coefficients of a level (the operator its
 // fused sweeps relax): t = (r*rm - emul * A e) * tinv.
 // ---------------------------------------------------------------------------
@@ -229,10 +205,26 @@ struct OpResidualW {
   const T* e;
   T* t;
   float rmul, emul, tinv;
-  int dyn;
+  int dyn;  // right-hand side scaled by the device rscale (finest level)
   float rm;
   const State* st;
-  double* partials;
+  static __device__ __forceinline__ OpResidualW make(const GridRec& G, const MarchLaunch& la) {
+    const LevelRec& l = G.lv[la.level];
+    const bool top = la.level == 0;
+    OpResidualW o;
+    o.ny = l.d[1];
+    o.nz = l.d[2];
+    o.wts = (const W4<T>*)l.w;
+    o.r = (const TR*)(top ? (const void*)G.r : l.r);
+    o.e = (const T*)(la.which_e ? l.e2 : l.e);
+    o.t = (T*)l.t;
+    o.rmul = (top ? 1.0f : l.Sr) / l.St;
+    o.emul = l.Se / l.St;
+    o.tinv = 1.0f;
+    o.dyn = top;
+    o.st = G.st;
+    return o;
+  }
   __device__ __forceinline__ void prepare() { rm = dyn ? rmul * st->rscale : rmul; }
   __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
     Cell c;
@@ -274,8 +266,17 @@ struct OpTrueResidual {
   const double* b;
   const double* xv;
   float* r;
-  State* st;
-  double* partials;
+  static __device__ __forceinline__ OpTrueResidual make(const GridRec& G, const MarchLaunch&) {
+    const LevelRec& l0 = G.lv[0];
+    OpTrueResidual o;
+    o.ny = l0.d[1];
+    o.nz = l0.d[2];
+    o.phi = l0.phi;
+    o.b = G.b;
+    o.xv = G.x;
+    o.r = G.r;
+    return o;
+  }
   __device__ __forceinline__ void prepare() {}
   __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
     Cell c{0.0, 1.0f};
@@ -302,18 +303,5 @@ struct OpTrueResidual {
     r[i] = (float)res;
   }
 };
-
-// Launches the march and returns its workgroup count (the partials to finish).
-template <class Op>
-inline unsigned launch_march(hipStream_t s, int nx, int ny, int nz, const Op& op, const State* st, int honour_done) {
-  const MarchGrid m = march_grid(nx, ny, nz);
-  if (m.TZ == 64)
-    hipLaunchKernelGGL((k_march<64, Op>), dim3(m.ngroups), dim3(256), 0, s, nx, ny, nz, op, m.chunk, m.tz, m.ty,
-                       m.ngroups, st, honour_done);
-  else
-    hipLaunchKernelGGL((k_march<32, Op>), dim3(m.ngroups), dim3(256), 0, s, nx, ny, nz, op, m.chunk, m.tz, m.ty,
-                       m.ngroups, st, honour_done);
-  return (unsigned)m.ngroups;
-}
 
 }  // namespace fp

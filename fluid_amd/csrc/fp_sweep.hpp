@@ -18,13 +18,9 @@
 #pragma once
 
 #include "fp_common.hpp"
+#include "fp_uniform.hpp"
 
 namespace fp {
-
-struct SweepGeom {
-  int nx, ny, nz;
-  float gx, gy, gz;
-};
 
 struct ProlongArgs {
   const void* ec;      // coarse correction (storage T; zero on coarse air cells), or nullptr
@@ -169,13 +165,25 @@ __device__ __forceinline__ void window_taps(int f, int s, int nc, int c0, int& a
 // first-colour cells of the ring-extended tile (CY rows of CZ/2), colour-2
 // lanes the second-colour cells of the tile (TY rows of TZ/2).
 template <int TZ, int D, typename TR, typename T, bool PROLONG, bool ZERO>
-__global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __restrict__ wts,
-                                                const TR* __restrict__ r, float rmul, int dyn,
-                                                const T* __restrict__ ein, T* __restrict__ eout, int first,
-                                                ProlongArgs P, int chunk, int tiles_z, int tiles_y, int ngroups,
-                                                const State* st) {
+__global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int level, int which_in, int first) {
   static_assert(D == 1 || D == 2 || D == 4, "the prefetch depth must divide the unroll of 4");
+  FP_PART(parts, np);
+  const State* st = G.st;
   if (st->done) return;
+  const LevelRec& lvl = G.lv[level];
+  const bool top = level == 0;
+  const W4<T>* __restrict__ wts = (const W4<T>*)lvl.w;
+  const TR* __restrict__ r = (const TR*)(top ? (const void*)G.r : lvl.r);
+  const float rmul = (top ? 1.0f : lvl.Sr) / lvl.Se;
+  const int dyn = top;
+  const T* __restrict__ ein = (const T*)(which_in ? lvl.e2 : lvl.e);
+  T* __restrict__ eout = (T*)(which_in ? lvl.e : lvl.e2);
+  ProlongArgs P{nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
+  if constexpr (PROLONG) {
+    const LevelRec& c = G.lv[level + 1];
+    P = ProlongArgs{c.efinal ? c.e2 : c.e, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / lvl.Se};
+  }
+  const int chunk = lvl.sw.chunk, tiles_z = lvl.sw.tz, tiles_y = lvl.sw.ty;
   constexpr int TY = 256 / TZ;
   constexpr int EY = TY + 4, EZ = TZ + 4;    // old correction: -2 .. T+1
   constexpr int WY = PROLONG ? EY : TY + 3;  // coefficients: -2 .. T (T+1)
@@ -195,9 +203,9 @@ __global__ __launch_bounds__(256) void k_sweep3(SweepGeom G, const W4<T>* __rest
   __shared__ T s_p[PROLONG ? PN : 1];
 
   const float rm = dyn ? rmul * st->rscale : rmul;
-  const int nx = G.nx, ny = G.ny, nz = G.nz;
+  const int nx = lvl.d[0], ny = lvl.d[1], nz = lvl.d[2];
   int tzi, tyi, cxi;
-  tile_of_group(blockIdx.x, tiles_z, tiles_y, ngroups, tzi, tyi, cxi);
+  tile_of_group(blk, tiles_z, tiles_y, nblk, tzi, tyi, cxi);
   const int z0 = tzi * TZ, y0 = tyi * TY, x0 = cxi * chunk;
   const int x1 = min(x0 + chunk, nx);
   const int tid = threadIdx.x;

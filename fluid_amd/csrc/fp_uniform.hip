@@ -1,8 +1,10 @@
-// fp_uniform.hip -- uniform MAC-grid pressure projection on gfx950.
+// fp_uniform.hip -- uniform MAC-grid pressure projection on gfx950, batched.
 //
 // One projection = divergence/rhs -> FP64 iterative refinement around an
 // FP32 multigrid-preconditioned CG (multigrid vectors in FP32, FP16 or BF16)
-// -> pressure-gradient subtract.  All arrays are C order, z fastest.
+// -> pressure-gradient subtract.  All arrays are C order, z fastest.  Every
+// kernel runs over a part table (fp_uniform.hpp), so one launch per stage
+// serves a whole batch of independent grids of different sizes.
 //
 // Reference algebra restated here (the kernels are written for CDNA4, not
 // translated from WGSL):
@@ -23,6 +25,8 @@
 #include "fp_march.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -49,7 +53,10 @@ struct LevelArgs {
   const float* phi;
 };
 
-#define GRID_STRIDE(idx, N) for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (N); idx += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ Grid3 grid_of(const LevelRec& l) { return Grid3{l.d[0], l.d[1], l.d[2]}; }
+__device__ __forceinline__ LevelArgs largs_of(const LevelRec& l) {
+  return LevelArgs{grid_of(l), l.g[0], l.g[1], l.g[2], l.phi};
+}
 
 // 4-wide loads of the storage types (arrays are 256-byte aligned).
 template <typename T> struct V4;
@@ -74,31 +81,90 @@ template <> struct V4<__hip_bfloat16> {
   }
 };
 
-// --------------------------------------------------------------------------
-// generic fills
-// --------------------------------------------------------------------------
 template <typename T>
-__global__ void k_fill(T* p, size_t n, T v) {
-  GRID_STRIDE(i, n) p[i] = v;
+__device__ __forceinline__ float ldT(const T* p, size_t i) { return Store<T>::ld(p, i); }
+template <typename T>
+__device__ __forceinline__ void stT(T* p, size_t i, float v) { Store<T>::st(p, i, v); }
+
+// --------------------------------------------------------------------------
+// per-projection state and vector resets
+// --------------------------------------------------------------------------
+// One workgroup per grid: a fresh State (rscale 1).
+__global__ void k_reset_state(const Part* parts) {
+  if (threadIdx.x != 0) return;
+  State* st = parts[blockIdx.x].g->st;
+  State z;
+  __builtin_memset(&z, 0, sizeof(z));
+  z.rscale = 1.0f;
+  *st = z;
 }
-__global__ void k_fill4(float* p, size_t n, float v) {
+
+// which: 0 x (FP64), 1 e, 2 d[0].  skip_converged: leave grids whose
+// refinement has converged.
+__global__ void k_zero(const Part* parts, int np, int which, int skip_converged) {
+  FP_PART(parts, np);
+  if (skip_converged && G.st->outer_done) return;
+  const size_t n = (size_t)G.n;
+  if (which == 0) {
+    if (G.warm) return;
+    double* p = G.x;
+    FP_STRIDE(i, n) p[i] = 0.0;
+    return;
+  }
+  float* p = which == 1 ? G.e : G.d[0];
   const size_t n4 = n / 4;
-  GRID_STRIDE(i, n4) reinterpret_cast<float4*>(p)[i] = make_float4(v, v, v, v);
-  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) p[n4 * 4 + threadIdx.x] = v;
+  FP_STRIDE(i, n4) reinterpret_cast<float4*>(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (blk == 0 && threadIdx.x < (n & 3)) p[n4 * 4 + threadIdx.x] = 0.0f;
+}
+
+// Pending work of a batch: counts[0] grids whose refinement has not
+// converged, counts[1] grids whose inner solve is still running.
+__global__ void k_status(const Part* parts, int np, int* counts) {
+  __shared__ int s_outer, s_inner;
+  if (threadIdx.x == 0) {
+    s_outer = 0;
+    s_inner = 0;
+  }
+  __syncthreads();
+  int o = 0, in = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    const State* st = parts[i].g->st;
+    o += st->outer_done ? 0 : 1;
+    in += st->done ? 0 : 1;
+  }
+  atomicAdd(&s_outer, o);
+  atomicAdd(&s_inner, in);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    counts[0] = s_outer;
+    counts[1] = s_inner;
+  }
+}
+
+// The States of a batch gathered in part order (one copy to the host).
+__global__ void k_gather_states(const Part* parts, int np, State* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) out[i] = *parts[i].g->st;
 }
 
 // --------------------------------------------------------------------------
 // rhs: b = -(rho h/dt) sum(u_out - u_in) + (rho h^2/dt) q on liquid cells.
 // Also ||b||^2 and max |div u| (1/s) before the projection.
 // --------------------------------------------------------------------------
-__global__ void k_rhs(Grid3 g, const float* __restrict__ phi, const float* __restrict__ u,
-                      const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ q,
-                      double s_flux, double s_q, double inv_h, double tol2, double* __restrict__ b, State* st,
-                      double* partials) {
+__global__ void k_rhs(const Part* parts, int np) {
+  FP_PART(parts, np);
+  const LevelRec& l0 = G.lv[0];
+  const Grid3 g = grid_of(l0);
+  const float* __restrict__ phi = l0.phi;
+  const float* __restrict__ u = G.in_u;
+  const float* __restrict__ v = G.in_v;
+  const float* __restrict__ w = G.in_w;
+  const float* __restrict__ q = G.in_q;
+  double* __restrict__ b = G.b;
+  const double s_flux = G.s_flux, s_q = G.s_q, inv_h = G.inv_h;
   double acc[2] = {0.0, 0.0};
   float dmax = 0.0f;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  GRID_STRIDE(idx, g.n()) {
+  FP_STRIDE(idx, g.n()) {
     int i, j, k;
     g.ijk(idx, i, j, k);
     double val = 0.0;
@@ -118,30 +184,38 @@ __global__ void k_rhs(Grid3 g, const float* __restrict__ phi, const float* __res
     }
     b[idx] = val;
   }
-  block_partials_max<2>(acc, dmax, partials);
+  block_partials_max<2>(acc, dmax, G.partials, blk, nblk);
 }
 
-__global__ void k_fin_rhs(const double* __restrict__ partials, unsigned G, double tol2, State* st) {
+// finishers run one workgroup per part of their producer's table
+__global__ void k_fin_rhs(const Part* parts) {
+  const Part p = parts[blockIdx.x];
+  const GridRec& G = *p.g;
+  State* st = G.st;
   double out[2];
   float mx;
-  sum_partials_max<2>(partials, G, out, mx);
+  sum_partials_max<2>(G.partials, p.nblocks, out, mx);
   if (threadIdx.x == 0) {
     st->divmax_before = __float_as_uint(mx);
     st->bb = out[0];
     // The refinement stops at a quarter of the requested relative residual:
     // the max-norm divergence left in the FP32 output then stays below that
     // of an FP64 solve stopped at the requested tolerance.
-    st->tol2_bb = 0.0625 * tol2 * out[0];
+    st->tol2_bb = 0.0625 * G.tol2 * out[0];
     st->liquid = (long long)out[1];
   }
 }
 
 // Refinement control after the FP64 true residual (OpTrueResidual): decides
 // convergence of the refinement and the target of the next inner solve.
-__global__ void k_fin_true(const double* __restrict__ partials, unsigned G, State* st) {
+// new_pass: an inner solve follows for grids that have not converged.
+__global__ void k_fin_true(const Part* parts, int new_pass) {
+  const Part p = parts[blockIdx.x];
+  State* st = p.g->st;
+  if (st->outer_done) return;
   double out[1];
   float m;
-  sum_partials_max<1>(partials, G, out, m);
+  sum_partials_max<1>(p.g->partials, p.nblocks, out, m);
   if (threadIdx.x == 0) {
     const double rr = out[0];
     st->rr64 = rr;
@@ -152,24 +226,35 @@ __global__ void k_fin_true(const double* __restrict__ partials, unsigned G, Stat
     // divergence), at most what FP32 CG can be trusted to reduce in one pass.
     st->inner_target2 = fmax(0.01 * st->tol2_bb, 1e-12 * rr);
     st->iter = 0;
-    st->done = st->outer_done;
+    st->done = st->outer_done || !new_pass;
+    if (!st->outer_done && new_pass) st->outer += 1;
     st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
   }
 }
 
-// x += e; e = 0 (folds one inner solve into the FP64 iterate).
-__global__ void k_accumulate(size_t n, double* __restrict__ x, float* __restrict__ e) {
+// x += e; e = 0 (folds one inner solve into the FP64 iterate).  A broken CG
+// recurrence leaves the FP64 iterate as it was (fail closed).
+__global__ void k_accumulate(const Part* parts, int np) {
+  FP_PART(parts, np);
+  const State* st = G.st;
+  if (st->outer_done) return;
+  const bool keep = !(st->flags & 0x7);
+  const size_t n = (size_t)G.n;
+  double* __restrict__ x = G.x;
+  float* __restrict__ e = G.e;
   const size_t n4 = n / 4;
-  GRID_STRIDE(i, n4) {
+  FP_STRIDE(i, n4) {
     const float4 ev = reinterpret_cast<const float4*>(e)[i];
-    double4 xv = reinterpret_cast<const double4*>(x)[i];
-    xv.x += ev.x; xv.y += ev.y; xv.z += ev.z; xv.w += ev.w;
-    reinterpret_cast<double4*>(x)[i] = xv;
+    if (keep) {
+      double4 xv = reinterpret_cast<const double4*>(x)[i];
+      xv.x += ev.x; xv.y += ev.y; xv.z += ev.z; xv.w += ev.w;
+      reinterpret_cast<double4*>(x)[i] = xv;
+    }
     reinterpret_cast<float4*>(e)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+  if (blk == 0 && threadIdx.x < (n & 3)) {
     const size_t i = n4 * 4 + threadIdx.x;
-    x[i] += (double)e[i];
+    if (keep) x[i] += (double)e[i];
     e[i] = 0.0f;
   }
 }
@@ -177,9 +262,16 @@ __global__ void k_accumulate(size_t n, double* __restrict__ x, float* __restrict
 // --------------------------------------------------------------------------
 // multigrid setup: coarse level set (CM11a Eq. 15-16 with C preserved levels)
 // --------------------------------------------------------------------------
-__global__ void k_downsample_phi(Grid3 f, const float* __restrict__ pf, Grid3 c, float* __restrict__ pc, int sx,
-                                 int sy, int sz, int use_positive) {
-  GRID_STRIDE(idx, c.n()) {
+__global__ void k_downsample_phi(const Part* parts, int np, int level, int preserve) {
+  FP_PART(parts, np);
+  const LevelRec& fl = G.lv[level - 1];
+  const LevelRec& cl = G.lv[level];
+  const Grid3 f = grid_of(fl), c = grid_of(cl);
+  const float* __restrict__ pf = fl.phi;
+  float* __restrict__ pc = cl.phi;
+  const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
+  const int use_positive = level <= preserve;
+  FP_STRIDE(idx, c.n()) {
     int I, J, K;
     c.ijk(idx, I, J, K);
     float sum = 0.0f, pos = 0.0f;
@@ -208,8 +300,8 @@ __global__ void k_downsample_phi(Grid3 f, const float* __restrict__ pf, Grid3 c,
 // Dense inverse of the coarsest level's liquid block (one workgroup, FP64
 // Gauss-Jordan in place; SPD so no pivoting).  A failed pivot falls back to
 // the diagonal (Jacobi) map so the preconditioner stays a fixed SPD map.
-__global__ void k_coarse_setup(LevelArgs L, int* __restrict__ cmap, int* __restrict__ ncount,
-                               double* __restrict__ a, int max_n, State* st) {
+__device__ void coarse_setup(LevelArgs L, int* __restrict__ cmap, int* __restrict__ ncount,
+                             double* __restrict__ a, int max_n, State* st) {
   __shared__ int s_n;
   __shared__ int s_fail;
   const Grid3 g = L.g;
@@ -309,26 +401,39 @@ __global__ void k_coarse_setup(LevelArgs L, int* __restrict__ cmap, int* __restr
   }
 }
 
+__global__ void k_coarse_setup(const Part* parts) {
+  const GridRec& G = *parts[blockIdx.x].g;
+  coarse_setup(largs_of(G.lv[G.L - 1]), G.cmap, G.ncoarse, G.minv, G.max_coarse, G.st);
+}
+
 // --------------------------------------------------------------------------
 // V-cycle kernels.  Stored value = true value / scale; TR is the storage of
 // the right-hand side (FP32 at the finest level, where it is the CG residual
-// scaled by the device-side rscale), T of everything else.
+// scaled by the device-side rscale), T of everything else.  `level` is the
+// same for every grid of a launch; the finest level reads the CG residual.
 // --------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ float ldT(const T* p, size_t i) { return Store<T>::ld(p, i); }
-template <typename T>
-__device__ __forceinline__ void stT(T* p, size_t i, float v) { Store<T>::st(p, i, v); }
+template <typename TR>
+__device__ __forceinline__ const TR* level_rhs(const GridRec& G, int level) {
+  return (const TR*)(level == 0 ? (const void*)G.r : G.lv[level].r);
+}
 
-// One colour of red-black Gauss-Seidel (colour = (i+j+k)&1).  zero_init:
-// the first pass from a zero guess, which also clears the other colour.
+// One colour of red-black Gauss-Seidel (colour = (i+j+k)&1) on levels too
+// small for the fused sweep.  zero_init: the first pass from a zero guess,
+// which also clears the other colour.
 template <typename TR, typename T>
-__global__ void k_smooth(LevelArgs L, const TR* __restrict__ r, float rmul, int dyn, T* __restrict__ e, int colour,
-                         int zero_init, const State* st) {
+__global__ void k_smooth(const Part* parts, int np, int level, int colour, int zero_init) {
+  FP_PART(parts, np);
+  const State* st = G.st;
   if (st->done) return;
-  const float rm = dyn ? rmul * st->rscale : rmul;
+  const LevelRec& lv = G.lv[level];
+  const LevelArgs L = largs_of(lv);
+  const TR* __restrict__ r = level_rhs<TR>(G, level);
+  T* __restrict__ e = (T*)lv.e;
+  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.Se;
+  const float rm = level == 0 ? rmul * st->rscale : rmul;
   const Grid3 g = L.g;
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
-  GRID_STRIDE(idx, g.n()) {
+  FP_STRIDE(idx, g.n()) {
     int i, j, k;
     g.ijk(idx, i, j, k);
     if (((i + j + k) & 1) != colour) {
@@ -360,15 +465,23 @@ __global__ void k_smooth(LevelArgs L, const TR* __restrict__ r, float rmul, int 
   }
 }
 
-// t = r - A e  (stored with multiplier tmul applied: t_s = (r*rm - emul*A e_s) * tinv)
+// t = r - A e on levels too small for the fused march (stored with St).
 template <typename TR, typename T>
-__global__ void k_residual(LevelArgs L, const TR* __restrict__ r, float rmul, int dyn, const T* __restrict__ e,
-                           float emul, float tinv, T* __restrict__ t, const State* st) {
+__global__ void k_residual(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  const State* st = G.st;
   if (st->done) return;
-  const float rm = dyn ? rmul * st->rscale : rmul;
+  const LevelRec& lv = G.lv[level];
+  const LevelArgs L = largs_of(lv);
+  const TR* __restrict__ r = level_rhs<TR>(G, level);
+  const T* __restrict__ e = (const T*)lv.e;
+  T* __restrict__ t = (T*)lv.t;
+  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.St;
+  const float rm = level == 0 ? rmul * st->rscale : rmul;
+  const float emul = lv.Se / lv.St;
   const Grid3 g = L.g;
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
-  GRID_STRIDE(idx, g.n()) {
+  FP_STRIDE(idx, g.n()) {
     int i, j, k;
     g.ijk(idx, i, j, k);
     const float pc = L.phi[idx];
@@ -390,7 +503,7 @@ __global__ void k_residual(LevelArgs L, const TR* __restrict__ r, float rmul, in
       RNB(k > 0, -1, L.gz)
       RNB(k + 1 < g.nz, 1, L.gz)
 #undef RNB
-      res = (ldT(r, idx) * rm - emul * ap) * tinv;
+      res = ldT(r, idx) * rm - emul * ap;
     }
     stT(t, idx, res);
   }
@@ -416,16 +529,25 @@ __device__ __forceinline__ int ptaps(int f, int s, int nc, int* c, float* w) {
   return 2;
 }
 
+// r_{l+1} = R t_l on the coarse level's liquid cells (R = P^T / 2^k).
 template <typename T>
-__global__ void k_restrict(Grid3 f, const T* __restrict__ tf, LevelArgs C, int sx, int sy, int sz, float fac,
-                           T* __restrict__ rc, const State* st) {
-  if (st->done) return;
-  const Grid3 c = C.g;
-  GRID_STRIDE(idx, c.n()) {
+__global__ void k_restrict(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  const LevelRec& fl = G.lv[level];
+  const LevelRec& cl = G.lv[level + 1];
+  const Grid3 f = grid_of(fl), c = grid_of(cl);
+  const T* __restrict__ tf = (const T*)fl.t;
+  T* __restrict__ rc = (T*)cl.r;
+  const float* __restrict__ phic = cl.phi;
+  const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
+  const int k2 = (sx == 2) + (sy == 2) + (sz == 2);
+  const float fac = (1.0f / (float)(1 << k2)) * fl.St / cl.Sr;
+  FP_STRIDE(idx, c.n()) {
     int I, J, K;
     c.ijk(idx, I, J, K);
     float acc = 0.0f;
-    if (C.phi[idx] < 0.0f) {
+    if (phic[idx] < 0.0f) {
       int fi[4], fj[4], fk[4];
       float wi[4], wj[4], wk[4];
       const int ni = rtaps(I, sx, f.nx, c.nx, fi, wi);
@@ -443,13 +565,22 @@ __global__ void k_restrict(Grid3 f, const T* __restrict__ tf, LevelArgs C, int s
   }
 }
 
+// e_l += P e_{l+1} on the fine level's liquid cells; which: the fine buffer.
 template <typename T>
-__global__ void k_prolong_add(LevelArgs F, T* __restrict__ ef, Grid3 c, const float* __restrict__ phic,
-                              const T* __restrict__ ec, int sx, int sy, int sz, float fac, const State* st) {
-  if (st->done) return;
-  const Grid3 f = F.g;
-  GRID_STRIDE(idx, f.n()) {
-    if (!(F.phi[idx] < 0.0f)) continue;
+__global__ void k_prolong_add(const Part* parts, int np, int level, int which) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  const LevelRec& fl = G.lv[level];
+  const LevelRec& cl = G.lv[level + 1];
+  const Grid3 f = grid_of(fl), c = grid_of(cl);
+  T* __restrict__ ef = (T*)(which ? fl.e2 : fl.e);
+  const T* __restrict__ ec = (const T*)(cl.efinal ? cl.e2 : cl.e);
+  const float* __restrict__ phif = fl.phi;
+  const float* __restrict__ phic = cl.phi;
+  const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
+  const float fac = cl.Se / fl.Se;
+  FP_STRIDE(idx, f.n()) {
+    if (!(phif[idx] < 0.0f)) continue;
     int i, j, k;
     f.ijk(idx, i, j, k);
     int ci[2], cj[2], ck[2];
@@ -471,15 +602,35 @@ __global__ void k_prolong_add(LevelArgs F, T* __restrict__ ef, Grid3 c, const fl
   }
 }
 
-// e = fac * Minv r on the coarsest liquid cells (one workgroup).
+// Clears a level's correction buffer (pre-smoothing with zero sweeps).
+template <typename T>
+__global__ void k_zero_level(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  const LevelRec& lv = G.lv[level];
+  T* e = (T*)lv.e;
+  FP_STRIDE(i, (size_t)lv.n) stT(e, i, 0.0f);
+}
+
+// e = fac * Minv r on each grid's coarsest liquid cells (one workgroup per
+// grid).  TR: storage of that level's right-hand side (FP32 when the coarsest
+// level is the finest).
 template <typename T, typename TR>
-__global__ void k_coarse_solve(Grid3 g, const int* __restrict__ cmap, const int* __restrict__ ncount,
-                               const double* __restrict__ minv, const TR* __restrict__ r, float rmul, int dyn,
-                               T* __restrict__ e, float fac, const State* st) {
+__global__ void k_coarse_solve(const Part* parts) {
+  const GridRec& G = *parts[blockIdx.x].g;
+  const State* st = G.st;
   if (st->done) return;
   __shared__ double rs[512];
-  const int n = *ncount;
-  const float rm = dyn ? rmul * st->rscale : rmul;
+  const int lc = G.L - 1;
+  const LevelRec& lv = G.lv[lc];
+  const Grid3 g = grid_of(lv);
+  const int* __restrict__ cmap = G.cmap;
+  const double* __restrict__ minv = G.minv;
+  const TR* __restrict__ r = level_rhs<TR>(G, lc);
+  T* __restrict__ e = (T*)lv.e;
+  const int n = *G.ncoarse;
+  const float rm = lc == 0 ? st->rscale : lv.Sr;
+  const float fac = 1.0f / lv.Se;
   for (size_t idx = threadIdx.x; idx < g.n(); idx += blockDim.x) stT(e, idx, 0.0f);
   for (int q = threadIdx.x; q < n; q += blockDim.x) rs[q] = (double)(ldT(r, (size_t)cmap[q]) * rm);
   __syncthreads();
@@ -490,37 +641,74 @@ __global__ void k_coarse_solve(Grid3 g, const int* __restrict__ cmap, const int*
   }
 }
 
+// Packed coefficients of a fused level.
+template <typename T>
+__global__ void k_build_weights(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  const LevelRec& lv = G.lv[level];
+  const int nx = lv.d[0], ny = lv.d[1], nz = lv.d[2];
+  const float gx = lv.g[0], gy = lv.g[1], gz = lv.g[2];
+  const float* __restrict__ phi = lv.phi;
+  W4<T>* __restrict__ w = (W4<T>*)lv.w;
+  const size_t n = (size_t)lv.n;
+  const size_t sx = (size_t)ny * nz, sy = (size_t)nz;
+  FP_STRIDE(idx, n) {
+    const int k = (int)(idx % (size_t)nz);
+    const size_t t = idx / (size_t)nz;
+    const int j = (int)(t % (size_t)ny);
+    const int i = (int)(t / (size_t)ny);
+    const float pc = phi[idx];
+    const float wxp = (i + 1 < nx) ? face_weight_fast(pc, phi[idx + sx], gx) : 0.0f;
+    const float wyp = (j + 1 < ny) ? face_weight_fast(pc, phi[idx + sy], gy) : 0.0f;
+    const float wzp = (k + 1 < nz) ? face_weight_fast(pc, phi[idx + 1], gz) : 0.0f;
+    const float wxm = (i > 0) ? face_weight_fast(pc, phi[idx - sx], gx) : 0.0f;
+    const float wym = (j > 0) ? face_weight_fast(pc, phi[idx - sy], gy) : 0.0f;
+    const float wzm = (k > 0) ? face_weight_fast(pc, phi[idx - 1], gz) : 0.0f;
+    const float diag = (wxp + wxm) + (wyp + wym) + (wzp + wzm);
+    W4<T> out;
+    Store<T>::st(&out.wx, 0, wxp);
+    Store<T>::st(&out.wy, 0, wyp);
+    Store<T>::st(&out.wz, 0, wzp);
+    Store<T>::st(&out.id, 0, (pc < 0.0f && diag > 0.0f) ? 1.0f / diag : 0.0f);
+    w[idx] = out;
+  }
+}
+
 // --------------------------------------------------------------------------
 // CG kernels (FP32 vectors, FP64 scalars)
 // --------------------------------------------------------------------------
-// gamma = r . z ; beta = gamma / gamma_prev (Fletcher-Reeves) or the
-// Polak-Ribiere form when z_prev is given.
+// gamma = r . z (z: the finest level's correction, stored scale Se / rscale).
 template <typename T>
-__global__ void k_cg_rz(size_t n, const float* __restrict__ r, const T* __restrict__ z, float zmul_static,
-                        State* st, double* partials) {
+__global__ void k_cg_rz(const Part* parts, int np) {
+  FP_PART(parts, np);
+  const State* st = G.st;
   if (st->done) return;
-  const float zm = zmul_static / st->rscale;
+  const LevelRec& l0 = G.lv[0];
+  const T* __restrict__ z = (const T*)(l0.efinal ? l0.e2 : l0.e);
+  const float* __restrict__ r = G.r;
+  const float zm = l0.Se / st->rscale;
+  const size_t n = (size_t)G.n;
   double acc[1] = {0.0};
-  {
-    const size_t n4 = n / 4;
-    GRID_STRIDE(i, n4) {
-      const float4 rv = reinterpret_cast<const float4*>(r)[i];
-      const float4 zv = V4<T>::ld(z, i);
-      acc[0] += ((double)rv.x * (double)(zv.x * zm) + (double)rv.y * (double)(zv.y * zm)) +
-                ((double)rv.z * (double)(zv.z * zm) + (double)rv.w * (double)(zv.w * zm));
-    }
-    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
-      const size_t i = n4 * 4 + threadIdx.x;
-      acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
-    }
+  const size_t n4 = n / 4;
+  FP_STRIDE(i, n4) {
+    const float4 rv = reinterpret_cast<const float4*>(r)[i];
+    const float4 zv = V4<T>::ld(z, i);
+    acc[0] += ((double)rv.x * (double)(zv.x * zm) + (double)rv.y * (double)(zv.y * zm)) +
+              ((double)rv.z * (double)(zv.z * zm) + (double)rv.w * (double)(zv.w * zm));
   }
-  block_partials<1>(acc, partials);
+  if (blk == 0 && threadIdx.x < (n & 3)) {
+    const size_t i = n4 * 4 + threadIdx.x;
+    acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
+  }
+  block_partials<1>(acc, G.partials, blk, nblk);
 }
 
-__global__ void k_fin_rz(const double* __restrict__ partials, unsigned G, State* st) {
+__global__ void k_fin_rz(const Part* parts) {
+  const Part p = parts[blockIdx.x];
+  State* st = p.g->st;
   if (st->done) return;
   double out[1];
-  sum_partials<1>(partials, G, out);
+  sum_partials<1>(p.g->partials, p.nblocks, out);
   if (threadIdx.x == 0) {
     const double rz = out[0];
     if (!(rz > 0.0) || !isfinite(rz)) {
@@ -533,10 +721,12 @@ __global__ void k_fin_rz(const double* __restrict__ partials, unsigned G, State*
   }
 }
 
-__global__ void k_fin_dq(const double* __restrict__ partials, unsigned G, State* st) {
+__global__ void k_fin_dq(const Part* parts) {
+  const Part p = parts[blockIdx.x];
+  State* st = p.g->st;
   if (st->done) return;
   double out[1];
-  sum_partials<1>(partials, G, out);
+  sum_partials<1>(p.g->partials, p.nblocks, out);
   if (threadIdx.x == 0) {
     const double dq = out[0];
     if (!(dq > 0.0) || !isfinite(dq)) {
@@ -549,14 +739,21 @@ __global__ void k_fin_dq(const double* __restrict__ partials, unsigned G, State*
 }
 
 // e += alpha d ; r -= alpha q ; ||r||^2, max|r| -> convergence and next rscale.
-__global__ void k_cg_update(size_t n, const float* __restrict__ d, const float* __restrict__ q,
-                            float* __restrict__ e, float* __restrict__ r, State* st, double* partials) {
+// d: the direction this iteration wrote (buffer (iter & 1) ^ 1).
+__global__ void k_cg_update(const Part* parts, int np) {
+  FP_PART(parts, np);
+  const State* st = G.st;
   if (st->done) return;
   const float alpha = (float)st->alpha;
+  const float* __restrict__ d = G.d[(st->iter & 1) ^ 1];
+  const float* __restrict__ q = G.q;
+  float* __restrict__ e = G.e;
+  float* __restrict__ r = G.r;
+  const size_t n = (size_t)G.n;
   double acc[1] = {0.0};
   float rmax = 0.0f;
   const size_t n4 = n / 4;
-  GRID_STRIDE(i, n4) {
+  FP_STRIDE(i, n4) {
     const float4 dv = reinterpret_cast<const float4*>(d)[i];
     const float4 qv = reinterpret_cast<const float4*>(q)[i];
     float4 ev = reinterpret_cast<const float4*>(e)[i];
@@ -568,7 +765,7 @@ __global__ void k_cg_update(size_t n, const float* __restrict__ d, const float* 
     acc[0] += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
     rmax = fmaxf(rmax, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
   }
-  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+  if (blk == 0 && threadIdx.x < (n & 3)) {
     const size_t i = n4 * 4 + threadIdx.x;
     e[i] += alpha * d[i];
     const float rn = r[i] - alpha * q[i];
@@ -576,14 +773,16 @@ __global__ void k_cg_update(size_t n, const float* __restrict__ d, const float* 
     acc[0] += (double)rn * (double)rn;
     rmax = fmaxf(rmax, fabsf(rn));
   }
-  block_partials_max<1>(acc, rmax, partials);
+  block_partials_max<1>(acc, rmax, G.partials, blk, nblk);
 }
 
-__global__ void k_fin_update(const double* __restrict__ partials, unsigned G, int max_iter, State* st) {
+__global__ void k_fin_update(const Part* parts) {
+  const Part p = parts[blockIdx.x];
+  State* st = p.g->st;
   if (st->done) return;
   double out[1];
   float m;
-  sum_partials_max<1>(partials, G, out, m);
+  sum_partials_max<1>(p.g->partials, p.nblocks, out, m);
   if (threadIdx.x == 0) {
     const double rr = out[0];
     st->rr = rr;
@@ -592,7 +791,7 @@ __global__ void k_fin_update(const double* __restrict__ partials, unsigned G, in
     if (!isfinite(rr)) {
       st->flags |= 0x1;
       st->done = 1;
-    } else if (rr <= st->inner_target2 || st->iter >= max_iter) {
+    } else if (rr <= st->inner_target2 || st->iter >= p.g->max_iter) {
       st->done = 1;
     }
     st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
@@ -602,17 +801,28 @@ __global__ void k_fin_update(const double* __restrict__ partials, unsigned G, in
 // --------------------------------------------------------------------------
 // output: pressure and gradient subtract (FP64 weights and pressures)
 // --------------------------------------------------------------------------
-__global__ void k_pressure_out(size_t n, const float* __restrict__ phi, const double* __restrict__ x,
-                               float* __restrict__ p) {
-  GRID_STRIDE(i, n) p[i] = phi[i] < 0.0f ? (float)x[i] : 0.0f;
+__global__ void k_pressure_out(const Part* parts, int np) {
+  FP_PART(parts, np);
+  float* __restrict__ p = G.out_p;
+  if (!p) return;
+  const float* __restrict__ phi = G.lv[0].phi;
+  const double* __restrict__ x = G.x;
+  FP_STRIDE(i, (size_t)G.n) p[i] = phi[i] < 0.0f ? (float)x[i] : 0.0f;
 }
 
 // axis 0: faces (nx+1, ny, nz); 1: (nx, ny+1, nz); 2: (nx, ny, nz+1).
-__global__ void k_subtract(Grid3 g, int axis, const float* __restrict__ phi, const double* __restrict__ x,
-                           const float* __restrict__ vin, float* __restrict__ vout, double scale) {
+__global__ void k_subtract(const Part* parts, int np, int axis) {
+  FP_PART(parts, np);
+  const float* __restrict__ vin = axis == 0 ? G.in_u : (axis == 1 ? G.in_v : G.in_w);
+  float* __restrict__ vout = axis == 0 ? G.out_u : (axis == 1 ? G.out_v : G.out_w);
+  if (!vout) return;
+  const Grid3 g{G.nx, G.ny, G.nz};
+  const float* __restrict__ phi = G.lv[0].phi;
+  const double* __restrict__ x = G.x;
+  const double scale = G.sub_scale;
   const int fx = g.nx + (axis == 0), fy = g.ny + (axis == 1), fz = g.nz + (axis == 2);
   const size_t nf = (size_t)fx * fy * fz;
-  GRID_STRIDE(f, nf) {
+  FP_STRIDE(f, nf) {
     const int k = (int)(f % (size_t)fz);
     const size_t t = f / (size_t)fz;
     const int j = (int)(t % (size_t)fy);
@@ -635,12 +845,19 @@ __global__ void k_subtract(Grid3 g, int axis, const float* __restrict__ phi, con
   }
 }
 
-__global__ void k_div_after(Grid3 g, const float* __restrict__ phi, const float* __restrict__ u,
-                            const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ q,
-                            double inv_h, double* partials) {
+__global__ void k_div_after(const Part* parts, int np) {
+  FP_PART(parts, np);
+  const float* __restrict__ u = G.out_u;
+  const float* __restrict__ v = G.out_v;
+  const float* __restrict__ w = G.out_w;
+  if (!u || !v || !w) return;
+  const float* __restrict__ phi = G.lv[0].phi;
+  const float* __restrict__ q = G.in_q;
+  const double inv_h = G.inv_h;
+  const Grid3 g{G.nx, G.ny, G.nz};
   float dmax = 0.0f;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  GRID_STRIDE(idx, g.n()) {
+  const int ny = g.ny, nz = g.nz;
+  FP_STRIDE(idx, g.n()) {
     if (!(phi[idx] < 0.0f)) continue;
     int i, j, k;
     g.ijk(idx, i, j, k);
@@ -650,21 +867,23 @@ __global__ void k_div_after(Grid3 g, const float* __restrict__ phi, const float*
     dmax = fmaxf(dmax, (float)fabs(flux * inv_h - (q ? (double)q[idx] : 0.0)));
   }
   double none[1] = {0.0};
-  block_partials_max<1>(none, dmax, partials);
+  block_partials_max<1>(none, dmax, G.partials, blk, nblk);
 }
 
-__global__ void k_fin_divafter(const double* __restrict__ partials, unsigned G, State* st) {
+__global__ void k_fin_divafter(const Part* parts) {
+  const Part p = parts[blockIdx.x];
+  if (!p.g->out_u || !p.g->out_v || !p.g->out_w) return;
   double out[1];
   float mx;
-  sum_partials_max<1>(partials, G, out, mx);
-  if (threadIdx.x == 0) st->divmax_after = __float_as_uint(mx);
+  sum_partials_max<1>(p.g->partials, p.nblocks, out, mx);
+  if (threadIdx.x == 0) p.g->st->divmax_after = __float_as_uint(mx);
 }
 
-// FP64 operator probe: out = A p.
+// FP64 operator probe: out = A p (single grid, plain launch).
 __global__ void k_apply_d(Grid3 g, const float* __restrict__ phi, const double* __restrict__ p,
                           double* __restrict__ out) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  GRID_STRIDE(idx, g.n()) {
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < g.n(); idx += (size_t)gridDim.x * blockDim.x) {
     int i, j, k;
     g.ijk(idx, i, j, k);
     const float pc = phi[idx];
@@ -692,32 +911,14 @@ __global__ void k_apply_d(Grid3 g, const float* __restrict__ phi, const double* 
 // ==========================================================================
 // host side
 // ==========================================================================
-static inline unsigned grid_for(size_t n) {
-  size_t b = (n + FP_BLOCK - 1) / FP_BLOCK;
-  if (b < 1) b = 1;
-  if (b > kMaxBlocks) b = kMaxBlocks;
-  return (unsigned)b;
-}
-
-#define LAUNCH(kern, n, stream, ...) hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(FP_BLOCK), 0, stream, __VA_ARGS__)
-// LAUNCH with HIP events around it when `cls` is the kernel class being timed.
-#define TLAUNCH(cls, kern, n, stream, ...)              \
-  do {                                                  \
-    const bool _t = (cls) == timed_kernel_;             \
-    if (_t) timer_begin(stream);                        \
-    LAUNCH(kern, n, stream, __VA_ARGS__);               \
-    if (_t) timer_end(stream);                          \
+#define HIPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t _e = (expr);                                            \
+    if (_e != hipSuccess) {                                            \
+      set_error("%s: %s", #expr, hipGetErrorString(_e));               \
+      return FP_ERR_DEVICE;                                            \
+    }                                                                  \
   } while (0)
-
-static LevelArgs largs(const UniformLevel& l) {
-  LevelArgs a;
-  a.g = Grid3{l.d[0], l.d[1], l.d[2]};
-  a.gx = l.g[0];
-  a.gy = l.g[1];
-  a.gz = l.g[2];
-  a.phi = l.phi;
-  return a;
-}
 
 int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3]) {
   // planUniformCM11aHierarchy (webgpu-uniform-pressure-multigrid.ts:149-169):
@@ -781,45 +982,44 @@ int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3])
   return count;
 }
 
-#define HIPCHK(expr)                                                   \
-  do {                                                                 \
-    hipError_t _e = (expr);                                            \
-    if (_e != hipSuccess) {                                            \
-      set_error("%s: %s", #expr, hipGetErrorString(_e));               \
-      return FP_ERR_DEVICE;                                            \
-    }                                                                  \
-  } while (0)
 
-UniformSolver::~UniformSolver() { release(); }
+#ifndef FP_SWEEP_DEPTH
+#define FP_SWEEP_DEPTH 1
+#endif
+constexpr int kSweepDepth = FP_SWEEP_DEPTH;  // planes of loads in flight per workgroup
 
-void UniformSolver::timer_begin(hipStream_t s) {
-  if ((size_t)(2 * timer_used_ + 2) > timer_events_.size()) {
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
-    timer_events_.push_back(a);
-    timer_events_.push_back(b);
-  }
-  (void)hipEventRecord(timer_events_[2 * timer_used_], s);
+// Fused sweeps run on levels large enough to fill a 256-lane column tile.
+static bool fused_dims(const int* d, long long n) { return d[2] >= 16 && d[1] >= 4 && d[0] >= 4 && n >= 4096; }
+
+// Sweep geometry: tiles of TZ x 256/TZ columns, about two rounds of 8
+// resident workgroups per CU on 256 CUs, at least 8 planes per workgroup
+// (the march carries 4 planes of start-up).
+static TileGeo sweep_geometry(const int* d, int TZ) {
+  const int TY = 256 / TZ;
+  TileGeo g;
+  g.tz = (d[2] + TZ - 1) / TZ;
+  g.ty = (d[1] + TY - 1) / TY;
+  const long tiles = (long)g.tz * g.ty;
+  const long cx_target = std::max<long>(1, 4096 / tiles);
+  int chunk = (int)std::max<long>(8, (d[0] + cx_target - 1) / cx_target);
+  chunk = std::min(chunk, d[0]);
+  g.chunk = chunk;
+  g.ngroups = (int)(tiles * ((d[0] + chunk - 1) / chunk));
+  return g;
 }
 
-void UniformSolver::timer_end(hipStream_t s) {
-  if ((size_t)(2 * timer_used_ + 2) > timer_events_.size()) return;
-  (void)hipEventRecord(timer_events_[2 * timer_used_ + 1], s);
-  ++timer_used_;
-}
+static std::atomic<int> g_next_grid_id{1};
 
-void UniformSolver::release() {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  graph_exec_ = nullptr;
+UniformGrid::~UniformGrid() { release(); }
+
+void UniformGrid::release() {
   for (void* p : allocations_) (void)hipFree(p);
   allocations_.clear();
-  for (void* p : pinned_) (void)hipHostFree(p);
-  pinned_.clear();
-  for (hipEvent_t e : timer_events_) (void)hipEventDestroy(e);
-  timer_events_.clear();
+  if (rec_) (void)hipHostFree(rec_);
+  rec_ = nullptr;
 }
 
-int UniformSolver::alloc(void** p, size_t bytes) {
+int UniformGrid::alloc(void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   bytes = (bytes + 255) & ~(size_t)255;
   hipError_t e = hipMalloc(p, bytes);
@@ -832,7 +1032,7 @@ int UniformSolver::alloc(void** p, size_t bytes) {
   return FP_OK;
 }
 
-size_t UniformSolver::estimate_bytes(int nx, int ny, int nz) {
+size_t UniformGrid::estimate_bytes(int nx, int ny, int nz) {
   int dims[kMaxLevels][3], strides[kMaxLevels][3];
   const int L = plan_uniform_levels(nx, ny, nz, dims, strides);
   if (L < 1) return 0;
@@ -842,79 +1042,159 @@ size_t UniformSolver::estimate_bytes(int nx, int ny, int nz) {
   add(n * 8);  // b
   add(n * 8);  // x
   for (int i = 0; i < 5; ++i) add(n * 4);  // r e q d0 d1
-  add(n * 4 * 3 + 4 * ((size_t)(nx + 1) * ny * nz)); // phi0 copy + u v w copies (upper bound)
+  add((size_t)(nx + 1) * ny * nz * 4);     // input staging u v w q
+  add((size_t)nx * (ny + 1) * nz * 4);
+  add((size_t)nx * ny * (nz + 1) * 4);
+  add(n * 4);
   for (int l = 0; l < L; ++l) {
     const size_t nl = (size_t)dims[l][0] * dims[l][1] * dims[l][2];
     add(nl * 4);      // phi
     add(nl * 4 * 4);  // r e e2 t
     add(nl * 16);     // packed coefficients
   }
-  add(512 * 512 * 8 + 4096);
-  add(sizeof(State) + 8 * kMaxBlocks * 8);
+  const size_t nc = std::min<size_t>((size_t)dims[L - 1][0] * dims[L - 1][1] * dims[L - 1][2], 512);
+  add(nc * 4 + 16);
+  add(16);
+  add(nc * nc * 8 + 16);
+  add(sizeof(State));
+  add(4 * (size_t)kMaxBlocks * 8);
+  add(sizeof(GridRec));
   return total;
 }
 
-int UniformSolver::create(int nx, int ny, int nz, double h) {
-  nx_ = nx; ny_ = ny; nz_ = nz; h_ = h;
-  n_ = (size_t)nx * ny * nz;
+int UniformGrid::create(int nx, int ny, int nz, double h) {
   int dims[kMaxLevels][3], strides[kMaxLevels][3];
-  L_ = plan_uniform_levels(nx, ny, nz, dims, strides);
-  if (L_ < 1) {
+  const int L = plan_uniform_levels(nx, ny, nz, dims, strides);
+  if (L < 1) {
     set_error("no multigrid hierarchy for %dx%dx%d", nx, ny, nz);
     return FP_ERR_LAYOUT;
   }
+  HIPCHK(hipHostMalloc((void**)&rec_, sizeof(GridRec), hipHostMallocDefault));
+  std::memset(rec_, 0, sizeof(GridRec));
+  GridRec& R = *rec_;
+  id_ = g_next_grid_id.fetch_add(1);
+  R.nx = nx; R.ny = ny; R.nz = nz; R.L = L; R.h = h;
+  R.n = (long long)nx * ny * nz;
+  R.nu = (long long)(nx + 1) * ny * nz;
+  R.nv = (long long)nx * (ny + 1) * nz;
+  R.nw = (long long)nx * ny * (nz + 1);
+  const size_t n = (size_t)R.n;
   int rc;
-  const size_t nu = (size_t)(nx + 1) * ny * nz, nv = (size_t)nx * (ny + 1) * nz, nw = (size_t)nx * ny * (nz + 1);
-  if ((rc = alloc((void**)&b_, n_ * 8))) return rc;
-  if ((rc = alloc((void**)&x_, n_ * 8))) return rc;
-  if ((rc = alloc((void**)&r_, n_ * 4))) return rc;
-  if ((rc = alloc((void**)&e_, n_ * 4))) return rc;
-  if ((rc = alloc((void**)&q_, n_ * 4))) return rc;
-  if ((rc = alloc((void**)&d_[0], n_ * 4))) return rc;
-  if ((rc = alloc((void**)&d_[1], n_ * 4))) return rc;
-  if ((rc = alloc((void**)&in_u_, nu * 4))) return rc;
-  if ((rc = alloc((void**)&in_v_, nv * 4))) return rc;
-  if ((rc = alloc((void**)&in_w_, nw * 4))) return rc;
-  if ((rc = alloc((void**)&in_q_, n_ * 4))) return rc;
-  for (int l = 0; l < L_; ++l) {
-    UniformLevel& lv = lv_[l];
+  if ((rc = alloc((void**)&R.b, n * 8))) return rc;
+  if ((rc = alloc((void**)&R.x, n * 8))) return rc;
+  if ((rc = alloc((void**)&R.r, n * 4))) return rc;
+  if ((rc = alloc((void**)&R.e, n * 4))) return rc;
+  if ((rc = alloc((void**)&R.q, n * 4))) return rc;
+  if ((rc = alloc((void**)&R.d[0], n * 4))) return rc;
+  if ((rc = alloc((void**)&R.d[1], n * 4))) return rc;
+  if ((rc = alloc((void**)&in_u_, (size_t)R.nu * 4))) return rc;
+  if ((rc = alloc((void**)&in_v_, (size_t)R.nv * 4))) return rc;
+  if ((rc = alloc((void**)&in_w_, (size_t)R.nw * 4))) return rc;
+  if ((rc = alloc((void**)&in_q_, n * 4))) return rc;
+  for (int l = 0; l < L; ++l) {
+    LevelRec& lv = R.lv[l];
     for (int a = 0; a < 3; ++a) {
       lv.d[a] = dims[l][a];
       lv.s[a] = strides[l][a];
-      lv.g[a] = l == 0 ? 1.0f : lv_[l - 1].g[a] / (float)(lv.s[a] * lv.s[a]);
+      lv.g[a] = l == 0 ? 1.0f : R.lv[l - 1].g[a] / (float)(lv.s[a] * lv.s[a]);
     }
-    lv.n = (size_t)lv.d[0] * lv.d[1] * lv.d[2];
-    if ((rc = alloc((void**)&lv.phi, lv.n * 4))) return rc;
-    if ((rc = alloc(&lv.r, lv.n * 4))) return rc;
-    if ((rc = alloc(&lv.e, lv.n * 4))) return rc;
-    if ((rc = alloc(&lv.e2, lv.n * 4))) return rc;
-    if ((rc = alloc(&lv.w, lv.n * 16))) return rc;
-    if ((rc = alloc(&lv.t, lv.n * 4))) return rc;
+    lv.n = (long long)lv.d[0] * lv.d[1] * lv.d[2];
+    const size_t nl = (size_t)lv.n;
+    if ((rc = alloc((void**)&lv.phi, nl * 4))) return rc;
+    if ((rc = alloc(&lv.r, nl * 4))) return rc;
+    if ((rc = alloc(&lv.e, nl * 4))) return rc;
+    if ((rc = alloc(&lv.e2, nl * 4))) return rc;
+    if ((rc = alloc(&lv.w, nl * 16))) return rc;
+    if ((rc = alloc(&lv.t, nl * 4))) return rc;
+    lv.Sr = lv.Se = lv.St = 1.0f;
+    lv.fused = (l < L - 1) && fused_dims(lv.d, lv.n);
+    lv.sweep_tz = lv.d[2] >= 32 ? 32 : 16;
+    lv.sw = sweep_geometry(lv.d, lv.sweep_tz);
+    lv.mg = march_geometry(lv.d[0], lv.d[1], lv.d[2]);
   }
-  const UniformLevel& c = lv_[L_ - 1];
-  max_coarse_ = (int)std::min<size_t>(c.n, 512);
-  if ((rc = alloc((void**)&cmap_, (size_t)max_coarse_ * 4 + 16))) return rc;
-  if ((rc = alloc((void**)&ncoarse_, 16))) return rc;
-  if ((rc = alloc((void**)&minv_, (size_t)max_coarse_ * max_coarse_ * 8 + 16))) return rc;
-  if ((rc = alloc((void**)&st_, sizeof(State)))) return rc;
-  if ((rc = alloc((void**)&partials_, 4 * (size_t)kMaxBlocks * 8))) return rc;
-  HIPCHK(hipMemset(st_, 0, sizeof(State)));
-  HIPCHK(hipMemset(x_, 0, n_ * 8));
-  HIPCHK(hipHostMalloc((void**)&host_state_, sizeof(State), hipHostMallocDefault));
-  pinned_.push_back(host_state_);
-  HIPCHK(hipHostMalloc((void**)&host_upload_, sizeof(State), hipHostMallocDefault));
-  pinned_.push_back(host_upload_);
+  const LevelRec& c = R.lv[L - 1];
+  R.max_coarse = (int)std::min<long long>(c.n, 512);
+  if ((rc = alloc((void**)&R.cmap, (size_t)R.max_coarse * 4 + 16))) return rc;
+  if ((rc = alloc((void**)&R.ncoarse, 16))) return rc;
+  if ((rc = alloc((void**)&R.minv, (size_t)R.max_coarse * R.max_coarse * 8 + 16))) return rc;
+  if ((rc = alloc((void**)&R.st, sizeof(State)))) return rc;
+  if ((rc = alloc((void**)&R.partials, 4 * (size_t)kMaxBlocks * 8))) return rc;
+  if ((rc = alloc((void**)&dev_rec_, sizeof(GridRec)))) return rc;
+  HIPCHK(hipMemset(R.st, 0, sizeof(State)));
+  HIPCHK(hipMemset(R.x, 0, n * 8));
+  R.inv_h = 1.0 / h;
+  R.max_iter = 1;
+  HIPCHK(hipMemcpy(dev_rec_, rec_, sizeof(GridRec), hipMemcpyHostToDevice));
   return FP_OK;
 }
 
-void UniformSolver::set_level_scales(int precision) {
+int UniformGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt) {
+  GridRec& R = *rec_;
+  const size_t n = (size_t)R.n;
+  host_io_ = in->memory == FP_MEMORY_HOST;
+  // level 0 phi is a private copy, so callers may reuse their buffer
+  if (host_io_) {
+    HIPCHK(hipMemcpyAsync(R.lv[0].phi, in->phi, n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(in_u_, in->u, (size_t)R.nu * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(in_v_, in->v, (size_t)R.nv * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(in_w_, in->w, (size_t)R.nw * 4, hipMemcpyHostToDevice, s));
+    if (in->target_divergence)
+      HIPCHK(hipMemcpyAsync(in_q_, in->target_divergence, n * 4, hipMemcpyHostToDevice, s));
+    R.in_u = in_u_;
+    R.in_v = in_v_;
+    R.in_w = in_w_;
+    R.in_q = in->target_divergence ? in_q_ : nullptr;
+    // projected in place on the device copies; FP32 pressure staged in r
+    R.out_u = out && out->u ? in_u_ : nullptr;
+    R.out_v = out && out->v ? in_v_ : nullptr;
+    R.out_w = out && out->w ? in_w_ : nullptr;
+    R.out_p = out && out->pressure ? R.r : nullptr;
+  } else {
+    HIPCHK(hipMemcpyAsync(R.lv[0].phi, in->phi, n * 4, hipMemcpyDeviceToDevice, s));
+    R.in_u = in->u;
+    R.in_v = in->v;
+    R.in_w = in->w;
+    R.in_q = in->target_divergence;
+    R.out_u = out ? out->u : nullptr;
+    R.out_v = out ? out->v : nullptr;
+    R.out_w = out ? out->w : nullptr;
+    R.out_p = out ? out->pressure : nullptr;
+  }
+  const double rho = in->rho, dt = in->dt, h = R.h;
+  R.s_flux = rho * h / dt;
+  R.s_q = rho * h * h / dt;
+  R.inv_h = 1.0 / h;
+  R.sub_scale = dt / (rho * h);
+  R.tol2 = opt ? opt->relative_tolerance * opt->relative_tolerance : 0.0;
+  R.max_iter = opt ? std::max(1, opt->max_iterations) : 1;
+  R.warm = opt ? (opt->warm_start ? 1 : 0) : 0;
+  phi_staged_ = true;
+  return FP_OK;
+}
+
+int UniformGrid::finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* out) {
+  if (!host_io_ || !out) return FP_OK;
+  const GridRec& R = *rec_;
+  if (out->u) HIPCHK(hipMemcpyAsync(out->u, in_u_, (size_t)R.nu * 4, hipMemcpyDeviceToHost, s));
+  if (out->v) HIPCHK(hipMemcpyAsync(out->v, in_v_, (size_t)R.nv * 4, hipMemcpyDeviceToHost, s));
+  if (out->w) HIPCHK(hipMemcpyAsync(out->w, in_w_, (size_t)R.nw * 4, hipMemcpyDeviceToHost, s));
+  if (out->pressure) HIPCHK(hipMemcpyAsync(out->pressure, R.r, (size_t)R.n * 4, hipMemcpyDeviceToHost, s));
+  (void)in;
+  return FP_OK;
+}
+
+void UniformGrid::set_level_scales(int precision, int pre, int post) {
   // FP32: every scale is 1.  FP16/BF16: the finest right-hand side arrives
   // normalised to |r| <= 1 (device rscale); error vectors grow like
   // A^-1 r, i.e. up to ~n^2 cells^2 on smooth modes, so each level stores
   // e / 2^(2 ceil(log2 n_l) - 8) to stay inside the FP16 range.
-  for (int l = 0; l < L_; ++l) {
-    UniformLevel& lv = lv_[l];
-    if (precision == 0) {
+  GridRec& R = *rec_;
+  for (int l = 0; l < R.L; ++l) {
+    LevelRec& lv = R.lv[l];
+    // fused levels ping-pong e/e2 once per sweep; the coarsest and the
+    // unfused levels work in place on e
+    lv.efinal = lv.fused ? ((pre + post) & 1) : 0;
+    if (precision == FP_PRECISION_FP32) {
       lv.Sr = lv.Se = lv.St = 1.0f;
       continue;
     }
@@ -929,389 +1209,445 @@ void UniformSolver::set_level_scales(int precision) {
   }
 }
 
-// Fused sweeps run on levels large enough to fill a 256-lane column tile.
-static bool fused_level(const UniformLevel& l) { return l.d[2] >= 16 && l.d[1] >= 4 && l.d[0] >= 4 && l.n >= 4096; }
-
-#ifndef FP_SWEEP_DEPTH
-#define FP_SWEEP_DEPTH 1
-#endif
-constexpr int kSweepDepth = FP_SWEEP_DEPTH;  // planes of loads in flight per workgroup
-
-template <int TZ, typename TR, typename T>
-static void launch_sweep_tz(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein,
-                            T* eout, int first, int zero_init, const ProlongArgs& P, const State* st) {
-  constexpr int TY = 256 / TZ;
-  const W4<T>* w = (const W4<T>*)f.w;
-  const SweepGeom G{f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2]};
-  const int tz = (f.d[2] + TZ - 1) / TZ, ty = (f.d[1] + TY - 1) / TY;
-  const long tiles = (long)tz * ty;
-  // about two rounds of 8 resident workgroups per CU on 256 CUs, at least 8
-  // planes per workgroup (the march carries 4 planes of start-up)
-  const long cx_target = std::max<long>(1, 4096 / tiles);
-  int chunk = (int)std::max<long>(8, (f.d[0] + cx_target - 1) / cx_target);
-  chunk = std::min(chunk, f.d[0]);
-  const int cx = (f.d[0] + chunk - 1) / chunk;
-  const int ng = (int)(tiles * cx);
-  const dim3 grid(ng), block(256);
-  if (zero_init)
-    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, false, true>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
-                       chunk, tz, ty, ng, st);
-  else if (P.ec)
-    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, true, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
-                       chunk, tz, ty, ng, st);
-  else
-    hipLaunchKernelGGL((k_sweep3<TZ, kSweepDepth, TR, T, false, false>), grid, block, 0, s, G, w, r, rmul, dyn, ein, eout, first, P,
-                       chunk, tz, ty, ng, st);
+int UniformGrid::upload(hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(dev_rec_, rec_, sizeof(GridRec), hipMemcpyHostToDevice, s));
+  return FP_OK;
 }
+
+// --------------------------------------------------------------------------
+// Projector: part tables of a batch and the solve schedule
+// --------------------------------------------------------------------------
+struct Projector::Plan {
+  std::vector<UniformGrid*> grids;
+  int precision = 1, pre = 2, post = 2, maxL = 1;
+  std::vector<Part> host;
+  Part* arena = nullptr;
+  struct Pending {
+    PartTable* t;
+    size_t off;
+  };
+  std::vector<Pending> pending;
+  PartTable one;        // one workgroup per grid
+  PartTable vec4;       // finest-level vectors, 4 per lane
+  PartTable cells;      // finest-level cells, 1 per lane
+  PartTable faces[3];   // face arrays
+  PartTable march0;     // finest-level column-tile march
+  PartTable coarse_top, coarse_deep;  // coarsest level = finest / deeper
+  struct Level {
+    PartTable sweep32, sweep16;  // fused grids by sweep tile width
+    PartTable res;               // fused grids: residual march
+    PartTable fusedcells;        // fused grids: cells of the level
+    PartTable smooth;            // unfused grids: cells of the level
+    PartTable restrict_;         // grids with a coarser level: coarse cells
+    PartTable down;              // grids with this level (l >= 1): its cells
+  } lv[kMaxLevels];
+  hipGraphExec_t graph = nullptr;
+  State* gathered = nullptr;     // device
+  State* gathered_host = nullptr;
+
+  static int blocks_for(size_t items, int per_lane) {
+    const size_t b = (items + (size_t)FP_BLOCK * per_lane - 1) / ((size_t)FP_BLOCK * per_lane);
+    return (int)std::max<size_t>(1, std::min<size_t>(b, 4096));
+  }
+  // Table of (grid, workgroups); part starts are multiples of 8.
+  void add(PartTable& t, const std::vector<std::pair<UniformGrid*, int>>& items) {
+    t.n = (int)items.size();
+    pending.push_back({&t, host.size()});
+    int b0 = 0;
+    for (const auto& it : items) {
+      host.push_back(Part{b0, it.second, it.first->dev_rec()});
+      b0 += (it.second + 7) & ~7;
+    }
+    t.blocks = b0;
+  }
+};
+
+Projector::~Projector() {
+  if (round_arena_) (void)hipFree(round_arena_);
+  if (round_staging_) (void)hipHostFree(round_staging_);
+  if (round_states_) (void)hipFree(round_states_);
+  if (round_states_host_) (void)hipHostFree(round_states_host_);
+  for (auto& kv : plans_) {
+    Plan* p = kv.second;
+    if (p->graph) (void)hipGraphExecDestroy(p->graph);
+    if (p->arena) (void)hipFree(p->arena);
+    if (p->gathered) (void)hipFree(p->gathered);
+    if (p->gathered_host) (void)hipHostFree(p->gathered_host);
+    delete p;
+  }
+  for (hipEvent_t e : timer_events_) (void)hipEventDestroy(e);
+  if (dev_counts_) (void)hipFree(dev_counts_);
+  if (host_counts_) (void)hipHostFree(host_counts_);
+}
+
+void Projector::forget(int grid_id) {
+  for (auto it = plans_.begin(); it != plans_.end();) {
+    bool uses = false;
+    for (size_t i = 0; i + 3 < it->first.size(); ++i)
+      if (it->first[i] == grid_id) uses = true;
+    if (uses) {
+      Plan* p = it->second;
+      if (p->graph) (void)hipGraphExecDestroy(p->graph);
+      if (p->arena) (void)hipFree(p->arena);
+      if (p->gathered) (void)hipFree(p->gathered);
+      if (p->gathered_host) (void)hipHostFree(p->gathered_host);
+      delete p;
+      it = plans_.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+// Host part tables of a grid set (dev pointers are set by the upload).
+void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int precision, int pre, int post) {
+  p->grids = grids;
+  p->precision = precision;
+  p->pre = pre;
+  p->post = post;
+  p->host.clear();
+  p->pending.clear();
+  using Items = std::vector<std::pair<UniformGrid*, int>>;
+  Items one, vec4, cells, faces[3], march0, ctop, cdeep;
+  int maxL = 1;
+  for (UniformGrid* g : grids) {
+    const GridRec& R = *g->host_rec();
+    maxL = std::max(maxL, R.L);
+    one.push_back({g, 1});
+    vec4.push_back({g, Plan::blocks_for((size_t)R.n, 4)});
+    cells.push_back({g, Plan::blocks_for((size_t)R.n, 1)});
+    faces[0].push_back({g, Plan::blocks_for((size_t)R.nu, 1)});
+    faces[1].push_back({g, Plan::blocks_for((size_t)R.nv, 1)});
+    faces[2].push_back({g, Plan::blocks_for((size_t)R.nw, 1)});
+    march0.push_back({g, R.lv[0].mg.ngroups});
+    (R.L == 1 ? ctop : cdeep).push_back({g, 1});
+  }
+  p->maxL = maxL;
+  p->add(p->one, one);
+  p->add(p->vec4, vec4);
+  p->add(p->cells, cells);
+  for (int a = 0; a < 3; ++a) p->add(p->faces[a], faces[a]);
+  p->add(p->march0, march0);
+  p->add(p->coarse_top, ctop);
+  p->add(p->coarse_deep, cdeep);
+  for (int l = 0; l < maxL; ++l) {
+    Items s32, s16, res, fc, sm, rs, dn;
+    for (UniformGrid* g : grids) {
+      const GridRec& R = *g->host_rec();
+      if (l >= R.L) continue;
+      const LevelRec& lv = R.lv[l];
+      if (l >= 1) dn.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
+      if (l >= R.L - 1) continue;  // the coarsest level is solved directly
+      rs.push_back({g, Plan::blocks_for((size_t)R.lv[l + 1].n, 1)});
+      if (lv.fused) {
+        (lv.sweep_tz == 32 ? s32 : s16).push_back({g, lv.sw.ngroups});
+        res.push_back({g, lv.mg.ngroups});
+        fc.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
+      } else {
+        sm.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
+      }
+    }
+    Plan::Level& T = p->lv[l];
+    p->add(T.sweep32, s32);
+    p->add(T.sweep16, s16);
+    p->add(T.res, res);
+    p->add(T.fusedcells, fc);
+    p->add(T.smooth, sm);
+    p->add(T.restrict_, rs);
+    p->add(T.down, dn);
+  }
+}
+
+// Copies a plan's parts into dev (capacity checked by the caller) and points
+// its tables there.
+static void set_table_pointers(Projector::Plan* p, Part* dev) {
+  for (const auto& pd : p->pending) pd.t->dev = dev + pd.off;
+}
+
+Projector::Plan* Projector::plan_for(const std::vector<UniformGrid*>& grids, const fp_options* opt) {
+  std::vector<long long> key;
+  for (UniformGrid* g : grids) key.push_back(g->id());
+  key.push_back(opt->precision);
+  key.push_back(opt->pre_sweeps);
+  key.push_back(opt->post_sweeps);
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return it->second;
+  Plan* p = new Plan();
+  fill_tables(p, grids, opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  bool ok = hipMalloc((void**)&p->arena, std::max<size_t>(1, p->host.size()) * sizeof(Part)) == hipSuccess;
+  ok = ok && hipMemcpy(p->arena, p->host.data(), p->host.size() * sizeof(Part), hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMalloc((void**)&p->gathered, grids.size() * sizeof(State)) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&p->gathered_host, grids.size() * sizeof(State), hipHostMallocDefault) == hipSuccess;
+  if (!ok) {
+    set_error("projector plan: device allocation failed");
+    if (p->arena) (void)hipFree(p->arena);
+    if (p->gathered) (void)hipFree(p->gathered);
+    if (p->gathered_host) (void)hipHostFree(p->gathered_host);
+    delete p;
+    return nullptr;
+  }
+  set_table_pointers(p, p->arena);
+  plans_[key] = p;
+  return p;
+}
+
+// Launch helpers: a part-table launch skips empty tables; a finisher runs one
+// workgroup per part of its producer's table.
+#define PLAUNCH(kern, tab, stream, ...)                                                               \
+  do {                                                                                                \
+    if ((tab).n > 0)                                                                                  \
+      hipLaunchKernelGGL(kern, dim3((tab).blocks), dim3(FP_BLOCK), 0, stream, (tab).dev, (tab).n, ##__VA_ARGS__); \
+  } while (0)
+#define FLAUNCH(kern, tab, stream, ...)                                                               \
+  do {                                                                                                \
+    if ((tab).n > 0) hipLaunchKernelGGL(kern, dim3((tab).n), dim3(FP_BLOCK), 0, stream, (tab).dev, ##__VA_ARGS__); \
+  } while (0)
+
+void Projector::timer_begin(hipStream_t s) {
+  if ((size_t)(2 * timer_used_ + 2) > timer_events_.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    timer_events_.push_back(a);
+    timer_events_.push_back(b);
+  }
+  (void)hipEventRecord(timer_events_[2 * timer_used_], s);
+}
+
+void Projector::timer_end(hipStream_t s) {
+  if ((size_t)(2 * timer_used_ + 2) > timer_events_.size()) return;
+  (void)hipEventRecord(timer_events_[2 * timer_used_ + 1], s);
+  ++timer_used_;
+}
+
+#define TIMED(cls, stmt)                        \
+  do {                                          \
+    const bool _t = (cls) == timed_kernel_;     \
+    if (_t) timer_begin(s);                     \
+    stmt;                                       \
+    if (_t) timer_end(s);                       \
+  } while (0)
 
 template <typename TR, typename T>
-static void launch_sweep(hipStream_t s, const UniformLevel& f, const TR* r, float rmul, int dyn, const T* ein, T* eout,
-                         int first, int zero_init, const ProlongArgs& P, const State* st) {
-  if (f.d[2] >= 32) launch_sweep_tz<32, TR, T>(s, f, r, rmul, dyn, ein, eout, first, zero_init, P, st);
-  else launch_sweep_tz<16, TR, T>(s, f, r, rmul, dyn, ein, eout, first, zero_init, P, st);
-}
-
-void UniformSolver::build_weights(hipStream_t s, int precision) {
-  for (int l = 0; l < L_ - 1; ++l) {
-    const UniformLevel& f = lv_[l];
-    if (!fused_level(f)) continue;
-    if (precision == FP_PRECISION_MIXED_FP16)
-      LAUNCH(k_build_weights<__half>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2], (const float*)f.phi,
-             (W4<__half>*)f.w);
-    else if (precision == FP_PRECISION_MIXED_BF16)
-      LAUNCH(k_build_weights<__hip_bfloat16>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2],
-             (const float*)f.phi, (W4<__hip_bfloat16>*)f.w);
-    else
-      LAUNCH(k_build_weights<float>, f.n, s, f.d[0], f.d[1], f.d[2], f.g[0], f.g[1], f.g[2], (const float*)f.phi,
-             (W4<float>*)f.w);
+static void sweep_launch(hipStream_t s, const PartTable& t32, const PartTable& t16, int level, int which_in, int first,
+                         bool zero, bool prolong) {
+  if (zero) {
+    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, false, true>), t32, s, level, which_in, first);
+    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, false, true>), t16, s, level, which_in, first);
+  } else if (prolong) {
+    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, true, false>), t32, s, level, which_in, first);
+    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, true, false>), t16, s, level, which_in, first);
+  } else {
+    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, false, false>), t32, s, level, which_in, first);
+    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, false, false>), t16, s, level, which_in, first);
   }
 }
 
 template <typename T>
-int UniformSolver::vcycle(hipStream_t s, int pre, int post) {
-  // level 0's right-hand side is the FP32 CG residual times the device rscale.
-  // cur[l] tracks which of the level's two correction buffers holds e.
-  int cur[kMaxLevels] = {0};
-  const ProlongArgs noP{nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
-  for (int l = 0; l < L_ - 1; ++l) {
-    UniformLevel& f = lv_[l];
-    UniformLevel& c = lv_[l + 1];
-    const LevelArgs fa = largs(f);
-    const bool top = (l == 0);
-    const void* rr = top ? (const void*)r_ : f.r;
-    const float rmul = (top ? 1.0f : f.Sr) / f.Se;  // r_true/Se in stored-e units
-    const bool fused = fused_level(f);
-    if (pre == 0) LAUNCH(k_fill<T>, f.n, s, (T*)f.e, f.n, T(0.0f));
+int Projector::vcycle_t(hipStream_t s, Plan& p) {
+  const int pre = p.pre, post = p.post;
+  for (int l = 0; l + 1 < p.maxL; ++l) {
+    Plan::Level& T_ = p.lv[l];
+    const bool top = l == 0;
+    // ---- pre-smoothing ----
+    if (pre == 0) {
+      PLAUNCH(k_zero_level<T>, T_.fusedcells, s, l);
+      PLAUNCH(k_zero_level<T>, T_.smooth, s, l);
+    }
     for (int sw = 0; sw < pre; ++sw) {
-      if (fused) {
-        T* ein = (T*)(cur[l] ? f.e2 : f.e);
-        T* eout = (T*)(cur[l] ? f.e : f.e2);
-        const bool t = top && timed_kernel_ == FP_KERNEL_SMOOTH_FINE;
-        if (t) timer_begin(s);
-        if (top) launch_sweep<float, T>(s, f, (const float*)rr, rmul, 1, ein, eout, 0, sw == 0, noP, st_);
-        else launch_sweep<T, T>(s, f, (const T*)rr, rmul, 0, ein, eout, 0, sw == 0, noP, st_);
-        if (t) timer_end(s);
-        cur[l] ^= 1;
-      } else {
-        T* e = (T*)(cur[l] ? f.e2 : f.e);
-        for (int colour = 0; colour < 2; ++colour) {
-          const int zi = (sw == 0 && colour == 0) ? 1 : 0;
-          if (top) LAUNCH((k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, zi, (const State*)st_);
-          else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, zi, (const State*)st_);
-        }
+      TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, false)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, false)));
+      for (int colour = 0; colour < 2; ++colour) {
+        const int zi = (sw == 0 && colour == 0) ? 1 : 0;
+        if (top) PLAUNCH((k_smooth<float, T>), T_.smooth, s, l, colour, zi);
+        else PLAUNCH((k_smooth<T, T>), T_.smooth, s, l, colour, zi);
       }
     }
-    T* e = (T*)(cur[l] ? f.e2 : f.e);
-    // t = r - A e, stored with scale St
-    const float r_t = (top ? 1.0f : f.Sr) / f.St;
-    const float e_t = f.Se / f.St;
-    if (fused) {
-      const bool t = top && timed_kernel_ == FP_KERNEL_RESIDUAL_FINE;
-      if (t) timer_begin(s);
+    // ---- residual, restriction ----
+    const MarchLaunch la{l, pre & 1};
+    TIMED(top ? FP_KERNEL_RESIDUAL_FINE : -1, {
       if (top) {
-        OpResidualW<float, T> op{f.d[1], f.d[2], (const W4<T>*)f.w, (const float*)rr, (const T*)e, (T*)f.t, r_t, e_t,
-                                 1.0f, 1, 0.0f, st_, nullptr};
-        launch_march(s, f.d[0], f.d[1], f.d[2], op, st_, 1);
+        PLAUNCH((k_march<OpResidualW<float, T>>), T_.res, s, 1, la);
+        PLAUNCH((k_residual<float, T>), T_.smooth, s, l);
       } else {
-        OpResidualW<T, T> op{f.d[1], f.d[2], (const W4<T>*)f.w, (const T*)rr, (const T*)e, (T*)f.t, r_t, e_t, 1.0f, 0,
-                             0.0f, st_, nullptr};
-        launch_march(s, f.d[0], f.d[1], f.d[2], op, st_, 1);
+        PLAUNCH((k_march<OpResidualW<T, T>>), T_.res, s, 1, la);
+        PLAUNCH((k_residual<T, T>), T_.smooth, s, l);
       }
-      if (t) timer_end(s);
-    } else if (top) {
-      TLAUNCH(FP_KERNEL_RESIDUAL_FINE, (k_residual<float, T>), f.n, s, fa, (const float*)rr, r_t, 1, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
-    } else {
-      LAUNCH((k_residual<T, T>), f.n, s, fa, (const T*)rr, r_t, 0, (const T*)e, e_t, 1.0f, (T*)f.t, (const State*)st_);
+    });
+    TIMED(top ? FP_KERNEL_RESTRICT_FINE : -1, PLAUNCH(k_restrict<T>, T_.restrict_, s, l));
+  }
+  FLAUNCH((k_coarse_solve<T, float>), p.coarse_top, s);
+  FLAUNCH((k_coarse_solve<T, T>), p.coarse_deep, s);
+  for (int l = p.maxL - 2; l >= 0; --l) {
+    Plan::Level& T_ = p.lv[l];
+    const bool top = l == 0;
+    // ---- fused levels: prolongation inside the first post-sweep ----
+    if (post == 0) {
+      PLAUNCH(k_prolong_add<T>, T_.fusedcells, s, l, pre & 1);
     }
-    const int k2 = (c.s[0] == 2) + (c.s[1] == 2) + (c.s[2] == 2);
-    const float fac = (1.0f / (float)(1 << k2)) * f.St / c.Sr;
-    TLAUNCH(top ? FP_KERNEL_RESTRICT_FINE : -1, k_restrict<T>, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const T*)f.t,
-            largs(c), c.s[0], c.s[1], c.s[2], fac, (T*)c.r, (const State*)st_);
-  }
-  {
-    UniformLevel& c = lv_[L_ - 1];
-    const bool top = (L_ == 1);
-    const float rmul = top ? 1.0f : c.Sr;
-    cur[L_ - 1] = 0;
-    if (top)
-      hipLaunchKernelGGL((k_coarse_solve<T, float>), dim3(1), dim3(FP_BLOCK), 0, s, Grid3{c.d[0], c.d[1], c.d[2]},
-                         (const int*)cmap_, (const int*)ncoarse_, (const double*)minv_, (const float*)r_, rmul, 1,
-                         (T*)c.e, 1.0f / c.Se, (const State*)st_);
-    else
-      hipLaunchKernelGGL((k_coarse_solve<T, T>), dim3(1), dim3(FP_BLOCK), 0, s, Grid3{c.d[0], c.d[1], c.d[2]},
-                         (const int*)cmap_, (const int*)ncoarse_, (const double*)minv_, (const T*)c.r, rmul, 0,
-                         (T*)c.e, 1.0f / c.Se, (const State*)st_);
-  }
-  for (int l = L_ - 2; l >= 0; --l) {
-    UniformLevel& f = lv_[l];
-    UniformLevel& c = lv_[l + 1];
-    const LevelArgs fa = largs(f);
-    const bool top = (l == 0);
-    const void* rr = top ? (const void*)r_ : f.r;
-    const float rmul = (top ? 1.0f : f.Sr) / f.Se;
-    const bool fused = fused_level(f);
-    const T* ec = (const T*)(cur[l + 1] ? c.e2 : c.e);
-    if (!fused || post == 0) {
-      T* e = (T*)(cur[l] ? f.e2 : f.e);
-      TLAUNCH(top ? FP_KERNEL_PROLONG_FINE : -1, k_prolong_add<T>, f.n, s, fa, e, Grid3{c.d[0], c.d[1], c.d[2]},
-              (const float*)c.phi, ec, c.s[0], c.s[1], c.s[2], c.Se / f.Se, (const State*)st_);
-      for (int sw = 0; sw < post; ++sw)
-        for (int cc = 0; cc < 2; ++cc) {
-          const int colour = 1 - cc;
-          if (top) LAUNCH((k_smooth<float, T>), f.n, s, fa, (const float*)rr, rmul, 1, e, colour, 0, (const State*)st_);
-          else LAUNCH((k_smooth<T, T>), f.n, s, fa, (const T*)rr, rmul, 0, e, colour, 0, (const State*)st_);
-        }
-    } else {
-      for (int sw = 0; sw < post; ++sw) {
-        ProlongArgs P = noP;
-        if (sw == 0) P = ProlongArgs{(const void*)ec, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / f.Se};
-        T* ein = (T*)(cur[l] ? f.e2 : f.e);
-        T* eout = (T*)(cur[l] ? f.e : f.e2);
-        const bool t = top && timed_kernel_ == FP_KERNEL_SMOOTH_FINE;
-        if (t) timer_begin(s);
-        if (top) launch_sweep<float, T>(s, f, (const float*)rr, rmul, 1, ein, eout, 1, 0, P, st_);
-        else launch_sweep<T, T>(s, f, (const T*)rr, rmul, 0, ein, eout, 1, 0, P, st_);
-        if (t) timer_end(s);
-        cur[l] ^= 1;
+    for (int sw = 0; sw < post; ++sw) {
+      TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, sw == 0)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, sw == 0)));
+    }
+    // ---- unfused levels ----
+    TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_add<T>, T_.smooth, s, l, 0));
+    for (int sw = 0; sw < post; ++sw)
+      for (int cc = 0; cc < 2; ++cc) {
+        const int colour = 1 - cc;
+        if (top) PLAUNCH((k_smooth<float, T>), T_.smooth, s, l, colour, 0);
+        else PLAUNCH((k_smooth<T, T>), T_.smooth, s, l, colour, 0);
       }
-    }
   }
-  zbuf_ = cur[0] ? lv_[0].e2 : lv_[0].e;
   return FP_OK;
+}
+
+int Projector::enqueue_vcycle(hipStream_t s, Plan& p) {
+  if (p.precision == FP_PRECISION_MIXED_FP16) return vcycle_t<__half>(s, p);
+  if (p.precision == FP_PRECISION_MIXED_BF16) return vcycle_t<__hip_bfloat16>(s, p);
+  return vcycle_t<float>(s, p);
 }
 
 template <typename T>
-int UniformSolver::iteration(hipStream_t s, int pre, int post, int max_iter, int parity) {
-  vcycle<T>(s, pre, post);
-  const UniformLevel& f = lv_[0];
-  const float zmul = f.Se;  // z_true * rscale = e_stored * Se
-  const T* z = (const T*)zbuf_;
-  LAUNCH(k_cg_rz<T>, n_ / 4 + 1, s, n_, (const float*)r_, z, zmul, st_, partials_);
-  hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_ / 4 + 1), st_);
-  {
-    const bool t = timed_kernel_ == FP_KERNEL_CG_DIRECTION;
-    if (t) timer_begin(s);
-    OpDirection<T> op{ny_, nz_, f.phi, z, (const float*)d_[parity], d_[parity ^ 1], q_, zmul, 0.0f, 0.0f,
-                      f.g[0], f.g[1], f.g[2], st_, partials_};
-    const unsigned G = launch_march(s, nx_, ny_, nz_, op, st_, 1);
-    if (t) timer_end(s);
-    hipLaunchKernelGGL(k_fin_dq, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, G, st_);
-  }
-  TLAUNCH(FP_KERNEL_CG_UPDATE, k_cg_update, n_ / 4 + 1, s, n_, (const float*)d_[parity ^ 1], (const float*)q_, e_, r_,
-          st_, partials_);
-  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_ / 4 + 1),
-                     max_iter, st_);
+int Projector::iteration_t(hipStream_t s, Plan& p) {
+  vcycle_t<T>(s, p);
+  PLAUNCH(k_cg_rz<T>, p.vec4, s);
+  FLAUNCH(k_fin_rz, p.vec4, s);
+  TIMED(FP_KERNEL_CG_DIRECTION, PLAUNCH(k_march<OpDirection<T>>, p.march0, s, 1, MarchLaunch{0, 0}));
+  FLAUNCH(k_fin_dq, p.march0, s);
+  TIMED(FP_KERNEL_CG_UPDATE, PLAUNCH(k_cg_update, p.vec4, s));
+  FLAUNCH(k_fin_update, p.vec4, s);
   return FP_OK;
 }
 
-void UniformSolver::run_iteration(hipStream_t s, int precision, int pre, int post, int max_iter, int parity) {
-  if (precision == FP_PRECISION_MIXED_FP16) iteration<__half>(s, pre, post, max_iter, parity);
-  else if (precision == FP_PRECISION_MIXED_BF16) iteration<__hip_bfloat16>(s, pre, post, max_iter, parity);
-  else iteration<float>(s, pre, post, max_iter, parity);
+int Projector::enqueue_iteration(hipStream_t s, Plan& p) {
+  if (p.precision == FP_PRECISION_MIXED_FP16) return iteration_t<__half>(s, p);
+  if (p.precision == FP_PRECISION_MIXED_BF16) return iteration_t<__hip_bfloat16>(s, p);
+  return iteration_t<float>(s, p);
 }
 
-int UniformSolver::ensure_graph(hipStream_t s, int precision, int pre, int post, int max_iter) {
-  if (graph_exec_ && graph_key_[0] == precision && graph_key_[1] == pre && graph_key_[2] == post &&
-      graph_key_[3] == max_iter)
-    return FP_OK;
-  if (graph_exec_) {
-    (void)hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
-  }
+int Projector::ensure_graph(hipStream_t s, Plan& p) {
+  if (p.graph) return FP_OK;
+  // one hipGraph holds two CG + V-cycle iterations (the d ping-pong);
+  // kernels of finished grids return at once
   hipGraph_t g = nullptr;
+  const int saved = timed_kernel_;
+  timed_kernel_ = 0;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  run_iteration(s, precision, pre, post, max_iter, 0);
-  run_iteration(s, precision, pre, post, max_iter, 1);
-  HIPCHK(hipStreamEndCapture(s, &g));
-  hipError_t e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+  enqueue_iteration(s, p);
+  enqueue_iteration(s, p);
+  hipError_t e = hipStreamEndCapture(s, &g);
+  timed_kernel_ = saved;
+  if (e != hipSuccess) {
+    set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    graph_exec_ = nullptr;
+    p.graph = nullptr;
     set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
     return FP_ERR_DEVICE;
   }
-  graph_key_[0] = precision;
-  graph_key_[1] = pre;
-  graph_key_[2] = post;
-  graph_key_[3] = max_iter;
   return FP_OK;
 }
 
-int UniformSolver::setup_levels(hipStream_t s, const float* phi0, int preserve) {
-  // level 0 phi is a private copy so callers may reuse their buffer
-  HIPCHK(hipMemcpyAsync(lv_[0].phi, phi0, n_ * 4, hipMemcpyDeviceToDevice, s));
-  for (int l = 1; l < L_; ++l) {
-    const UniformLevel& f = lv_[l - 1];
-    UniformLevel& c = lv_[l];
-    LAUNCH(k_downsample_phi, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const float*)f.phi, Grid3{c.d[0], c.d[1], c.d[2]},
-           c.phi, c.s[0], c.s[1], c.s[2], (l <= preserve) ? 1 : 0);
+int Projector::poll(hipStream_t s, Plan& p, int* counts) {
+  if (!dev_counts_) {
+    HIPCHK(hipMalloc((void**)&dev_counts_, 16));
+    HIPCHK(hipHostMalloc((void**)&host_counts_, 16, hipHostMallocDefault));
   }
-  const UniformLevel& c = lv_[L_ - 1];
-  hipLaunchKernelGGL(k_coarse_setup, dim3(1), dim3(FP_BLOCK), 0, s, largs(c), cmap_, ncoarse_, minv_, max_coarse_, st_);
+  hipLaunchKernelGGL(k_status, dim3(1), dim3(FP_BLOCK), 0, s, p.one.dev, p.one.n, dev_counts_);
+  HIPCHK(hipMemcpyAsync(host_counts_, dev_counts_, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  counts[0] = host_counts_[0];
+  counts[1] = host_counts_[1];
   return FP_OK;
 }
 
-int UniformSolver::project(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt,
-                           fp_report* rep) {
-  const size_t nu = (size_t)(nx_ + 1) * ny_ * nz_, nv = (size_t)nx_ * (ny_ + 1) * nz_,
-               nw = (size_t)nx_ * ny_ * (nz_ + 1);
-  const bool host = in->memory == FP_MEMORY_HOST;
-  const float *phi = in->phi, *u = in->u, *v = in->v, *w = in->w, *q = in->target_divergence;
+template <typename T>
+static void build_weights_t(hipStream_t s, const Projector* self, int maxL, const PartTable* tabs) {
+  (void)self;
+  for (int l = 0; l + 1 < maxL; ++l) PLAUNCH(k_build_weights<T>, tabs[l], s, l);
+}
+
+int Projector::setup_tables(hipStream_t s, Plan& p, int preserve) {
+  for (int l = 1; l < p.maxL; ++l) PLAUNCH(k_downsample_phi, p.lv[l].down, s, l, preserve);
+  FLAUNCH(k_coarse_setup, p.one, s);
+  PartTable wt[kMaxLevels];
+  for (int l = 0; l < p.maxL; ++l) wt[l] = p.lv[l].fusedcells;
+  if (p.precision == FP_PRECISION_MIXED_FP16) build_weights_t<__half>(s, this, p.maxL, wt);
+  else if (p.precision == FP_PRECISION_MIXED_BF16) build_weights_t<__hip_bfloat16>(s, this, p.maxL, wt);
+  else build_weights_t<float>(s, this, p.maxL, wt);
+  return FP_OK;
+}
+
+int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps) {
+  if (grids.empty()) return FP_OK;
+  for (UniformGrid* g : grids) {
+    g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
+    int rc = g->upload(s);
+    if (rc) return rc;
+  }
+  Plan* pp = plan_for(grids, opt);
+  if (!pp) return FP_ERR_OUT_OF_MEMORY;
+  Plan& p = *pp;
   timed_kernel_ = opt->timed_kernel;
   timer_used_ = 0;
   hipEvent_t ev0, ev1;
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, s));
-  if (host) {
-    // stage host inputs in the grid's resident buffers (level-0 phi doubles as
-    // the phi staging buffer)
-    HIPCHK(hipMemcpyAsync(lv_[0].phi, phi, n_ * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(in_u_, u, nu * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(in_v_, v, nv * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(in_w_, w, nw * 4, hipMemcpyHostToDevice, s));
-    if (q) HIPCHK(hipMemcpyAsync(in_q_, q, n_ * 4, hipMemcpyHostToDevice, s));
-    phi = lv_[0].phi;
-    u = in_u_; v = in_v_; w = in_w_;
-    if (q) q = in_q_;
-  }
-  // reset scalars (fill kernels rather than memsets so the sequence is graph-safe)
-  const double tol = opt->relative_tolerance;
-  std::memset(host_upload_, 0, sizeof(State));
-  host_upload_->rscale = 1.0f;
-  HIPCHK(hipMemcpyAsync(st_, host_upload_, offsetof(State, counters), hipMemcpyHostToDevice, s));
-  if (phi != lv_[0].phi) {
-    if (setup_levels(s, phi, opt->phi_preserve_levels)) return FP_ERR_DEVICE;
-  } else {
-    for (int l = 1; l < L_; ++l) {
-      const UniformLevel& f = lv_[l - 1];
-      UniformLevel& c = lv_[l];
-      LAUNCH(k_downsample_phi, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const float*)f.phi,
-             Grid3{c.d[0], c.d[1], c.d[2]}, c.phi, c.s[0], c.s[1], c.s[2], (l <= opt->phi_preserve_levels) ? 1 : 0);
-    }
-    const UniformLevel& c = lv_[L_ - 1];
-    hipLaunchKernelGGL(k_coarse_setup, dim3(1), dim3(FP_BLOCK), 0, s, largs(c), cmap_, ncoarse_, minv_, max_coarse_,
-                       st_);
-  }
-  const float* phi_l0 = lv_[0].phi;
-  const Grid3 g{nx_, ny_, nz_};
-  const double rho = in->rho, dt = in->dt;
-  LAUNCH(k_rhs, n_, s, g, phi_l0, u, v, w, q, rho * h_ / dt, rho * h_ * h_ / dt, 1.0 / h_, tol * tol, b_, st_,
-         partials_);
-  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), tol * tol, st_);
-  if (!opt->warm_start) LAUNCH(k_fill<double>, n_, s, x_, n_, 0.0);
-  LAUNCH(k_fill4, n_ / 4 + 1, s, e_, n_, 0.0f);
-  LAUNCH(k_fill4, n_ / 4 + 1, s, d_[0], n_, 0.0f);
-  set_level_scales(opt->precision);
-  build_weights(s, opt->precision);
-  int outer = 0, total = 0;
-  int flags = 0;
+  FLAUNCH(k_reset_state, p.one, s);
+  setup_tables(s, p, opt->phi_preserve_levels);
+  PLAUNCH(k_rhs, p.cells, s);
+  FLAUNCH(k_fin_rhs, p.cells, s);
+  PLAUNCH(k_zero, p.vec4, s, 0, 0);  // x, except on warm-started grids
+  PLAUNCH(k_zero, p.vec4, s, 1, 0);
+  PLAUNCH(k_zero, p.vec4, s, 2, 0);
+  const int max_outer = std::max(1, opt->max_outer);
   const int max_iter = std::max(1, opt->max_iterations);
   const int check = std::max(1, opt->check_every);
-  bool converged = false;
-  for (outer = 0; outer < std::max(1, opt->max_outer) + 1; ++outer) {
-    {
-      const bool t = timed_kernel_ == FP_KERNEL_TRUE_RESIDUAL;
-      if (t) timer_begin(s);
-      OpTrueResidual op{ny_, nz_, phi_l0, (const double*)b_, (const double*)x_, r_, st_, partials_};
-      const unsigned G = launch_march(s, nx_, ny_, nz_, op, st_, 0);
-      if (t) timer_end(s);
-      hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, G, st_);
-    }
-    HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (host_state_->outer_done) {
-      converged = !(host_state_->flags & 0x1);
-      break;
-    }
-    if (outer == std::max(1, opt->max_outer)) break;
-    const bool graphs = opt->graphs && timed_kernel_ == 0;
+  const bool graphs = opt->graphs && timed_kernel_ == 0;
+  int counts[2] = {0, 0};
+  for (int pass = 0; pass <= max_outer; ++pass) {
+    const int new_pass = pass < max_outer;
+    TIMED(FP_KERNEL_TRUE_RESIDUAL, PLAUNCH(k_march<OpTrueResidual>, p.march0, s, 2, MarchLaunch{0, 0}));
+    FLAUNCH(k_fin_true, p.march0, s, new_pass);
+    int rc = poll(s, p, counts);
+    if (rc) return rc;
+    if (counts[0] == 0 || !new_pass) break;
     if (graphs) {
-      // one hipGraph holds two CG + V-cycle iterations (the d ping-pong);
-      // kernels of finished iterations return at once, so replays past
-      // convergence cost only their launch
-      int rc = ensure_graph(s, opt->precision, opt->pre_sweeps, opt->post_sweeps, max_iter);
+      rc = ensure_graph(s, p);
       if (rc) return rc;
+      const int period = std::max(2, check + (check & 1));
       for (int it = 0; it < max_iter; it += 2) {
-        HIPCHK(hipGraphLaunch(graph_exec_, s));
-        if ((it + 2) % std::max(2, check + (check & 1)) == 0 || it + 2 >= max_iter) {
-          HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
-          HIPCHK(hipStreamSynchronize(s));
-          if (host_state_->done) break;
+        HIPCHK(hipGraphLaunch(p.graph, s));
+        if ((it + 2) % period == 0 || it + 2 >= max_iter) {
+          if ((rc = poll(s, p, counts))) return rc;
+          if (counts[1] == 0) break;
         }
       }
     } else {
-      int parity = 0;
       for (int it = 0; it < max_iter; ++it) {
-        run_iteration(s, opt->precision, opt->pre_sweeps, opt->post_sweeps, max_iter, parity);
-        parity ^= 1;
+        enqueue_iteration(s, p);
         if ((it + 1) % check == 0 || it + 1 == max_iter) {
-          HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
-          HIPCHK(hipStreamSynchronize(s));
-          if (host_state_->done) break;
+          if ((rc = poll(s, p, counts))) return rc;
+          if (counts[1] == 0) break;
         }
       }
     }
-    if (host_state_->flags & 0x7) {
-      // a broken recurrence: keep what the FP64 iterate has (fail closed)
-      flags |= host_state_->flags;
-      LAUNCH(k_fill4, n_ / 4 + 1, s, e_, n_, 0.0f);
-    }
-    LAUNCH(k_accumulate, n_ / 4 + 1, s, n_, x_, e_);
-    LAUNCH(k_fill4, n_ / 4 + 1, s, d_[0], n_, 0.0f);
+    PLAUNCH(k_accumulate, p.vec4, s);
+    PLAUNCH(k_zero, p.vec4, s, 2, 1);
   }
-  total = host_state_->total_iter;
-  flags |= host_state_->flags;
-  if (!converged) flags |= 0x8;  // FP_FLAG_NONCONVERGENCE
-  // outputs
-  float* pout = out->pressure;
-  float *uo = out->u, *vo = out->v, *wo = out->w;
-  const bool host_out = host;
-  float* dev_p = nullptr;
-  float *dev_u = nullptr, *dev_v = nullptr, *dev_w = nullptr;
-  if (host_out) {
-    dev_u = in_u_; dev_v = in_v_; dev_w = in_w_;  // projected in place on the device copy
-    dev_p = r_;  // reuse r as the FP32 pressure staging buffer
-  } else {
-    dev_u = uo; dev_v = vo; dev_w = wo; dev_p = pout;
-  }
-  const double sc = dt / (rho * h_);
-  if (dev_u) LAUNCH(k_subtract, nu, s, g, 0, phi_l0, (const double*)x_, u, dev_u, sc);
-  if (dev_v) LAUNCH(k_subtract, nv, s, g, 1, phi_l0, (const double*)x_, v, dev_v, sc);
-  if (dev_w) LAUNCH(k_subtract, nw, s, g, 2, phi_l0, (const double*)x_, w, dev_w, sc);
-  if (dev_u && dev_v && dev_w) {
-    LAUNCH(k_div_after, n_, s, g, phi_l0, dev_u, dev_v, dev_w, q, 1.0 / h_, partials_);
-    hipLaunchKernelGGL(k_fin_divafter, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), st_);
-  }
-  if (pout && dev_p) LAUNCH(k_pressure_out, n_, s, n_, phi_l0, (const double*)x_, dev_p);
+  for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, p.faces[a], s, a);
+  PLAUNCH(k_div_after, p.cells, s);
+  FLAUNCH(k_fin_divafter, p.cells, s);
+  PLAUNCH(k_pressure_out, p.cells, s);
   HIPCHK(hipEventRecord(ev1, s));
-  if (host_out) {
-    if (uo) HIPCHK(hipMemcpyAsync(uo, dev_u, nu * 4, hipMemcpyDeviceToHost, s));
-    if (vo) HIPCHK(hipMemcpyAsync(vo, dev_v, nv * 4, hipMemcpyDeviceToHost, s));
-    if (wo) HIPCHK(hipMemcpyAsync(wo, dev_w, nw * 4, hipMemcpyDeviceToHost, s));
-    if (pout) HIPCHK(hipMemcpyAsync(pout, dev_p, n_ * 4, hipMemcpyDeviceToHost, s));
-  }
-  HIPCHK(hipMemcpyAsync(host_state_, st_, sizeof(State), hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_gather_states, dim3((unsigned)((grids.size() + 255) / 256)), dim3(256), 0, s, p.one.dev,
+                     p.one.n, p.gathered);
+  HIPCHK(hipMemcpyAsync(p.gathered_host, p.gathered, grids.size() * sizeof(State), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -1326,131 +1662,285 @@ int UniformSolver::project(hipStream_t s, const fp_inputs* in, fp_outputs* out, 
   const int timed_launches = timer_used_;
   timed_kernel_ = 0;
   timer_used_ = 0;
-  if (rep) {
-    std::memset(rep, 0, sizeof(*rep));
-    rep->flags = flags;
-    rep->iterations = total;
-    rep->outer_iterations = outer;
-    rep->levels = L_;
-    rep->rhs_norm = std::sqrt(std::max(0.0, host_state_->bb));
-    rep->relative_residual = host_state_->bb > 0 ? std::sqrt(std::max(0.0, host_state_->rr64) / host_state_->bb) : 0.0;
-    unsigned a = host_state_->divmax_before, b = host_state_->divmax_after;
-    float fa, fb;
-    std::memcpy(&fa, &a, 4);
-    std::memcpy(&fb, &b, 4);
-    rep->max_div_before = fa;
-    rep->max_div_after = fb;
-    rep->device_ms = ms;
-    rep->samples = (int64_t)n_;
-    rep->liquid_samples = (int64_t)host_state_->liquid;
-    rep->timed_kernel_ms = timed_ms;
-    rep->timed_kernel_launches = timed_launches;
+  if (reps) {
+    for (size_t i = 0; i < grids.size(); ++i) {
+      const State& S = p.gathered_host[i];
+      const GridRec& R = *grids[i]->host_rec();
+      fp_report* rep = &reps[i];
+      std::memset(rep, 0, sizeof(*rep));
+      int flags = S.flags;
+      if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;  // FP_FLAG_NONCONVERGENCE
+      rep->flags = flags;
+      rep->iterations = S.total_iter;
+      rep->outer_iterations = S.outer;
+      rep->levels = R.L;
+      rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
+      rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
+      float fa, fb;
+      std::memcpy(&fa, &S.divmax_before, 4);
+      std::memcpy(&fb, &S.divmax_after, 4);
+      rep->max_div_before = fa;
+      rep->max_div_after = fb;
+      rep->device_ms = ms;
+      rep->samples = (int64_t)R.n;
+      rep->liquid_samples = (int64_t)S.liquid;
+      rep->timed_kernel_ms = timed_ms;
+      rep->timed_kernel_launches = timed_launches;
+    }
   }
   return FP_OK;
 }
 
-int UniformSolver::probe_apply(hipStream_t s, const float* phi_host, const double* p_host, double* out_host) {
-  float* phi;
-  double *p, *o;
-  HIPCHK(hipMalloc(&phi, n_ * 4));
-  HIPCHK(hipMalloc(&p, n_ * 8));
-  HIPCHK(hipMalloc(&o, n_ * 8));
-  HIPCHK(hipMemcpy(phi, phi_host, n_ * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(p, p_host, n_ * 8, hipMemcpyHostToDevice));
-  LAUNCH(k_apply_d, n_, s, Grid3{nx_, ny_, nz_}, (const float*)phi, (const double*)p, o);
+
+// ---- continuous batching ---------------------------------------------------
+static double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
+  struct Group {
+    fp_options opt;
+    std::vector<Job*> jobs;
+  };
+  auto key_of = [](const fp_options& o) {
+    return ((long long)o.precision << 32) | ((long long)o.pre_sweeps << 24) | ((long long)o.post_sweeps << 16) |
+           (long long)(o.phi_preserve_levels & 0xffff);
+  };
+  std::map<long long, Group> setup, accum, iter, resid[2];
+  std::vector<Job*> finish, live;
+  for (Job* j : jobs) {
+    if (j->stage == Job::DONE) continue;
+    live.push_back(j);
+    const long long k = key_of(j->opt);
+    Group* g = nullptr;
+    if (j->stage == Job::SETUP) g = &setup[k];
+    else if (j->stage == Job::ACCUMULATE) g = &accum[k];
+    else if (j->stage == Job::ITERATE) g = &iter[k];
+    else finish.push_back(j);
+    if (g) {
+      g->opt = j->opt;
+      g->jobs.push_back(j);
+    }
+    if (j->stage == Job::SETUP || j->stage == Job::ACCUMULATE) {
+      // an FP64 residual follows in this round; another inner pass may follow it
+      Group& r = resid[j->passes < std::max(1, j->opt.max_outer) ? 1 : 0][k];
+      r.opt = j->opt;
+      r.jobs.push_back(j);
+    }
+  }
+  if (live.empty()) return FP_OK;
+  std::vector<std::unique_ptr<Plan>> plans;
+  auto make = [&](const std::vector<Job*>& js, const fp_options& o) {
+    std::vector<UniformGrid*> gs;
+    for (Job* j : js) gs.push_back(j->grid);
+    plans.emplace_back(new Plan());
+    fill_tables(plans.back().get(), gs, o.precision, o.pre_sweeps, o.post_sweeps);
+    return plans.back().get();
+  };
+  std::vector<std::pair<Plan*, const Group*>> setup_p, accum_p, iter_p, resid_p[2];
+  for (auto& kv : setup) setup_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (auto& kv : accum) accum_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (int np = 0; np < 2; ++np)
+    for (auto& kv : resid[np]) resid_p[np].push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (auto& kv : iter) iter_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  fp_options plain;
+  std::memset(&plain, 0, sizeof(plain));
+  Plan* fin_p = finish.empty() ? nullptr : make(finish, plain);
+  Plan* all_p = make(live, plain);
+  // tables of the round: one staging copy into the reusable arena (the
+  // previous round ended with a stream synchronisation, so it is free)
+  size_t total = 0;
+  for (auto& p : plans) total += p->host.size();
+  if (total > round_arena_cap_) {
+    if (round_arena_) (void)hipFree(round_arena_);
+    if (round_staging_) (void)hipHostFree(round_staging_);
+    round_arena_ = nullptr;
+    round_staging_ = nullptr;
+    const size_t cap = std::max<size_t>(total * 2, 4096);
+    HIPCHK(hipMalloc((void**)&round_arena_, cap * sizeof(Part)));
+    HIPCHK(hipHostMalloc((void**)&round_staging_, cap * sizeof(Part), hipHostMallocDefault));
+    round_arena_cap_ = cap;
+  }
+  if (live.size() > round_states_cap_) {
+    if (round_states_) (void)hipFree(round_states_);
+    if (round_states_host_) (void)hipHostFree(round_states_host_);
+    round_states_ = nullptr;
+    round_states_host_ = nullptr;
+    const size_t cap = std::max<size_t>(live.size() * 2, 64);
+    HIPCHK(hipMalloc((void**)&round_states_, cap * sizeof(State)));
+    HIPCHK(hipHostMalloc((void**)&round_states_host_, cap * sizeof(State), hipHostMallocDefault));
+    round_states_cap_ = cap;
+  }
+  size_t off = 0;
+  for (auto& p : plans) {
+    std::memcpy(round_staging_ + off, p->host.data(), p->host.size() * sizeof(Part));
+    set_table_pointers(p.get(), round_arena_ + off);
+    off += p->host.size();
+  }
+  HIPCHK(hipMemcpyAsync(round_arena_, round_staging_, std::max<size_t>(total, 1) * sizeof(Part),
+                        hipMemcpyHostToDevice, s));
+  // ---- set-up: records, level sets, coarse inverses, coefficients, rhs ----
+  for (auto& sp : setup_p)
+    for (Job* j : sp.second->jobs) {
+      j->t_start = now_seconds();
+      int rc = j->grid->bind(s, &j->in, &j->out, &j->opt);
+      if (rc) return rc;
+      j->grid->set_level_scales(j->opt.precision, j->opt.pre_sweeps, j->opt.post_sweeps);
+      if ((rc = j->grid->upload(s))) return rc;
+    }
+  for (auto& sp : setup_p) {
+    Plan& p = *sp.first;
+    FLAUNCH(k_reset_state, p.one, s);
+    setup_tables(s, p, sp.second->opt.phi_preserve_levels);
+    PLAUNCH(k_rhs, p.cells, s);
+    FLAUNCH(k_fin_rhs, p.cells, s);
+    PLAUNCH(k_zero, p.vec4, s, 0, 0);
+    PLAUNCH(k_zero, p.vec4, s, 1, 0);
+    PLAUNCH(k_zero, p.vec4, s, 2, 0);
+  }
+  // ---- finished inner solves fold into the FP64 iterate ----
+  for (auto& ap : accum_p) {
+    PLAUNCH(k_accumulate, ap.first->vec4, s);
+    PLAUNCH(k_zero, ap.first->vec4, s, 2, 1);
+  }
+  // ---- FP64 true residuals (refinement control) ----
+  for (int np = 0; np < 2; ++np)
+    for (auto& rp : resid_p[np]) {
+      PLAUNCH(k_march<OpTrueResidual>, rp.first->march0, s, 2, MarchLaunch{0, 0});
+      FLAUNCH(k_fin_true, rp.first->march0, s, np);
+    }
+  // ---- one CG + V-cycle iteration of every iterating grid ----
+  for (auto& ip : iter_p) enqueue_iteration(s, *ip.first);
+  // ---- gradient subtract of converged grids ----
+  if (fin_p) {
+    for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, fin_p->faces[a], s, a);
+    PLAUNCH(k_div_after, fin_p->cells, s);
+    FLAUNCH(k_fin_divafter, fin_p->cells, s);
+    PLAUNCH(k_pressure_out, fin_p->cells, s);
+    for (Job* j : finish) {
+      int rc = j->grid->finish_outputs(s, &j->in, &j->out);
+      if (rc) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_gather_states, dim3((unsigned)((live.size() + 255) / 256)), dim3(256), 0, s, all_p->one.dev,
+                     all_p->one.n, round_states_);
+  HIPCHK(hipMemcpyAsync(round_states_host_, round_states_, live.size() * sizeof(State), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipMemcpy(out_host, o, n_ * 8, hipMemcpyDeviceToHost));
-  (void)hipFree(phi); (void)hipFree(p); (void)hipFree(o);
+  const double t = now_seconds();
+  for (size_t i = 0; i < live.size(); ++i) {
+    Job* j = live[i];
+    const State& S = round_states_host_[i];
+    switch (j->stage) {
+      case Job::SETUP:
+      case Job::ACCUMULATE: {
+        const bool more = j->passes < std::max(1, j->opt.max_outer);
+        j->passes += 1;
+        j->stage = (S.outer_done || !more) ? Job::FINISH : Job::ITERATE;
+        break;
+      }
+      case Job::ITERATE:
+        if (S.done) j->stage = Job::ACCUMULATE;
+        break;
+      case Job::FINISH: {
+        const GridRec& R = *j->grid->host_rec();
+        fp_report* rep = &j->report;
+        std::memset(rep, 0, sizeof(*rep));
+        int flags = S.flags;
+        if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;
+        rep->flags = flags;
+        rep->iterations = S.total_iter;
+        rep->outer_iterations = S.outer;
+        rep->levels = R.L;
+        rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
+        rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
+        float fa, fb;
+        std::memcpy(&fa, &S.divmax_before, 4);
+        std::memcpy(&fb, &S.divmax_after, 4);
+        rep->max_div_before = fa;
+        rep->max_div_after = fb;
+        rep->device_ms = (t - j->t_start) * 1e3;
+        rep->samples = (int64_t)R.n;
+        rep->liquid_samples = (int64_t)S.liquid;
+        j->t_done = t;
+        j->stage = Job::DONE;
+        break;
+      }
+      default:
+        break;
+    }
+  }
   return FP_OK;
 }
 
-int UniformSolver::probe_rhs(hipStream_t s, const fp_inputs* in, double* b_host) {
-  State init;
-  std::memset(&init, 0, sizeof(init));
-  HIPCHK(hipMemcpy(st_, &init, sizeof(State), hipMemcpyHostToDevice));
-  const size_t nu = (size_t)(nx_ + 1) * ny_ * nz_, nv = (size_t)nx_ * (ny_ + 1) * nz_,
-               nw = (size_t)nx_ * ny_ * (nz_ + 1);
-  HIPCHK(hipMemcpy(lv_[0].phi, in->phi, n_ * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(in_u_, in->u, nu * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(in_v_, in->v, nv * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(in_w_, in->w, nw * 4, hipMemcpyHostToDevice));
-  const float* q = nullptr;
-  if (in->target_divergence) {
-    HIPCHK(hipMemcpy(in_q_, in->target_divergence, n_ * 4, hipMemcpyHostToDevice));
-    q = in_q_;
-  }
-  const double rho = in->rho, dt = in->dt;
-  LAUNCH(k_rhs, n_, s, Grid3{nx_, ny_, nz_}, (const float*)lv_[0].phi, (const float*)in_u_, (const float*)in_v_,
-         (const float*)in_w_, q, rho * h_ / dt, rho * h_ * h_ / dt, 1.0 / h_, 0.0, b_, st_, partials_);
-  hipLaunchKernelGGL(k_fin_rhs, dim3(1), dim3(FP_BLOCK), 0, s, (const double*)partials_, grid_for(n_), 0.0, st_);
+// ---- parity probes (single grid) -------------------------------------------
+int Projector::probe_rhs(hipStream_t s, UniformGrid* g, double* b_host) {
+  fp_options o;
+  std::memset(&o, 0, sizeof(o));
+  o.precision = FP_PRECISION_FP32;
+  Plan* p = plan_for({g}, &o);
+  if (!p) return FP_ERR_OUT_OF_MEMORY;
+  g->host_rec()->tol2 = 0.0;
+  int rc = g->upload(s);
+  if (rc) return rc;
+  FLAUNCH(k_reset_state, p->one, s);
+  PLAUNCH(k_rhs, p->cells, s);
+  FLAUNCH(k_fin_rhs, p->cells, s);
+  HIPCHK(hipMemcpyAsync(b_host, g->host_rec()->b, (size_t)g->host_rec()->n * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipMemcpy(b_host, b_, n_ * 8, hipMemcpyDeviceToHost));
   return FP_OK;
 }
 
-int UniformSolver::probe_vcycle(hipStream_t s, const float* phi_host, const fp_options* opt, const float* r_host,
-                                float* z_host) {
-  State init;
-  std::memset(&init, 0, sizeof(init));
-  init.rscale = 1.0f;
-  HIPCHK(hipMemcpy(st_, &init, sizeof(State), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(lv_[0].phi, phi_host, n_ * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(r_, r_host, n_ * 4, hipMemcpyHostToDevice));
-  for (int l = 1; l < L_; ++l) {
-    const UniformLevel& f = lv_[l - 1];
-    UniformLevel& c = lv_[l];
-    LAUNCH(k_downsample_phi, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const float*)f.phi, Grid3{c.d[0], c.d[1], c.d[2]},
-           c.phi, c.s[0], c.s[1], c.s[2], (l <= opt->phi_preserve_levels) ? 1 : 0);
-  }
-  {
-    const UniformLevel& c = lv_[L_ - 1];
-    hipLaunchKernelGGL(k_coarse_setup, dim3(1), dim3(FP_BLOCK), 0, s, largs(c), cmap_, ncoarse_, minv_, max_coarse_,
-                       st_);
-  }
-  set_level_scales(opt->precision);
-  build_weights(s, opt->precision);
-  // normalise like the CG does: rscale = 2^-ceil(log2 max|r|)
-  float m = 0.0f;
-  for (size_t i = 0; i < n_; ++i) m = std::max(m, std::fabs(r_host[i]));
-  const float rs = m > 0.0f ? std::exp2(-std::ceil(std::log2(m))) : 1.0f;
-  HIPCHK(hipMemcpy(&st_->rscale, &rs, sizeof(float), hipMemcpyHostToDevice));
-  const UniformLevel& f = lv_[0];
-  std::vector<float> tmp(n_);
-  if (opt->precision == FP_PRECISION_MIXED_FP16) {
-    vcycle<__half>(s, opt->pre_sweeps, opt->post_sweeps);
-    std::vector<__half> hz(n_);
-    HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(hz.data(), zbuf_, n_ * 2, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n_; ++i) z_host[i] = __half2float(hz[i]) * f.Se / rs;
-  } else if (opt->precision == FP_PRECISION_MIXED_BF16) {
-    vcycle<__hip_bfloat16>(s, opt->pre_sweeps, opt->post_sweeps);
-    std::vector<__hip_bfloat16> hz(n_);
-    HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(hz.data(), zbuf_, n_ * 2, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n_; ++i) z_host[i] = __bfloat162float(hz[i]) * f.Se / rs;
-  } else {
-    vcycle<float>(s, opt->pre_sweeps, opt->post_sweeps);
-    HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(tmp.data(), zbuf_, n_ * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n_; ++i) z_host[i] = tmp[i] * f.Se / rs;
-  }
-  return FP_OK;
-}
-
-int UniformSolver::probe_level_phi(hipStream_t s, const float* phi_host, int level, int preserve, float* out_host) {
-  if (level < 0 || level >= L_) {
-    set_error("level %d out of range [0,%d)", level, L_);
-    return FP_ERR_INVALID_ARGUMENT;
-  }
-  HIPCHK(hipMemcpy(lv_[0].phi, phi_host, n_ * 4, hipMemcpyHostToDevice));
-  for (int l = 1; l <= level; ++l) {
-    const UniformLevel& f = lv_[l - 1];
-    UniformLevel& c = lv_[l];
-    LAUNCH(k_downsample_phi, c.n, s, Grid3{f.d[0], f.d[1], f.d[2]}, (const float*)f.phi, Grid3{c.d[0], c.d[1], c.d[2]},
-           c.phi, c.s[0], c.s[1], c.s[2], (l <= preserve) ? 1 : 0);
-  }
+int Projector::probe_vcycle(hipStream_t s, UniformGrid* g, const fp_options* opt, float rscale, void* z_out) {
+  Plan* p = plan_for({g}, opt);
+  if (!p) return FP_ERR_OUT_OF_MEMORY;
+  g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  int rc = g->upload(s);
+  if (rc) return rc;
+  FLAUNCH(k_reset_state, p->one, s);
+  setup_tables(s, *p, opt->phi_preserve_levels);
+  HIPCHK(hipMemcpyAsync(&g->host_rec()->st->rscale, &rscale, sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipMemcpy(out_host, lv_[level].phi, lv_[level].n * 4, hipMemcpyDeviceToHost));
+  enqueue_vcycle(s, *p);
+  const LevelRec& l0 = g->level(0);
+  const size_t bytes = (size_t)l0.n * (opt->precision == FP_PRECISION_FP32 ? 4 : 2);
+  HIPCHK(hipMemcpyAsync(z_out, l0.efinal ? l0.e2 : l0.e, bytes, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FP_OK;
+}
+
+int Projector::probe_level_phi(hipStream_t s, UniformGrid* g, int preserve, int level, float* out) {
+  fp_options o;
+  std::memset(&o, 0, sizeof(o));
+  o.precision = FP_PRECISION_FP32;
+  Plan* p = plan_for({g}, &o);
+  if (!p) return FP_ERR_OUT_OF_MEMORY;
+  int rc = g->upload(s);
+  if (rc) return rc;
+  FLAUNCH(k_reset_state, p->one, s);
+  for (int l = 1; l <= level; ++l) PLAUNCH(k_downsample_phi, p->lv[l].down, s, l, preserve);
+  HIPCHK(hipMemcpyAsync(out, g->level(level).phi, (size_t)g->level(level).n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FP_OK;
+}
+
+int probe_apply(hipStream_t s, const UniformGrid* g, const float* phi_host, const double* p_host, double* out_host) {
+  const GridRec& R = *const_cast<UniformGrid*>(g)->host_rec();
+  const size_t n = (size_t)R.n;
+  float* phi = nullptr;
+  double *p = nullptr, *o = nullptr;
+  HIPCHK(hipMalloc(&phi, n * 4));
+  HIPCHK(hipMalloc(&p, n * 8));
+  HIPCHK(hipMalloc(&o, n * 8));
+  HIPCHK(hipMemcpy(phi, phi_host, n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p, p_host, n * 8, hipMemcpyHostToDevice));
+  const unsigned blocks = (unsigned)std::min<size_t>(4096, (n + FP_BLOCK - 1) / FP_BLOCK);
+  hipLaunchKernelGGL(k_apply_d, dim3(blocks), dim3(FP_BLOCK), 0, s, Grid3{R.nx, R.ny, R.nz}, (const float*)phi,
+                     (const double*)p, o);
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(out_host, o, n * 8, hipMemcpyDeviceToHost));
+  (void)hipFree(phi);
+  (void)hipFree(p);
+  (void)hipFree(o);
   return FP_OK;
 }
 

@@ -1,9 +1,23 @@
-// fp_uniform.hpp -- uniform-grid projection solver (host class + device state).
+// fp_uniform.hpp -- uniform-grid projection: per-grid device records, the
+// grid resource holder and the (batched) projector.
+//
+// Every kernel of the solve is launched over a table of parts: part i covers
+// workgroups [block0_i, block0_i + nblocks_i) of the launch and names the
+// grid record it works on.  One launch per solver stage therefore serves any
+// number of independent grids of different sizes (a single projection is a
+// table of one part); each grid keeps its own block decomposition, scalars and
+// reductions, so its result does not depend on what it is batched with.
 #pragma once
 
 #include "../../include/fluid_projection.h"
 #include "fp_common.hpp"
 
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace fp {
@@ -24,91 +38,244 @@ struct State {
   double alpha, beta;
   double rr;             // inner ||r||^2
   long long liquid;      // liquid samples
-  int done;              // inner solve finished
+  int done;              // inner solve finished (or nothing left to do)
   int outer_done;        // FP64 tolerance met
   int flags;             // FP_FLAG_*
   int iter;              // inner iterations this pass
   int total_iter;
+  int outer;             // refinement passes run
   float rscale;          // power-of-two normalisation of the V-cycle input
-  unsigned rmax_bits;    // max |r| accumulator (float bits)
   unsigned divmax_before, divmax_after;
-  unsigned counters[8];  // grid-reduction tickets
+  int pad;
 };
 
-struct UniformLevel {
-  int d[3] = {0, 0, 0};
-  int s[3] = {1, 1, 1};
-  float g[3] = {1, 1, 1};
-  size_t n = 0;
-  float* phi = nullptr;
-  void* r = nullptr;  // restricted right-hand side (storage T)
-  void* e = nullptr;  // correction (storage T); level 0: the preconditioned residual z
-  void* e2 = nullptr; // second correction buffer (fused sweeps ping-pong)
-  void* w = nullptr;  // packed coefficients W4<T> (fused levels)
-  void* t = nullptr;  // residual scratch (storage T)
-  float Sr = 1, Se = 1, St = 1;  // stored = true / S
+// Launch geometry of the column-tile kernels on one level.
+struct TileGeo {
+  int tz = 0, ty = 0, chunk = 0, ngroups = 0;
 };
+
+struct LevelRec {
+  int d[3];
+  int s[3];           // strides fine/coarse per axis from the previous level
+  float g[3];         // per-axis coefficient factors (h0/h_l,a)^2
+  long long n;
+  float* phi;
+  void* r;            // restricted right-hand side (storage T)
+  void* e;            // correction (storage T); level 0: the preconditioned residual z
+  void* e2;           // second correction buffer (fused sweeps ping-pong)
+  void* w;            // packed coefficients W4<T> (fused levels)
+  void* t;            // residual scratch (storage T)
+  float Sr, Se, St;   // stored = true / S
+  int fused;          // sweeps and residual run as column-tile marches
+  int sweep_tz;       // sweep tile width along z (32 or 16)
+  int efinal;         // buffer (0: e, 1: e2) holding e after the post-smoothing
+  TileGeo sw;         // sweep geometry
+  TileGeo mg;         // march geometry (tile 32 x 8)
+};
+
+struct GridRec {
+  int nx, ny, nz, L;
+  long long n, nu, nv, nw;
+  double h;
+  LevelRec lv[kMaxLevels];
+  double* b;          // FP64 right-hand side (h^2-scaled)
+  double* x;          // FP64 pressure iterate (resident: warm start)
+  float* r;           // FP32 CG residual
+  float* e;           // FP32 inner solution
+  float* q;           // A d
+  float* d[2];
+  int* cmap;
+  int* ncoarse;
+  double* minv;
+  int max_coarse;
+  int max_iter;
+  State* st;
+  double* partials;
+  // per-projection inputs / outputs (device) and constants
+  const float* in_u;
+  const float* in_v;
+  const float* in_w;
+  const float* in_q;
+  float* out_u;
+  float* out_v;
+  float* out_w;
+  float* out_p;
+  double s_flux, s_q, inv_h, tol2, sub_scale;
+  int warm;           // keep x from the previous projection (warm start)
+  int pad;
+};
+
+struct Part {
+  int block0;
+  int nblocks;
+  const GridRec* g;
+};
+
+// Workgroups of a part-table launch: the part covering blockIdx.x, the grid it
+// names and the workgroup's index within that grid.  Part starts are multiples
+// of 8, so a grid's workgroup b runs on XCD b % 8 as tile_of_group assumes;
+// workgroups in the padding return at once.
+__device__ __forceinline__ int part_index(const Part* parts, int n) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define FP_PART(parts, np)                           \
+  const Part fp_part_ = (parts)[part_index(parts, np)]; \
+  const GridRec& G = *fp_part_.g;                    \
+  const int blk = (int)blockIdx.x - fp_part_.block0; \
+  const int nblk = fp_part_.nblocks;                 \
+  if (blk >= nblk) return;
+
+// Grid-stride loop over one grid's workgroups.
+#define FP_STRIDE(idx, N) \
+  for (size_t idx = (size_t)blk * blockDim.x + threadIdx.x; idx < (size_t)(N); idx += (size_t)nblk * blockDim.x)
 
 int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3]);
 
-class UniformSolver {
+// Resources of one grid: the multigrid hierarchy, CG vectors and its record.
+class UniformGrid {
  public:
-  ~UniformSolver();
+  ~UniformGrid();
   static size_t estimate_bytes(int nx, int ny, int nz);
   int create(int nx, int ny, int nz, double h);
-  int project(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt, fp_report* rep);
-  int probe_apply(hipStream_t s, const float* phi, const double* p, double* out);
-  int probe_rhs(hipStream_t s, const fp_inputs* in, double* b);
-  int probe_vcycle(hipStream_t s, const float* phi, const fp_options* opt, const float* r, float* z);
-  int probe_level_phi(hipStream_t s, const float* phi, int level, int preserve, float* out);
 
-  int levels() const { return L_; }
-  const UniformLevel& level(int l) const { return lv_[l]; }
+  // Per-projection set-up of the record: stage host inputs (async on s) or
+  // take device pointers, the output targets and the scalars.
+  int bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt);
+  void set_level_scales(int precision, int pre, int post);
+  int upload(hipStream_t s);  // record -> device (async)
+  int finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* out);
+
+  int levels() const { return rec_->L; }
+  const LevelRec& level(int l) const { return rec_->lv[l]; }
   size_t bytes() const { return bytes_; }
-  size_t cells() const { return n_; }
+  size_t cells() const { return (size_t)rec_->n; }
+  GridRec* host_rec() { return rec_; }
+  const GridRec* dev_rec() const { return dev_rec_; }
+  State* dev_state() const { return rec_->st; }
+  bool phi_is_staged() const { return phi_staged_; }
+  int id() const { return id_; }
 
  private:
   int alloc(void** p, size_t bytes);
   void release();
-  void set_level_scales(int precision);
-  void build_weights(hipStream_t s, int precision);
-  int setup_levels(hipStream_t s, const float* phi0, int preserve);
-  template <typename T> int vcycle(hipStream_t s, int pre, int post);
-  template <typename T> int iteration(hipStream_t s, int pre, int post, int max_iter, int parity);
-  void run_iteration(hipStream_t s, int precision, int pre, int post, int max_iter, int parity);
-  int ensure_graph(hipStream_t s, int precision, int pre, int post, int max_iter);
-  hipGraphExec_t graph_exec_ = nullptr;
-  int graph_key_[4] = {-1, -1, -1, -1};
-
-  int nx_ = 0, ny_ = 0, nz_ = 0;
-  double h_ = 0;
-  size_t n_ = 0;
-  int L_ = 0;
-  UniformLevel lv_[kMaxLevels];
-  double* b_ = nullptr;   // FP64 right-hand side (h^2-scaled)
-  double* x_ = nullptr;   // FP64 pressure iterate (resident: warm start)
-  float* r_ = nullptr;    // FP32 CG residual
-  float* e_ = nullptr;    // FP32 inner solution
-  float* q_ = nullptr;    // A d
-  float* d_[2] = {nullptr, nullptr};
+  GridRec* rec_ = nullptr;        // pinned host copy
+  GridRec* dev_rec_ = nullptr;    // device copy
   float *in_u_ = nullptr, *in_v_ = nullptr, *in_w_ = nullptr, *in_q_ = nullptr;  // host-input staging
-  int* cmap_ = nullptr;
-  int* ncoarse_ = nullptr;
-  double* minv_ = nullptr;
-  int max_coarse_ = 0;
-  State* st_ = nullptr;
-  double* partials_ = nullptr;
-  State* host_state_ = nullptr;
-  State* host_upload_ = nullptr;
+  bool host_io_ = false;
+  bool phi_staged_ = false;
+  int id_ = 0;
+  std::vector<void*> allocations_;
+  size_t bytes_ = 0;
+};
+
+// One asynchronous projection request (fp_submit) and its progress.
+struct Job {
+  enum Stage { SETUP, ITERATE, ACCUMULATE, FINISH, DONE };
+  uint64_t ticket = 0;
+  UniformGrid* grid = nullptr;
+  fp_inputs in{};
+  fp_outputs out{};
+  fp_options opt{};
+  int stage = SETUP;
+  int passes = 0;     // FP64 true residuals evaluated
+  int rc = FP_OK;
+  fp_report report{};
+  double t_submit = 0, t_start = 0, t_done = 0;  // host seconds
+};
+
+// Device part table of one stage.
+struct PartTable {
+  const Part* dev = nullptr;
+  int n = 0;        // parts (grids)
+  int blocks = 0;   // workgroups of the launch
+};
+
+// Runs projections of a set of bound grids: one launch per solver stage
+// covers every grid of the set.  Plans (part tables and the replayed
+// iteration graph) are cached per grid set and options.
+class Projector {
+ public:
+  struct Plan;
+  ~Projector();
+  int run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps);
+  // Drops cached plans that name the grid (before the grid is destroyed).
+  void forget(int grid_id);
+  // Parity probes on one grid.
+  int probe_rhs(hipStream_t s, UniformGrid* g, double* b_host);
+  int probe_vcycle(hipStream_t s, UniformGrid* g, const fp_options* opt, float rscale, void* z_out);
+  int probe_level_phi(hipStream_t s, UniformGrid* g, int preserve, int level, float* out);
+
+  // One round of continuous batching: every job advances one stage (set-up +
+  // FP64 residual, one CG + V-cycle iteration, accumulation + FP64 residual,
+  // or the gradient subtract); each stage group is one batched launch set.
+  // Jobs whose stage became DONE carry their report.
+  int serve_round(hipStream_t s, std::vector<Job*>& jobs);
+  void fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int precision, int pre, int post);
+
+ private:
+  Plan* plan_for(const std::vector<UniformGrid*>& grids, const fp_options* opt);
+  int setup_tables(hipStream_t s, Plan& p, int preserve);
+  template <typename T> int vcycle_t(hipStream_t s, Plan& p);
+  template <typename T> int iteration_t(hipStream_t s, Plan& p);
+  int enqueue_vcycle(hipStream_t s, Plan& p);
+  int enqueue_iteration(hipStream_t s, Plan& p);
+  int ensure_graph(hipStream_t s, Plan& p);
+  int poll(hipStream_t s, Plan& p, int* counts);
   void timer_begin(hipStream_t s);
   void timer_end(hipStream_t s);
-  void* zbuf_ = nullptr;  // level-0 buffer holding z after the last V-cycle
+  std::map<std::vector<long long>, Plan*> plans_;
   int timed_kernel_ = 0;
   int timer_used_ = 0;
   std::vector<hipEvent_t> timer_events_;
-  std::vector<void*> allocations_;
-  std::vector<void*> pinned_;
-  size_t bytes_ = 0;
+  int* host_counts_ = nullptr;
+  int* dev_counts_ = nullptr;
+  // serving: per-round tables in one reusable arena, states gathered per round
+  Part* round_arena_ = nullptr;
+  size_t round_arena_cap_ = 0;
+  Part* round_staging_ = nullptr;
+  State* round_states_ = nullptr;
+  State* round_states_host_ = nullptr;
+  size_t round_states_cap_ = 0;
 };
+
+// The serving worker of a context: fp_submit queues jobs, a host thread runs
+// serve_round while jobs are active, fp_poll waits for a ticket.  GPU work of
+// the worker and of synchronous fp_project calls is serialised by gpu_mutex.
+class Server {
+ public:
+  Server(hipStream_t s, int device, Projector* proj, std::mutex* gpu_mutex);
+  ~Server();
+  int submit(UniformGrid* g, const fp_inputs* in, const fp_outputs* out, const fp_options* opt, uint64_t* ticket);
+  // FP_OK with the report when done; 1 while pending (after waiting up to timeout_ms).
+  int poll(uint64_t ticket, int timeout_ms, fp_report* rep, double* latency_ms);
+  int jobs_on(const UniformGrid* g);
+  int pending();
+
+ private:
+  void loop();
+  hipStream_t stream_;
+  int device_;
+  Projector* proj_;
+  std::mutex* gpu_mutex_;
+  std::mutex mu_;
+  std::condition_variable work_cv_, done_cv_;
+  std::deque<std::unique_ptr<Job>> incoming_;
+  std::vector<std::unique_ptr<Job>> active_;
+  std::map<uint64_t, std::unique_ptr<Job>> done_;
+  uint64_t next_ticket_ = 1;
+  bool stop_ = false;
+  std::thread thread_;
+};
+
+// FP64 operator probe out = A p on the grid's finest level (host arrays).
+int probe_apply(hipStream_t s, const UniformGrid* g, const float* phi_host, const double* p_host, double* out_host);
 
 }  // namespace fp

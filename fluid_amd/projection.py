@@ -42,9 +42,10 @@ KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "
 EXPORTED_SYMBOLS = [
     "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
     "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
-    "fp_grid_face_counts", "fp_project", "fp_probe_apply", "fp_probe_rhs", "fp_probe_vcycle",
-    "fp_probe_level_phi",
+    "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_submit", "fp_poll", "fp_probe_apply",
+    "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi",
 ]
+PENDING = 1
 
 
 class GridDesc(C.Structure):
@@ -120,6 +121,10 @@ def load_library(path: str = LIB_PATH):
     L.fp_grid_levels.argtypes = [vp, C.c_int32, vp, vp, C.POINTER(C.c_int32)]
     L.fp_grid_face_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fp_project.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(Report)]
+    L.fp_project_batch.argtypes = [vp, C.c_int32, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options),
+                                   C.POINTER(Report)]
+    L.fp_submit.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(C.c_uint64)]
+    L.fp_poll.argtypes = [vp, C.c_uint64, C.c_int32, C.POINTER(Report), C.POINTER(C.c_double)]
     L.fp_probe_apply.argtypes = [vp, vp, vp, vp]
     L.fp_probe_rhs.argtypes = [vp, C.POINTER(Inputs), vp]
     L.fp_probe_vcycle.argtypes = [vp, vp, C.POINTER(Options), vp, vp]
@@ -244,12 +249,10 @@ class UniformPressureProjection:
         _check(_lib.fp_grid_levels(self._g, 16, dims.ctypes.data, strides.ctypes.data, C.byref(L)))
         return dims[: 3 * L.value].reshape(-1, 3), strides[: 3 * L.value].reshape(-1, 3)
 
-    def project(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, options: Options | None = None,
-                **opts) -> ProjectionResult:
-        """One projection.  Host arrays in -> host arrays out; CUDA tensors in ->
-        results written into `out` tensors (or fresh tensors) on the device."""
+    def _prepare(self, phi, u, v, w, dt, rho, target_divergence=None, out=None):
+        """C input/output records for one projection, the result arrays and
+        the objects that must stay alive until the call returns."""
         nx, ny, nz = self.dims
-        o = options or default_options(**opts)
         parts = [_ptr(phi, np.float32), _ptr(u, np.float32), _ptr(v, np.float32), _ptr(w, np.float32),
                  _ptr(target_divergence, np.float32)]
         kinds = {p[1] for p in parts if p[1] is not None}
@@ -275,10 +278,29 @@ class UniformPressureProjection:
                        torch.empty(shapes[2], device=dev), torch.empty(shapes[3], device=dev))
             p, uo, vo, wo = out
             outs = Outputs(p.data_ptr() if p is not None else None, uo.data_ptr(), vo.data_ptr(), wo.data_ptr())
+        return ins, outs, (p, uo, vo, wo), [q[2] for q in parts]
+
+    def project(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, options: Options | None = None,
+                **opts) -> ProjectionResult:
+        """One projection.  Host arrays in -> host arrays out; CUDA tensors in ->
+        results written into `out` tensors (or fresh tensors) on the device."""
+        o = options or default_options(**opts)
+        ins, outs, res, keep = self._prepare(phi, u, v, w, dt, rho, target_divergence, out)
         rep = Report()
         _check(_lib.fp_project(self._g, C.byref(ins), C.byref(outs), C.byref(o), C.byref(rep)))
+        del keep
         self.diagnostics = rep.as_dict()
-        return ProjectionResult(p, uo, vo, wo, self.diagnostics)
+        return ProjectionResult(*res, self.diagnostics)
+
+    def submit(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, options: Options | None = None,
+               **opts) -> "Ticket":
+        """Queue one projection on the context's serving worker (fp_submit);
+        the arrays stay referenced by the ticket until it completes."""
+        o = options or default_options(**opts)
+        ins, outs, res, keep = self._prepare(phi, u, v, w, dt, rho, target_divergence, out)
+        t = C.c_uint64()
+        _check(_lib.fp_submit(self._g, C.byref(ins), C.byref(outs), C.byref(o), C.byref(t)))
+        return Ticket(self, t.value, res, (keep, ins, outs, o))
 
     # ---- parity probes --------------------------------------------------------
     def probe_apply(self, phi, p):
@@ -331,3 +353,61 @@ def project_scene(scene, **opts) -> ProjectionResult:
         return proj.project(scene.phi, scene.u, scene.v, scene.w, scene.dt, scene.rho, **opts)
     finally:
         proj.destroy()
+
+
+def project_batch(grids, requests, options: Options | None = None, **opts):
+    """Independent projections of several grids (one context) in one batched
+    solve (fp_project_batch).  requests: per grid a dict with phi, u, v, w, dt,
+    rho and optionally target_divergence / out.  Returns a ProjectionResult
+    per grid; each equals what the grid's projection alone returns."""
+    if len(grids) != len(requests) or not grids:
+        raise ValueError("one request per grid")
+    o = options or default_options(**opts)
+    n = len(grids)
+    ins_arr = (Inputs * n)()
+    outs_arr = (Outputs * n)()
+    results, keep = [], []
+    for i, (g, rq) in enumerate(zip(grids, requests)):
+        ins, outs, res, k = g._prepare(rq["phi"], rq["u"], rq["v"], rq["w"], rq["dt"], rq["rho"],
+                                       rq.get("target_divergence"), rq.get("out"))
+        ins_arr[i] = ins
+        outs_arr[i] = outs
+        results.append(res)
+        keep.append(k)
+    reps = (Report * n)()
+    handles = (C.c_void_p * n)(*[g._g.value for g in grids])
+    _check(_lib.fp_project_batch(C.cast(handles, C.c_void_p), n, ins_arr, outs_arr, C.byref(o), reps))
+    del keep
+    out = []
+    for g, res, rep in zip(grids, results, reps):
+        g.diagnostics = rep.as_dict()
+        out.append(ProjectionResult(*res, g.diagnostics))
+    return out
+
+
+class Ticket:
+    """A submitted projection (fp_submit); wait() returns its result."""
+
+    def __init__(self, grid, ticket, res, keep):
+        self.grid = grid
+        self.ticket = ticket
+        self._res = res
+        self._keep = keep
+        self.result = None
+        self.latency_ms = None
+
+    def wait(self, timeout_ms: int = -1):
+        """The ProjectionResult once done, else None after timeout_ms."""
+        if self.result is not None:
+            return self.result
+        rep = Report()
+        lat = C.c_double()
+        rc = _lib.fp_poll(self.grid.ctx.handle, C.c_uint64(self.ticket), int(timeout_ms), C.byref(rep), C.byref(lat))
+        if rc == PENDING:
+            return None
+        _check(rc)
+        self.latency_ms = lat.value
+        self.grid.diagnostics = rep.as_dict()
+        self.result = ProjectionResult(*self._res, self.grid.diagnostics)
+        self._keep = None
+        return self.result

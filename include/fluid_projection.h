@@ -179,6 +179,31 @@ int fp_grid_face_counts(const fp_grid* grid, int64_t* nu, int64_t* nv, int64_t* 
  * MGPCG to the FP64 relative tolerance -> gradient subtract.  Synchronous. */
 int fp_project(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_options* options, fp_report* report);
 
+/* Independent projections of several grids of one context in one batched
+ * solve: every solver stage is one launch over all the grids, each grid keeps
+ * its own scalars and convergence test, and its result is the same as when
+ * projected alone.  in/out/reports are arrays of `count`; the options are
+ * shared.  Synchronous.  (No reference counterpart: the reference projects one
+ * grid per frame.) */
+int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, fp_outputs* out,
+                     const fp_options* options, fp_report* reports);
+
+/* Serving (continuous batching).  fp_submit queues one projection and returns
+ * at once; the context's worker thread advances every queued and running
+ * projection by one stage per round (set-up, one CG + V-cycle iteration,
+ * refinement, gradient subtract), each stage of all of them in one batched
+ * launch, so requests arriving while others run join them on the device.
+ * The input and output arrays must stay valid until the ticket completes.
+ * fp_poll waits up to timeout_ms (-1: forever, 0: no wait) and returns FP_OK
+ * with the report (and the submit-to-completion latency) once the ticket is
+ * done, FP_PENDING while it is not.  A grid with running tickets cannot be
+ * destroyed (FP_ERR_STATE).  Synchronous calls on the context wait for the
+ * worker's current round.  (No reference counterpart: the reference runs one
+ * projection per frame on its render thread.) */
+#define FP_PENDING 1
+int fp_submit(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_options* options, uint64_t* ticket);
+int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report, double* latency_ms);
+
 /* ---- parity probes (device work on host arrays; used by the test suite) ---- */
 /* out = A p (FP64) on the finest level, h^2-scaled. */
 int fp_probe_apply(fp_grid* grid, const float* phi, const double* p, double* out);

@@ -1,0 +1,85 @@
+"""Serving (fp_submit / fp_poll): continuous batching of uneven projections.
+
+Bars: every served result is bitwise equal to the same projection run alone
+through fp_project (the same per-grid arithmetic, whatever it shares rounds
+with); requests submitted while others are running complete; warm-started
+session steps reach the tolerance; a grid with running work refuses
+destruction."""
+import time
+
+import numpy as np
+import pytest
+
+from fluid_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [48, 32, 64, 37, 24, 40]
+
+
+def scene(g, n):
+    rng = np.random.default_rng(300 + g)
+    fill = tuple(float(x) for x in rng.uniform(0.3, 0.8, 3))
+    return scenes.dam_break_scene(n, seed=400 + g, fill=fill)
+
+
+def test_served_equals_alone_with_staggered_arrivals(fpl):
+    ss = [scene(g, n) for g, n in enumerate(SIZES)]
+    grids = [fpl.UniformPressureProjection(s.shape, s.h) for s in ss]
+    try:
+        alone = [g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp16") for g, s in zip(grids, ss)]
+        tickets = []
+        for i, (g, s) in enumerate(zip(grids, ss)):
+            tickets.append(g.submit(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp16"))
+            if i == 2:
+                time.sleep(0.01)  # later requests arrive while the first ones run
+        for t, a, s in zip(tickets, alone, ss):
+            r = t.wait(60000)
+            assert r is not None, "served projection did not complete"
+            print(s.shape, "latency ms", round(t.latency_ms, 2), "iters", r.report["iterations"])
+            assert r.report["flags"] == 0 and r.report["relative_residual"] <= 1e-6
+            assert r.report["iterations"] == a.report["iterations"]
+            for x, y in ((a.pressure, r.pressure), (a.u, r.u), (a.v, r.v), (a.w, r.w)):
+                assert np.array_equal(x, y)
+    finally:
+        for g in grids:
+            g.destroy()
+
+
+def test_session_warm_start_steps(fpl):
+    s = scene(7, 40)
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    try:
+        u, v, w = s.u, s.v, s.w
+        iters = []
+        for step in range(4):
+            t = g.submit(s.phi, u, v, w, s.dt, s.rho, precision="fp16", warm_start=1 if step else 0)
+            r = t.wait(60000)
+            assert r is not None and r.report["flags"] == 0
+            assert r.report["relative_residual"] <= 1e-6
+            iters.append(r.report["iterations"])
+            # next step: the projected field plus a fresh gravity impulse
+            u, v, w = r.u, r.v.copy(), r.w
+            v[:, 1:-1, :] -= 9.81 * s.dt
+        print("iterations per step", iters)
+        assert iters[-1] <= iters[0]
+    finally:
+        g.destroy()
+
+
+def test_destroy_refused_while_running(fpl):
+    s = scene(9, 64)
+    g = fpl.UniformPressureProjection(s.shape, s.h)
+    t = g.submit(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp32", relative_tolerance=1e-10)
+    try:
+        try:
+            g.destroy()
+            refused = False
+        except fpl.FluidProjectionError:
+            refused = True
+        r = t.wait(60000)
+        assert r is not None
+        if not refused:
+            pytest.skip("the projection finished before destroy was attempted")
+    finally:
+        g.destroy()
