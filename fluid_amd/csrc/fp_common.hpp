@@ -41,19 +41,73 @@ __device__ __forceinline__ double face_weight_d(float pc, float pn) {
   return lc ? 1.0 / ghost_theta_d(pc, pn) : 1.0 / ghost_theta_d(pn, pc);
 }
 
+// ---- explicit address spaces ------------------------------------------------
+// Arrays reached through a grid record are generic pointers to the compiler,
+// which would issue FLAT instructions for them; FLAT loads also count against
+// the LDS counter, so an LDS-only barrier would wait for an in-flight global
+// prefetch.  ldg/stg move the bits of a value through a global (address space
+// 1) pointer, and cload reads a record from constant memory (address space 4)
+// into scalar registers, readfirstlane keeping every word uniform.
+#define FP_AS1 __attribute__((address_space(1)))
+typedef unsigned int fp_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int fp_u32x4 __attribute__((ext_vector_type(4)));
+typedef float fp_f32x4 __attribute__((ext_vector_type(4)));
+typedef double fp_f64x4 __attribute__((ext_vector_type(4)));
+
+template <int N> struct RawOf;
+template <> struct RawOf<2> { typedef unsigned short type; };
+template <> struct RawOf<4> { typedef unsigned int type; };
+template <> struct RawOf<8> { typedef fp_u32x2 type; };
+template <> struct RawOf<16> { typedef fp_u32x4 type; };
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, size_t i) {
+  typedef typename RawOf<sizeof(T)>::type R;
+  const R r = ((const FP_AS1 R*)p)[i];
+  return __builtin_bit_cast(T, r);
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, size_t i, T v) {
+  typedef typename RawOf<sizeof(T)>::type R;
+  ((FP_AS1 R*)p)[i] = __builtin_bit_cast(R, v);
+}
+__device__ __forceinline__ fp_f32x4 ldg4(const float* p, size_t i4) { return ((const FP_AS1 fp_f32x4*)p)[i4]; }
+__device__ __forceinline__ void stg4(float* p, size_t i4, fp_f32x4 v) { ((FP_AS1 fp_f32x4*)p)[i4] = v; }
+
+template <typename T>
+__device__ __forceinline__ T cload(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
+
+  typedef const __attribute__((address_space(4))) unsigned CU;
+  CU* src = (CU*)(p);
+  T out;
+  unsigned* dst = reinterpret_cast<unsigned*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
 // Storage conversions of multigrid vectors.
 template <typename T> struct Store;
 template <> struct Store<float> {
   static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
   static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+  static __device__ __forceinline__ float gld(const float* p, size_t i) { return ldg(p, i); }
+  static __device__ __forceinline__ void gst(float* p, size_t i, float v) { stg(p, i, v); }
 };
 template <> struct Store<__half> {
   static __device__ __forceinline__ float ld(const __half* p, size_t i) { return __half2float(p[i]); }
   static __device__ __forceinline__ void st(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
+  static __device__ __forceinline__ float gld(const __half* p, size_t i) { return __half2float(ldg(p, i)); }
+  static __device__ __forceinline__ void gst(__half* p, size_t i, float v) { stg(p, i, __float2half_rn(v)); }
 };
 template <> struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ float ld(const __hip_bfloat16* p, size_t i) { return __bfloat162float(p[i]); }
   static __device__ __forceinline__ void st(__hip_bfloat16* p, size_t i, float v) { p[i] = __float2bfloat16(v); }
+  static __device__ __forceinline__ float gld(const __hip_bfloat16* p, size_t i) { return __bfloat162float(ldg(p, i)); }
+  static __device__ __forceinline__ void gst(__hip_bfloat16* p, size_t i, float v) {
+    stg(p, i, __float2bfloat16(v));
+  }
 };
 
 // Wave (64 lanes) and block sums of doubles.

@@ -38,17 +38,18 @@ constexpr int kMarchTZ = 32;
 template <class Op>
 __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int honour, MarchLaunch la) {
   FP_PART(parts, np);
-  if (honour == 1 && G.st->done) return;
-  if (honour == 2 && G.st->outer_done) return;
+  const State* st_ = cload(&Gp->st);
+  if (honour == 1 && cload(&st_->done)) return;
+  if (honour == 2 && cload(&st_->outer_done)) return;
   constexpr int TZ = kMarchTZ;
   constexpr int TY = 256 / TZ;
   constexpr int RZ = TZ + 2, RY = TY + 2;
   using Cell = typename Op::Cell;
   __shared__ Cell s[3][RY][RZ];
-  const LevelRec& lv = G.lv[la.level];
+  const LevelRec lv = cload(&Gp->lv[la.level]);
   const int nx = lv.d[0], ny = lv.d[1], nz = lv.d[2];
   const int chunk = lv.mg.chunk;
-  Op op = Op::make(G, la);
+  Op op = Op::make(Gp, la);
   op.prepare();
   float mx = 0.0f;
   int tzi, tyi, cxi;
@@ -101,8 +102,9 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
     lds_barrier();
   }
   if constexpr (Op::NV > 0) {
-    if constexpr (Op::WITH_MAX) block_partials_max<Op::NV>(acc, mx, G.partials, blk, nblk);
-    else block_partials<Op::NV>(acc, G.partials, blk, nblk);
+    double* partials = cload(&Gp->partials);
+    if constexpr (Op::WITH_MAX) block_partials_max<Op::NV>(acc, mx, partials, blk, nblk);
+    else block_partials<Op::NV>(acc, partials, blk, nblk);
   }
 }
 
@@ -142,35 +144,35 @@ struct OpDirection {
   float zm, beta;
   float gx, gy, gz;
   const State* st;
-  static __device__ __forceinline__ OpDirection make(const GridRec& G, const MarchLaunch& la) {
-    const LevelRec& l0 = G.lv[0];
+  static __device__ __forceinline__ OpDirection make(const GridRec* Gp, const MarchLaunch&) {
+    const LevelRec l0 = cload(&Gp->lv[0]);
     OpDirection o;
     o.ny = l0.d[1];
     o.nz = l0.d[2];
     o.phi = l0.phi;
     o.z = (const T*)(l0.efinal ? l0.e2 : l0.e);
+    o.st = cload(&Gp->st);
     // CG direction buffers alternate with the grid's own iteration count
-    const int parity = G.st->iter & 1;
-    o.dold = G.d[parity];
-    o.dnew = G.d[parity ^ 1];
-    o.q = G.q;
+    const int parity = cload(&o.st->iter) & 1;
+    o.dold = cload(&Gp->d[parity]);
+    o.dnew = cload(&Gp->d[parity ^ 1]);
+    o.q = cload(&Gp->q);
     o.zmul = l0.Se;
     o.gx = l0.g[0];
     o.gy = l0.g[1];
     o.gz = l0.g[2];
-    o.st = G.st;
     return o;
   }
   __device__ __forceinline__ void prepare() {
-    zm = zmul / st->rscale;
-    beta = (float)st->beta;
+    zm = zmul / cload(&st->rscale);
+    beta = (float)cload(&st->beta);
   }
   __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
     Cell c{0.0f, 1.0f};
     if (inside) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.p = phi[i];
-      if (c.p < 0.0f) c.d = Store<T>::ld(z, i) * zm + beta * dold[i];
+      c.p = ldg(phi, i);
+      if (c.p < 0.0f) c.d = Store<T>::gld(z, i) * zm + beta * ldg(dold, i);
     }
     return c;
   }
@@ -185,8 +187,8 @@ struct OpDirection {
         if (has[f]) qc += face_weight_fast(c.p, nb[f].p, g[f]) * (c.d - nb[f].d);
       acc[0] += (double)c.d * (double)qc;
     }
-    dnew[i] = c.d;
-    q[i] = qc;
+    stg(dnew, i, c.d);
+    stg(q, i, qc);
   }
 };
 
@@ -208,24 +210,24 @@ struct OpResidualW {
   int dyn;  // right-hand side scaled by the device rscale (finest level)
   float rm;
   const State* st;
-  static __device__ __forceinline__ OpResidualW make(const GridRec& G, const MarchLaunch& la) {
-    const LevelRec& l = G.lv[la.level];
+  static __device__ __forceinline__ OpResidualW make(const GridRec* Gp, const MarchLaunch& la) {
+    const LevelRec l = cload(&Gp->lv[la.level]);
     const bool top = la.level == 0;
     OpResidualW o;
     o.ny = l.d[1];
     o.nz = l.d[2];
     o.wts = (const W4<T>*)l.w;
-    o.r = (const TR*)(top ? (const void*)G.r : l.r);
+    o.r = (const TR*)(top ? (const void*)cload(&Gp->r) : l.r);
     o.e = (const T*)(la.which_e ? l.e2 : l.e);
     o.t = (T*)l.t;
     o.rmul = (top ? 1.0f : l.Sr) / l.St;
     o.emul = l.Se / l.St;
     o.tinv = 1.0f;
     o.dyn = top;
-    o.st = G.st;
+    o.st = cload(&Gp->st);
     return o;
   }
-  __device__ __forceinline__ void prepare() { rm = dyn ? rmul * st->rscale : rmul; }
+  __device__ __forceinline__ void prepare() { rm = dyn ? rmul * cload(&st->rscale) : rmul; }
   __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
     Cell c;
     c.e = 0.0f;
@@ -235,8 +237,8 @@ struct OpResidualW {
     Store<T>::st(&c.w.id, 0, 0.0f);
     if (inside) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.w = wts[i];
-      if (wf(c.w.id) > 0.0f) c.e = Store<T>::ld(e, i);
+      c.w = ldg(wts, i);
+      if (wf(c.w.id) > 0.0f) c.e = Store<T>::gld(e, i);
     }
     return c;
   }
@@ -248,9 +250,9 @@ struct OpResidualW {
       const float ap = wf(c.w.wx) * (c.e - nb[1].e) + wf(nb[0].w.wx) * (c.e - nb[0].e) +
                        wf(c.w.wy) * (c.e - nb[3].e) + wf(nb[2].w.wy) * (c.e - nb[2].e) +
                        wf(c.w.wz) * (c.e - nb[5].e) + wf(nb[4].w.wz) * (c.e - nb[4].e);
-      res = (Store<TR>::ld(r, i) * rm - emul * ap) * tinv;
+      res = (Store<TR>::gld(r, i) * rm - emul * ap) * tinv;
     }
-    Store<T>::st(t, i, res);
+    Store<T>::gst(t, i, res);
   }
 };
 
@@ -266,15 +268,15 @@ struct OpTrueResidual {
   const double* b;
   const double* xv;
   float* r;
-  static __device__ __forceinline__ OpTrueResidual make(const GridRec& G, const MarchLaunch&) {
-    const LevelRec& l0 = G.lv[0];
+  static __device__ __forceinline__ OpTrueResidual make(const GridRec* Gp, const MarchLaunch&) {
+    const LevelRec l0 = cload(&Gp->lv[0]);
     OpTrueResidual o;
     o.ny = l0.d[1];
     o.nz = l0.d[2];
     o.phi = l0.phi;
-    o.b = G.b;
-    o.xv = G.x;
-    o.r = G.r;
+    o.b = cload(&Gp->b);
+    o.xv = cload(&Gp->x);
+    o.r = cload(&Gp->r);
     return o;
   }
   __device__ __forceinline__ void prepare() {}
@@ -282,8 +284,8 @@ struct OpTrueResidual {
     Cell c{0.0, 1.0f};
     if (inside) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.p = phi[i];
-      if (c.p < 0.0f) c.x = xv[i];
+      c.p = ldg(phi, i);
+      if (c.p < 0.0f) c.x = ldg(xv, i);
     }
     return c;
   }
@@ -296,11 +298,11 @@ struct OpTrueResidual {
 #pragma unroll
       for (int f = 0; f < 6; ++f)
         if (has[f]) ap += face_weight_d(c.p, nb[f].p) * (c.x - nb[f].x);
-      res = b[i] - ap;
+      res = ldg(b, i) - ap;
       acc[0] += res * res;
       *mx = fmaxf(*mx, (float)fabs(res));
     }
-    r[i] = (float)res;
+    stg(r, i, (float)res);
   }
 };
 

@@ -100,12 +100,12 @@ __device__ __forceinline__ float wf(const T& v) { return Store<T>::ld(&v, 0); }
 
 // Bit-level views for branch-free masking of staged loads.
 template <typename T> struct W4Bits;
-template <> struct W4Bits<__half> { using type = uint2; };
-template <> struct W4Bits<__hip_bfloat16> { using type = uint2; };
-template <> struct W4Bits<float> { using type = uint4; };
+template <> struct W4Bits<__half> { typedef fp_u32x2 type; };
+template <> struct W4Bits<__hip_bfloat16> { typedef fp_u32x2 type; };
+template <> struct W4Bits<float> { typedef fp_u32x4 type; };
 
-__device__ __forceinline__ uint2 bits_unless(uint2 b, bool keep) { return keep ? b : make_uint2(0u, 0u); }
-__device__ __forceinline__ uint4 bits_unless(uint4 b, bool keep) { return keep ? b : make_uint4(0u, 0u, 0u, 0u); }
+__device__ __forceinline__ fp_u32x2 bits_unless(fp_u32x2 b, bool keep) { return keep ? b : fp_u32x2(0u); }
+__device__ __forceinline__ fp_u32x4 bits_unless(fp_u32x4 b, bool keep) { return keep ? b : fp_u32x4(0u); }
 
 template <typename T>
 __device__ __forceinline__ T value_unless(T v, bool keep) {
@@ -168,19 +168,19 @@ template <int TZ, int D, typename TR, typename T, bool PROLONG, bool ZERO>
 __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int level, int which_in, int first) {
   static_assert(D == 1 || D == 2 || D == 4, "the prefetch depth must divide the unroll of 4");
   FP_PART(parts, np);
-  const State* st = G.st;
-  if (st->done) return;
-  const LevelRec& lvl = G.lv[level];
+  const State* st = cload(&Gp->st);
+  if (cload(&st->done)) return;
+  const LevelRec lvl = cload(&Gp->lv[level]);
   const bool top = level == 0;
   const W4<T>* __restrict__ wts = (const W4<T>*)lvl.w;
-  const TR* __restrict__ r = (const TR*)(top ? (const void*)G.r : lvl.r);
+  const TR* __restrict__ r = (const TR*)(top ? (const void*)cload(&Gp->r) : lvl.r);
   const float rmul = (top ? 1.0f : lvl.Sr) / lvl.Se;
   const int dyn = top;
   const T* __restrict__ ein = (const T*)(which_in ? lvl.e2 : lvl.e);
   T* __restrict__ eout = (T*)(which_in ? lvl.e : lvl.e2);
   ProlongArgs P{nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
   if constexpr (PROLONG) {
-    const LevelRec& c = G.lv[level + 1];
+    const LevelRec c = cload(&Gp->lv[level + 1]);
     P = ProlongArgs{c.efinal ? c.e2 : c.e, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / lvl.Se};
   }
   const int chunk = lvl.sw.chunk, tiles_z = lvl.sw.tz, tiles_y = lvl.sw.ty;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   __shared__ T s_c[4][CN];
   __shared__ T s_p[PROLONG ? PN : 1];
 
-  const float rm = dyn ? rmul * st->rscale : rmul;
+  const float rm = dyn ? rmul * cload(&st->rscale) : rmul;
   const int nx = lvl.d[0], ny = lvl.d[1], nz = lvl.d[2];
   int tzi, tyi, cxi;
   tile_of_group(blk, tiles_z, tiles_y, nblk, tzi, tyi, cxi);
@@ -259,21 +259,21 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       const W4<T>* wp = wts + fo;
       const TR* rp = r + fo;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) pw[d][k] = *reinterpret_cast<const WB*>(wp + max(ow[k], 0));
+      for (int k = 0; k < NW; ++k) pw[d][k] = ldg(reinterpret_cast<const WB*>(wp), (size_t)max(ow[k], 0));
       if constexpr (!ZERO) {
         const T* ep = ein + fo;
 #pragma unroll
-        for (int k = 0; k < NE; ++k) pe[d][k] = ep[max(oe[k], 0)];
+        for (int k = 0; k < NE; ++k) pe[d][k] = ldg(ep, (size_t)max(oe[k], 0));
       }
 #pragma unroll
-      for (int k = 0; k < NR; ++k) pr[d][k] = rp[max(orr[k], 0)];
+      for (int k = 0; k < NR; ++k) pr[d][k] = ldg(rp, (size_t)max(orr[k], 0));
       if constexpr (PROLONG) {
         const int pa = coarse_plane0(lf);
         const T* ec = (const T*)P.ec;
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
           const int c = min(max(pa + opl[k], 0), P.cnx - 1);
-          pp[d][k] = ec[(long long)c * cplane + max(op[k], 0)];
+          pp[d][k] = ldg(ec, (size_t)((long long)c * cplane + max(op[k], 0)));
         }
       }
     }
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
         const float val = idg > 0.0f ? acc * idg : 0.0f;
         s_c[s1][ci] = to_store<T>(val);
         if (R >= x0 && R < x1 && c1st[h])
-          Store<T>::st(eout + oR, (size_t)(c1gy * nz + z0 + lz - 1), val);
+          Store<T>::gst(eout + oR, (size_t)(c1gy * nz + z0 + lz - 1), val);
       }
       // colour-2 lanes read the -x coefficient of the next step's plane B - 1
       // (plane L - 2) now: its slot is refilled at the next commit
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       acc += wf(wc.wz) * wf(cc[ci + 1]);
       acc += wf(w0[wi - 1].wz) * wf(cc[ci - 1]);
       const float val = idg > 0.0f ? acc * idg : 0.0f;
-      if (c2st[h]) Store<T>::st(eout + (oR - plane), (size_t)(c2gy * nz + z0 + lz), val);
+      if (c2st[h]) Store<T>::gst(eout + (oR - plane), (size_t)(c2gy * nz + z0 + lz), val);
     }
     wxm2 = wxm;
     oR += plane;

@@ -25,6 +25,7 @@
 #include "fp_march.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -533,9 +534,9 @@ __device__ __forceinline__ int ptaps(int f, int s, int nc, int* c, float* w) {
 template <typename T>
 __global__ void k_restrict(const Part* parts, int np, int level) {
   FP_PART(parts, np);
-  if (G.st->done) return;
-  const LevelRec& fl = G.lv[level];
-  const LevelRec& cl = G.lv[level + 1];
+  if (cload(&cload(&Gp->st)->done)) return;
+  const LevelRec fl = cload(&Gp->lv[level]);
+  const LevelRec cl = cload(&Gp->lv[level + 1]);
   const Grid3 f = grid_of(fl), c = grid_of(cl);
   const T* __restrict__ tf = (const T*)fl.t;
   T* __restrict__ rc = (T*)cl.r;
@@ -547,7 +548,7 @@ __global__ void k_restrict(const Part* parts, int np, int level) {
     int I, J, K;
     c.ijk(idx, I, J, K);
     float acc = 0.0f;
-    if (phic[idx] < 0.0f) {
+    if (ldg(phic, idx) < 0.0f) {
       int fi[4], fj[4], fk[4];
       float wi[4], wj[4], wk[4];
       const int ni = rtaps(I, sx, f.nx, c.nx, fi, wi);
@@ -557,11 +558,11 @@ __global__ void k_restrict(const Part* parts, int np, int level) {
         for (int b = 0; b < nj; ++b) {
           const float wab = wi[a] * wj[b];
           const size_t base = ((size_t)fi[a] * f.ny + fj[b]) * f.nz;
-          for (int q = 0; q < nk; ++q) acc += wab * wk[q] * ldT(tf, base + fk[q]);
+          for (int q = 0; q < nk; ++q) acc += wab * wk[q] * Store<T>::gld(tf, base + fk[q]);
         }
       acc *= fac;
     }
-    stT(rc, idx, acc);
+    Store<T>::gst(rc, idx, acc);
   }
 }
 
@@ -678,21 +679,37 @@ __global__ void k_build_weights(const Part* parts, int np, int level) {
 // CG kernels (FP32 vectors, FP64 scalars)
 // --------------------------------------------------------------------------
 // gamma = r . z (z: the finest level's correction, stored scale Se / rscale).
+__device__ __forceinline__ fp_f32x4 ldg4T(const float* p, size_t i4) { return ldg4(p, i4); }
+__device__ __forceinline__ fp_f32x4 ldg4T(const __half* p, size_t i4) {
+  const fp_u32x2 u = ldg((const fp_u32x2*)p, i4);
+  const unsigned ux = u.x, uy = u.y;
+  __half2 a, b;
+  __builtin_memcpy(&a, &ux, 4);
+  __builtin_memcpy(&b, &uy, 4);
+  const float2 fa = __half22float2(a), fb = __half22float2(b);
+  return fp_f32x4{fa.x, fa.y, fb.x, fb.y};
+}
+__device__ __forceinline__ fp_f32x4 ldg4T(const __hip_bfloat16* p, size_t i4) {
+  const fp_u32x2 u = ldg((const fp_u32x2*)p, i4);
+  return fp_f32x4{__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u), __uint_as_float(u.y << 16),
+                  __uint_as_float(u.y & 0xffff0000u)};
+}
+
 template <typename T>
 __global__ void k_cg_rz(const Part* parts, int np) {
   FP_PART(parts, np);
-  const State* st = G.st;
-  if (st->done) return;
-  const LevelRec& l0 = G.lv[0];
+  const State* st = cload(&Gp->st);
+  if (cload(&st->done)) return;
+  const LevelRec l0 = cload(&Gp->lv[0]);
   const T* __restrict__ z = (const T*)(l0.efinal ? l0.e2 : l0.e);
-  const float* __restrict__ r = G.r;
-  const float zm = l0.Se / st->rscale;
-  const size_t n = (size_t)G.n;
+  const float* __restrict__ r = cload(&Gp->r);
+  const float zm = l0.Se / cload(&st->rscale);
+  const size_t n = (size_t)cload(&Gp->n);
   double acc[1] = {0.0};
   const size_t n4 = n / 4;
   FP_STRIDE(i, n4) {
-    const float4 rv = reinterpret_cast<const float4*>(r)[i];
-    const float4 zv = V4<T>::ld(z, i);
+    const fp_f32x4 rv = ldg4(r, i);
+    const fp_f32x4 zv = ldg4T(z, i);
     acc[0] += ((double)rv.x * (double)(zv.x * zm) + (double)rv.y * (double)(zv.y * zm)) +
               ((double)rv.z * (double)(zv.z * zm) + (double)rv.w * (double)(zv.w * zm));
   }
@@ -700,7 +717,7 @@ __global__ void k_cg_rz(const Part* parts, int np) {
     const size_t i = n4 * 4 + threadIdx.x;
     acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
   }
-  block_partials<1>(acc, G.partials, blk, nblk);
+  block_partials<1>(acc, cload(&Gp->partials), blk, nblk);
 }
 
 __global__ void k_fin_rz(const Part* parts) {
@@ -742,26 +759,26 @@ __global__ void k_fin_dq(const Part* parts) {
 // d: the direction this iteration wrote (buffer (iter & 1) ^ 1).
 __global__ void k_cg_update(const Part* parts, int np) {
   FP_PART(parts, np);
-  const State* st = G.st;
-  if (st->done) return;
-  const float alpha = (float)st->alpha;
-  const float* __restrict__ d = G.d[(st->iter & 1) ^ 1];
-  const float* __restrict__ q = G.q;
-  float* __restrict__ e = G.e;
-  float* __restrict__ r = G.r;
-  const size_t n = (size_t)G.n;
+  const State* st = cload(&Gp->st);
+  if (cload(&st->done)) return;
+  const float alpha = (float)cload(&st->alpha);
+  const float* __restrict__ d = cload(&Gp->d[(cload(&st->iter) & 1) ^ 1]);
+  const float* __restrict__ q = cload(&Gp->q);
+  float* __restrict__ e = cload(&Gp->e);
+  float* __restrict__ r = cload(&Gp->r);
+  const size_t n = (size_t)cload(&Gp->n);
   double acc[1] = {0.0};
   float rmax = 0.0f;
   const size_t n4 = n / 4;
   FP_STRIDE(i, n4) {
-    const float4 dv = reinterpret_cast<const float4*>(d)[i];
-    const float4 qv = reinterpret_cast<const float4*>(q)[i];
-    float4 ev = reinterpret_cast<const float4*>(e)[i];
-    float4 rv = reinterpret_cast<const float4*>(r)[i];
+    const fp_f32x4 dv = ldg4(d, i);
+    const fp_f32x4 qv = ldg4(q, i);
+    fp_f32x4 ev = ldg4(e, i);
+    fp_f32x4 rv = ldg4(r, i);
     ev.x += alpha * dv.x; ev.y += alpha * dv.y; ev.z += alpha * dv.z; ev.w += alpha * dv.w;
     rv.x -= alpha * qv.x; rv.y -= alpha * qv.y; rv.z -= alpha * qv.z; rv.w -= alpha * qv.w;
-    reinterpret_cast<float4*>(e)[i] = ev;
-    reinterpret_cast<float4*>(r)[i] = rv;
+    stg4(e, i, ev);
+    stg4(r, i, rv);
     acc[0] += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
     rmax = fmaxf(rmax, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
   }
@@ -773,7 +790,7 @@ __global__ void k_cg_update(const Part* parts, int np) {
     acc[0] += (double)rn * (double)rn;
     rmax = fmaxf(rmax, fabsf(rn));
   }
-  block_partials_max<1>(acc, rmax, G.partials, blk, nblk);
+  block_partials_max<1>(acc, rmax, cload(&Gp->partials), blk, nblk);
 }
 
 __global__ void k_fin_update(const Part* parts) {

@@ -120,15 +120,16 @@ __device__ __forceinline__ int part_index(const Part* parts, int n) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (parts[mid].block0 <= b) lo = mid;
+    if (cload(&parts[mid].block0) <= b) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-#define FP_PART(parts, np)                           \
-  const Part fp_part_ = (parts)[part_index(parts, np)]; \
-  const GridRec& G = *fp_part_.g;                    \
+#define FP_PART(parts, np)                                \
+  const Part fp_part_ = cload((parts) + part_index(parts, np)); \
+  const GridRec* Gp = fp_part_.g;                         \
+  const GridRec& G = *Gp;                                 \
   const int blk = (int)blockIdx.x - fp_part_.block0; \
   const int nblk = fp_part_.nblocks;                 \
   if (blk >= nblk) return;

@@ -250,3 +250,20 @@ def test_graph_replay_matches_plain_launches(fpl, c0, precision):
         assert np.array_equal(gr.pressure, plain.pressure)
         assert np.array_equal(gr.u, plain.u) and np.array_equal(gr.w, plain.w)
     g.destroy()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_iteration_counts_stay_multigrid_like(fpl, precision):
+    """The preconditioner must keep MGPCG's grid-independent iteration counts:
+    a broken storage path still converges (CG is robust) but needs tens of
+    times more iterations, which parity checks alone do not see."""
+    counts = {}
+    for n in (32, 64, 96):
+        s = scenes.dam_spheres_scene(n, seed=1)
+        g = fpl.UniformPressureProjection(s.shape, s.h)
+        res = g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision=precision, relative_tolerance=1e-6)
+        g.destroy()
+        assert res.report["flags"] == 0
+        counts[n] = res.report["iterations"]
+    print(precision, counts)
+    assert max(counts.values()) <= 14, counts
