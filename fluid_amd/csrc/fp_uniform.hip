@@ -534,9 +534,9 @@ __device__ __forceinline__ int ptaps(int f, int s, int nc, int* c, float* w) {
 template <typename T>
 __global__ void k_restrict(const Part* parts, int np, int level) {
   FP_PART(parts, np);
-  if (cload(&cload(&Gp->st)->done)) return;
-  const LevelRec fl = cload(&Gp->lv[level]);
-  const LevelRec cl = cload(&Gp->lv[level + 1]);
+  if (G.st->done) return;
+  const LevelRec& fl = G.lv[level];
+  const LevelRec& cl = G.lv[level + 1];
   const Grid3 f = grid_of(fl), c = grid_of(cl);
   const T* __restrict__ tf = (const T*)fl.t;
   T* __restrict__ rc = (T*)cl.r;
@@ -548,7 +548,7 @@ __global__ void k_restrict(const Part* parts, int np, int level) {
     int I, J, K;
     c.ijk(idx, I, J, K);
     float acc = 0.0f;
-    if (ldg(phic, idx) < 0.0f) {
+    if (phic[idx] < 0.0f) {
       int fi[4], fj[4], fk[4];
       float wi[4], wj[4], wk[4];
       const int ni = rtaps(I, sx, f.nx, c.nx, fi, wi);
@@ -558,11 +558,11 @@ __global__ void k_restrict(const Part* parts, int np, int level) {
         for (int b = 0; b < nj; ++b) {
           const float wab = wi[a] * wj[b];
           const size_t base = ((size_t)fi[a] * f.ny + fj[b]) * f.nz;
-          for (int q = 0; q < nk; ++q) acc += wab * wk[q] * Store<T>::gld(tf, base + fk[q]);
+          for (int q = 0; q < nk; ++q) acc += wab * wk[q] * ldT(tf, base + fk[q]);
         }
       acc *= fac;
     }
-    Store<T>::gst(rc, idx, acc);
+    stT(rc, idx, acc);
   }
 }
 
