@@ -170,3 +170,208 @@ def batch_scenes(count: int = 96, seed: int = 0, lo: int = 32, hi: int = 192, ma
         scenes.append(UniformScene(phi, u, v, w, 1.0 / n, 1.0 / 120.0, name=f"batch{g}",
                                    meta={"n": n, "fill": fill, "index": g}))
     return scenes
+
+
+# --------------------------------------------------------------------------
+# Restricted tall-cell scenes
+# --------------------------------------------------------------------------
+@dataclass
+class TallScene:
+    """Inputs of one tall-cell projection (see fluid_amd.tall_layout)."""
+
+    layout: object   # TallCellLayout
+    phi: np.ndarray  # (nx, P, nz) float32 samples
+    u: np.ndarray    # (nx+1, S, nz) float32 slots
+    v: np.ndarray    # (nx, P, nz)
+    w: np.ndarray    # (nx, S, nz+1)
+    dt: float
+    rho: float = RHO_WATER
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def h(self):
+        return self.layout.h
+
+    @property
+    def samples(self) -> int:
+        return int(self.phi.size)
+
+
+def tall_row_phi(layout, phi, y):
+    """phi of every column at fine row(s) y ((nx, nz) or broadcastable):
+    linear between the tall samples below the base, the band sample inside
+    the band, +5h above it (representedInitialPhiVolumeCells,
+    lib/core/tall-cell-grid.ts:228-238)."""
+    b = np.asarray(layout.bases, np.int64)
+    y = np.broadcast_to(np.asarray(y, np.int64), b.shape)
+    bot = phi[:, 0, :].astype(np.float64)
+    top = phi[:, 1, :].astype(np.float64)
+    k = y / np.maximum(b - 1, 1).astype(np.float64)
+    tall = bot + (top - bot) * k
+    py = np.clip(y - b + 2, 2, layout.packed_ny - 1)
+    band = np.take_along_axis(phi, py[:, None, :], axis=1)[:, 0, :].astype(np.float64)
+    in_band = (y >= b) & (y - b < layout.band) & (y < layout.fine_ny)
+    out = np.where(y < b, tall, np.where(in_band, band, 5.0 * layout.h))
+    return out
+
+
+def tall_fluid_slots(layout, phi):
+    """Slots with an interior face that has a liquid side."""
+    from . import tall_layout as tl
+    mu, mv, mw = layout.slot_masks()
+    b = np.asarray(layout.bases, np.int64)
+    liq = phi < 0
+    fv = np.zeros_like(mv)
+    fv[:, 0, :] = liq[:, 0, :] | liq[:, 1, :]
+    fv[:, 1, :] = liq[:, 1, :] | liq[:, 2, :] if layout.packed_ny > 2 else liq[:, 1, :]
+    for s in range(2, layout.band + 2):
+        upper = liq[:, s + 1, :] if s + 1 < layout.packed_ny else np.zeros_like(liq[:, s, :])
+        fv[:, s, :] = liq[:, s, :] | upper
+    fu = np.zeros_like(mu)
+    fw = np.zeros_like(mw)
+    for arr, sl_a, sl_n, ax in ((fu, (slice(0, -1), slice(None)), (slice(1, None), slice(None)), 0),
+                                (fw, (slice(None), slice(0, -1)), (slice(None), slice(1, None)), 2)):
+        ba, bn = b[sl_a], b[sl_n]
+        m = np.minimum(ba, bn)
+        tall_liq = (liq[:, 0, :] | liq[:, 1, :])
+        for s in range(layout.slots):
+            if s < 2:
+                f = tall_liq[sl_a] | tall_liq[sl_n]
+            else:
+                y = m + s - 2
+                pa = tall_row_phi(layout, phi, np.pad(y, ((0, 1), (0, 0)) if ax == 0 else ((0, 0), (0, 1)),
+                                                      constant_values=0))[sl_a]
+                pn = tall_row_phi(layout, phi, np.pad(y, ((1, 0), (0, 0)) if ax == 0 else ((0, 0), (1, 0)),
+                                                      constant_values=0))[sl_n]
+                f = (pa < 0) | (pn < 0)
+            if ax == 0:
+                arr[1:-1, s, :] = f
+            else:
+                arr[:, s, 1:-1] = f
+    return fu & mu, fv & mv, fw & mw
+
+
+def _tall_velocities(rng, layout, phi, dt, noise: str, sigma: float):
+    """Slot velocities: noise on fluid slots (slope slots per fine row, i.e.
+    scaled by 2/m so the end rows deviate by about sigma), gravity on v."""
+    fu, fv, fw = tall_fluid_slots(layout, phi)
+    b = np.asarray(layout.bases, np.float64)
+    out = []
+    for shape, mask, grav, ax in ((fu.shape, fu, 0.0, 0), (fv.shape, fv, -GRAVITY * dt, 1),
+                                  (fw.shape, fw, 0.0, 2)):
+        if noise == "uniform":
+            a = rng.uniform(-sigma, sigma, size=shape).astype(np.float32)
+        else:
+            a = rng.standard_normal(size=shape, dtype=np.float32) * np.float32(sigma)
+        if ax == 0:
+            m = np.minimum(b[:-1], b[1:])
+            a[1:-1, 1, :] *= (2.0 / m).astype(np.float32)
+        elif ax == 2:
+            m = np.minimum(b[:, :-1], b[:, 1:])
+            a[:, 1, 1:-1] *= (2.0 / m).astype(np.float32)
+        else:
+            a += np.float32(grav)
+        a[~mask] = 0.0
+        out.append(np.ascontiguousarray(a))
+    return out
+
+
+def tall_wave_scene(n: int = 512, fine_ny: int = 512, band: int = 32, seed: int = 0, dt: float = 1.0 / 120.0,
+                    sigma: float = 0.3, max_delta: int = 4) -> TallScene:
+    """Restricted tall-cell grid with a wavy free surface.
+
+    Domain 1 m tall, n x n columns of h = 1/fine_ny.  Surface height
+    s(x, z) = 0.6 + 0.05 sin(2 pi 3 x) cos(2 pi 2 z) + U[-0.01, 0.01] at the
+    column centres (noise drawn first from PCG64(seed), shape (n, n)); liquid
+    below.  phi = (sample height - s), clamped to +-5h (buildInitialPhi's
+    narrow band, lib/core/tall-cell-grid.ts:171-190).  Bases: the surface
+    cells of each column (last wet row and the dry row above it,
+    isInitialSurfaceCell, tall-cell-grid.ts:296-307) through choose_base with
+    regularLayers = band, halos 16/8, then limit_bases with delta 4.
+    Velocities: N(0, sigma^2) on fluid slots, u, v, w in that order, plus the
+    gravity impulse on v.
+    """
+    from . import tall_layout as tl
+    rng = _rng(seed)
+    h = 1.0 / fine_ny
+    c = (np.arange(n, dtype=np.float64) + 0.5) / n
+    noise = rng.uniform(-0.01, 0.01, size=(n, n))
+    surf = 0.6 + 0.05 * np.sin(2 * np.pi * 3 * c)[:, None] * np.cos(2 * np.pi * 2 * c)[None, :] + noise
+    # last wet row J: (J + 0.5) h <= s
+    J = np.floor(surf / h - 0.5).astype(np.int64)
+    J = np.clip(J, -1, fine_ny - 1)
+    lowest = np.where(J >= 0, J, J + 1)
+    highest = np.where(J + 1 < fine_ny, J + 1, J)
+    raw = tl.choose_base(lowest, highest, fine_ny, band, tl.LIQUID_HALO, tl.AIR_HALO)
+    bases = tl.limit_bases(raw, max_delta)
+    layout = tl.TallCellLayout(bases, fine_ny, band, h, max_delta)
+    yc = layout.sample_heights() * h
+    phi = np.clip(yc - surf[:, None, :], -5 * h, 5 * h)
+    phi = np.where(layout.active(), phi, 5 * h).astype(np.float32)
+    u, v, w = _tall_velocities(rng, layout, phi, dt, "gaussian", sigma)
+    return TallScene(layout, phi, u, v, w, dt, name="tall_wave", meta={"n": n, "fine_ny": fine_ny, "band": band,
+                                                                       "seed": seed})
+
+
+def tall_dam_scene(n: int = 256, fine_ny: int | None = None, band: int = 32, seed: int = 0,
+                   fill=(0.4, 0.6, 0.4), dt: float = 1.0 / 120.0, noise: float = 0.25,
+                   max_delta: int = 4) -> TallScene:
+    """Corner dam break on a restricted tall-cell grid (n x n columns).
+
+    Liquid box [0, fill] in the unit cube (h = 1/fine_ny vertically, 1/n
+    across); phi is the box signed distance at the sample centres clamped
+    to +-5h.  Wet columns take choose_base of their surface rows; dry
+    columns take the maximum base (tall-cell-grid.ts:380-389) before the
+    limiter.  Velocities U[-noise, noise] on fluid slots plus gravity on v,
+    drawn from PCG64(seed).
+    """
+    from . import tall_layout as tl
+    rng = _rng(seed)
+    ny = fine_ny or n
+    h = 1.0 / ny
+    c = (np.arange(n, dtype=np.float64) + 0.5) / n
+    inside = (c[:, None] <= fill[0]) & (c[None, :] <= fill[2])
+    J = np.floor(fill[1] * ny - 0.5).astype(np.int64)  # last wet row: (J + 0.5) h <= fill_y
+    J = min(max(J, -1), ny - 1)
+    max_base = max(0, ny - band)
+    wet_base = tl.choose_base(np.full((n, n), J), np.full((n, n), min(J + 1, ny - 1)), ny, band,
+                              tl.LIQUID_HALO, tl.AIR_HALO)
+    dry_base = min(2, max_base) if max_base <= 3 else max_base
+    raw = np.where(inside & (J >= 0), wet_base, max(2, dry_base)).astype(np.int32)
+    bases = tl.limit_bases(raw, max_delta)
+    layout = tl.TallCellLayout(bases, ny, band, h, max_delta)
+    yc = layout.sample_heights() * h
+    px = c[:, None, None]
+    pz = c[None, None, :]
+    phi = box_sdf(px, yc, pz, (0.0, 0.0, 0.0), tuple(fill))
+    phi = np.where(layout.active(), np.clip(phi, -5 * h, 5 * h), 5 * h).astype(np.float32)
+    u, v, w = _tall_velocities(rng, layout, phi, dt, "uniform", noise)
+    return TallScene(layout, phi, u, v, w, dt, name="tall_dam", meta={"n": n, "fine_ny": ny, "band": band,
+                                                                      "fill": tuple(fill), "seed": seed})
+
+
+def tall_random_scene(nx: int, nz: int, fine_ny: int, band: int, seed: int = 0, max_delta: int = 4,
+                      surface=(0.3, 0.7), deep_mixed: bool = False, dt: float = 1.0 / 120.0) -> TallScene:
+    """Small random layouts for the oracle and parity tests: a random rough
+    surface per column (liquid below), bases through choose_base with halos
+    scaled to the band, optional air pockets, Gaussian slot velocities."""
+    from . import tall_layout as tl
+    rng = _rng(seed)
+    h = 1.0 / fine_ny
+    surf = rng.uniform(surface[0], surface[1], size=(nx, nz))
+    J = np.clip(np.floor(surf / h - 0.5).astype(np.int64), 0, fine_ny - 1)
+    lh, ah = max(1, band // 3), max(1, band // 4)
+    raw = tl.choose_base(J, np.minimum(J + 1, fine_ny - 1), fine_ny, band, lh, ah)
+    raw = np.maximum(raw, 2)
+    bases = tl.limit_bases(raw, max_delta)
+    layout = tl.TallCellLayout(bases, fine_ny, band, h, max_delta)
+    yc = layout.sample_heights() * h
+    phi = yc - surf[:, None, :] + rng.uniform(-0.3, 0.3, size=(nx, band + 2, nz)) * h
+    if deep_mixed:
+        # flip some tall tops to air: mixed tall cells deeper than two rows
+        flip = rng.uniform(size=(nx, nz)) < 0.2
+        phi[:, 1, :] = np.where(flip, np.abs(phi[:, 1, :]) + 0.5 * h, phi[:, 1, :])
+    phi = np.where(layout.active(), np.clip(phi, -5 * h, 5 * h), 5 * h).astype(np.float32)
+    u, v, w = _tall_velocities(rng, layout, phi, dt, "gaussian", 0.3)
+    return TallScene(layout, phi, u, v, w, dt, name="tall_random", meta={"seed": seed})

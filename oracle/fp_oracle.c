@@ -982,3 +982,403 @@ void orc_tall_limit_bases(int32_t* bases, int nx, int nz, int max_delta, int pas
   }
   free(next);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Restricted tall-cell projection                                          */
+/* ------------------------------------------------------------------------ */
+/* The tall-cell pressure space is p_fine = E p.  Along a column (x, z) of
+ * base b the fine rows 0..b-1 interpolate linearly between the bottom sample
+ * (row 0) and the top sample (row b-1) -- the represented-field rule of
+ * representedInitialPhiVolumeCells, tall-cell-grid.ts:206-249, applied to phi
+ * and p alike -- rows b..b+B-1 are the cubic band samples, and rows above the
+ * band are air (phi = +5h, the narrow-band clamp of buildInitialPhi,
+ * tall-cell-grid.ts:171-190; p = 0).
+ *
+ * Face velocities live in slots.  v: slot 0 is the one velocity of the b-1
+ * faces inside a tall cell, slot 1 the face tall top -> band row 0, slot 1+k
+ * the face band k-1 -> k, slot B+1 the face band top -> air (the lid when the
+ * band reaches it).  u and w: slots 0 and 1 are the mean and the slope (per
+ * fine row, about row (m-1)/2) of an affine velocity over rows 0..m-1, m the
+ * smaller base of the two columns; slot 2+j is the single fine row m+j.
+ *
+ * The operator, right-hand side and gradient subtract are the variational
+ * reduction of the uniform ghost-fluid projection on the expanded fine grid
+ * (docs/papers/sg2025narita.md:120-172; the restricted form of
+ * tallCells.txt Eq. 13-18): A = E^T A_u E, b = E^T b_u(expanded u), and the
+ * subtracted fine velocities are projected back onto the slots by least
+ * squares (identity for single-row slots, mean for v slot 0, mean and slope
+ * for u/w slots 0-1).  Because each pressure coefficient c(y) is affine in y
+ * over a slot's rows, E^T D of the projected field equals E^T D of the fine
+ * field, so the reduced divergence vanishes exactly where the fine one does.
+ * This restatement walks every fine face; the device kernels evaluate the
+ * same sums in closed form from per-face weight moments. */
+#define TALL_P(t) ((t)->band + 2)
+#define TALL_S(t) ((t)->band + 2 + (t)->max_delta)
+
+enum { TALL_ROLE_ONE = 0, TALL_ROLE_MEAN = 1, TALL_ROLE_AFFINE = 2 };
+
+typedef struct {
+  int n;           /* g = sum_q c[q] p[idx[q]] (p of air samples = 0) */
+  size_t idx[4];
+  double c[4];
+  double w;        /* ghost-fluid weight of the fine face */
+  int array;       /* 0 u, 1 v, 2 w */
+  int role;
+  size_t slot, slot1; /* AFFINE: mean slot, slope slot */
+  double t;        /* AFFINE: row - (m-1)/2 */
+  double count;    /* MEAN: rows sharing the slot; AFFINE: m */
+  double q;        /* AFFINE: sum of t^2 over the m rows */
+} tall_face;
+
+static size_t tall_sample(const orc_tall* t, int x, int py, int z) {
+  return ((size_t)x * TALL_P(t) + py) * t->nz + z;
+}
+
+static int tall_base(const orc_tall* t, int x, int z) { return t->bases[(size_t)x * t->nz + z]; }
+
+/* phi of fine cell (x, y, z) and p there as sum c[q] p[idx[q]]. */
+static double tall_cell(const orc_tall* t, int x, int z, int y, int* nd, size_t* idx, double* c) {
+  const int b = tall_base(t, x, z);
+  *nd = 0;
+  if (y < b) {
+    const double k = y / (double)(b - 1);
+    const double pb = t->phi[tall_sample(t, x, 0, z)], pt = t->phi[tall_sample(t, x, 1, z)];
+    if (y == 0 || y == b - 1) {
+      idx[0] = tall_sample(t, x, y == 0 ? 0 : 1, z);
+      c[0] = 1.0;
+      *nd = 1;
+      return y == 0 ? pb : pt;
+    }
+    idx[0] = tall_sample(t, x, 0, z);
+    c[0] = 1.0 - k;
+    idx[1] = tall_sample(t, x, 1, z);
+    c[1] = k;
+    *nd = 2;
+    return pb + (pt - pb) * k;
+  }
+  if (y - b < t->band && y < t->ny) {
+    idx[0] = tall_sample(t, x, 2 + y - b, z);
+    c[0] = 1.0;
+    *nd = 1;
+    return t->phi[idx[0]];
+  }
+  return 5.0 * t->h;
+}
+
+typedef void (*tall_face_cb)(void* ctx, const tall_face* f);
+
+/* The fine face between cells a (-) and b (+); walls (n = 0) are skipped. */
+static void tall_emit(const orc_tall* t, tall_face* f, int xa, int za, int xb, int zb, int ya, int yb,
+                      tall_face_cb cb, void* ctx) {
+  int na, nb;
+  size_t ia[2], ib[2];
+  double ca[2], cbv[2];
+  const double phia = tall_cell(t, xa, za, ya, &na, ia, ca);
+  const double phib = tall_cell(t, xb, zb, yb, &nb, ib, cbv);
+  f->n = 0;
+  for (int q = 0; q < nb; ++q) { f->idx[f->n] = ib[q]; f->c[f->n] = cbv[q]; ++f->n; }
+  for (int q = 0; q < na; ++q) {
+    int merged = 0; /* a face inside one tall cell names its samples twice */
+    for (int r = 0; r < f->n; ++r)
+      if (f->idx[r] == ia[q]) { f->c[r] -= ca[q]; merged = 1; }
+    if (!merged) { f->idx[f->n] = ia[q]; f->c[f->n] = -ca[q]; ++f->n; }
+  }
+  f->w = face_weight(phia, phib, 1.0);
+  cb(ctx, f);
+}
+
+/* Every fine face of the domain interior (walls carry no flux) once. */
+static int tall_faces(const orc_tall* t, tall_face_cb cb, void* ctx) {
+  const int nx = t->nx, ny = t->ny, nz = t->nz, B = t->band, P = TALL_P(t), S = TALL_S(t);
+  for (size_t i = 0; i < (size_t)nx * nz; ++i)
+    if (t->bases[i] < 2 || t->bases[i] > ny) return ORC_TALL_BAD_BASE;
+  for (int x = 0; x < nx; ++x)
+    for (int z = 0; z < nz; ++z) {
+      const int b = tall_base(t, x, z);
+      tall_face f;
+      memset(&f, 0, sizeof f);
+      /* vertical faces y -> y+1 below the lid and below the band top + 1 */
+      f.array = 1;
+      for (int y = 0; y + 1 < ny && y < b + B; ++y) {
+        if (y < b - 1) {
+          f.role = TALL_ROLE_MEAN;
+          f.slot = ((size_t)x * P + 0) * nz + z;
+          f.count = b - 1;
+        } else {
+          f.role = TALL_ROLE_ONE;
+          f.slot = ((size_t)x * P + (y - b + 2)) * nz + z;
+        }
+        tall_emit(t, &f, x, z, x, z, y, y + 1, cb, ctx);
+      }
+      /* horizontal faces to the +x and +z neighbours */
+      for (int dir = 0; dir < 2; ++dir) {
+        const int xn = x + (dir == 0), zn = z + (dir == 1);
+        if (xn >= nx || zn >= nz) continue;
+        const int bn = tall_base(t, xn, zn);
+        if (abs(bn - b) > t->max_delta) return ORC_TALL_BAD_BASE;
+        const int m = b < bn ? b : bn, M = b > bn ? b : bn;
+        const size_t slot0 = dir == 0 ? ((size_t)(x + 1) * S) * nz + z : ((size_t)x * S) * (nz + 1) + (z + 1);
+        const size_t sstride = dir == 0 ? (size_t)nz : (size_t)(nz + 1);
+        int ytop = M + B;
+        if (ytop > ny) ytop = ny;
+        f.array = dir == 0 ? 0 : 2;
+        for (int y = 0; y < ytop; ++y) {
+          if (y < m) {
+            f.role = TALL_ROLE_AFFINE;
+            f.slot = slot0;
+            f.slot1 = slot0 + sstride;
+            f.t = y - 0.5 * (m - 1);
+            f.count = m;
+            f.q = m * ((double)m * m - 1.0) / 12.0;
+          } else {
+            f.role = TALL_ROLE_ONE;
+            f.slot = slot0 + (size_t)(2 + y - m) * sstride;
+          }
+          tall_emit(t, &f, x, z, xn, zn, y, y, cb, ctx);
+        }
+      }
+    }
+  return 0;
+}
+
+static double tall_face_velocity(const tall_face* f, const double* U) {
+  if (f->role == TALL_ROLE_AFFINE) return U[f->slot] + f->t * U[f->slot1];
+  return U[f->slot];
+}
+
+int orc_tall_check(const orc_tall* t) {
+  for (int x = 0; x < t->nx; ++x)
+    for (int z = 0; z < t->nz; ++z) {
+      const int b = tall_base(t, x, z);
+      if (b < 2 || b > t->ny) return ORC_TALL_BAD_BASE;
+      if (x + 1 < t->nx && abs(tall_base(t, x + 1, z) - b) > t->max_delta) return ORC_TALL_BAD_BASE;
+      if (z + 1 < t->nz && abs(tall_base(t, x, z + 1) - b) > t->max_delta) return ORC_TALL_BAD_BASE;
+    }
+  return 0;
+}
+
+/* Unknown samples: liquid, inside the domain. */
+void orc_tall_unknowns(const orc_tall* t, uint8_t* mask) {
+  const int P = TALL_P(t);
+  for (int x = 0; x < t->nx; ++x)
+    for (int z = 0; z < t->nz; ++z) {
+      const int b = tall_base(t, x, z);
+      for (int py = 0; py < P; ++py) {
+        const size_t i = tall_sample(t, x, py, z);
+        const int active = py < 2 ? 1 : (b + py - 2 < t->ny);
+        mask[i] = active && t->phi[i] < 0.0f;
+      }
+    }
+}
+
+static size_t tall_samples(const orc_tall* t) { return (size_t)t->nx * TALL_P(t) * t->nz; }
+
+typedef struct {
+  const double* p;
+  double* out;
+  const uint8_t* mask;
+} tall_apply_ctx;
+
+static double tall_grad(const tall_face* f, const double* p, const uint8_t* mask) {
+  double g = 0.0;
+  for (int q = 0; q < f->n; ++q)
+    if (mask[f->idx[q]]) g += f->c[q] * p[f->idx[q]];
+  return g;
+}
+
+static void tall_apply_cb(void* vctx, const tall_face* f) {
+  tall_apply_ctx* c = (tall_apply_ctx*)vctx;
+  if (f->w == 0.0) return;
+  const double s = f->w * tall_grad(f, c->p, c->mask);
+  for (int q = 0; q < f->n; ++q)
+    if (c->mask[f->idx[q]]) c->out[f->idx[q]] += s * f->c[q];
+}
+
+int orc_tall_apply(const orc_tall* t, const double* p, double* out) {
+  const size_t n = tall_samples(t);
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  memset(out, 0, n * sizeof(double));
+  tall_apply_ctx c = {p, out, mask};
+  const int rc = tall_faces(t, tall_apply_cb, &c);
+  free(mask);
+  return rc;
+}
+
+typedef struct {
+  const double* U[3];
+  double* b;
+  const uint8_t* mask;
+  double s;
+} tall_rhs_ctx;
+
+static void tall_rhs_cb(void* vctx, const tall_face* f) {
+  tall_rhs_ctx* c = (tall_rhs_ctx*)vctx;
+  const double u = tall_face_velocity(f, c->U[f->array]);
+  for (int q = 0; q < f->n; ++q)
+    if (c->mask[f->idx[q]]) c->b[f->idx[q]] += c->s * f->c[q] * u;
+}
+
+/* b = (rho h / dt) E^T (-div u_fine) on unknown samples; the uniform rhs
+ * (orc_rhs) of the expanded velocity, gathered through E^T. */
+int orc_tall_rhs(const orc_tall* t, const double* u, const double* v, const double* w, double dt, double rho,
+                 double* b) {
+  const size_t n = tall_samples(t);
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  memset(b, 0, n * sizeof(double));
+  tall_rhs_ctx c = {{u, v, w}, b, mask, rho * t->h / dt};
+  const int rc = tall_faces(t, tall_rhs_cb, &c);
+  free(mask);
+  return rc;
+}
+
+typedef struct {
+  const double* p;
+  const double* U[3];
+  double* O[3];
+  uint8_t* live[3];
+  const uint8_t* mask;
+  double s;
+} tall_sub_ctx;
+
+static void tall_sub_cb(void* vctx, const tall_face* f) {
+  tall_sub_ctx* c = (tall_sub_ctx*)vctx;
+  double u = tall_face_velocity(f, c->U[f->array]);
+  int live = f->w != 0.0;
+  if (live) u -= c->s * f->w * tall_grad(f, c->p, c->mask);
+  for (int q = 0; q < f->n; ++q) live |= c->mask[f->idx[q]] && f->c[q] != 0.0;
+  double* o = c->O[f->array];
+  if (live) c->live[f->array][f->slot] = 1;
+  switch (f->role) {
+    case TALL_ROLE_ONE: o[f->slot] = u; break;
+    case TALL_ROLE_MEAN: o[f->slot] += u / f->count; break;
+    default:
+      o[f->slot] += u / f->count;
+      o[f->slot1] += u * f->t / f->q;
+      if (live) c->live[f->array][f->slot1] = 1;
+  }
+}
+
+/* Fine velocities u - dt/(rho h) w grad p, projected back onto the slots.  A
+ * slot none of whose faces has a liquid side or an unknown sample in its
+ * expansion is set to 0 (the faces orc_project zeroes); unused slots are 0.
+ * Faces without a liquid side keep their velocity inside a live slot, so the
+ * reduced divergence of the result is exactly b - A p. */
+int orc_tall_subtract(const orc_tall* t, const double* p, const double* u, const double* v, const double* w,
+                      double dt, double rho, double* uo, double* vo, double* wo) {
+  const size_t n = tall_samples(t);
+  const size_t nu = (size_t)(t->nx + 1) * TALL_S(t) * t->nz, nw = (size_t)t->nx * TALL_S(t) * (t->nz + 1);
+  memset(uo, 0, nu * sizeof(double));
+  memset(vo, 0, n * sizeof(double));
+  memset(wo, 0, nw * sizeof(double));
+  uint8_t* mask = (uint8_t*)malloc(n);
+  uint8_t* lu = (uint8_t*)calloc(nu, 1);
+  uint8_t* lv = (uint8_t*)calloc(n, 1);
+  uint8_t* lw = (uint8_t*)calloc(nw, 1);
+  orc_tall_unknowns(t, mask);
+  tall_sub_ctx c = {p, {u, v, w}, {uo, vo, wo}, {lu, lv, lw}, mask, dt / (rho * t->h)};
+  const int rc = tall_faces(t, tall_sub_cb, &c);
+  for (size_t i = 0; i < nu; ++i) if (!lu[i]) uo[i] = 0.0;
+  for (size_t i = 0; i < n; ++i) if (!lv[i]) vo[i] = 0.0;
+  for (size_t i = 0; i < nw; ++i) if (!lw[i]) wo[i] = 0.0;
+  free(mask); free(lu); free(lv); free(lw);
+  return rc;
+}
+
+typedef struct {
+  double* d;
+  const uint8_t* mask;
+} tall_diag_ctx;
+
+static void tall_diag_cb(void* vctx, const tall_face* f) {
+  tall_diag_ctx* c = (tall_diag_ctx*)vctx;
+  for (int q = 0; q < f->n; ++q)
+    if (c->mask[f->idx[q]]) c->d[f->idx[q]] += f->w * f->c[q] * f->c[q];
+}
+
+int orc_tall_diagonal(const orc_tall* t, double* d) {
+  const size_t n = tall_samples(t);
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  memset(d, 0, n * sizeof(double));
+  tall_diag_ctx c = {d, mask};
+  const int rc = tall_faces(t, tall_diag_cb, &c);
+  free(mask);
+  return rc;
+}
+
+/* FP64 Jacobi-PCG on the unknown samples, from p = 0. */
+int orc_tall_solve(const orc_tall* t, const double* b, double rel_tol, int max_iter, double* p, int* iterations,
+                   double* relative_residual) {
+  const size_t n = tall_samples(t);
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  double* d = (double*)calloc(n, sizeof(double));
+  double *r = (double*)calloc(n, sizeof(double)), *z = (double*)calloc(n, sizeof(double));
+  double *s = (double*)calloc(n, sizeof(double)), *q = (double*)calloc(n, sizeof(double));
+  int rc = orc_tall_diagonal(t, d);
+  memset(p, 0, n * sizeof(double));
+  for (size_t i = 0; i < n; ++i) r[i] = mask[i] ? b[i] : 0.0;
+  const double bb = orc_dot_neumaier(r, r, n);
+  int it = 0;
+  if (rc == 0 && bb > 0.0) {
+    for (size_t i = 0; i < n; ++i) {
+      z[i] = (mask[i] && d[i] > 0.0) ? r[i] / d[i] : 0.0;
+      s[i] = z[i];
+    }
+    double rz = orc_dot_neumaier(r, z, n);
+    while (it < max_iter) {
+      orc_tall_apply(t, s, q);
+      const double sq = orc_dot_neumaier(s, q, n);
+      if (!(sq > 0.0)) { rc = 4; break; }
+      const double a = rz / sq;
+      for (size_t i = 0; i < n; ++i) { p[i] += a * s[i]; r[i] -= a * q[i]; }
+      ++it;
+      if (orc_dot_neumaier(r, r, n) <= rel_tol * rel_tol * bb) break;
+      for (size_t i = 0; i < n; ++i) z[i] = (mask[i] && d[i] > 0.0) ? r[i] / d[i] : 0.0;
+      const double rz2 = orc_dot_neumaier(r, z, n);
+      const double beta = rz2 / rz;
+      rz = rz2;
+      for (size_t i = 0; i < n; ++i) s[i] = z[i] + beta * s[i];
+    }
+  }
+  double rel = 0.0;
+  if (rc == 0 && bb > 0.0) {
+    orc_tall_apply(t, p, q);
+    double rr = 0.0;
+    for (size_t i = 0; i < n; ++i)
+      if (mask[i]) rr += (b[i] - q[i]) * (b[i] - q[i]);
+    rel = sqrt(rr / bb);
+  }
+  if (iterations) *iterations = it;
+  if (relative_residual) *relative_residual = rel;
+  free(mask); free(d); free(r); free(z); free(s); free(q);
+  return rc;
+}
+
+/* max over unknown samples of |E^T div u_fine| / (h * support), support the
+ * fine rows of the sample's hat function (b/2 for a tall end, 1 in the band):
+ * a per-volume divergence in 1/s comparable with the uniform diagnostic. */
+double orc_tall_max_divergence(const orc_tall* t, const double* u, const double* v, const double* w) {
+  const size_t n = tall_samples(t);
+  double* dv = (double*)calloc(n, sizeof(double));
+  if (orc_tall_rhs(t, u, v, w, 1.0, 1.0, dv) != 0) { free(dv); return -1.0; }
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  double m = 0.0;
+  for (int x = 0; x < t->nx; ++x)
+    for (int z = 0; z < t->nz; ++z) {
+      const int b = tall_base(t, x, z);
+      for (int py = 0; py < TALL_P(t); ++py) {
+        const size_t i = tall_sample(t, x, py, z);
+        if (!mask[i]) continue;
+        const double support = py < 2 ? 0.5 * b : 1.0;
+        const double d = fabs(dv[i]) / (t->h * t->h) / support;
+        if (d > m) m = d;
+      }
+    }
+  free(dv);
+  free(mask);
+  return m;
+}

@@ -142,6 +142,29 @@ int orc_tall_choose_base(int lowest_surface, int highest_surface, int fine_ny,
 /* limitNeighboringTallCellBases (tall-cell-grid.ts:84-106), in place. */
 void orc_tall_limit_bases(int32_t* bases, int nx, int nz, int max_delta, int passes);
 
+/* Tall-cell projection (variational reduction of the uniform operator; the
+ * layout and slot conventions are described in fp_oracle.c).  Samples are
+ * packed [nx][B+2][nz]; u faces [nx+1][S][nz], w faces [nx][S][nz+1], v faces
+ * [nx][B+2][nz], S = B + 2 + max_delta. */
+typedef struct {
+  int nx, ny, nz, band, max_delta;
+  const int32_t* bases; /* [nx][nz], 2 <= base <= ny */
+  const float* phi;     /* packed samples */
+  double h;
+} orc_tall;
+#define ORC_TALL_BAD_BASE 31
+int orc_tall_check(const orc_tall* t);
+void orc_tall_unknowns(const orc_tall* t, uint8_t* mask);
+int orc_tall_apply(const orc_tall* t, const double* p, double* out);
+int orc_tall_rhs(const orc_tall* t, const double* u, const double* v, const double* w, double dt, double rho,
+                 double* b);
+int orc_tall_subtract(const orc_tall* t, const double* p, const double* u, const double* v, const double* w,
+                      double dt, double rho, double* uo, double* vo, double* wo);
+int orc_tall_diagonal(const orc_tall* t, double* d);
+int orc_tall_solve(const orc_tall* t, const double* b, double rel_tol, int max_iter, double* p, int* iterations,
+                   double* relative_residual);
+double orc_tall_max_divergence(const orc_tall* t, const double* u, const double* v, const double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,11 @@ class MGOptions(C.Structure):
     _fields_ = [("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("phi_preserve_levels", C.c_int32)]
 
 
+class TallLayout(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("band", C.c_int32),
+                ("max_delta", C.c_int32), ("bases", C.c_void_p), ("phi", C.c_void_p), ("h", C.c_double)]
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (gcc) into oracle/_build/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -86,6 +91,22 @@ def lib():
         L.orc_tall_choose_base.restype = i
         L.orc_tall_choose_base.argtypes = [i, i, i, i, i, i]
         L.orc_tall_limit_bases.argtypes = [_ip, i, i, i, i]
+        tp = C.POINTER(TallLayout)
+        L.orc_tall_check.restype = i
+        L.orc_tall_check.argtypes = [tp]
+        L.orc_tall_unknowns.argtypes = [tp, _u8]
+        L.orc_tall_apply.restype = i
+        L.orc_tall_apply.argtypes = [tp, _dp, _dp]
+        L.orc_tall_rhs.restype = i
+        L.orc_tall_rhs.argtypes = [tp, _dp, _dp, _dp, d, d, _dp]
+        L.orc_tall_subtract.restype = i
+        L.orc_tall_subtract.argtypes = [tp, _dp, _dp, _dp, _dp, d, d, _dp, _dp, _dp]
+        L.orc_tall_diagonal.restype = i
+        L.orc_tall_diagonal.argtypes = [tp, _dp]
+        L.orc_tall_solve.restype = i
+        L.orc_tall_solve.argtypes = [tp, _dp, d, i, _dp, C.POINTER(i), C.POINTER(d)]
+        L.orc_tall_max_divergence.restype = d
+        L.orc_tall_max_divergence.argtypes = [tp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -279,3 +300,81 @@ def _wall_mask(a, axis):
     sl[axis] = -1
     m[tuple(sl)] = 0
     return m
+
+
+# --------------------------------------------------------------------------
+# Restricted tall-cell projection.  Samples (nx, B+2, nz): py 0 tall bottom
+# (fine row 0), py 1 tall top (row base-1), py 2+k band row base+k.  Faces:
+# v (nx, B+2, nz), u (nx+1, S, nz), w (nx, S, nz+1), S = B+2+max_delta.
+# --------------------------------------------------------------------------
+class Tall:
+    """A tall-cell layout bound for the oracle calls (keeps its arrays alive)."""
+
+    def __init__(self, bases, phi, ny, band, h, max_delta=4):
+        self.bases = np.ascontiguousarray(bases, dtype=np.int32)
+        self.phi = np.ascontiguousarray(phi, dtype=np.float32)
+        nx, nz = self.bases.shape
+        assert self.phi.shape == (nx, band + 2, nz), self.phi.shape
+        self.nx, self.ny, self.nz, self.band, self.max_delta, self.h = nx, int(ny), nz, int(band), int(max_delta), h
+        self.S = self.band + 2 + self.max_delta
+        self.P = self.band + 2
+        self._c = TallLayout(nx, self.ny, nz, self.band, self.max_delta, self.bases.ctypes.data,
+                             self.phi.ctypes.data, float(h))
+
+    @property
+    def ref(self):
+        return C.byref(self._c)
+
+    def shapes(self):
+        return (self.nx + 1, self.S, self.nz), (self.nx, self.P, self.nz), (self.nx, self.S, self.nz + 1)
+
+    def check(self):
+        return lib().orc_tall_check(self.ref)
+
+    def unknowns(self):
+        m = np.zeros((self.nx, self.P, self.nz), np.uint8)
+        lib().orc_tall_unknowns(self.ref, m)
+        return m.astype(bool)
+
+    def _rc(self, rc):
+        if rc != 0:
+            raise ValueError(f"tall-cell oracle error {rc}")
+
+    def apply(self, p):
+        out = np.zeros((self.nx, self.P, self.nz))
+        self._rc(lib().orc_tall_apply(self.ref, _f64(p), out))
+        return out
+
+    def diagonal(self):
+        out = np.zeros((self.nx, self.P, self.nz))
+        self._rc(lib().orc_tall_diagonal(self.ref, out))
+        return out
+
+    def rhs(self, u, v, w, dt, rho):
+        out = np.zeros((self.nx, self.P, self.nz))
+        self._rc(lib().orc_tall_rhs(self.ref, _f64(u), _f64(v), _f64(w), dt, rho, out))
+        return out
+
+    def subtract(self, p, u, v, w, dt, rho):
+        su, sv, sw = self.shapes()
+        uo, vo, wo = np.zeros(su), np.zeros(sv), np.zeros(sw)
+        self._rc(lib().orc_tall_subtract(self.ref, _f64(p), _f64(u), _f64(v), _f64(w), dt, rho, uo, vo, wo))
+        return uo, vo, wo
+
+    def solve(self, b, rel_tol=1e-12, max_iter=20000):
+        p = np.zeros((self.nx, self.P, self.nz))
+        it, rel = C.c_int(0), C.c_double(0)
+        rc = lib().orc_tall_solve(self.ref, _f64(b), rel_tol, int(max_iter), p, C.byref(it), C.byref(rel))
+        return p, {"status": rc, "iterations": it.value, "relative_residual": rel.value}
+
+    def max_divergence(self, u, v, w):
+        return lib().orc_tall_max_divergence(self.ref, _f64(u), _f64(v), _f64(w))
+
+    def projection(self, u, v, w, dt, rho, rel_tol=1e-12):
+        """FP64 projection: rhs -> Jacobi-PCG -> subtract -> divergence."""
+        b = self.rhs(u, v, w, dt, rho)
+        p, rep = self.solve(b, rel_tol)
+        uo, vo, wo = self.subtract(p, u, v, w, dt, rho)
+        rep["max_div_before"] = self.max_divergence(u, v, w)
+        rep["max_div_after"] = self.max_divergence(uo, vo, wo)
+        return p, (uo, vo, wo), b, rep
