@@ -83,6 +83,117 @@ def cpu_baseline(seconds_budget: float = 20.0):
     }
 
 
+def tall_cpu_baseline(n: int = 96):
+    """The FP64 tall-cell oracle (oracle/fp_oracle.c, Jacobi-PCG over the
+    expanded fine faces) on a bounded sample of the tall-cell workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from fluid_amd import scenes
+
+    oracle.build()
+    sc = scenes.tall_wave_scene(n, n, band=32, seed=0)
+    L = sc.layout
+    t = oracle.Tall(L.bases, sc.phi, L.fine_ny, L.band, L.h, L.max_delta)
+    t0 = time.perf_counter()
+    p, vel, b, rep = t.projection(sc.u, sc.v, sc.w, sc.dt, sc.rho, rel_tol=1e-6)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(sc.samples / dt / 1e6, 4),
+        "unit": "Msamples/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"wave-surface tall-cell scene at {n}^2 columns x {n} rows, band 32 (seed 0), one projection, "
+                  f"Jacobi-PCG to 1e-6 on the oracle's fine-face walk (scalar C): {rep['iterations']} "
+                  f"iterations in {dt:.2f} s",
+    }
+
+
+def bench_tall(args):
+    """Restricted tall-cell workload: 512^2 columns, 512 fine rows, 32-row band."""
+    import torch
+    from fluid_amd import scenes
+    from fluid_amd.projection import TallCellPressureProjection, default_options
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    sc = scenes.tall_wave_scene(args.tall_n, args.tall_n, band=32, seed=0)
+    L = sc.layout
+    print(f"# tall scene {args.tall_n}^2 columns generated in {time.perf_counter() - t0:.1f} s, bases "
+          f"{L.bases.min()}..{L.bases.max()}", file=sys.stderr, flush=True)
+    phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (sc.phi, sc.u, sc.v, sc.w))
+    out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
+    g = TallCellPressureProjection.from_layout(L)
+    common = dict(precision=args.precision, relative_tolerance=1e-6, pre_sweeps=args.sweeps,
+                  post_sweeps=args.sweeps, graphs=0 if args.no_graphs else 1)
+    opts = default_options(**common)
+    for i in range(args.warmup):
+        r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=opts)
+        print(f"# warmup {i}: {r.report['device_ms']:.2f} ms iters={r.report['iterations']} "
+              f"rel={r.report['relative_residual']:.2e}", file=sys.stderr, flush=True)
+    timed = default_options(timed_kernel=args.timed_kernel, **common)
+    torch.cuda.synchronize()
+    lat, reports = [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=timed)
+        lat.append((time.perf_counter() - t1) * 1e3)
+        reports.append(r.report)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    samples = sc.samples
+    value = samples * args.steps / elapsed / 1e6
+    rep = reports[-1]
+    kms = sum(x["timed_kernel_ms"] for x in reports)
+    kl = sum(x["timed_kernel_launches"] for x in reports)
+    avg_ms = kms / max(kl, 1)
+    bps = {"smooth_fine": 20.0, "residual_fine": 16.0, "cg_direction": 24.0, "cg_update": 24.0}.get(
+        args.timed_kernel, 0.0)
+    achieved = bps * samples / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    err = None
+    if not args.no_error_solve:
+        ref = g.project(phi, u, v, w, sc.dt, sc.rho, relative_tolerance=1e-12)
+        p_ref = ref.pressure.double()
+        err = float(torch.linalg.norm(out[0].double() - p_ref) / torch.linalg.norm(p_ref))
+    lat_sorted = sorted(lat)
+    line = {
+        "metric": "MGPCG projection Mcells/s at 1e-6 rel. residual",
+        "value": round(value, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 multigrid / fp32 CG / fp64 refinement",
+        "data": "synthetic (seeded wave free surface, Gaussian slot velocities)",
+        "config": {"workload": f"restricted tall-cell grid {args.tall_n}^2 columns x {args.tall_n} rows, band 32, "
+                               "one projection to 1e-6 (cells = packed samples)",
+                   "samples": samples, "equivalent_fine_cells": L.equivalent_cells,
+                   "equivalent_mcells_per_s": round(L.equivalent_cells * args.steps / elapsed / 1e6, 1),
+                   "liquid_samples": int(rep["liquid_samples"]), "sweeps": args.sweeps},
+        "latency_ms": {"p50": round(statistics.median(lat), 3),
+                       "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
+        "solve": {"iterations": rep["iterations"], "outer": rep["outer_iterations"], "levels": rep["levels"],
+                  "relative_residual": rep["relative_residual"], "max_div_after": rep["max_div_after"],
+                  "max_div_before": rep["max_div_before"], "fp64_pressure_rel_l2_error": err,
+                  "device_ms": round(rep["device_ms"], 3)},
+        "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(avg_ms, 5), "bytes_per_sample": bps,
+                     "launches": kl},
+    }
+    g.destroy()
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = tall_cpu_baseline()
+        except Exception as e:
+            line["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
 def bench_batch(args):
     """96 uneven cubic grids (edges log-uniform in [32,192]) solved to 1e-6 each:
     one batched FP16-multigrid solve against the sequential per-grid FP32 loop
@@ -208,7 +319,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (profiling)")
     ap.add_argument("--no-error-solve", action="store_true", help="skip the 1e-12 reference solve")
     ap.add_argument("--copy-probe-mib", type=int, default=0, help="only run a device copy of this size")
-    ap.add_argument("--workload", default="uniform", choices=["uniform", "batch", "serve"],
+    ap.add_argument("--tall-n", type=int, default=512, help="tall: columns per side (and fine rows)")
+    ap.add_argument("--workload", default="uniform", choices=["uniform", "batch", "serve", "tall"],
                     help="uniform: one 384^3 grid; batch: 96 uneven grids in one batched solve; "
                          "serve: Poisson-arriving 20-step sessions through fp_submit/fp_poll")
     ap.add_argument("--batch-count", type=int, default=96)
@@ -228,6 +340,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "tall":
+        bench_tall(args)
         return
 
     import numpy as np

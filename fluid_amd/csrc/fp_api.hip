@@ -1,4 +1,5 @@
 // fp_api.hip -- the C ABI (include/fluid_projection.h) over the HIP solvers.
+#include "fp_tall.hpp"
 #include "fp_uniform.hpp"
 
 #include <algorithm>
@@ -29,6 +30,7 @@ struct fp_ctx {
   hipStream_t stream = nullptr;
   int grids = 0;
   fp::Projector* proj = nullptr;
+  fp::TallProjector* tall = nullptr;
   fp::Server* server = nullptr;  // created by the first fp_submit
   std::mutex gpu;                // serialises synchronous calls with the serving worker
 };
@@ -36,9 +38,14 @@ struct fp_ctx {
 struct fp_grid {
   fp_ctx* ctx = nullptr;
   fp_grid_desc desc{};
-  fp::UniformGrid* uniform = nullptr;
+  fp::UniformGrid* uniform = nullptr;  // FP_LAYOUT_UNIFORM
+  fp::TallGrid* tall = nullptr;        // FP_LAYOUT_TALL_CELL
   uint64_t bytes = 0;
 };
+
+// Tall-cell max neighbour base delta: fp_grid_desc.reserved, 0 meaning the
+// maximumNeighborDelta of the high preset (lib/core/tall-cell-grid.ts:55).
+static int tall_delta(const fp_grid_desc* d) { return d->reserved > 0 ? d->reserved : 4; }
 
 #define CHECK_ARG(cond, ...)              \
   do {                                    \
@@ -74,7 +81,10 @@ int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out) {
   ctx->device = device;
   ctx->budget = hbm_budget_bytes;
   ctx->proj = new (std::nothrow) fp::Projector();
-  if (!ctx->proj) {
+  ctx->tall = new (std::nothrow) fp::TallProjector();
+  if (!ctx->proj || !ctx->tall) {
+    delete ctx->proj;
+    delete ctx->tall;
     delete ctx;
     return FP_ERR_OUT_OF_MEMORY;
   }
@@ -83,6 +93,7 @@ int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out) {
   if (e != hipSuccess) {
     set_error("hipStreamCreate: %s", hipGetErrorString(e));
     delete ctx->proj;
+    delete ctx->tall;
     delete ctx;
     return FP_ERR_DEVICE;
   }
@@ -101,6 +112,7 @@ int fp_ctx_destroy(fp_ctx* ctx) {
   ctx->server = nullptr;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   delete ctx->proj;
+  delete ctx->tall;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return FP_OK;
@@ -136,8 +148,10 @@ static int validate_desc(const fp_grid_desc* d) {
   CHECK_ARG((uint64_t)d->nx * d->ny * d->nz < (1ull << 34), "grid too large");
   CHECK_ARG(d->h > 0.0, "cell size must be positive");
   if (d->layout == FP_LAYOUT_TALL_CELL) {
-    set_error("tall-cell layout not built into this library");
-    return FP_ERR_UNSUPPORTED;
+    CHECK_ARG(d->column_bases != nullptr, "tall-cell grid: column_bases is NULL");
+    CHECK_ARG(d->band >= 1 && d->band <= 1024, "tall-cell band rows must be in [1,1024] (got %d)", d->band);
+    CHECK_ARG(d->reserved >= 0 && tall_delta(d) <= d->band, "tall-cell max neighbour delta %d must be in [1, band]",
+              tall_delta(d));
   }
   return FP_OK;
 }
@@ -147,7 +161,9 @@ int fp_grid_bytes(const fp_grid_desc* desc, int32_t precision, uint64_t* bytes) 
   int rc = validate_desc(desc);
   if (rc) return rc;
   CHECK_ARG(bytes != nullptr, "bytes is NULL");
-  const size_t b = fp::UniformGrid::estimate_bytes(desc->nx, desc->ny, desc->nz);
+  const size_t b = desc->layout == FP_LAYOUT_TALL_CELL
+                       ? fp::TallGrid::estimate_bytes(desc->nx, desc->ny, desc->nz, desc->band, tall_delta(desc))
+                       : fp::UniformGrid::estimate_bytes(desc->nx, desc->ny, desc->nz);
   if (b == 0) {
     set_error("no multigrid hierarchy for %d x %d x %d", desc->nx, desc->ny, desc->nz);
     return FP_ERR_LAYOUT;
@@ -175,15 +191,27 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
   g->ctx = ctx;
   g->desc = *desc;
   g->desc.column_bases = nullptr;
-  g->uniform = new (std::nothrow) fp::UniformGrid();
-  if (!g->uniform) { delete g; return FP_ERR_OUT_OF_MEMORY; }
-  rc = g->uniform->create(desc->nx, desc->ny, desc->nz, desc->h);
-  if (rc) {
-    delete g->uniform;
-    delete g;
-    return rc;
+  if (desc->layout == FP_LAYOUT_TALL_CELL) {
+    g->tall = new (std::nothrow) fp::TallGrid();
+    if (!g->tall) { delete g; return FP_ERR_OUT_OF_MEMORY; }
+    rc = g->tall->create(desc->nx, desc->ny, desc->nz, desc->band, tall_delta(desc), desc->h, desc->column_bases);
+    if (rc) {
+      delete g->tall;
+      delete g;
+      return rc;
+    }
+    g->bytes = g->tall->bytes();
+  } else {
+    g->uniform = new (std::nothrow) fp::UniformGrid();
+    if (!g->uniform) { delete g; return FP_ERR_OUT_OF_MEMORY; }
+    rc = g->uniform->create(desc->nx, desc->ny, desc->nz, desc->h);
+    if (rc) {
+      delete g->uniform;
+      delete g;
+      return rc;
+    }
+    g->bytes = g->uniform->bytes();
   }
-  g->bytes = g->uniform->bytes();
   ctx->in_use += g->bytes;
   ctx->grids += 1;
   *out = g;
@@ -192,7 +220,7 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
 
 int fp_grid_destroy(fp_grid* g) {
   if (!g) return FP_OK;
-  if (g->ctx->server && g->ctx->server->jobs_on(g->uniform) > 0) {
+  if (g->uniform && g->ctx->server && g->ctx->server->jobs_on(g->uniform) > 0) {
     set_error("fp_grid_destroy: the grid has submitted projections still running (fp_poll them first)");
     return FP_ERR_STATE;
   }
@@ -201,14 +229,31 @@ int fp_grid_destroy(fp_grid* g) {
   (void)hipStreamSynchronize(g->ctx->stream);
   g->ctx->in_use -= g->bytes;
   g->ctx->grids -= 1;
-  g->ctx->proj->forget(g->uniform->id());
+  if (g->uniform) g->ctx->proj->forget(g->uniform->id());
+  if (g->tall) g->ctx->tall->forget(g->tall->id());
   delete g->uniform;
+  delete g->tall;
   delete g;
   return FP_OK;
 }
 
 int fp_grid_levels(const fp_grid* g, int32_t max_levels, int32_t* dims, int32_t* strides, int32_t* levels) {
   CHECK_ARG(g && levels, "fp_grid_levels: NULL argument");
+  if (g->tall) {
+    // sample level, then the lattice levels; strides: the aggregation factors
+    const fp::TallRec& R = *g->tall->host_rec();
+    *levels = R.L;
+    for (int l = 0; l < R.L && l < max_levels; ++l) {
+      const fp::TallLevel& lv = R.lv[l];
+      const int d3[3] = {lv.nx, lv.ny, lv.nz};
+      const int s3[3] = {l == 0 ? 1 : 2, l == 0 ? 1 : lv.cy, l == 0 ? 1 : 2};
+      for (int a = 0; a < 3; ++a) {
+        if (dims) dims[l * 3 + a] = d3[a];
+        if (strides) strides[l * 3 + a] = s3[a];
+      }
+    }
+    return FP_OK;
+  }
   const int L = g->uniform->levels();
   *levels = L;
   for (int l = 0; l < L && l < max_levels; ++l)
@@ -222,6 +267,13 @@ int fp_grid_levels(const fp_grid* g, int32_t max_levels, int32_t* dims, int32_t*
 int fp_grid_face_counts(const fp_grid* g, int64_t* nu, int64_t* nv, int64_t* nw) {
   CHECK_ARG(g && nu && nv && nw, "fp_grid_face_counts: NULL argument");
   const fp_grid_desc& d = g->desc;
+  if (g->tall) {
+    const int P = d.band + 2, S = d.band + 2 + tall_delta(&d);
+    *nu = (int64_t)(d.nx + 1) * S * d.nz;
+    *nv = (int64_t)d.nx * P * d.nz;
+    *nw = (int64_t)d.nx * S * (d.nz + 1);
+    return FP_OK;
+  }
   *nu = (int64_t)(d.nx + 1) * d.ny * d.nz;
   *nv = (int64_t)d.nx * (d.ny + 1) * d.nz;
   *nw = (int64_t)d.nx * d.ny * (d.nz + 1);
@@ -254,24 +306,51 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
   fp_ctx* ctx = grids[0] ? grids[0]->ctx : nullptr;
   CHECK_ARG(ctx != nullptr, "fp_project_batch: grid 0 is NULL");
   std::vector<fp::UniformGrid*> list;
+  std::vector<fp::TallGrid*> tlist;
+  std::vector<int> ui, ti;  // batch positions of the uniform / tall-cell grids
   for (int i = 0; i < count; ++i) {
     CHECK_ARG(grids[i] != nullptr, "fp_project_batch: grid %d is NULL", i);
     CHECK_ARG(grids[i]->ctx == ctx, "fp_project_batch: grid %d belongs to another context", i);
     for (int j = 0; j < i; ++j) CHECK_ARG(grids[j] != grids[i], "fp_project_batch: grid %d repeated", i);
     int rc = validate_inputs(&in[i], &out[i], o);
     if (rc) return rc;
-    list.push_back(grids[i]->uniform);
+    if (grids[i]->tall) {
+      if (in[i].target_divergence) {
+        set_error("fp_project: target_divergence is not supported on tall-cell grids");
+        return FP_ERR_UNSUPPORTED;
+      }
+      tlist.push_back(grids[i]->tall);
+      ti.push_back(i);
+    } else {
+      list.push_back(grids[i]->uniform);
+      ui.push_back(i);
+    }
   }
   std::lock_guard<std::mutex> lock(ctx->gpu);
   (void)hipSetDevice(ctx->device);
-  for (int i = 0; i < count; ++i) {
-    int rc = list[i]->bind(ctx->stream, &in[i], &out[i], o);
+  for (size_t k = 0; k < list.size(); ++k) {
+    int rc = list[k]->bind(ctx->stream, &in[ui[k]], &out[ui[k]], o);
     if (rc) return rc;
   }
-  int rc = ctx->proj->run(ctx->stream, list, o, reps);
+  for (size_t k = 0; k < tlist.size(); ++k) {
+    int rc = tlist[k]->bind(ctx->stream, &in[ti[k]], &out[ti[k]], o);
+    if (rc) return rc;
+  }
+  std::vector<fp_report> ureps(list.size()), treps(tlist.size());
+  int rc = ctx->proj->run(ctx->stream, list, o, reps ? ureps.data() : nullptr);
   if (rc) return rc;
-  for (int i = 0; i < count; ++i) {
-    rc = list[i]->finish_outputs(ctx->stream, &in[i], &out[i]);
+  rc = ctx->tall->run(ctx->stream, tlist, o, reps ? treps.data() : nullptr);
+  if (rc) return rc;
+  if (reps) {
+    for (size_t k = 0; k < list.size(); ++k) reps[ui[k]] = ureps[k];
+    for (size_t k = 0; k < tlist.size(); ++k) reps[ti[k]] = treps[k];
+  }
+  for (size_t k = 0; k < list.size(); ++k) {
+    rc = list[k]->finish_outputs(ctx->stream, &in[ui[k]], &out[ui[k]]);
+    if (rc) return rc;
+  }
+  for (size_t k = 0; k < tlist.size(); ++k) {
+    rc = tlist[k]->finish_outputs(ctx->stream, &in[ti[k]], &out[ti[k]]);
     if (rc) return rc;
   }
   hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -287,6 +366,10 @@ int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options
   int rc = validate_inputs(in, out, o);
   if (rc) return rc;
   fp_ctx* ctx = g->ctx;
+  if (g->tall) {
+    set_error("fp_submit: tall-cell grids are projected with fp_project / fp_project_batch");
+    return FP_ERR_UNSUPPORTED;
+  }
   if (!ctx->server) {
     ctx->server = new (std::nothrow) fp::Server(ctx->stream, ctx->device, ctx->proj, &ctx->gpu);
     if (!ctx->server) return FP_ERR_OUT_OF_MEMORY;
@@ -307,6 +390,7 @@ int fp_probe_apply(fp_grid* g, const float* phi, const double* p, double* out) {
   CHECK_ARG(g && phi && p && out, "fp_probe_apply: NULL argument");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
+  if (g->tall) return g->ctx->tall->probe_apply(g->ctx->stream, g->tall, phi, p, out);
   return fp::probe_apply(g->ctx->stream, g->uniform, phi, p, out);
 }
 
@@ -315,6 +399,7 @@ int fp_probe_rhs(fp_grid* g, const fp_inputs* in, double* b) {
   CHECK_ARG(in->memory == FP_MEMORY_HOST, "fp_probe_rhs takes host arrays");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
+  if (g->tall) return g->ctx->tall->probe_rhs(g->ctx->stream, g->tall, in, b);
   int rc = g->uniform->bind(g->ctx->stream, in, nullptr, nullptr);
   if (rc) return rc;
   return g->ctx->proj->probe_rhs(g->ctx->stream, g->uniform, b);
@@ -322,6 +407,7 @@ int fp_probe_rhs(fp_grid* g, const fp_inputs* in, double* b) {
 
 int fp_probe_vcycle(fp_grid* g, const float* phi, const fp_options* o, const float* r, float* z) {
   CHECK_ARG(g && phi && o && r && z, "fp_probe_vcycle: NULL argument");
+  CHECK_ARG(g->uniform != nullptr, "fp_probe_vcycle: uniform grids only");
   CHECK_ARG(o->precision >= 0 && o->precision <= 2, "unknown precision %d", o->precision);
   CHECK_ARG(o->pre_sweeps == o->post_sweeps && o->pre_sweeps >= 0 && o->pre_sweeps <= 16,
             "pre and post sweeps must match and lie in [0,16]");
@@ -358,6 +444,7 @@ int fp_probe_vcycle(fp_grid* g, const float* phi, const fp_options* o, const flo
 
 int fp_probe_level_phi(fp_grid* g, const float* phi, int32_t level, int32_t preserve, float* out) {
   CHECK_ARG(g && phi && out, "fp_probe_level_phi: NULL argument");
+  CHECK_ARG(g->uniform != nullptr, "fp_probe_level_phi: uniform grids only");
   CHECK_ARG(level >= 0 && level < g->uniform->levels(), "level %d out of range [0,%d)", level, g->uniform->levels());
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
@@ -368,6 +455,14 @@ int fp_probe_level_phi(fp_grid* g, const float* phi, int32_t level, int32_t pres
     return FP_ERR_DEVICE;
   }
   return g->ctx->proj->probe_level_phi(g->ctx->stream, g->uniform, preserve, level, out);
+}
+
+int fp_probe_tall_layout(fp_grid* g, const float* phi, int32_t* rows, uint8_t* unknown) {
+  CHECK_ARG(g && phi && rows && unknown, "fp_probe_tall_layout: NULL argument");
+  CHECK_ARG(g->tall != nullptr, "fp_probe_tall_layout: tall-cell grids only");
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
+  (void)hipSetDevice(g->ctx->device);
+  return g->ctx->tall->probe_layout(g->ctx->stream, g->tall, phi, rows, unknown);
 }
 
 }  // extern "C"

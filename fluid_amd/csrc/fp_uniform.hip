@@ -23,6 +23,7 @@
 #include "fp_uniform.hpp"
 #include "fp_sweep.hpp"
 #include "fp_march.hpp"
+#include "fp_state.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -87,19 +88,6 @@ __device__ __forceinline__ float ldT(const T* p, size_t i) { return Store<T>::ld
 template <typename T>
 __device__ __forceinline__ void stT(T* p, size_t i, float v) { Store<T>::st(p, i, v); }
 
-// --------------------------------------------------------------------------
-// per-projection state and vector resets
-// --------------------------------------------------------------------------
-// One workgroup per grid: a fresh State (rscale 1).
-__global__ void k_reset_state(const Part* parts) {
-  if (threadIdx.x != 0) return;
-  State* st = parts[blockIdx.x].g->st;
-  State z;
-  __builtin_memset(&z, 0, sizeof(z));
-  z.rscale = 1.0f;
-  *st = z;
-}
-
 // which: 0 x (FP64), 1 e, 2 d[0].  skip_converged: leave grids whose
 // refinement has converged.
 __global__ void k_zero(const Part* parts, int np, int which, int skip_converged) {
@@ -116,35 +104,6 @@ __global__ void k_zero(const Part* parts, int np, int which, int skip_converged)
   const size_t n4 = n / 4;
   FP_STRIDE(i, n4) reinterpret_cast<float4*>(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (blk == 0 && threadIdx.x < (n & 3)) p[n4 * 4 + threadIdx.x] = 0.0f;
-}
-
-// Pending work of a batch: counts[0] grids whose refinement has not
-// converged, counts[1] grids whose inner solve is still running.
-__global__ void k_status(const Part* parts, int np, int* counts) {
-  __shared__ int s_outer, s_inner;
-  if (threadIdx.x == 0) {
-    s_outer = 0;
-    s_inner = 0;
-  }
-  __syncthreads();
-  int o = 0, in = 0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) {
-    const State* st = parts[i].g->st;
-    o += st->outer_done ? 0 : 1;
-    in += st->done ? 0 : 1;
-  }
-  atomicAdd(&s_outer, o);
-  atomicAdd(&s_inner, in);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    counts[0] = s_outer;
-    counts[1] = s_inner;
-  }
-}
-
-// The States of a batch gathered in part order (one copy to the host).
-__global__ void k_gather_states(const Part* parts, int np, State* out) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) out[i] = *parts[i].g->st;
 }
 
 // --------------------------------------------------------------------------
@@ -186,51 +145,6 @@ __global__ void k_rhs(const Part* parts, int np) {
     b[idx] = val;
   }
   block_partials_max<2>(acc, dmax, G.partials, blk, nblk);
-}
-
-// finishers run one workgroup per part of their producer's table
-__global__ void k_fin_rhs(const Part* parts) {
-  const Part p = parts[blockIdx.x];
-  const GridRec& G = *p.g;
-  State* st = G.st;
-  double out[2];
-  float mx;
-  sum_partials_max<2>(G.partials, p.nblocks, out, mx);
-  if (threadIdx.x == 0) {
-    st->divmax_before = __float_as_uint(mx);
-    st->bb = out[0];
-    // The refinement stops at a quarter of the requested relative residual:
-    // the max-norm divergence left in the FP32 output then stays below that
-    // of an FP64 solve stopped at the requested tolerance.
-    st->tol2_bb = 0.0625 * G.tol2 * out[0];
-    st->liquid = (long long)out[1];
-  }
-}
-
-// Refinement control after the FP64 true residual (OpTrueResidual): decides
-// convergence of the refinement and the target of the next inner solve.
-// new_pass: an inner solve follows for grids that have not converged.
-__global__ void k_fin_true(const Part* parts, int new_pass) {
-  const Part p = parts[blockIdx.x];
-  State* st = p.g->st;
-  if (st->outer_done) return;
-  double out[1];
-  float m;
-  sum_partials_max<1>(p.g->partials, p.nblocks, out, m);
-  if (threadIdx.x == 0) {
-    const double rr = out[0];
-    st->rr64 = rr;
-    const bool finite = isfinite(rr);
-    st->outer_done = (!finite) || rr <= st->tol2_bb;
-    if (!finite) st->flags |= 0x1;  // FP_FLAG_NONFINITE
-    // inner solve: a tenth of the refinement target (margin for the max-norm
-    // divergence), at most what FP32 CG can be trusted to reduce in one pass.
-    st->inner_target2 = fmax(0.01 * st->tol2_bb, 1e-12 * rr);
-    st->iter = 0;
-    st->done = st->outer_done || !new_pass;
-    if (!st->outer_done && new_pass) st->outer += 1;
-    st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
-  }
 }
 
 // x += e; e = 0 (folds one inner solve into the FP64 iterate).  A broken CG
@@ -720,41 +634,6 @@ __global__ void k_cg_rz(const Part* parts, int np) {
   block_partials<1>(acc, cload(&Gp->partials), blk, nblk);
 }
 
-__global__ void k_fin_rz(const Part* parts) {
-  const Part p = parts[blockIdx.x];
-  State* st = p.g->st;
-  if (st->done) return;
-  double out[1];
-  sum_partials<1>(p.g->partials, p.nblocks, out);
-  if (threadIdx.x == 0) {
-    const double rz = out[0];
-    if (!(rz > 0.0) || !isfinite(rz)) {
-      st->flags |= isfinite(rz) ? 0x4 : 0x1;  // NONPOSITIVE_PRECONDITIONER / NONFINITE
-      st->done = 1;
-      return;
-    }
-    st->beta = (st->iter == 0) ? 0.0 : rz / st->rz;
-    st->rz = rz;
-  }
-}
-
-__global__ void k_fin_dq(const Part* parts) {
-  const Part p = parts[blockIdx.x];
-  State* st = p.g->st;
-  if (st->done) return;
-  double out[1];
-  sum_partials<1>(p.g->partials, p.nblocks, out);
-  if (threadIdx.x == 0) {
-    const double dq = out[0];
-    if (!(dq > 0.0) || !isfinite(dq)) {
-      st->flags |= isfinite(dq) ? 0x2 : 0x1;  // NONPOSITIVE_CURVATURE / NONFINITE
-      st->done = 1;
-      return;
-    }
-    st->alpha = st->rz / dq;
-  }
-}
-
 // e += alpha d ; r -= alpha q ; ||r||^2, max|r| -> convergence and next rscale.
 // d: the direction this iteration wrote (buffer (iter & 1) ^ 1).
 __global__ void k_cg_update(const Part* parts, int np) {
@@ -791,28 +670,6 @@ __global__ void k_cg_update(const Part* parts, int np) {
     rmax = fmaxf(rmax, fabsf(rn));
   }
   block_partials_max<1>(acc, rmax, cload(&Gp->partials), blk, nblk);
-}
-
-__global__ void k_fin_update(const Part* parts) {
-  const Part p = parts[blockIdx.x];
-  State* st = p.g->st;
-  if (st->done) return;
-  double out[1];
-  float m;
-  sum_partials_max<1>(p.g->partials, p.nblocks, out, m);
-  if (threadIdx.x == 0) {
-    const double rr = out[0];
-    st->rr = rr;
-    st->iter += 1;
-    st->total_iter += 1;
-    if (!isfinite(rr)) {
-      st->flags |= 0x1;
-      st->done = 1;
-    } else if (rr <= st->inner_target2 || st->iter >= p.g->max_iter) {
-      st->done = 1;
-    }
-    st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
-  }
 }
 
 // --------------------------------------------------------------------------
@@ -887,15 +744,6 @@ __global__ void k_div_after(const Part* parts, int np) {
   block_partials_max<1>(none, dmax, G.partials, blk, nblk);
 }
 
-__global__ void k_fin_divafter(const Part* parts) {
-  const Part p = parts[blockIdx.x];
-  if (!p.g->out_u || !p.g->out_v || !p.g->out_w) return;
-  double out[1];
-  float mx;
-  sum_partials_max<1>(p.g->partials, p.nblocks, out, mx);
-  if (threadIdx.x == 0) p.g->st->divmax_after = __float_as_uint(mx);
-}
-
 // FP64 operator probe: out = A p (single grid, plain launch).
 __global__ void k_apply_d(Grid3 g, const float* __restrict__ phi, const double* __restrict__ p,
                           double* __restrict__ out) {
@@ -928,14 +776,6 @@ __global__ void k_apply_d(Grid3 g, const float* __restrict__ phi, const double* 
 // ==========================================================================
 // host side
 // ==========================================================================
-#define HIPCHK(expr)                                                   \
-  do {                                                                 \
-    hipError_t _e = (expr);                                            \
-    if (_e != hipSuccess) {                                            \
-      set_error("%s: %s", #expr, hipGetErrorString(_e));               \
-      return FP_ERR_DEVICE;                                            \
-    }                                                                  \
-  } while (0)
 
 int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3]) {
   // planUniformCM11aHierarchy (webgpu-uniform-pressure-multigrid.ts:149-169):
@@ -1408,18 +1248,6 @@ Projector::Plan* Projector::plan_for(const std::vector<UniformGrid*>& grids, con
   return p;
 }
 
-// Launch helpers: a part-table launch skips empty tables; a finisher runs one
-// workgroup per part of its producer's table.
-#define PLAUNCH(kern, tab, stream, ...)                                                               \
-  do {                                                                                                \
-    if ((tab).n > 0)                                                                                  \
-      hipLaunchKernelGGL(kern, dim3((tab).blocks), dim3(FP_BLOCK), 0, stream, (tab).dev, (tab).n, ##__VA_ARGS__); \
-  } while (0)
-#define FLAUNCH(kern, tab, stream, ...)                                                               \
-  do {                                                                                                \
-    if ((tab).n > 0) hipLaunchKernelGGL(kern, dim3((tab).n), dim3(FP_BLOCK), 0, stream, (tab).dev, ##__VA_ARGS__); \
-  } while (0)
-
 void Projector::timer_begin(hipStream_t s) {
   if ((size_t)(2 * timer_used_ + 2) > timer_events_.size()) {
     hipEvent_t a, b;
@@ -1529,11 +1357,11 @@ template <typename T>
 int Projector::iteration_t(hipStream_t s, Plan& p) {
   vcycle_t<T>(s, p);
   PLAUNCH(k_cg_rz<T>, p.vec4, s);
-  FLAUNCH(k_fin_rz, p.vec4, s);
+  FLAUNCH(k_fin_rz<Part>, p.vec4, s);
   TIMED(FP_KERNEL_CG_DIRECTION, PLAUNCH(k_march<OpDirection<T>>, p.march0, s, 1, MarchLaunch{0, 0}));
-  FLAUNCH(k_fin_dq, p.march0, s);
+  FLAUNCH(k_fin_dq<Part>, p.march0, s);
   TIMED(FP_KERNEL_CG_UPDATE, PLAUNCH(k_cg_update, p.vec4, s));
-  FLAUNCH(k_fin_update, p.vec4, s);
+  FLAUNCH(k_fin_update<Part>, p.vec4, s);
   return FP_OK;
 }
 
@@ -1574,7 +1402,7 @@ int Projector::poll(hipStream_t s, Plan& p, int* counts) {
     HIPCHK(hipMalloc((void**)&dev_counts_, 16));
     HIPCHK(hipHostMalloc((void**)&host_counts_, 16, hipHostMallocDefault));
   }
-  hipLaunchKernelGGL(k_status, dim3(1), dim3(FP_BLOCK), 0, s, p.one.dev, p.one.n, dev_counts_);
+  hipLaunchKernelGGL(k_status<Part>, dim3(1), dim3(FP_BLOCK), 0, s, p.one.dev, p.one.n, dev_counts_);
   HIPCHK(hipMemcpyAsync(host_counts_, dev_counts_, 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   counts[0] = host_counts_[0];
@@ -1615,10 +1443,10 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, s));
-  FLAUNCH(k_reset_state, p.one, s);
+  FLAUNCH(k_reset_state<Part>, p.one, s);
   setup_tables(s, p, opt->phi_preserve_levels);
   PLAUNCH(k_rhs, p.cells, s);
-  FLAUNCH(k_fin_rhs, p.cells, s);
+  FLAUNCH(k_fin_rhs<Part>, p.cells, s);
   PLAUNCH(k_zero, p.vec4, s, 0, 0);  // x, except on warm-started grids
   PLAUNCH(k_zero, p.vec4, s, 1, 0);
   PLAUNCH(k_zero, p.vec4, s, 2, 0);
@@ -1630,7 +1458,7 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   for (int pass = 0; pass <= max_outer; ++pass) {
     const int new_pass = pass < max_outer;
     TIMED(FP_KERNEL_TRUE_RESIDUAL, PLAUNCH(k_march<OpTrueResidual>, p.march0, s, 2, MarchLaunch{0, 0}));
-    FLAUNCH(k_fin_true, p.march0, s, new_pass);
+    FLAUNCH(k_fin_true<Part>, p.march0, s, new_pass);
     int rc = poll(s, p, counts);
     if (rc) return rc;
     if (counts[0] == 0 || !new_pass) break;
@@ -1659,10 +1487,10 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   }
   for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, p.faces[a], s, a);
   PLAUNCH(k_div_after, p.cells, s);
-  FLAUNCH(k_fin_divafter, p.cells, s);
+  FLAUNCH(k_fin_divafter<Part>, p.cells, s);
   PLAUNCH(k_pressure_out, p.cells, s);
   HIPCHK(hipEventRecord(ev1, s));
-  hipLaunchKernelGGL(k_gather_states, dim3((unsigned)((grids.size() + 255) / 256)), dim3(256), 0, s, p.one.dev,
+  hipLaunchKernelGGL(k_gather_states<Part>, dim3((unsigned)((grids.size() + 255) / 256)), dim3(256), 0, s, p.one.dev,
                      p.one.n, p.gathered);
   HIPCHK(hipMemcpyAsync(p.gathered_host, p.gathered, grids.size() * sizeof(State), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1807,10 +1635,10 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
     }
   for (auto& sp : setup_p) {
     Plan& p = *sp.first;
-    FLAUNCH(k_reset_state, p.one, s);
+    FLAUNCH(k_reset_state<Part>, p.one, s);
     setup_tables(s, p, sp.second->opt.phi_preserve_levels);
     PLAUNCH(k_rhs, p.cells, s);
-    FLAUNCH(k_fin_rhs, p.cells, s);
+    FLAUNCH(k_fin_rhs<Part>, p.cells, s);
     PLAUNCH(k_zero, p.vec4, s, 0, 0);
     PLAUNCH(k_zero, p.vec4, s, 1, 0);
     PLAUNCH(k_zero, p.vec4, s, 2, 0);
@@ -1824,7 +1652,7 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
   for (int np = 0; np < 2; ++np)
     for (auto& rp : resid_p[np]) {
       PLAUNCH(k_march<OpTrueResidual>, rp.first->march0, s, 2, MarchLaunch{0, 0});
-      FLAUNCH(k_fin_true, rp.first->march0, s, np);
+      FLAUNCH(k_fin_true<Part>, rp.first->march0, s, np);
     }
   // ---- one CG + V-cycle iteration of every iterating grid ----
   for (auto& ip : iter_p) enqueue_iteration(s, *ip.first);
@@ -1832,14 +1660,14 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
   if (fin_p) {
     for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, fin_p->faces[a], s, a);
     PLAUNCH(k_div_after, fin_p->cells, s);
-    FLAUNCH(k_fin_divafter, fin_p->cells, s);
+    FLAUNCH(k_fin_divafter<Part>, fin_p->cells, s);
     PLAUNCH(k_pressure_out, fin_p->cells, s);
     for (Job* j : finish) {
       int rc = j->grid->finish_outputs(s, &j->in, &j->out);
       if (rc) return rc;
     }
   }
-  hipLaunchKernelGGL(k_gather_states, dim3((unsigned)((live.size() + 255) / 256)), dim3(256), 0, s, all_p->one.dev,
+  hipLaunchKernelGGL(k_gather_states<Part>, dim3((unsigned)((live.size() + 255) / 256)), dim3(256), 0, s, all_p->one.dev,
                      all_p->one.n, round_states_);
   HIPCHK(hipMemcpyAsync(round_states_host_, round_states_, live.size() * sizeof(State), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1899,9 +1727,9 @@ int Projector::probe_rhs(hipStream_t s, UniformGrid* g, double* b_host) {
   g->host_rec()->tol2 = 0.0;
   int rc = g->upload(s);
   if (rc) return rc;
-  FLAUNCH(k_reset_state, p->one, s);
+  FLAUNCH(k_reset_state<Part>, p->one, s);
   PLAUNCH(k_rhs, p->cells, s);
-  FLAUNCH(k_fin_rhs, p->cells, s);
+  FLAUNCH(k_fin_rhs<Part>, p->cells, s);
   HIPCHK(hipMemcpyAsync(b_host, g->host_rec()->b, (size_t)g->host_rec()->n * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return FP_OK;
@@ -1913,7 +1741,7 @@ int Projector::probe_vcycle(hipStream_t s, UniformGrid* g, const fp_options* opt
   g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
   int rc = g->upload(s);
   if (rc) return rc;
-  FLAUNCH(k_reset_state, p->one, s);
+  FLAUNCH(k_reset_state<Part>, p->one, s);
   setup_tables(s, *p, opt->phi_preserve_levels);
   HIPCHK(hipMemcpyAsync(&g->host_rec()->st->rscale, &rscale, sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1933,7 +1761,7 @@ int Projector::probe_level_phi(hipStream_t s, UniformGrid* g, int preserve, int 
   if (!p) return FP_ERR_OUT_OF_MEMORY;
   int rc = g->upload(s);
   if (rc) return rc;
-  FLAUNCH(k_reset_state, p->one, s);
+  FLAUNCH(k_reset_state<Part>, p->one, s);
   for (int l = 1; l <= level; ++l) PLAUNCH(k_downsample_phi, p->lv[l].down, s, l, preserve);
   HIPCHK(hipMemcpyAsync(out, g->level(level).phi, (size_t)g->level(level).n * 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

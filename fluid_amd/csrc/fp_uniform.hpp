@@ -105,17 +105,21 @@ struct GridRec {
   int pad;
 };
 
-struct Part {
+// One grid's slice of a part-table launch (R: the grid record type).
+template <class R>
+struct PartOf {
   int block0;
   int nblocks;
-  const GridRec* g;
+  const R* g;
 };
+using Part = PartOf<GridRec>;
 
 // Workgroups of a part-table launch: the part covering blockIdx.x, the grid it
 // names and the workgroup's index within that grid.  Part starts are multiples
 // of 8, so a grid's workgroup b runs on XCD b % 8 as tile_of_group assumes;
 // workgroups in the padding return at once.
-__device__ __forceinline__ int part_index(const Part* parts, int n) {
+template <class P>
+__device__ __forceinline__ int part_index(const P* parts, int n) {
   const int b = (int)blockIdx.x;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -127,9 +131,9 @@ __device__ __forceinline__ int part_index(const Part* parts, int n) {
 }
 
 #define FP_PART(parts, np)                                \
-  const Part fp_part_ = cload((parts) + part_index(parts, np)); \
-  const GridRec* Gp = fp_part_.g;                         \
-  const GridRec& G = *Gp;                                 \
+  const auto fp_part_ = cload((parts) + part_index(parts, np)); \
+  const auto* Gp = fp_part_.g;                            \
+  const auto& G = *Gp;                                    \
   const int blk = (int)blockIdx.x - fp_part_.block0; \
   const int nblk = fp_part_.nblocks;                 \
   if (blk >= nblk) return;
@@ -139,6 +143,29 @@ __device__ __forceinline__ int part_index(const Part* parts, int n) {
   for (size_t idx = (size_t)blk * blockDim.x + threadIdx.x; idx < (size_t)(N); idx += (size_t)nblk * blockDim.x)
 
 int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3]);
+
+// ---- host helpers -----------------------------------------------------------
+#define HIPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t _e = (expr);                                            \
+    if (_e != hipSuccess) {                                            \
+      set_error("%s: %s", #expr, hipGetErrorString(_e));               \
+      return FP_ERR_DEVICE;                                            \
+    }                                                                  \
+  } while (0)
+
+// Launch helpers: a part-table launch skips empty tables; a finisher runs one
+// workgroup per part of its producer's table.
+#define PLAUNCH(kern, tab, stream, ...)                                                               \
+  do {                                                                                                \
+    if ((tab).n > 0)                                                                                  \
+      hipLaunchKernelGGL(kern, dim3((tab).blocks), dim3(FP_BLOCK), 0, stream, (tab).dev, (tab).n, ##__VA_ARGS__); \
+  } while (0)
+#define FLAUNCH(kern, tab, stream, ...)                                                               \
+  do {                                                                                                \
+    if ((tab).n > 0) hipLaunchKernelGGL(kern, dim3((tab).n), dim3(FP_BLOCK), 0, stream, (tab).dev, ##__VA_ARGS__); \
+  } while (0)
+
 
 // Resources of one grid: the multigrid hierarchy, CG vectors and its record.
 class UniformGrid {
@@ -193,11 +220,13 @@ struct Job {
 };
 
 // Device part table of one stage.
-struct PartTable {
-  const Part* dev = nullptr;
+template <class P>
+struct PartTableOf {
+  const P* dev = nullptr;
   int n = 0;        // parts (grids)
   int blocks = 0;   // workgroups of the launch
 };
+using PartTable = PartTableOf<Part>;
 
 // Runs projections of a set of bound grids: one launch per solver stage
 // covers every grid of the set.  Plans (part tables and the replayed

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = [
     "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
     "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
     "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_submit", "fp_poll", "fp_probe_apply",
-    "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi",
+    "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout",
 ]
 PENDING = 1
 
@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_probe_rhs.argtypes = [vp, C.POINTER(Inputs), vp]
     L.fp_probe_vcycle.argtypes = [vp, vp, C.POINTER(Options), vp, vp]
     L.fp_probe_level_phi.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
+    L.fp_probe_tall_layout.argtypes = [vp, vp, vp, vp]
     for name in EXPORTED_SYMBOLS:
         if name not in ("fp_last_error", "fp_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -238,6 +239,11 @@ class UniformPressureProjection:
     def handle(self):
         return self._g
 
+    def shapes(self):
+        """Shapes of (phi / pressure, u, v, w)."""
+        nx, ny, nz = self.dims
+        return [(nx, ny, nz), (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+
     @property
     def level_count(self) -> int:
         return len(self.levels()[0])
@@ -252,20 +258,19 @@ class UniformPressureProjection:
     def _prepare(self, phi, u, v, w, dt, rho, target_divergence=None, out=None):
         """C input/output records for one projection, the result arrays and
         the objects that must stay alive until the call returns."""
-        nx, ny, nz = self.dims
         parts = [_ptr(phi, np.float32), _ptr(u, np.float32), _ptr(v, np.float32), _ptr(w, np.float32),
                  _ptr(target_divergence, np.float32)]
         kinds = {p[1] for p in parts if p[1] is not None}
         if len(kinds) != 1:
             raise TypeError("all arrays must live in the same memory (host or device)")
         mem = kinds.pop()
-        shapes = [(nx, ny, nz), (nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+        shapes = self.shapes()
         for arr, shp in zip((phi, u, v, w), shapes):
             if tuple(arr.shape) != shp:
                 raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
         ins = Inputs(mem, 0, parts[0][0], parts[1][0], parts[2][0], parts[3][0], parts[4][0], float(dt), float(rho))
         if mem == MEMORY_HOST:
-            p = np.zeros((nx, ny, nz), np.float32)
+            p = np.zeros(shapes[0], np.float32)
             uo = np.zeros(shapes[1], np.float32)
             vo = np.zeros(shapes[2], np.float32)
             wo = np.zeros(shapes[3], np.float32)
@@ -306,7 +311,7 @@ class UniformPressureProjection:
     def probe_apply(self, phi, p):
         phi = np.ascontiguousarray(phi, np.float32)
         p = np.ascontiguousarray(p, np.float64)
-        out = np.zeros(self.dims, np.float64)
+        out = np.zeros(self.shapes()[0], np.float64)
         _check(_lib.fp_probe_apply(self._g, phi.ctypes.data, p.ctypes.data, out.ctypes.data))
         return out
 
@@ -315,7 +320,7 @@ class UniformPressureProjection:
         q = np.ascontiguousarray(target_divergence, np.float32) if target_divergence is not None else None
         ins = Inputs(MEMORY_HOST, 0, arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data,
                      arrs[3].ctypes.data, q.ctypes.data if q is not None else None, float(dt), float(rho))
-        b = np.zeros(self.dims, np.float64)
+        b = np.zeros(self.shapes()[0], np.float64)
         _check(_lib.fp_probe_rhs(self._g, C.byref(ins), b.ctypes.data))
         return b
 
@@ -344,6 +349,59 @@ class UniformPressureProjection:
             self.destroy()
         except Exception:
             pass
+
+
+class TallCellPressureProjection(UniformPressureProjection):
+    """One resident restricted tall-cell grid: nx x nz columns of fine_ny rows
+    of cubic cells h, each column a tall cell of `bases[x, z]` rows under a
+    band of `band` cubic cells (fluid_amd.tall_layout for the sample and slot
+    conventions).  project() takes phi on the samples (nx, band+2, nz) and
+    the slot velocities u (nx+1, S, nz), v (nx, band+2, nz), w (nx, S, nz+1)."""
+
+    def __init__(self, bases, fine_ny, band, spacing, max_delta: int = 4, context: Context | None = None):
+        bases = np.ascontiguousarray(bases, np.int32)
+        if bases.ndim != 2:
+            raise ValueError("bases must be (nx, nz)")
+        nx, nz = bases.shape
+        self.ctx = context or default_context()
+        self.bases = bases
+        self.fine_ny, self.band, self.max_delta = int(fine_ny), int(band), int(max_delta)
+        self.h = float(spacing)
+        self.dims = (nx, self.fine_ny, nz)
+        desc = GridDesc(LAYOUT_TALL_CELL, nx, self.fine_ny, nz, self.band, self.max_delta, self.h,
+                        bases.ctypes.data)
+        self._g = C.c_void_p()
+        rc = load_library().fp_grid_create(self.ctx.handle, C.byref(desc), C.byref(self._g))
+        if rc == -1 or rc == -4:
+            raise ValueError(_lib.fp_last_error().decode())
+        _check(rc)
+        self.diagnostics = None
+
+    @classmethod
+    def from_layout(cls, layout, context: Context | None = None):
+        return cls(layout.bases, layout.fine_ny, layout.band, layout.h, layout.max_delta, context)
+
+    def shapes(self):
+        nx, _, nz = self.dims
+        P, S = self.band + 2, self.band + 2 + self.max_delta
+        return [(nx, P, nz), (nx + 1, S, nz), (nx, P, nz), (nx, S, nz + 1)]
+
+    def submit(self, *a, **k):
+        raise FluidProjectionError(-7, "tall-cell grids are projected with project() / project_batch()")
+
+    def probe_vcycle(self, *a, **k):
+        raise FluidProjectionError(-1, "the V-cycle probe covers uniform grids")
+
+    def probe_level_phi(self, *a, **k):
+        raise FluidProjectionError(-1, "the level-set probe covers uniform grids")
+
+    def probe_layout(self, phi):
+        """(fine row, unknown flag) of every sample, computed on the device."""
+        phi = np.ascontiguousarray(phi, np.float32)
+        rows = np.zeros(self.shapes()[0], np.int32)
+        unk = np.zeros(self.shapes()[0], np.uint8)
+        _check(_lib.fp_probe_tall_layout(self._g, phi.ctypes.data, rows.ctypes.data, unk.ctypes.data))
+        return rows, unk.astype(bool)
 
 
 def project_scene(scene, **opts) -> ProjectionResult:

@@ -303,7 +303,8 @@ def tall_wave_scene(n: int = 512, fine_ny: int = 512, band: int = 32, seed: int 
     J = np.clip(J, -1, fine_ny - 1)
     lowest = np.where(J >= 0, J, J + 1)
     highest = np.where(J + 1 < fine_ny, J + 1, J)
-    raw = tl.choose_base(lowest, highest, fine_ny, band, tl.LIQUID_HALO, tl.AIR_HALO)
+    # halos clamped to the band as createTallCellLayout does (tall-cell-grid.ts:348)
+    raw = tl.choose_base(lowest, highest, fine_ny, band, min(tl.LIQUID_HALO, band), min(tl.AIR_HALO, band))
     bases = tl.limit_bases(raw, max_delta)
     layout = tl.TallCellLayout(bases, fine_ny, band, h, max_delta)
     yc = layout.sample_heights() * h
@@ -336,7 +337,7 @@ def tall_dam_scene(n: int = 256, fine_ny: int | None = None, band: int = 32, see
     J = min(max(J, -1), ny - 1)
     max_base = max(0, ny - band)
     wet_base = tl.choose_base(np.full((n, n), J), np.full((n, n), min(J + 1, ny - 1)), ny, band,
-                              tl.LIQUID_HALO, tl.AIR_HALO)
+                              min(tl.LIQUID_HALO, band), min(tl.AIR_HALO, band))
     dry_base = min(2, max_base) if max_base <= 3 else max_base
     raw = np.where(inside & (J >= 0), wet_base, max(2, dry_base)).astype(np.int32)
     bases = tl.limit_bases(raw, max_delta)

@@ -28,9 +28,16 @@
  *     input values are ignored and they are written as 0.
  *   Tall-cell grid (nx, fine_ny, nz, band B): packed samples (nx, B+2, nz);
  *     packed row 0 is the tall-cell bottom sample, 1 its top sample, 2+k the
- *     cubic band cell base+k (lib/core/tall-cell-grid.ts:206-249 indexing,
- *     transposed to z-fastest).  Face layouts are described at
- *     fp_tall_face_counts below.
+ *     cubic band cell base+k (lib/core/tall-cell-grid.ts:171-190 indexing,
+ *     transposed to z-fastest).  Velocities live in face slots: v (nx, B+2, nz)
+ *     with slot 0 the velocity of the base-1 faces inside the tall cell, 1 the
+ *     face tall top -> band row 0, 1+k the face band k-1 -> k, B+1 the face
+ *     band top -> air; u (nx+1, S, nz) and w (nx, S, nz+1), S = B+2+D (D the
+ *     max neighbour base delta), between columns A (-) and N (+): slots 0 and 1
+ *     the mean and per-row slope (about row (m-1)/2, m = min base) of an affine
+ *     velocity over fine rows 0..m-1, slot 2+j the single fine row m+j up to
+ *     the higher band top; other slots are unused and written as 0.  Pressure
+ *     is linear between the two samples of a tall cell.
  *   Liquid iff phi < 0.  Pressure is in Pa, velocity in m/s, h in m, dt in s,
  *   rho in kg/m^3.
  *
@@ -89,9 +96,10 @@ typedef struct {
   int32_t ny;       /* uniform: cells along y; tall-cell: fine rows (fineNy) */
   int32_t nz;       /* cells along z */
   int32_t band;     /* tall-cell: cubic band rows B (regularLayers); else 0 */
-  int32_t reserved;
+  int32_t reserved; /* tall-cell: max neighbour base delta D (0: 4, maximumNeighborDelta); else 0 */
   double h;         /* cubic cell size (m) */
-  const int32_t* column_bases; /* tall-cell: (nx, nz) host array, tall height per column (>= 2) */
+  const int32_t* column_bases; /* tall-cell: (nx, nz) host array, tall height per column in [2, ny],
+                                   neighbours within D (limitNeighboringTallCellBases) */
 } fp_grid_desc;
 
 typedef struct {
@@ -213,6 +221,10 @@ int fp_probe_rhs(fp_grid* grid, const fp_inputs* in, double* b);
 int fp_probe_vcycle(fp_grid* grid, const float* phi, const fp_options* options, const float* r, float* z);
 /* The level-l level set of the multigrid hierarchy (FP32). */
 int fp_probe_level_phi(fp_grid* grid, const float* phi, int32_t level, int32_t phi_preserve_levels, float* out);
+/* Tall-cell layout maps: the fine row of every packed sample and whether it
+ * is an unknown (liquid and inside the domain).  No reference counterpart
+ * (the reference computes these in buildInitialPhi, tall-cell-grid.ts:171-190). */
+int fp_probe_tall_layout(fp_grid* grid, const float* phi, int32_t* rows, uint8_t* unknown);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,222 @@
+"""Restricted tall-cell projection on the GPU against the FP64 oracle.
+
+Bars: sample rows and unknown flags bit-exact; the FP64 operator and the
+right-hand side to 1e-12; pressures and slot velocities within 1e-5 relative
+L2 of the FP64 oracle solve; max |div| of the FP32 output no larger than the
+oracle's at the same tolerance; the requested FP64 relative residual reached.
+"""
+import numpy as np
+import pytest
+
+from fluid_amd import scenes
+from fluid_amd import tall_layout as tl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+SCENES = {
+    "random": lambda: scenes.tall_random_scene(16, 12, 40, 8, seed=5),
+    "mixed": lambda: scenes.tall_random_scene(14, 10, 36, 8, seed=6, deep_mixed=True),
+    "lid": lambda: scenes.tall_random_scene(6, 5, 14, 12, seed=7),
+    "dam": lambda: scenes.tall_dam_scene(24, 32, band=12, fill=(0.5, 0.6, 0.45), seed=8),
+    "wave": lambda: scenes.tall_wave_scene(32, 64, band=16, seed=9),
+}
+
+
+@pytest.fixture(scope="module", params=list(SCENES))
+def case(request, oracle):
+    sc = SCENES[request.param]()
+    L = sc.layout
+    t = oracle.Tall(L.bases, sc.phi, L.fine_ny, L.band, L.h, L.max_delta)
+    return request.param, sc, t
+
+
+@pytest.fixture(scope="module")
+def oracle_solutions():
+    return {}
+
+
+def oracle_projection(oracle_solutions, name, sc, t, tol):
+    key = (name, tol)
+    if key not in oracle_solutions:
+        oracle_solutions[key] = t.projection(sc.u, sc.v, sc.w, sc.dt, sc.rho, rel_tol=tol)
+    return oracle_solutions[key]
+
+
+def grid_of(fpl, sc):
+    return fpl.TallCellPressureProjection.from_layout(sc.layout)
+
+
+def test_layout_maps_bit_exact(fpl, case):
+    name, sc, t = case
+    g = grid_of(fpl, sc)
+    try:
+        rows, unk = g.probe_layout(sc.phi)
+        assert np.array_equal(rows, sc.layout.sample_rows())
+        assert np.array_equal(unk, t.unknowns())
+        dims, strides = g.levels()
+        assert tuple(dims[0]) == (sc.layout.nx, sc.layout.packed_ny, sc.layout.nz)
+        assert dims[-1].prod() <= 512
+    finally:
+        g.destroy()
+
+
+def test_operator_matches_oracle(fpl, case):
+    name, sc, t = case
+    g = grid_of(fpl, sc)
+    try:
+        rng = np.random.default_rng(1)
+        p = rng.standard_normal(sc.phi.shape) * t.unknowns()
+        assert rel_l2(g.probe_apply(sc.phi, p), t.apply(p)) < 1e-12
+    finally:
+        g.destroy()
+
+
+def test_rhs_matches_oracle(fpl, case):
+    name, sc, t = case
+    g = grid_of(fpl, sc)
+    try:
+        b = g.probe_rhs(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho)
+        assert rel_l2(b, t.rhs(sc.u, sc.v, sc.w, sc.dt, sc.rho)) < 1e-12
+    finally:
+        g.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_projection_parity(fpl, oracle_solutions, case, precision):
+    name, sc, t = case
+    op, (ou, ov, ow), ob, orep = oracle_projection(oracle_solutions, name, sc, t, 1e-12)
+    g = grid_of(fpl, sc)
+    try:
+        res = g.project(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho, precision=precision, relative_tolerance=1e-6)
+        rep = res.report
+        print(name, precision, {k: rep[k] for k in ("iterations", "outer_iterations", "levels",
+                                                    "relative_residual", "max_div_after", "flags")})
+        assert rep["flags"] == 0, rep
+        assert rep["relative_residual"] <= 1e-6
+        err = rel_l2(res.pressure, op)
+        print("pressure rel L2 vs FP64 oracle", err)
+        assert err < 1e-5
+        for a, b in ((res.u, ou), (res.v, ov), (res.w, ow)):
+            assert rel_l2(a, b) < 1e-5
+        gdiv = t.max_divergence(res.u.astype(np.float64), res.v.astype(np.float64), res.w.astype(np.float64))
+        assert gdiv == pytest.approx(rep["max_div_after"], rel=1e-3, abs=1e-6)
+        _, (u6, v6, w6), _, _ = oracle_projection(oracle_solutions, name, sc, t, 1e-6)
+        odiv = t.max_divergence(*(x.astype(np.float32).astype(np.float64) for x in (u6, v6, w6)))
+        print("max div gpu", gdiv, "oracle at the same tolerance", odiv)
+        assert gdiv <= odiv
+        assert rep["max_div_before"] == pytest.approx(orep["max_div_before"], rel=1e-5)
+        assert rep["liquid_samples"] == int(t.unknowns().sum())
+    finally:
+        g.destroy()
+
+
+def test_tight_tolerance(fpl, oracle_solutions, case):
+    name, sc, t = case
+    op = oracle_projection(oracle_solutions, name, sc, t, 1e-12)[0]
+    g = grid_of(fpl, sc)
+    try:
+        res = g.project(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho, relative_tolerance=1e-10)
+        assert res.report["relative_residual"] <= 1e-10
+        assert rel_l2(res.pressure, op) < 1e-7
+    finally:
+        g.destroy()
+
+
+def test_hydrostatic_tall_columns(fpl):
+    """Flat surface and the gravity impulse: p = rho g h (theta + J - row)."""
+    nx, nz, ny, B = 24, 20, 96, 16
+    h = 1.0 / ny
+    s_height = 0.63 * ny
+    # bases 47..53: the band rows base..base+15 hold the surface row 59
+    raw = np.full((nx, nz), 50, np.int32) + np.random.default_rng(3).integers(-3, 4, (nx, nz)).astype(np.int32)
+    bases = tl.limit_bases(raw, 4)
+    L = tl.TallCellLayout(bases, ny, B, h)
+    yc = L.sample_heights()
+    phi = np.where(L.active(), np.clip((yc - s_height) * h, -5 * h, 5 * h), 5 * h).astype(np.float32)
+    dt, rho, grav = 1 / 120, 998.2, 9.81
+    fu, fv, fw = scenes.tall_fluid_slots(L, phi)
+    u = np.zeros(fu.shape, np.float32)
+    w = np.zeros(fw.shape, np.float32)
+    v = np.where(fv, -grav * dt, 0.0).astype(np.float32)
+    g = fpl.TallCellPressureProjection.from_layout(L)
+    try:
+        res = g.project(phi, u, v, w, dt, rho, relative_tolerance=1e-9)
+        J = int(np.floor(s_height - 0.5))
+        phl = float(np.float32((J + 0.5 - s_height) * h))
+        pha = float(np.float32((J + 1.5 - s_height) * h))
+        theta = min(1.0, max(0.05, abs(phl) / (abs(phl) + abs(pha))))
+        rows = L.sample_rows()
+        liq = (phi < 0) & L.active()
+        exact = rho * grav * h * (theta + J - rows)
+        assert rel_l2(res.pressure[liq], exact[liq]) < 1e-6
+        assert np.abs(res.v).max() < 1e-4 and np.abs(res.u).max() < 1e-4 and np.abs(res.w).max() < 1e-4
+    finally:
+        g.destroy()
+
+
+def test_batch_equals_alone_and_mixes_layouts(fpl):
+    tall = [scenes.tall_random_scene(16, 12, 40, 8, seed=11), scenes.tall_wave_scene(24, 48, band=12, seed=12)]
+    uni = scenes.dam_break_scene(24, seed=13)
+    grids = [grid_of(fpl, s) for s in tall] + [fpl.UniformPressureProjection(uni.shape, uni.h)]
+    reqs = [dict(phi=s.phi, u=s.u, v=s.v, w=s.w, dt=s.dt, rho=s.rho) for s in tall + [uni]]
+    try:
+        batch = fpl.project_batch(grids, reqs, relative_tolerance=1e-6)
+        for g, rq, rb in zip(grids, reqs, batch):
+            ra = g.project(rq["phi"], rq["u"], rq["v"], rq["w"], rq["dt"], rq["rho"], relative_tolerance=1e-6)
+            assert rb.report["flags"] == 0
+            assert rb.report["iterations"] == ra.report["iterations"]
+            for a, b in ((ra.pressure, rb.pressure), (ra.u, rb.u), (ra.v, rb.v), (ra.w, rb.w)):
+                assert np.array_equal(a, b)
+    finally:
+        for g in grids:
+            g.destroy()
+
+
+def test_warm_start_and_device_memory(fpl):
+    import torch
+    sc = scenes.tall_wave_scene(32, 64, band=16, seed=14)
+    g = grid_of(fpl, sc)
+    try:
+        cold = g.project(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho, relative_tolerance=1e-7)
+        warm = g.project(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho, relative_tolerance=1e-7, warm_start=1)
+        assert warm.report["iterations"] < cold.report["iterations"]
+        assert rel_l2(warm.pressure, cold.pressure) < 1e-6
+        dev = torch.device("cuda:0")
+        t = [torch.from_numpy(a).to(dev) for a in (sc.phi, sc.u, sc.v, sc.w)]
+        res = g.project(*t, sc.dt, sc.rho, relative_tolerance=1e-7)
+        torch.cuda.synchronize()
+        assert np.array_equal(res.pressure.cpu().numpy(), cold.pressure)
+        assert np.array_equal(res.u.cpu().numpy(), cold.u)
+    finally:
+        g.destroy()
+
+
+def test_layout_errors(fpl):
+    bases = np.full((6, 5), 10, np.int32)
+    bases[2, 2] = 20  # neighbour delta 10 > 4
+    with pytest.raises(ValueError):
+        fpl.TallCellPressureProjection(bases, 40, 8, 1 / 40)
+    bases[2, 2] = 1  # below the minimum tall cell
+    with pytest.raises(ValueError):
+        fpl.TallCellPressureProjection(bases, 40, 8, 1 / 40)
+
+
+@pytest.mark.parametrize("n,ny,band", [(64, 96, 16), (128, 128, 32)])
+def test_iteration_counts_stay_multigrid_like(fpl, n, ny, band):
+    sc = scenes.tall_wave_scene(n, ny, band=band, seed=15)
+    g = grid_of(fpl, sc)
+    try:
+        res = g.project(sc.phi, sc.u, sc.v, sc.w, sc.dt, sc.rho, relative_tolerance=1e-6)
+        print(n, ny, band, res.report["iterations"], res.report["levels"])
+        assert res.report["flags"] == 0
+        assert res.report["iterations"] <= 40
+    finally:
+        g.destroy()
