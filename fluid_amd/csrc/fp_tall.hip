@@ -18,9 +18,10 @@
 // true operator (each face touches two blocks, so A <= 2 blockdiag(A) and any
 // damping below 1 keeps the cycle SPD).  Coarser levels: a 7-point lattice
 // surrogate (tall rows kept as two layers, band rows aggregated in pairs,
-// columns in 2 x 2), coarsened by aggregation with the Galerkin sums halved
-// along coarsened axes, smoothed by red-black Gauss-Seidel; dense FP64
-// inverse on the coarsest lattice.
+// columns in 2 x 2), whose couplings are the aggregation (Galerkin) sums
+// halved along coarsened axes, smoothed by red-black Gauss-Seidel, with
+// cell-centred linear transfers across columns; dense FP64 inverse on the
+// coarsest lattice.
 #include "fp_tall.hpp"
 #include "fp_state.hpp"
 
@@ -262,12 +263,19 @@ struct VecDir {
   __device__ __forceinline__ float operator()(size_t i) const { return z[i] + beta * d[i]; }
 };
 
-// Sample-index decomposition of a grid-stride loop over (nx, P, nz).
-#define TALL_SAMPLE(idx, x, py, z)                      \
-  const int z = (int)((idx) % (size_t)g.nz);            \
-  const size_t t_ = (idx) / (size_t)g.nz;               \
-  const int py = (int)(t_ % (size_t)g.P);               \
-  const int x = (int)(t_ / (size_t)g.P);
+// Sample loops run over lines of FP_BLOCK samples along z at one (x, packed
+// row): the line index is uniform across the workgroup (scalar division),
+// each lane takes one z, and a wave shares its packed row (no divergence
+// between tall and band rows).
+#define TALL_LINES(g) for (int line_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK; \
+                           line_ < (g).nx * (g).P * zch_; line_ += nblk)
+#define TALL_LINE(g, idx, x, py, z)                       \
+  const int xp_ = line_ / zch_;                           \
+  const int z = (line_ - xp_ * zch_) * FP_BLOCK + (int)threadIdx.x; \
+  const int py = xp_ % (g).P;                             \
+  const int x = xp_ / (g).P;                              \
+  if (z >= (g).nz) continue;                              \
+  const size_t idx = ((size_t)x * (g).P + py) * (g).nz + z;
 
 // --------------------------------------------------------------------------
 // set-up
@@ -397,8 +405,8 @@ __global__ void k_tall_diag(const TallPart* parts, int np) {
   fp_f32x4* __restrict__ binv = G.blk;
   fp_f32x4* __restrict__ w0 = G.w0;
   double* __restrict__ xv = G.x;
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     const int b = bases[g.col(x, z)];
     const bool unk = is_unknown(g, phi, b, x, py, z);
     if (!unk) xv[idx] = 0.0;
@@ -661,8 +669,8 @@ __global__ void k_tall_rhs(const TallPart* parts, int np, int mode) {
   const double sf = G.s_flux, inv_h = G.inv_h;
   double acc[2] = {0.0, 0.0};
   float dmax = 0.0f;
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     const int b = bases[g.col(x, z)];
     if (py == 1) continue;  // written with the bottom sample
     if (py >= 2) {
@@ -720,8 +728,8 @@ __global__ void k_tall_true(const TallPart* parts, int np) {
   const VecD xv{G.x};
   double acc[1] = {0.0};
   float rmax = 0.0f;
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     if (py == 1) continue;
     const int b = bases[g.col(x, z)];
     if (py >= 2) {
@@ -761,8 +769,8 @@ __global__ void k_tall_jacobi(const TallPart* parts, int np, int sel, int first)
   float* __restrict__ zout = first ? G.z : (sel ? G.z : G.z2);
   const float om = G.omega;
   const VecF zv{zin};
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     if (py == 1) continue;
     const int b = bases[g.col(x, z)];
     if (py >= 2) {
@@ -806,8 +814,8 @@ __global__ void k_tall_residual(const TallPart* parts, int np, int sel) {
   const float* __restrict__ r = G.r;
   const VecF zv{sel ? G.z2 : G.z};
   float* __restrict__ t = G.t;
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     if (py == 1) continue;
     const int b = bases[g.col(x, z)];
     if (py >= 2) {
@@ -842,8 +850,8 @@ __global__ void k_tall_direction(const TallPart* parts, int np) {
   float* __restrict__ q = G.q;
   const VecDir dv{zz, dold, (float)st->beta};
   double acc[1] = {0.0};
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     if (py == 1) continue;
     const int b = bases[g.col(x, z)];
     if (py >= 2) {
@@ -930,27 +938,54 @@ __global__ void k_tall_accumulate(const TallPart* parts, int np) {
   }
 }
 
-// ---- transfers between the sample level and lattice level 1 ---------------
-// r1 = P^T t (sum over each aggregate's samples).
+// ---- transfers ----------------------------------------------------------------
+// Cell-centred linear interpolation along x and z (the uniform grids' taps),
+// injection of the tall rows and of each aggregated pair of band rows along y;
+// restriction is its transpose (sums, not means: the lattice couplings are
+// Galerkin sums).
+__device__ __forceinline__ float restrict_gather(const float* __restrict__ t, int nxf, int nyf, int nzf, int nxc,
+                                                 int nzc, int X, int Y, int Z, int cy) {
+  int fx[4], fz[4];
+  float wx[4], wz[4];
+  const int na = rtaps(X, nxf > 1 ? 2 : 1, nxf, nxc, fx, wx);
+  const int nc = rtaps(Z, nzf > 1 ? 2 : 1, nzf, nzc, fz, wz);
+  int y0, y1;
+  member_rows(Y, cy, nyf, y0, y1);
+  float s = 0.0f;
+  for (int a = 0; a < na; ++a)
+    for (int y = y0; y < y1; ++y) {
+      const size_t base = ((size_t)fx[a] * nyf + y) * nzf;
+      for (int c = 0; c < nc; ++c) s += wx[a] * wz[c] * t[base + fz[c]];
+    }
+  return s;
+}
+
+__device__ __forceinline__ float prolong_gather(const float* __restrict__ ec, int nxf, int nzf, int nxc, int nyc,
+                                                int nzc, int x, int y, int z, int cy) {
+  int cxs[2], czs[2];
+  float wx[2], wz[2];
+  const int na = ptaps(x, nxf > 1 ? 2 : 1, nxc, cxs, wx);
+  const int nc = ptaps(z, nzf > 1 ? 2 : 1, nzc, czs, wz);
+  const int Y = y < 2 ? y : 2 + (y - 2) / cy;
+  float s = 0.0f;
+  for (int a = 0; a < na; ++a) {
+    const size_t base = ((size_t)cxs[a] * nyc + Y) * nzc;
+    for (int c = 0; c < nc; ++c) s += wx[a] * wz[c] * ec[base + czs[c]];
+  }
+  return s;
+}
+
+// r1 = P^T t from the sample level.
 __global__ void k_tall_restrict0(const TallPart* parts, int np) {
   FP_PART(parts, np);
   if (G.st->done) return;
   const TallLevel& C = G.lv[1];
-  const float* __restrict__ t = G.t;
-  const int cy = C.cy;
   FP_STRIDE(ci, (size_t)C.n) {
     const int Z = (int)(ci % (size_t)C.nz);
     const size_t tt = ci / (size_t)C.nz;
     const int Y = (int)(tt % (size_t)C.ny);
     const int X = (int)(tt / (size_t)C.ny);
-    int y0, y1;
-    member_rows(Y, cy, G.P, y0, y1);
-    float s = 0.0f;
-    if (C.w[ci].w > 0.0f)
-      for (int x = 2 * X; x < min(2 * X + 2, G.nx); ++x)
-        for (int y = y0; y < y1; ++y)
-          for (int z = 2 * Z; z < min(2 * Z + 2, G.nz); ++z) s += t[((size_t)x * G.P + y) * G.nz + z];
-    C.r[ci] = s;
+    C.r[ci] = C.w[ci].w > 0.0f ? restrict_gather(G.t, G.nx, G.P, G.nz, C.nx, C.nz, X, Y, Z, C.cy) : 0.0f;
   }
 }
 
@@ -963,8 +998,8 @@ __global__ void k_tall_prolong0(const TallPart* parts, int np, int sel) {
   float* __restrict__ zz = sel ? G.z2 : G.z;
   const float* __restrict__ dinv = G.dinv;
   const fp_f32x4* __restrict__ binv = G.blk;
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     bool unk;
     if (py >= 2) {
       unk = dinv[idx] != 0.0f;
@@ -973,8 +1008,7 @@ __global__ void k_tall_prolong0(const TallPart* parts, int np, int sel) {
       unk = py == 0 ? (iv.x != 0.0f || iv.y != 0.0f) : (iv.z != 0.0f || iv.y != 0.0f);
     }
     if (!unk) continue;
-    const int Y = py < 2 ? py : 2 + (py - 2) / C.cy;
-    zz[idx] += C.e[((size_t)(x >> 1) * C.ny + Y) * C.nz + (z >> 1)];
+    zz[idx] += prolong_gather(C.e, g.nx, g.nz, C.nx, C.ny, C.nz, x, py, z, C.cy);
   }
 }
 
@@ -1056,14 +1090,7 @@ __global__ void k_lat_restrict(const TallPart* parts, int np, int level) {
     const size_t tt = ci / (size_t)C.nz;
     const int Y = (int)(tt % (size_t)C.ny);
     const int X = (int)(tt / (size_t)C.ny);
-    int y0, y1;
-    member_rows(Y, C.cy, F.ny, y0, y1);
-    float s = 0.0f;
-    if (C.w[ci].w > 0.0f)
-      for (int x = 2 * X; x < min(2 * X + 2, F.nx); ++x)
-        for (int y = y0; y < y1; ++y)
-          for (int z = 2 * Z; z < min(2 * Z + 2, F.nz); ++z) s += F.t[((size_t)x * F.ny + y) * F.nz + z];
-    C.r[ci] = s;
+    C.r[ci] = C.w[ci].w > 0.0f ? restrict_gather(F.t, F.nx, F.ny, F.nz, C.nx, C.nz, X, Y, Z, C.cy) : 0.0f;
   }
 }
 
@@ -1079,8 +1106,7 @@ __global__ void k_lat_prolong(const TallPart* parts, int np, int level) {
     const size_t tt = i / (size_t)F.nz;
     const int y = (int)(tt % (size_t)F.ny);
     const int x = (int)(tt / (size_t)F.ny);
-    const int Y = y < 2 ? y : 2 + (y - 2) / C.cy;
-    F.e[i] += C.e[((size_t)(x >> 1) * C.ny + Y) * C.nz + (z >> 1)];
+    F.e[i] += prolong_gather(C.e, F.nx, F.nz, C.nx, C.ny, C.nz, x, y, z, C.cy);
   }
 }
 
@@ -1108,8 +1134,8 @@ __global__ void k_tall_pressure_out(const TallPart* parts, int np) {
   float* __restrict__ p = G.out_p;
   if (!p) return;
   const TallGeo g = geo_of(G);
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     p[idx] = is_unknown(g, G.phi, G.bases[g.col(x, z)], x, py, z) ? (float)G.x[idx] : 0.0f;
   }
 }
@@ -1216,8 +1242,8 @@ __global__ void k_tall_subtract(const TallPart* parts, int np, int axis) {
 __global__ void k_tall_layout(const TallPart* parts, int np, int* rows, unsigned char* unk) {
   FP_PART(parts, np);
   const TallGeo g = geo_of(G);
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     const int b = G.bases[g.col(x, z)];
     rows[idx] = py == 0 ? 0 : (py == 1 ? b - 1 : b + py - 2);
     unk[idx] = is_unknown(g, G.phi, b, x, py, z) ? 1 : 0;
@@ -1229,8 +1255,8 @@ __global__ void k_tall_apply_d(const TallPart* parts, int np, const double* p, d
   FP_PART(parts, np);
   const TallGeo g = geo_of(G);
   const VecD pv{p};
-  FP_STRIDE(idx, (size_t)G.n) {
-    TALL_SAMPLE(idx, x, py, z);
+  TALL_LINES(g) {
+    TALL_LINE(g, idx, x, py, z);
     if (py == 1) continue;
     const int b = G.bases[g.col(x, z)];
     if (py >= 2) {

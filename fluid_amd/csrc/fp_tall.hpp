@@ -16,7 +16,7 @@
 namespace fp {
 
 constexpr int kTallMaxLevels = 14;
-constexpr int kTallMaxCoarse = 512;
+constexpr int kTallMaxCoarse = 64;  // nodes of the coarsest lattice (dense FP64 inverse)
 
 // Per-column FP64 face data: the weight moments sum w, sum w t, sum w t^2
 // (t = row - (m-1)/2) over the rows 0..m-1 shared with the +x and the +z
@@ -104,7 +104,6 @@ class TallGrid {
   TallRec* host_rec() { return rec_; }
   const TallRec* dev_rec() const { return dev_rec_; }
   int id() const { return id_; }
-  int pre = 2, post = 2;   // sweeps of the plan this record was set up for
 
  private:
   int alloc(void** p, size_t bytes);

@@ -424,26 +424,6 @@ __global__ void k_residual(const Part* parts, int np, int level) {
   }
 }
 
-// 1-D taps of the transpose of the prolongation (restriction gather).
-__device__ __forceinline__ int rtaps(int I, int s, int nf, int nc, int* f, float* w) {
-  if (s == 1) { f[0] = I; w[0] = 1.0f; return 1; }
-  int c = 0;
-  if (I >= 1) { f[c] = 2 * I - 1; w[c] = 0.25f; ++c; }
-  f[c] = 2 * I; w[c] = (I - 1 < 0) ? 1.0f : 0.75f; ++c;
-  if (2 * I + 1 < nf) { f[c] = 2 * I + 1; w[c] = (I + 1 >= nc) ? 1.0f : 0.75f; ++c; }
-  if (2 * I + 2 < nf) { f[c] = 2 * I + 2; w[c] = 0.25f; ++c; }
-  return c;
-}
-// 1-D taps of the prolongation for fine index f.
-__device__ __forceinline__ int ptaps(int f, int s, int nc, int* c, float* w) {
-  if (s == 1) { c[0] = f; w[0] = 1.0f; return 1; }
-  const int I = f >> 1;
-  const int J = (f & 1) ? I + 1 : I - 1;
-  if (J < 0 || J >= nc) { c[0] = I; w[0] = 1.0f; return 1; }
-  c[0] = I; w[0] = 0.75f; c[1] = J; w[1] = 0.25f;
-  return 2;
-}
-
 // r_{l+1} = R t_l on the coarse level's liquid cells (R = P^T / 2^k).
 template <typename T>
 __global__ void k_restrict(const Part* parts, int np, int level) {

@@ -63,7 +63,7 @@ def test_layout_maps_bit_exact(fpl, case):
         assert np.array_equal(unk, t.unknowns())
         dims, strides = g.levels()
         assert tuple(dims[0]) == (sc.layout.nx, sc.layout.packed_ny, sc.layout.nz)
-        assert dims[-1].prod() <= 512
+        assert dims[-1].prod() <= 64
     finally:
         g.destroy()
 
