@@ -21,25 +21,44 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
+# HBM bytes per launch of the timed kernel, measured with rocprofv3 FETCH_SIZE
+# and WRITE_SIZE (each in its own run, scaled by a 1 GiB copy probe: reads 2x,
+# writes 1x) -- profiles/uniform_384_sweep_and_tall_512_jacobi_hbm_traffic.md.
+# The bench cannot read hardware counters itself; a configuration without a
+# measurement reports null.
+MEASURED_TRAFFIC_MB = {
+    ("uniform", 384, "fp16", "smooth_fine", 2): 1246.1,
+    ("tall", 512, "fp32", "smooth_fine", 2): 381.8,
+}
 
-def kernel_bytes_per_cell(kernel: str, precision: str) -> float:
+
+def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float:
     """Algorithmic (compulsory) HBM bytes per grid cell for one launch.
 
-    smooth_fine: one fused red-black sweep over the finest level: read the
-    packed face coefficients (4 values of the storage type, 4s B), the FP32
-    right-hand side (4 B) and the old correction (s B; the first pre-smoothing
-    sweep skips it), write the new correction (s B).
+    smooth_fine: the fused red-black sweeps over the finest level, averaged
+    over the launches one V-cycle times (per side `sweeps`): each reads the
+    packed face coefficients (4 values of the storage type, 4s B) and the FP32
+    right-hand side (4 B) and writes the new correction (s B); all but the
+    first pre-smoothing sweep (zero guess) read the old correction (s B); the
+    first post-smoothing sweep also reads the coarse correction (s/8 B).
     residual_fine: read coefficients (4s), r (4), e (s); write t (s).
     cg_direction: read z (s), d_old (4), phi (4); write d (4), q (4).
     cg_update: read d, q, e, r (16); write e, r (8).
     """
     s = 4 if precision == "fp32" else 2
+    plain = 4 * s + 4 + 2 * s
+    smooth = (2 * sweeps * plain - s + s / 8) / (2 * sweeps) if sweeps > 0 else plain
     return {
-        "smooth_fine": 4 * s + 4 + s + s,
+        "smooth_fine": smooth,
         "cg_direction": s + 4 + 4 + 4 + 4,
         "cg_update": 24.0,
         "residual_fine": 4 * s + 4 + s + s,
     }[kernel]
+
+
+def traffic_bytes(key):
+    mb = MEASURED_TRAFFIC_MB.get(key)
+    return None if mb is None else round(mb * 1e6)
 
 
 def copy_probe(mib: int, reps: int) -> None:
@@ -182,8 +201,9 @@ def bench_tall(args):
                   "device_ms": round(rep["device_ms"], 3)},
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(avg_ms, 5), "bytes_per_sample": bps,
-                     "launches": kl},
+                     "traffic": traffic_bytes(("tall", args.tall_n, "fp32", args.timed_kernel, args.sweeps)),
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_sample": bps,
+                     "algorithmic_bytes": round(bps * samples), "launches": kl},
     }
     g.destroy()
     if not args.no_cpu_baseline:
@@ -385,7 +405,7 @@ def main():
     kms = sum(x["timed_kernel_ms"] for x in reports)
     kl = sum(x["timed_kernel_launches"] for x in reports)
     avg_ms = kms / max(kl, 1)
-    bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision)
+    bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps)
     achieved = bpc * cells / (avg_ms * 1e-3) / 1e9
     # error against a tight FP64-refined solve of the same system (1e-12)
     err = None
@@ -418,8 +438,9 @@ def main():
                   "device_ms": round(rep["device_ms"], 3)},
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc,
-                     "launches": kl},
+                     "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps)),
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc,
+                     "algorithmic_bytes": round(bpc * cells), "launches": kl},
     }
     g.destroy()
     if not args.no_cpu_baseline:

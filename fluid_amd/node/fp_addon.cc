@@ -119,6 +119,7 @@ napi_value create_grid(napi_env env, napi_callback_info info) {
   d.ny = (int32_t)get_number(env, argv[1], "ny", 0);
   d.nz = (int32_t)get_number(env, argv[1], "nz", 0);
   d.band = (int32_t)get_number(env, argv[1], "band", 0);
+  d.reserved = (int32_t)get_number(env, argv[1], "maxDelta", 0);
   d.h = get_number(env, argv[1], "h", 0.0);
   std::vector<int32_t> bases;
   napi_value cb;
@@ -257,6 +258,17 @@ napi_value project(napi_env env, napi_callback_info info) {
   ProjectJob* j = new ProjectJob();
   j->grid = (fp_grid*)unwrap_external(env, argv[0]);
   fp_default_options(&j->opt);
+  // array lengths the grid reads and writes: level-0 samples and the three face arrays
+  int32_t dims[3 * 16], levels = 0;
+  int64_t nf[3] = {0, 0, 0};
+  if (fp_grid_levels(j->grid, 16, dims, nullptr, &levels) != FP_OK ||
+      fp_grid_face_counts(j->grid, &nf[0], &nf[1], &nf[2]) != FP_OK) {
+    delete j;
+    return throw_fp(env, FP_ERR_INVALID_ARGUMENT);
+  }
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  const size_t want_in[5] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2], cells};
+  const size_t want_out[4] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2]};
   j->in.memory = FP_MEMORY_HOST;
   const char* in_names[5] = {"phi", "u", "v", "w", "targetDivergence"};
   const float** in_ptrs[5] = {&j->in.phi, &j->in.u, &j->in.v, &j->in.w, &j->in.target_divergence};
@@ -268,6 +280,11 @@ napi_value project(napi_env env, napi_callback_info info) {
     if (!get_f32(env, v, &p, &len)) {
       delete j;
       napi_throw_type_error(env, nullptr, "inputs must be Float32Array");
+      return nullptr;
+    }
+    if (len != want_in[k]) {
+      delete j;
+      napi_throw_range_error(env, nullptr, "input array length does not match the grid");
       return nullptr;
     }
     *in_ptrs[k] = p;
@@ -287,6 +304,11 @@ napi_value project(napi_env env, napi_callback_info info) {
     if (!get_f32(env, v, &p, &len)) {
       delete j;
       napi_throw_type_error(env, nullptr, "outputs must be Float32Array");
+      return nullptr;
+    }
+    if (len != want_out[k]) {
+      delete j;
+      napi_throw_range_error(env, nullptr, "output array length does not match the grid");
       return nullptr;
     }
     *out_ptrs[k] = p;

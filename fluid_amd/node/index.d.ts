@@ -50,6 +50,25 @@ export declare class UniformPressureProjection {
   destroy(): void;
 }
 
+export interface TallCellLayoutDesc {
+  nx: number;                      // columns along x (z-fastest arrays)
+  fineNy: number;                  // fine rows
+  nz: number;
+  band: number;                    // cubic band rows B
+  h: number;                       // cubic cell size (m)
+  columnBases: ArrayLike<number>;  // (nx, nz) tall heights in [2, fineNy]
+  maxDelta?: number;               // neighbour base limit (default 4)
+}
+
+export declare class TallCellPressureProjection {
+  constructor(layout: TallCellLayoutDesc, context?: unknown);
+  readonly levelCount: number;
+  diagnostics: ProjectionReport | undefined;
+  // phi (nx, B+2, nz); u (nx+1, S, nz); v (nx, B+2, nz); w (nx, S, nz+1); S = B+2+maxDelta
+  project(inputs: Omit<ProjectionInputs, 'targetDivergence'>, options?: ProjectionOptions): Promise<ProjectionResult>;
+  destroy(): void;
+}
+
 export declare const mi355xProjectionMethod: {
   id: 'mi355x-projection';
   label: string;

@@ -70,6 +70,51 @@ class UniformPressureProjection {
   }
 }
 
+/**
+ * Restricted tall-cell grid: nx x nz columns of fineNy rows of cubic cells h,
+ * each column a tall cell of columnBases[x * nz + z] rows under `band` cubic
+ * cells (the reference's createTallCellLayout output, tall-cell-grid.ts:309-436,
+ * with x and z named the other way round: this library's arrays are z-fastest).
+ * project() takes phi on the samples (nx, band+2, nz) and the face slots
+ * u (nx+1, S, nz), v (nx, band+2, nz), w (nx, S, nz+1), S = band+2+maxDelta.
+ */
+class TallCellPressureProjection {
+  constructor({ nx, fineNy, nz, band, h, columnBases, maxDelta = 4 }, context) {
+    this.context = context || defaultContext();
+    this.nx = nx; this.fineNy = fineNy; this.nz = nz; this.band = band; this.maxDelta = maxDelta;
+    this.h = h;
+    this.grid = binding.createGrid(this.context, {
+      layout: 1, nx, ny: fineNy, nz, band, h, maxDelta, columnBases: Int32Array.from(columnBases),
+    });
+    this.diagnostics = undefined;
+  }
+
+  get levelCount() { return binding.gridLevels(this.grid).length; }
+
+  async project({ phi, u, v, w, dt, rho }, options = {}) {
+    const P = this.band + 2, S = this.band + 2 + this.maxDelta;
+    const { nx, nz } = this;
+    const out = {
+      pressure: new Float32Array(nx * P * nz),
+      u: new Float32Array((nx + 1) * S * nz),
+      v: new Float32Array(nx * P * nz),
+      w: new Float32Array(nx * S * (nz + 1)),
+    };
+    const opts = Object.assign({}, options);
+    if (typeof opts.precision === 'string') opts.precision = PRECISION[opts.precision];
+    const res = await binding.project(this.grid, { phi, u, v, w, dt, rho }, out, opts);
+    this.diagnostics = res.report;
+    return res;
+  }
+
+  destroy() {
+    if (this.grid) {
+      binding.destroyGrid(this.grid);
+      this.grid = null;
+    }
+  }
+}
+
 // ---- SimulationMethod-shaped plugin (headless projection harness) -----------
 function damBreakScene(n, fill, seed) {
   // corner dam-break level set (box signed distance) and zero velocities
@@ -135,4 +180,4 @@ const mi355xProjectionMethod = {
   },
 };
 
-module.exports = { binding, UniformPressureProjection, mi355xProjectionMethod, PRECISION, FLAGS };
+module.exports = { binding, UniformPressureProjection, TallCellPressureProjection, mi355xProjectionMethod, PRECISION, FLAGS };
