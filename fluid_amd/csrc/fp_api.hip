@@ -220,7 +220,7 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
 
 int fp_grid_destroy(fp_grid* g) {
   if (!g) return FP_OK;
-  if (g->uniform && g->ctx->server && g->ctx->server->jobs_on(g->uniform) > 0) {
+  if (g->ctx->server && g->ctx->server->jobs_on(g->uniform ? (const void*)g->uniform : (const void*)g->tall) > 0) {
     set_error("fp_grid_destroy: the grid has submitted projections still running (fp_poll them first)");
     return FP_ERR_STATE;
   }
@@ -366,15 +366,15 @@ int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options
   int rc = validate_inputs(in, out, o);
   if (rc) return rc;
   fp_ctx* ctx = g->ctx;
-  if (g->tall) {
-    set_error("fp_submit: tall-cell grids are projected with fp_project / fp_project_batch");
+  if (g->tall && in->target_divergence) {
+    set_error("fp_submit: target_divergence is not supported on tall-cell grids");
     return FP_ERR_UNSUPPORTED;
   }
   if (!ctx->server) {
-    ctx->server = new (std::nothrow) fp::Server(ctx->stream, ctx->device, ctx->proj, &ctx->gpu);
+    ctx->server = new (std::nothrow) fp::Server(ctx->stream, ctx->device, ctx->proj, ctx->tall, &ctx->gpu);
     if (!ctx->server) return FP_ERR_OUT_OF_MEMORY;
   }
-  return ctx->server->submit(g->uniform, in, out, o, ticket);
+  return ctx->server->submit(g->uniform, g->tall, in, out, o, ticket);
 }
 
 int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report, double* latency_ms) {

@@ -4,6 +4,7 @@
 // the next round; every round advances each active projection by one stage
 // (Projector::serve_round), so new work overlaps running work on the device
 // and a grid's result does not depend on what it shares rounds with.
+#include "fp_tall.hpp"
 #include "fp_uniform.hpp"
 
 #include <chrono>
@@ -15,8 +16,8 @@ static double wall_seconds() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-Server::Server(hipStream_t s, int device, Projector* proj, std::mutex* gpu_mutex)
-    : stream_(s), device_(device), proj_(proj), gpu_mutex_(gpu_mutex) {
+Server::Server(hipStream_t s, int device, Projector* proj, TallProjector* tall, std::mutex* gpu_mutex)
+    : stream_(s), device_(device), proj_(proj), tall_(tall), gpu_mutex_(gpu_mutex) {
   thread_ = std::thread(&Server::loop, this);
 }
 
@@ -29,10 +30,11 @@ Server::~Server() {
   if (thread_.joinable()) thread_.join();
 }
 
-int Server::submit(UniformGrid* g, const fp_inputs* in, const fp_outputs* out, const fp_options* opt,
+int Server::submit(UniformGrid* g, TallGrid* t, const fp_inputs* in, const fp_outputs* out, const fp_options* opt,
                    uint64_t* ticket) {
   std::unique_ptr<Job> j(new Job());
   j->grid = g;
+  j->tall = t;
   j->in = *in;
   j->out = *out;
   j->opt = *opt;
@@ -47,11 +49,11 @@ int Server::submit(UniformGrid* g, const fp_inputs* in, const fp_outputs* out, c
   return FP_OK;
 }
 
-int Server::jobs_on(const UniformGrid* g) {
+int Server::jobs_on(const void* g) {
   std::lock_guard<std::mutex> lk(mu_);
   int n = 0;
-  for (const auto& j : incoming_) n += j->grid == g;
-  for (const auto& j : active_) n += j->grid == g;
+  for (const auto& j : incoming_) n += (const void*)j->grid == g || (const void*)j->tall == g;
+  for (const auto& j : active_) n += (const void*)j->grid == g || (const void*)j->tall == g;
   return n;
 }
 
@@ -99,12 +101,15 @@ void Server::loop() {
         incoming_.pop_front();
       }
     }
-    std::vector<Job*> jobs;
-    for (auto& j : active_) jobs.push_back(j.get());
-    int rc;
+    std::vector<Job*> jobs, tjobs;
+    for (auto& j : active_) (j->tall ? tjobs : jobs).push_back(j.get());
+    int rc = FP_OK;
     {
+      // uniform and tall-cell jobs advance in the same round, each kind in
+      // its own batched launch set
       std::lock_guard<std::mutex> g(*gpu_mutex_);
-      rc = proj_->serve_round(stream_, jobs);
+      if (!jobs.empty()) rc = proj_->serve_round(stream_, jobs);
+      if (rc == FP_OK && !tjobs.empty()) rc = tall_->serve_round(stream_, tjobs);
     }
     {
       std::lock_guard<std::mutex> lk(mu_);

@@ -1547,6 +1547,10 @@ struct TallProjector::Plan {
 };
 
 TallProjector::~TallProjector() {
+  if (round_arena_) (void)hipFree(round_arena_);
+  if (round_staging_) (void)hipHostFree(round_staging_);
+  if (round_states_) (void)hipFree(round_states_);
+  if (round_states_host_) (void)hipHostFree(round_states_host_);
   for (auto& kv : plans_) {
     Plan* p = kv.second;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
@@ -1578,18 +1582,13 @@ void TallProjector::forget(int grid_id) {
   }
 }
 
-TallProjector::Plan* TallProjector::plan_for(const std::vector<TallGrid*>& grids, const fp_options* opt) {
-  std::vector<long long> key;
-  for (TallGrid* g : grids) key.push_back(g->id());
-  key.push_back(opt->pre_sweeps);
-  key.push_back(opt->post_sweeps);
-  auto it = plans_.find(key);
-  if (it != plans_.end()) return it->second;
-  Plan* p = new Plan();
+void TallProjector::fill_tables(Plan* p, const std::vector<TallGrid*>& grids, int pre, int post) {
   p->grids = grids;
-  p->pre = opt->pre_sweeps;
-  p->post = opt->post_sweeps;
+  p->pre = pre;
+  p->post = post;
   p->maxL = 0;
+  p->host.clear();
+  p->pending.clear();
   std::vector<std::pair<TallGrid*, int>> one, samples, columns, faces[3];
   for (TallGrid* g : grids) {
     const TallRec& R = *g->host_rec();
@@ -1618,11 +1617,20 @@ TallProjector::Plan* TallProjector::plan_for(const std::vector<TallGrid*>& grids
     p->add(p->lv[l].nodes, nodes);
     p->add(p->lv[l].coarser, coarser);
   }
-  {
-    std::vector<std::pair<TallGrid*, int>> coarser;
-    for (TallGrid* g : grids) coarser.push_back({g, Plan::blocks_for((size_t)g->host_rec()->lv[1].n)});
-    p->add(p->lv[0].coarser, coarser);
-  }
+  std::vector<std::pair<TallGrid*, int>> coarser;
+  for (TallGrid* g : grids) coarser.push_back({g, Plan::blocks_for((size_t)g->host_rec()->lv[1].n)});
+  p->add(p->lv[0].coarser, coarser);
+}
+
+TallProjector::Plan* TallProjector::plan_for(const std::vector<TallGrid*>& grids, const fp_options* opt) {
+  std::vector<long long> key;
+  for (TallGrid* g : grids) key.push_back(g->id());
+  key.push_back(opt->pre_sweeps);
+  key.push_back(opt->post_sweeps);
+  auto it = plans_.find(key);
+  if (it != plans_.end()) return it->second;
+  Plan* p = new Plan();
+  fill_tables(p, grids, opt->pre_sweeps, opt->post_sweeps);
   bool ok = hipMalloc((void**)&p->arena, std::max<size_t>(1, p->host.size()) * sizeof(TallPart)) == hipSuccess;
   ok = ok && hipMemcpy(p->arena, p->host.data(), p->host.size() * sizeof(TallPart), hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMalloc((void**)&p->gathered, grids.size() * sizeof(State)) == hipSuccess;
@@ -1855,6 +1863,175 @@ int TallProjector::run(hipStream_t s, const std::vector<TallGrid*>& grids, const
       rep->liquid_samples = (int64_t)S.liquid;
       rep->timed_kernel_ms = timed_ms;
       rep->timed_kernel_launches = timed_launches;
+    }
+  }
+  return FP_OK;
+}
+
+// ---- continuous batching ---------------------------------------------------
+static double tall_now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int TallProjector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
+  struct Group {
+    fp_options opt;
+    std::vector<Job*> jobs;
+  };
+  auto key_of = [](const fp_options& o) { return ((long long)o.pre_sweeps << 16) | (long long)o.post_sweeps; };
+  std::map<long long, Group> setups, accum, iter, resid[2];
+  std::vector<Job*> finish, live;
+  for (Job* j : jobs) {
+    if (j->stage == Job::DONE) continue;
+    live.push_back(j);
+    const long long k = key_of(j->opt);
+    Group* g = nullptr;
+    if (j->stage == Job::SETUP) g = &setups[k];
+    else if (j->stage == Job::ACCUMULATE) g = &accum[k];
+    else if (j->stage == Job::ITERATE) g = &iter[k];
+    else finish.push_back(j);
+    if (g) {
+      g->opt = j->opt;
+      g->jobs.push_back(j);
+    }
+    if (j->stage == Job::SETUP || j->stage == Job::ACCUMULATE) {
+      Group& r = resid[j->passes < std::max(1, j->opt.max_outer) ? 1 : 0][k];
+      r.opt = j->opt;
+      r.jobs.push_back(j);
+    }
+  }
+  if (live.empty()) return FP_OK;
+  std::vector<std::unique_ptr<Plan>> plans;
+  auto make = [&](const std::vector<Job*>& js, const fp_options& o) {
+    std::vector<TallGrid*> gs;
+    for (Job* j : js) gs.push_back(j->tall);
+    plans.emplace_back(new Plan());
+    fill_tables(plans.back().get(), gs, o.pre_sweeps, o.post_sweeps);
+    return plans.back().get();
+  };
+  std::vector<std::pair<Plan*, const Group*>> setup_p, accum_p, iter_p, resid_p[2];
+  for (auto& kv : setups) setup_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (auto& kv : accum) accum_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (int np = 0; np < 2; ++np)
+    for (auto& kv : resid[np]) resid_p[np].push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  for (auto& kv : iter) iter_p.push_back({make(kv.second.jobs, kv.second.opt), &kv.second});
+  fp_options plain;
+  std::memset(&plain, 0, sizeof(plain));
+  Plan* fin_p = finish.empty() ? nullptr : make(finish, plain);
+  Plan* all_p = make(live, plain);
+  size_t total = 0;
+  for (auto& p : plans) total += p->host.size();
+  if (total > round_arena_cap_) {
+    if (round_arena_) (void)hipFree(round_arena_);
+    if (round_staging_) (void)hipHostFree(round_staging_);
+    round_arena_ = nullptr;
+    round_staging_ = nullptr;
+    const size_t cap = std::max<size_t>(total * 2, 1024);
+    HIPCHK(hipMalloc((void**)&round_arena_, cap * sizeof(TallPart)));
+    HIPCHK(hipHostMalloc((void**)&round_staging_, cap * sizeof(TallPart), hipHostMallocDefault));
+    round_arena_cap_ = cap;
+  }
+  if (live.size() > round_states_cap_) {
+    if (round_states_) (void)hipFree(round_states_);
+    if (round_states_host_) (void)hipHostFree(round_states_host_);
+    round_states_ = nullptr;
+    round_states_host_ = nullptr;
+    const size_t cap = std::max<size_t>(live.size() * 2, 64);
+    HIPCHK(hipMalloc((void**)&round_states_, cap * sizeof(State)));
+    HIPCHK(hipHostMalloc((void**)&round_states_host_, cap * sizeof(State), hipHostMallocDefault));
+    round_states_cap_ = cap;
+  }
+  size_t off = 0;
+  for (auto& p : plans) {
+    std::memcpy(round_staging_ + off, p->host.data(), p->host.size() * sizeof(TallPart));
+    for (const auto& pd : p->pending) pd.t->dev = round_arena_ + off + pd.off;
+    off += p->host.size();
+  }
+  HIPCHK(hipMemcpyAsync(round_arena_, round_staging_, std::max<size_t>(total, 1) * sizeof(TallPart),
+                        hipMemcpyHostToDevice, s));
+  // ---- set-up: records, face moments, block inverses, lattice, rhs ----
+  for (auto& sp : setup_p)
+    for (Job* j : sp.second->jobs) {
+      j->t_start = tall_now_seconds();
+      int rc = j->tall->bind(s, &j->in, &j->out, &j->opt);
+      if (rc) return rc;
+      if ((rc = j->tall->upload(s))) return rc;
+    }
+  for (auto& sp : setup_p) {
+    Plan& p = *sp.first;
+    FLAUNCH(k_reset_state<TallPart>, p.one, s);
+    PLAUNCH(k_tall_zero, p.samples, s, 0, 0);
+    setup(s, p);
+    PLAUNCH(k_tall_rhs, p.samples, s, 0);
+    FLAUNCH(k_fin_rhs<TallPart>, p.samples, s);
+    PLAUNCH(k_tall_zero, p.samples, s, 1, 0);
+    PLAUNCH(k_tall_zero, p.samples, s, 2, 0);
+  }
+  for (auto& ap : accum_p) {
+    PLAUNCH(k_tall_accumulate, ap.first->samples, s);
+    PLAUNCH(k_tall_zero, ap.first->samples, s, 2, 1);
+  }
+  for (int np = 0; np < 2; ++np)
+    for (auto& rp : resid_p[np]) {
+      PLAUNCH(k_tall_true, rp.first->samples, s);
+      FLAUNCH(k_fin_true<TallPart>, rp.first->samples, s, np);
+    }
+  for (auto& ip : iter_p) enqueue_iteration(s, *ip.first);
+  if (fin_p) {
+    for (int a = 0; a < 3; ++a) PLAUNCH(k_tall_subtract, fin_p->faces[a], s, a);
+    PLAUNCH(k_tall_rhs, fin_p->samples, s, 1);
+    FLAUNCH(k_tall_fin_divafter<TallPart>, fin_p->samples, s);
+    PLAUNCH(k_tall_pressure_out, fin_p->samples, s);
+    for (Job* j : finish) {
+      int rc = j->tall->finish_outputs(s, &j->in, &j->out);
+      if (rc) return rc;
+    }
+  }
+  hipLaunchKernelGGL(k_gather_states<TallPart>, dim3((unsigned)((live.size() + 255) / 256)), dim3(256), 0, s,
+                     all_p->one.dev, all_p->one.n, round_states_);
+  HIPCHK(hipMemcpyAsync(round_states_host_, round_states_, live.size() * sizeof(State), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const double t = tall_now_seconds();
+  for (size_t i = 0; i < live.size(); ++i) {
+    Job* j = live[i];
+    const State& S = round_states_host_[i];
+    switch (j->stage) {
+      case Job::SETUP:
+      case Job::ACCUMULATE: {
+        const bool more = j->passes < std::max(1, j->opt.max_outer);
+        j->passes += 1;
+        j->stage = (S.outer_done || !more) ? Job::FINISH : Job::ITERATE;
+        break;
+      }
+      case Job::ITERATE:
+        if (S.done) j->stage = Job::ACCUMULATE;
+        break;
+      case Job::FINISH: {
+        const TallRec& R = *j->tall->host_rec();
+        fp_report* rep = &j->report;
+        std::memset(rep, 0, sizeof(*rep));
+        int flags = S.flags;
+        if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;
+        rep->flags = flags;
+        rep->iterations = S.total_iter;
+        rep->outer_iterations = S.outer;
+        rep->levels = R.L;
+        rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
+        rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
+        float fa, fb;
+        std::memcpy(&fa, &S.divmax_before, 4);
+        std::memcpy(&fb, &S.divmax_after, 4);
+        rep->max_div_before = fa;
+        rep->max_div_after = fb;
+        rep->device_ms = (t - j->t_start) * 1e3;
+        rep->samples = (int64_t)R.n;
+        rep->liquid_samples = (int64_t)S.liquid;
+        j->t_done = t;
+        j->stage = Job::DONE;
+        break;
+      }
+      default:
+        break;
     }
   }
   return FP_OK;

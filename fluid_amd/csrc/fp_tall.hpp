@@ -128,8 +128,12 @@ class TallProjector {
   int probe_apply(hipStream_t s, TallGrid* g, const float* phi, const double* p, double* out);
   int probe_rhs(hipStream_t s, TallGrid* g, const fp_inputs* in, double* b);
   int probe_layout(hipStream_t s, TallGrid* g, const float* phi, int* rows, unsigned char* unknown);
+  // One round of continuous batching over tall-cell jobs (the stages of
+  // Projector::serve_round).
+  int serve_round(hipStream_t s, std::vector<Job*>& jobs);
 
  private:
+  void fill_tables(Plan* p, const std::vector<TallGrid*>& grids, int pre, int post);
   Plan* plan_for(const std::vector<TallGrid*>& grids, const fp_options* opt);
   int setup(hipStream_t s, Plan& p);
   int enqueue_vcycle(hipStream_t s, Plan& p);
@@ -144,6 +148,12 @@ class TallProjector {
   std::vector<hipEvent_t> timer_events_;
   int* host_counts_ = nullptr;
   int* dev_counts_ = nullptr;
+  TallPart* round_arena_ = nullptr;
+  size_t round_arena_cap_ = 0;
+  TallPart* round_staging_ = nullptr;
+  State* round_states_ = nullptr;
+  State* round_states_host_ = nullptr;
+  size_t round_states_cap_ = 0;
 };
 
 }  // namespace fp

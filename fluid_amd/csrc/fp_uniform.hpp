@@ -204,11 +204,16 @@ class UniformGrid {
   size_t bytes_ = 0;
 };
 
-// One asynchronous projection request (fp_submit) and its progress.
+class TallGrid;
+class TallProjector;
+
+// One asynchronous projection request (fp_submit) and its progress; exactly
+// one of grid / tall is set.
 struct Job {
   enum Stage { SETUP, ITERATE, ACCUMULATE, FINISH, DONE };
   uint64_t ticket = 0;
   UniformGrid* grid = nullptr;
+  TallGrid* tall = nullptr;
   fp_inputs in{};
   fp_outputs out{};
   fp_options opt{};
@@ -281,12 +286,14 @@ class Projector {
 // the worker and of synchronous fp_project calls is serialised by gpu_mutex.
 class Server {
  public:
-  Server(hipStream_t s, int device, Projector* proj, std::mutex* gpu_mutex);
+  Server(hipStream_t s, int device, Projector* proj, TallProjector* tall, std::mutex* gpu_mutex);
   ~Server();
-  int submit(UniformGrid* g, const fp_inputs* in, const fp_outputs* out, const fp_options* opt, uint64_t* ticket);
+  // g: a UniformGrid or a TallGrid (the other NULL)
+  int submit(UniformGrid* g, TallGrid* t, const fp_inputs* in, const fp_outputs* out, const fp_options* opt,
+             uint64_t* ticket);
   // FP_OK with the report when done; 1 while pending (after waiting up to timeout_ms).
   int poll(uint64_t ticket, int timeout_ms, fp_report* rep, double* latency_ms);
-  int jobs_on(const UniformGrid* g);
+  int jobs_on(const void* grid);
   int pending();
 
  private:
@@ -294,6 +301,7 @@ class Server {
   hipStream_t stream_;
   int device_;
   Projector* proj_;
+  TallProjector* tall_;
   std::mutex* gpu_mutex_;
   std::mutex mu_;
   std::condition_variable work_cv_, done_cv_;

@@ -386,9 +386,6 @@ class TallCellPressureProjection(UniformPressureProjection):
         P, S = self.band + 2, self.band + 2 + self.max_delta
         return [(nx, P, nz), (nx + 1, S, nz), (nx, P, nz), (nx, S, nz + 1)]
 
-    def submit(self, *a, **k):
-        raise FluidProjectionError(-7, "tall-cell grids are projected with project() / project_batch()")
-
     def probe_vcycle(self, *a, **k):
         raise FluidProjectionError(-1, "the V-cycle probe covers uniform grids")
 

@@ -1,10 +1,12 @@
 """Request loop for served projections (host side of fp_submit / fp_poll).
 
 A stream of sessions arrives by a Poisson process.  Each session owns one grid
-for its lifetime and runs a fixed number of time steps; each step is one
-projection submitted to the context's continuous-batching worker, warm-started
-from the previous step's pressure, followed by a host-side gravity impulse on
-the projected velocities (on the device) that feeds the next step.  Grids are
+for its lifetime -- a cubic uniform grid, or a restricted tall-cell grid
+("tall<n>": n x n columns of n rows under a 32-row band) -- and runs a fixed
+number of time steps; each step is one projection submitted to the context's
+continuous-batching worker, warm-started from the previous step's pressure,
+followed by a gravity impulse on the projected velocities (on the device) that
+feeds the next step.  Grids are
 admitted against an HBM budget: a session whose grid does not fit waits in the
 admission queue until finished sessions free memory.  Latency is measured per
 projection from submit to completion.
@@ -25,7 +27,7 @@ from . import scenes
 @dataclass
 class Session:
     sid: int
-    n: int
+    n: object  # edge of a cubic grid, or "tall<n>"
     arrival: float
     steps_left: int
     scene: object = None
@@ -35,11 +37,14 @@ class Session:
     admitted: float | None = None
     latencies: list = field(default_factory=list)
     iterations: list = field(default_factory=list)
+    vmask: object = None
+    cells: int = 0
 
 
 class ServeLoop:
-    """Poisson arrivals of `steps`-step sessions of cubic grids drawn from
-    `mix` ({edge: probability}); returns latency / throughput statistics."""
+    """Poisson arrivals of `steps`-step sessions drawn from `mix`
+    ({edge or "tall<n>": probability}); returns latency / throughput
+    statistics."""
 
     def __init__(self, rate_hz: float, duration_s: float, mix=None, steps: int = 20, seed: int = 0,
                  hbm_budget_bytes: int = 200 << 30, precision: str = "fp16", tolerance: float = 1e-6):
@@ -47,7 +52,7 @@ class ServeLoop:
         self.torch = torch
         self.rate = rate_hz
         self.duration = duration_s
-        self.mix = mix or {64: 0.4, 128: 0.3, 192: 0.2}
+        self.mix = mix or {64: 0.4, 128: 0.3, 192: 0.2, "tall256": 0.1}
         self.steps = steps
         self.rng = np.random.default_rng(seed)
         self.ctx = fpl.Context(0, hbm_budget_bytes)
@@ -65,21 +70,33 @@ class ServeLoop:
             t += self.rng.exponential(1.0 / self.rate)
             if t >= self.duration:
                 return out
-            n = int(self.rng.choice(sizes, p=probs))
+            n = sizes[int(self.rng.choice(len(sizes), p=probs))]
             out.append(Session(sid, n, t, self.steps))
             sid += 1
 
     def _admit(self, s: Session) -> bool:
-        try:
-            s.grid = fpl.UniformPressureProjection((s.n,) * 3, 1.0 / s.n, context=self.ctx)
-        except (fpl.FluidProjectionError, ValueError):
-            return False  # over the HBM budget: wait
         fill = tuple(float(x) for x in np.random.default_rng(1000 + s.sid).uniform(0.2, 0.8, 3))
-        s.scene = scenes.dam_break_scene(s.n, seed=2000 + s.sid, fill=fill)
+        tall = isinstance(s.n, str)
+        try:
+            if tall:
+                n = int(s.n[4:])
+                s.scene = scenes.tall_dam_scene(n, n, band=32, seed=2000 + s.sid, fill=fill)
+                s.grid = fpl.TallCellPressureProjection.from_layout(s.scene.layout, context=self.ctx)
+            else:
+                s.grid = fpl.UniformPressureProjection((s.n,) * 3, 1.0 / s.n, context=self.ctx)
+        except (fpl.FluidProjectionError, ValueError):
+            s.scene = None
+            return False  # over the HBM budget: wait
+        if not tall:
+            s.scene = scenes.dam_break_scene(s.n, seed=2000 + s.sid, fill=fill)
         torch = self.torch
         phi, u, v, w = (torch.from_numpy(a).to(self.dev) for a in (s.scene.phi, s.scene.u, s.scene.v, s.scene.w))
         out = (torch.empty_like(phi), torch.empty_like(u), torch.empty_like(v), torch.empty_like(w))
         s.tensors = (phi, u, v, w, out)
+        s.cells = int(s.scene.phi.size)
+        if tall:
+            # gravity on every v slot that carries an interior face
+            s.vmask = torch.from_numpy(s.scene.layout.slot_masks()[1].astype(np.float32)).to(self.dev)
         return True
 
     def _submit(self, s: Session, first: bool):
@@ -95,7 +112,10 @@ class ServeLoop:
         u.copy_(uo)
         v.copy_(vo)
         w.copy_(wo)
-        v[:, 1:-1, :] -= 9.81 * s.scene.dt
+        if s.vmask is not None:
+            v.sub_(s.vmask * (9.81 * s.scene.dt))
+        else:
+            v[:, 1:-1, :] -= 9.81 * s.scene.dt
 
     def run(self):
         pending = collections.deque(self._arrivals())
@@ -130,7 +150,7 @@ class ServeLoop:
                 s.iterations.append(res.report["iterations"])
                 if res.report["flags"] != 0 or res.report["relative_residual"] > self.opts_cold.relative_tolerance:
                     raise RuntimeError(f"session {s.sid} step failed: {res.report}")
-                cells_solved += s.n ** 3
+                cells_solved += s.cells
                 solves += 1
                 s.steps_left -= 1
                 if s.steps_left > 0:
@@ -151,10 +171,13 @@ class ServeLoop:
         its = np.array([x for s in done for x in s.iterations], float)
         self.ctx.destroy()
         by_size = {}
-        for n in sorted({s.n for s in done}):
+        for n in sorted({s.n for s in done}, key=str):
             ls = np.array([x for s in done if s.n == n for x in s.latencies])
-            by_size[f"{n}^3"] = {"solves": int(ls.size), "p50_ms": round(float(np.percentile(ls, 50)), 3),
-                                 "p99_ms": round(float(np.percentile(ls, 99)), 3)}
+            its_n = np.array([x for s in done if s.n == n for x in s.iterations], float)
+            label = f"{n}^3" if not isinstance(n, str) else f"{n} columns x 32 band"
+            by_size[label] = {"solves": int(ls.size), "p50_ms": round(float(np.percentile(ls, 50)), 3),
+                              "p99_ms": round(float(np.percentile(ls, 99)), 3),
+                              "iterations_mean": round(float(its_n.mean()), 2)}
         return {
             "sessions": len(done),
             "solves": solves,

@@ -83,3 +83,33 @@ def test_destroy_refused_while_running(fpl):
             pytest.skip("the projection finished before destroy was attempted")
     finally:
         g.destroy()
+
+
+def test_tall_and_uniform_served_together_equal_alone(fpl):
+    """Tall-cell and uniform requests interleaved on one worker: each result
+    bitwise equal to its projection alone."""
+    tall = [scenes.tall_dam_scene(32, 48, band=16, seed=501, fill=(0.6, 0.5, 0.4)),
+            scenes.tall_wave_scene(24, 64, band=16, seed=502)]
+    uni = [scene(10, 32), scene(11, 40)]
+    grids = [fpl.TallCellPressureProjection.from_layout(t.layout) for t in tall] + \
+            [fpl.UniformPressureProjection(s.shape, s.h) for s in uni]
+    reqs = tall + uni
+    try:
+        alone = [g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp16") for g, s in zip(grids, reqs)]
+        order = [0, 2, 1, 3]
+        tickets = {}
+        for k, i in enumerate(order):
+            g, s = grids[i], reqs[i]
+            tickets[i] = g.submit(s.phi, s.u, s.v, s.w, s.dt, s.rho, precision="fp16")
+            if k == 1:
+                time.sleep(0.01)
+        for i, a in enumerate(alone):
+            r = tickets[i].wait(60000)
+            assert r is not None
+            assert r.report["flags"] == 0 and r.report["relative_residual"] <= 1e-6
+            assert r.report["iterations"] == a.report["iterations"]
+            for x, y in ((a.pressure, r.pressure), (a.u, r.u), (a.v, r.v), (a.w, r.w)):
+                assert np.array_equal(x, y)
+    finally:
+        for g in grids:
+            g.destroy()
