@@ -347,49 +347,6 @@ struct ZeroVec {
   __device__ __forceinline__ Real operator()(size_t) const { return (Real)0; }
 };
 
-// Coupling strength of sample (x, py, z) towards direction dir (0 -x, 1 +x,
-// 2 -z, 3 +z, 4 -y, 5 +y) in the lattice surrogate: horizontal band rows take
-// the mean weight of the two fine faces the lattice edge stands for, tall
-// rows half the zeroth face moment each, vertical edges the fine face (the
-// internal tall faces as sum w / (b-1)^2).  0 where the domain ends.
-__device__ float surrogate_edge(const TallGeo& g, const int* __restrict__ bases, const float* __restrict__ phi,
-                                const TallCol* __restrict__ cols, int x, int py, int z, int b, int dir) {
-  if (dir < 4) {
-    int xn, zn;
-    if (!neighbour(g, x, z, dir, xn, zn)) return 0.0f;
-    if (py < 2) return (float)(0.5 * face_moments(g, cols, x, z, dir)[0]);
-    const int bn = bases[g.col(xn, zn)];
-    float s = 0.0f;
-    const int ya = b + py - 2, yn = bn + py - 2;
-    if (ya < g.ny) {
-      const RowRep<float> r = row_rep<float>(g, phi, bn, xn, zn, ya);
-      s += fwr<float>(phi[g.sidx(x, py, z)], r.phi);
-    }
-    if (yn < g.ny) {
-      const RowRep<float> r = row_rep<float>(g, phi, b, x, z, yn);
-      s += fwr<float>(r.phi, phi[g.sidx(xn, py, zn)]);
-    }
-    return 0.5f * s;
-  }
-  const int y = py == 0 ? 0 : (py == 1 ? b - 1 : b + py - 2);
-  if (dir == 4) {  // down
-    if (py == 0) return 0.0f;
-    if (py == 1) {
-      const double k = 1.0 / (double)(b - 1);
-      return (float)(cols[g.col(x, z)].wint * k * k);
-    }
-    return fw<float>(phi[g.sidx(x, py - 1, z)], phi[g.sidx(x, py, z)]);
-  }
-  // up
-  if (py == 0) {
-    const double k = 1.0 / (double)(b - 1);
-    return (float)(cols[g.col(x, z)].wint * k * k);
-  }
-  if (y + 1 >= g.ny) return 0.0f;
-  if (py + 1 < g.P) return fw<float>(phi[g.sidx(x, py, z)], phi[g.sidx(x, py + 1, z)]);
-  return fwr<float>(phi[g.sidx(x, py, z)], (float)(5.0 * g.h));
-}
-
 __device__ __forceinline__ bool sample_unknown(const TallGeo& g, const int* __restrict__ bases,
                                                const float* __restrict__ phi, int x, int py, int z) {
   return is_unknown(g, phi, bases[g.col(x, z)], x, py, z);
@@ -403,7 +360,6 @@ __global__ void k_tall_diag(const TallPart* parts, int np) {
   const TallCol* __restrict__ cols = G.cols;
   float* __restrict__ dinv = G.dinv;
   fp_f32x4* __restrict__ binv = G.blk;
-  fp_f32x4* __restrict__ w0 = G.w0;
   double* __restrict__ xv = G.x;
   TALL_LINES(g) {
     TALL_LINE(g, idx, x, py, z);
@@ -432,157 +388,14 @@ __global__ void k_tall_diag(const TallPart* parts, int np) {
       }
       binv[g.col(x, z)] = inv;
     }
-    // lattice surrogate
-    fp_f32x4 wv = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (unk) {
-      float e[6];
-      float dg = 0.0f;
-#pragma unroll
-      for (int dir = 0; dir < 6; ++dir) {
-        e[dir] = surrogate_edge(g, bases, phi, cols, x, py, z, b, dir);
-        dg += e[dir];
-      }
-      wv.x = (x + 1 < g.nx && sample_unknown(g, bases, phi, x + 1, py, z)) ? e[1] : 0.0f;
-      wv.y = (py + 1 < g.P && is_unknown(g, phi, b, x, py + 1, z)) ? e[5] : 0.0f;
-      wv.z = (z + 1 < g.nz && sample_unknown(g, bases, phi, x, py, z + 1)) ? e[3] : 0.0f;
-      wv.w = dg;
-    }
-    w0[idx] = wv;
   }
 }
 
-// ---- lattice coarsening: level l -> l + 1 (l = 0 reads the surrogate) -------
-struct LatView {
-  int nx, ny, nz;
-  const fp_f32x4* w;
-  __device__ __forceinline__ size_t idx(int x, int y, int z) const { return ((size_t)x * ny + y) * nz + z; }
-};
+}  // namespace fp
 
-__device__ __forceinline__ LatView fine_view(const TallRec& R, int l) {
-  if (l == 0) return LatView{R.nx, R.P, R.nz, R.w0};
-  const TallLevel& L = R.lv[l];
-  return LatView{L.nx, L.ny, L.nz, L.w};
-}
+#include "fp_tall_galerkin.hpp"
 
-// Row range of coarse row Y in the finer lattice.
-__device__ __forceinline__ void member_rows(int Y, int cy, int nyf, int& y0, int& y1) {
-  if (Y < 2) {
-    y0 = Y;
-    y1 = Y + 1;
-  } else {
-    y0 = 2 + cy * (Y - 2);
-    y1 = min(y0 + cy, nyf);
-  }
-}
-
-__global__ void k_tall_coarsen(const TallPart* parts, int np, int level) {
-  FP_PART(parts, np);
-  const LatView f = fine_view(G, level);
-  const TallLevel& C = G.lv[level + 1];
-  const int cy = C.cy;
-  const float sx = f.nx > 1 ? 0.5f : 1.0f, sz = f.nz > 1 ? 0.5f : 1.0f;
-  FP_STRIDE(ci, (size_t)C.n) {
-    const int Z = (int)(ci % (size_t)C.nz);
-    const size_t tt = ci / (size_t)C.nz;
-    const int Y = (int)(tt % (size_t)C.ny);
-    const int X = (int)(tt / (size_t)C.ny);
-    const int x0 = 2 * X, x1 = min(2 * X + 2, f.nx);
-    const int z0 = 2 * Z, z1 = min(2 * Z + 2, f.nz);
-    int y0, y1;
-    member_rows(Y, cy, f.ny, y0, y1);
-    const float sy = Y == 0 ? 1.0f : (cy == 2 ? 0.5f : 1.0f);
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f, exm = 0.0f, eym = 0.0f, ezm = 0.0f, dir = 0.0f;
-    bool active = false;
-    for (int x = x0; x < x1; ++x)
-      for (int y = y0; y < y1; ++y)
-        for (int z = z0; z < z1; ++z) {
-          const fp_f32x4 w = f.w[f.idx(x, y, z)];
-          if (!(w.w > 0.0f)) continue;
-          active = true;
-          const float wxm = x > 0 ? f.w[f.idx(x - 1, y, z)].x : 0.0f;
-          const float wym = y > 0 ? f.w[f.idx(x, y - 1, z)].y : 0.0f;
-          const float wzm = z > 0 ? f.w[f.idx(x, y, z - 1)].z : 0.0f;
-          // Dirichlet part of the fine diagonal
-          dir += fmaxf(0.0f, w.w - (w.x + w.y + w.z + wxm + wym + wzm));
-          if (x == x1 - 1 && X + 1 < C.nx) ex += w.x;
-          if (z == z1 - 1 && Z + 1 < C.nz) ez += w.z;
-          if (y == y1 - 1 && Y + 1 < C.ny) ey += w.y;
-          if (x == x0 && X > 0) exm += wxm;
-          if (z == z0 && Z > 0) ezm += wzm;
-          if (y == y0 && Y > 0) eym += wym;
-        }
-    fp_f32x4 out = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (active) {
-      const float sym = Y - 1 == 0 ? 1.0f : (cy == 2 ? 0.5f : 1.0f);  // edge below: row Y-1 -> Y
-      out.x = sx * ex;
-      out.y = sy * ey;
-      out.z = sz * ez;
-      out.w = sx * (ex + exm) + sz * (ez + ezm) + sy * ey + sym * eym + 0.5f * dir;
-    }
-    C.w[ci] = out;
-  }
-}
-
-// Dense FP64 inverse of the coarsest lattice (one workgroup per grid).
-__global__ void k_tall_coarse_setup(const TallPart* parts) {
-  const TallRec& G = *parts[blockIdx.x].g;
-  const TallLevel& L = G.lv[G.L - 1];
-  int* __restrict__ cmap = G.cmap;
-  double* __restrict__ a = G.minv;
-  __shared__ int s_n, s_fail;
-  if (threadIdx.x == 0) {
-    int n = 0;
-    for (size_t i = 0; i < (size_t)L.n && n < G.max_coarse; ++i)
-      if (L.w[i].w > 0.0f) cmap[n++] = (int)i;
-    s_n = n;
-    s_fail = 0;
-    *G.ncoarse = n;
-  }
-  __syncthreads();
-  const int n = s_n;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int r = e / n, c = e % n;
-    const size_t ir = (size_t)cmap[r], ic = (size_t)cmap[c];
-    double val = 0.0;
-    if (r == c) {
-      val = L.w[ir].w;
-    } else {
-      const size_t lo = ir < ic ? ir : ic, hi = ir < ic ? ic : ir;
-      if (hi == lo + 1 && (lo % (size_t)L.nz) + 1 < (size_t)L.nz) val = -(double)L.w[lo].z;
-      else if (hi == lo + (size_t)L.nz && ((lo / (size_t)L.nz) % (size_t)L.ny) + 1 < (size_t)L.ny) val = -(double)L.w[lo].y;
-      else if (hi == lo + (size_t)L.nz * L.ny) val = -(double)L.w[lo].x;
-    }
-    a[e] = val;
-  }
-  __syncthreads();
-  for (int p = 0; p < n; ++p) {
-    const double piv = a[p * n + p];
-    if (!(piv > 0.0) && threadIdx.x == 0) s_fail = 1;
-    __syncthreads();
-    if (s_fail) break;
-    const double inv = 1.0 / piv;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-      const int r = e / n, c = e % n;
-      if (r == p || c == p) continue;
-      a[e] -= a[r * n + p] * a[p * n + c] * inv;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < n; e += blockDim.x)
-      if (e != p) {
-        a[e * n + p] = -a[e * n + p] * inv;
-        a[p * n + e] = a[p * n + e] * inv;
-      }
-    __syncthreads();
-    if (threadIdx.x == 0) a[p * n + p] = inv;
-    __syncthreads();
-  }
-  if (s_fail) {
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) a[e] = 0.0;
-    __syncthreads();
-    for (int r = threadIdx.x; r < n; r += blockDim.x) a[r * n + r] = 1.0 / (double)L.w[cmap[r]].w;
-    if (threadIdx.x == 0) G.st->flags |= 0x40;  // FP_FLAG_COARSE_SINGULAR
-  }
-}
+namespace fp {
 
 // --------------------------------------------------------------------------
 // right-hand side and divergence: b_i = (rho h/dt) sum_f c_f,i u_f (E^T of
@@ -938,64 +751,15 @@ __global__ void k_tall_accumulate(const TallPart* parts, int np) {
   }
 }
 
-// ---- transfers ----------------------------------------------------------------
-// Cell-centred linear interpolation along x and z (the uniform grids' taps),
-// injection of the tall rows and of each aggregated pair of band rows along y;
-// restriction is its transpose (sums, not means: the lattice couplings are
-// Galerkin sums).
-__device__ __forceinline__ float restrict_gather(const float* __restrict__ t, int nxf, int nyf, int nzf, int nxc,
-                                                 int nzc, int X, int Y, int Z, int cy) {
-  int fx[4], fz[4];
-  float wx[4], wz[4];
-  const int na = rtaps(X, nxf > 1 ? 2 : 1, nxf, nxc, fx, wx);
-  const int nc = rtaps(Z, nzf > 1 ? 2 : 1, nzf, nzc, fz, wz);
-  int y0, y1;
-  member_rows(Y, cy, nyf, y0, y1);
-  float s = 0.0f;
-  for (int a = 0; a < na; ++a)
-    for (int y = y0; y < y1; ++y) {
-      const size_t base = ((size_t)fx[a] * nyf + y) * nzf;
-      for (int c = 0; c < nc; ++c) s += wx[a] * wz[c] * t[base + fz[c]];
-    }
-  return s;
-}
-
-__device__ __forceinline__ float prolong_gather(const float* __restrict__ ec, int nxf, int nzf, int nxc, int nyc,
-                                                int nzc, int x, int y, int z, int cy) {
-  int cxs[2], czs[2];
-  float wx[2], wz[2];
-  const int na = ptaps(x, nxf > 1 ? 2 : 1, nxc, cxs, wx);
-  const int nc = ptaps(z, nzf > 1 ? 2 : 1, nzc, czs, wz);
-  const int Y = y < 2 ? y : 2 + (y - 2) / cy;
-  float s = 0.0f;
-  for (int a = 0; a < na; ++a) {
-    const size_t base = ((size_t)cxs[a] * nyc + Y) * nzc;
-    for (int c = 0; c < nc; ++c) s += wx[a] * wz[c] * ec[base + czs[c]];
-  }
-  return s;
-}
-
-// r1 = P^T t from the sample level.
-__global__ void k_tall_restrict0(const TallPart* parts, int np) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
-  const TallLevel& C = G.lv[1];
-  FP_STRIDE(ci, (size_t)C.n) {
-    const int Z = (int)(ci % (size_t)C.nz);
-    const size_t tt = ci / (size_t)C.nz;
-    const int Y = (int)(tt % (size_t)C.ny);
-    const int X = (int)(tt / (size_t)C.ny);
-    C.r[ci] = C.w[ci].w > 0.0f ? restrict_gather(G.t, G.nx, G.P, G.nz, C.nx, C.nz, X, Y, Z, C.cy) : 0.0f;
-  }
-}
-
-// z += P e1 on unknown samples (sel: which buffer holds z).
+// z += P e1 on unknown samples (piecewise constant: the level-1 node holding
+// the sample; sel: z buffer).
 __global__ void k_tall_prolong0(const TallPart* parts, int np, int sel) {
   FP_PART(parts, np);
   if (G.st->done) return;
   const TallGeo g = geo_of(G);
   const TallLevel& C = G.lv[1];
   float* __restrict__ zz = sel ? G.z2 : G.z;
+  const float* __restrict__ ec = final_e(G, 1);
   const float* __restrict__ dinv = G.dinv;
   const fp_f32x4* __restrict__ binv = G.blk;
   TALL_LINES(g) {
@@ -1008,121 +772,8 @@ __global__ void k_tall_prolong0(const TallPart* parts, int np, int sel) {
       unk = py == 0 ? (iv.x != 0.0f || iv.y != 0.0f) : (iv.z != 0.0f || iv.y != 0.0f);
     }
     if (!unk) continue;
-    zz[idx] += prolong_gather(C.e, g.nx, g.nz, C.nx, C.ny, C.nz, x, py, z, C.cy);
-  }
-}
-
-// ---- lattice levels ---------------------------------------------------------
-// One colour of red-black Gauss-Seidel ((x + y + z) & 1 == colour) on level l.
-__global__ void k_lat_smooth(const TallPart* parts, int np, int level, int colour, int zero_init) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
-  const TallLevel& L = G.lv[level];
-  const fp_f32x4* __restrict__ W = L.w;
-  const float* __restrict__ r = L.r;
-  float* __restrict__ e = L.e;
-  const size_t sx = (size_t)L.ny * L.nz, sy = (size_t)L.nz;
-  FP_STRIDE(i, (size_t)L.n) {
-    const int z = (int)(i % (size_t)L.nz);
-    const size_t tt = i / (size_t)L.nz;
-    const int y = (int)(tt % (size_t)L.ny);
-    const int x = (int)(tt / (size_t)L.ny);
-    if (((x + y + z) & 1) != colour) {
-      if (zero_init) e[i] = 0.0f;
-      continue;
-    }
-    const fp_f32x4 w = W[i];
-    if (!(w.w > 0.0f)) {
-      e[i] = 0.0f;
-      continue;
-    }
-    float acc = r[i];
-    if (!zero_init) {
-      if (x + 1 < L.nx) acc += w.x * e[i + sx];
-      if (y + 1 < L.ny) acc += w.y * e[i + sy];
-      if (z + 1 < L.nz) acc += w.z * e[i + 1];
-      if (x > 0) acc += W[i - sx].x * e[i - sx];
-      if (y > 0) acc += W[i - sy].y * e[i - sy];
-      if (z > 0) acc += W[i - 1].z * e[i - 1];
-    }
-    e[i] = acc / w.w;
-  }
-}
-
-__global__ void k_lat_residual(const TallPart* parts, int np, int level) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
-  const TallLevel& L = G.lv[level];
-  const fp_f32x4* __restrict__ W = L.w;
-  const float* __restrict__ r = L.r;
-  const float* __restrict__ e = L.e;
-  float* __restrict__ t = L.t;
-  const size_t sx = (size_t)L.ny * L.nz, sy = (size_t)L.nz;
-  FP_STRIDE(i, (size_t)L.n) {
-    const int z = (int)(i % (size_t)L.nz);
-    const size_t tt = i / (size_t)L.nz;
-    const int y = (int)(tt % (size_t)L.ny);
-    const int x = (int)(tt / (size_t)L.ny);
-    const fp_f32x4 w = W[i];
-    float res = 0.0f;
-    if (w.w > 0.0f) {
-      float ae = w.w * e[i];
-      if (x + 1 < L.nx) ae -= w.x * e[i + sx];
-      if (y + 1 < L.ny) ae -= w.y * e[i + sy];
-      if (z + 1 < L.nz) ae -= w.z * e[i + 1];
-      if (x > 0) ae -= W[i - sx].x * e[i - sx];
-      if (y > 0) ae -= W[i - sy].y * e[i - sy];
-      if (z > 0) ae -= W[i - 1].z * e[i - 1];
-      res = r[i] - ae;
-    }
-    t[i] = res;
-  }
-}
-
-// r_{l+1} = P^T t_l
-__global__ void k_lat_restrict(const TallPart* parts, int np, int level) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
-  const TallLevel& F = G.lv[level];
-  const TallLevel& C = G.lv[level + 1];
-  FP_STRIDE(ci, (size_t)C.n) {
-    const int Z = (int)(ci % (size_t)C.nz);
-    const size_t tt = ci / (size_t)C.nz;
-    const int Y = (int)(tt % (size_t)C.ny);
-    const int X = (int)(tt / (size_t)C.ny);
-    C.r[ci] = C.w[ci].w > 0.0f ? restrict_gather(F.t, F.nx, F.ny, F.nz, C.nx, C.nz, X, Y, Z, C.cy) : 0.0f;
-  }
-}
-
-// e_l += P e_{l+1} on active nodes
-__global__ void k_lat_prolong(const TallPart* parts, int np, int level) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
-  const TallLevel& F = G.lv[level];
-  const TallLevel& C = G.lv[level + 1];
-  FP_STRIDE(i, (size_t)F.n) {
-    if (!(F.w[i].w > 0.0f)) continue;
-    const int z = (int)(i % (size_t)F.nz);
-    const size_t tt = i / (size_t)F.nz;
-    const int y = (int)(tt % (size_t)F.ny);
-    const int x = (int)(tt / (size_t)F.ny);
-    F.e[i] += prolong_gather(C.e, F.nx, F.nz, C.nx, C.ny, C.nz, x, y, z, C.cy);
-  }
-}
-
-__global__ void k_tall_coarse_solve(const TallPart* parts) {
-  const TallRec& G = *parts[blockIdx.x].g;
-  if (G.st->done) return;
-  __shared__ double rs[kTallMaxCoarse];
-  const TallLevel& L = G.lv[G.L - 1];
-  const int n = *G.ncoarse;
-  for (size_t i = threadIdx.x; i < (size_t)L.n; i += blockDim.x) L.e[i] = 0.0f;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) rs[q] = (double)L.r[G.cmap[q]];
-  __syncthreads();
-  for (int row = threadIdx.x; row < n; row += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < n; ++q) s += G.minv[(size_t)row * n + q] * rs[q];
-    L.e[G.cmap[row]] = (float)s;
+    const int Y = py < 2 ? py : 2 + (py - 2) / C.cy;
+    zz[idx] += ec[((size_t)(x >> 1) * C.ny + Y) * C.nz + (z >> 1)];
   }
 }
 
@@ -1339,17 +990,17 @@ size_t TallGrid::estimate_bytes(int nx, int ny, int nz, int band, int max_delta)
   add(ncol * sizeof(TallCol));
   add(ncol * 16);                // blk
   add(n * 4);                    // dinv
-  add(n * 16);                   // w0
   add(n * 8);                    // b
   add(n * 8);                    // x
   for (int i = 0; i < 8; ++i) add(n * 4);  // r e q d0 d1 z z2 t
   add((size_t)(nx + 1) * S * nz * 4);
   add(n * 4);
   add((size_t)nx * S * (nz + 1) * 4);
+  const int K = (max_delta + 5) / 2;
   for (int l = 1; l < L; ++l) {
     const size_t nl = (size_t)dims[l][0] * dims[l][1] * dims[l][2];
-    add(nl * 16);
-    add(nl * 4 * 3);
+    add(nl * 4 * (5 * K + 3));     // banded couplings
+    add(nl * 4 * 5);               // dinv r e e2 t
   }
   add(kTallMaxCoarse * 4 + 16);
   add(16);
@@ -1396,6 +1047,11 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   R.nv = R.n;
   R.nw = (long long)nx * R.S * (nz + 1);
   R.omega = 0.8f;
+  R.K = (max_delta + 5) / 2;
+  if (R.K > kTallMaxK) {
+    set_error("tall-cell max neighbour delta %d exceeds %d", max_delta, 2 * kTallMaxK - 5);
+    return FP_ERR_LAYOUT;
+  }
   const size_t n = (size_t)R.n, ncol = (size_t)R.ncol;
   int rc;
   int* bases = nullptr;
@@ -1406,7 +1062,6 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   if ((rc = alloc((void**)&R.cols, ncol * sizeof(TallCol)))) return rc;
   if ((rc = alloc((void**)&R.blk, ncol * 16))) return rc;
   if ((rc = alloc((void**)&R.dinv, n * 4))) return rc;
-  if ((rc = alloc((void**)&R.w0, n * 16))) return rc;
   if ((rc = alloc((void**)&R.b, n * 8))) return rc;
   if ((rc = alloc((void**)&R.x, n * 8))) return rc;
   float** vecs[] = {&R.r, &R.e, &R.q, &R.d[0], &R.d[1], &R.z, &R.z2, &R.t};
@@ -1423,10 +1078,10 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
     lv.cy = cy[l];
     lv.n = (long long)lv.nx * lv.ny * lv.nz;
     const size_t nl = (size_t)lv.n;
-    if ((rc = alloc((void**)&lv.w, nl * 16))) return rc;
-    if ((rc = alloc((void**)&lv.r, nl * 4))) return rc;
-    if ((rc = alloc((void**)&lv.e, nl * 4))) return rc;
-    if ((rc = alloc((void**)&lv.t, nl * 4))) return rc;
+    if ((rc = alloc((void**)&lv.w, nl * 4 * (size_t)(5 * R.K + 3)))) return rc;
+    float** lvecs[] = {&lv.dinv, &lv.r, &lv.e, &lv.e2, &lv.t};
+    for (float** v : lvecs)
+      if ((rc = alloc((void**)v, nl * 4))) return rc;
   }
   R.lv[0].nx = nx;
   R.lv[0].ny = R.P;
@@ -1675,8 +1330,12 @@ void TallProjector::timer_end(hipStream_t s) {
 int TallProjector::setup(hipStream_t s, Plan& p) {
   PLAUNCH(k_tall_columns, p.columns, s);
   PLAUNCH(k_tall_diag, p.samples, s);
-  PLAUNCH(k_tall_coarsen, p.lv[0].coarser, s, 0);
-  for (int l = 1; l + 1 < p.maxL; ++l) PLAUNCH(k_tall_coarsen, p.lv[l].coarser, s, l);
+  PLAUNCH(k_tall_galerkin1, p.lv[0].coarser, s);
+  PLAUNCH(k_tall_l1diag, p.lv[0].coarser, s, 1);
+  for (int l = 1; l + 1 < p.maxL; ++l) {
+    PLAUNCH(k_tall_galerkin, p.lv[l].coarser, s, l);
+    PLAUNCH(k_tall_l1diag, p.lv[l].coarser, s, l + 1);
+  }
   FLAUNCH(k_tall_coarse_setup, p.one, s);
   return FP_OK;
 }
@@ -1690,19 +1349,23 @@ int TallProjector::enqueue_vcycle(hipStream_t s, Plan& p) {
     if (sw > 0) cur ^= 1;
   }
   TTIMED(FP_KERNEL_RESIDUAL_FINE, PLAUNCH(k_tall_residual, p.samples, s, cur));
-  TTIMED(FP_KERNEL_RESTRICT_FINE, PLAUNCH(k_tall_restrict0, p.lv[0].coarser, s));
-  // lattice levels
+  TTIMED(FP_KERNEL_RESTRICT_FINE, PLAUNCH(k_lat_restrict, p.lv[0].coarser, s, 0));
+  // lattice levels: kTallCoarseSweeps sweeps before (the first from zero
+  // writes e, then e and e2 alternate) and after the coarse correction; the
+  // post-sweeps start where the pre-sweeps ended, so with equal counts the
+  // correction of a smoothed level always ends in e2 (final_e); the coarsest
+  // solve writes e
+  const int cpre = (kTallCoarseSweeps - 1) & 1;  // buffer of e after the pre-sweeps
   for (int l = 1; l + 1 < p.maxL; ++l) {
-    for (int sw = 0; sw < 2; ++sw)
-      for (int c = 0; c < 2; ++c) PLAUNCH(k_lat_smooth, p.lv[l].nodes, s, l, c, (sw == 0 && c == 0) ? 1 : 0);
-    PLAUNCH(k_lat_residual, p.lv[l].nodes, s, l);
+    PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, 0, 1);
+    for (int sw = 1, c = 0; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, c, 0);
+    PLAUNCH(k_lat_residual, p.lv[l].nodes, s, l, cpre);
     PLAUNCH(k_lat_restrict, p.lv[l].coarser, s, l);
   }
   FLAUNCH(k_tall_coarse_solve, p.coarse, s);
   for (int l = p.maxL - 2; l >= 1; --l) {
-    PLAUNCH(k_lat_prolong, p.lv[l].nodes, s, l);
-    for (int sw = 0; sw < 2; ++sw)
-      for (int c = 0; c < 2; ++c) PLAUNCH(k_lat_smooth, p.lv[l].nodes, s, l, 1 - c, 0);
+    PLAUNCH(k_lat_prolong, p.lv[l].nodes, s, l, cpre);
+    for (int sw = 0, c = cpre; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, c, 0);
   }
   TTIMED(FP_KERNEL_PROLONG_FINE, PLAUNCH(k_tall_prolong0, p.samples, s, cur));
   for (int sw = 0; sw < post; ++sw) {

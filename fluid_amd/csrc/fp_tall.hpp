@@ -17,6 +17,8 @@ namespace fp {
 
 constexpr int kTallMaxLevels = 14;
 constexpr int kTallMaxCoarse = 64;  // nodes of the coarsest lattice (dense FP64 inverse)
+constexpr int kTallMaxK = 8;        // banded coupling reach of the coarse levels (max neighbour delta 11)
+constexpr int kTallSlotsMax = 5 * kTallMaxK + 3;
 
 // Per-column FP64 face data: the weight moments sum w, sum w t, sum w t^2
 // (t = row - (m-1)/2) over the rows 0..m-1 shared with the +x and the +z
@@ -30,15 +32,18 @@ struct TallCol {
 
 // One lattice level of the multigrid (levels >= 1; level 0 is the sample
 // space).  Nodes (nx, ny, nz): rows 0/1 keep the tall bottom/top layers, rows
-// >= 2 aggregate the band.  w holds per node the coupling to the +x, +y, +z
-// neighbours and the diagonal (0 on inactive nodes).
+// >= 2 aggregate the band.  w holds the banded Galerkin couplings
+// (fp_tall_galerkin.hpp, slot-major), dinv the scaled l1-Jacobi weight (0 on
+// inactive nodes).
 struct TallLevel {
   int nx, ny, nz;
   int cy;              // band rows of the finer level per band row of this one (1 or 2)
   long long n;
-  fp_f32x4* w;
+  float* w;
+  float* dinv;
   float* r;
   float* e;
+  float* e2;
   float* t;
 };
 
@@ -46,6 +51,7 @@ struct TallRec {
   int nx, ny, nz;      // columns and fine rows
   int band, P, S, D;   // band rows, packed rows, face slots, max neighbour base delta
   int L;               // levels including the sample level
+  int K;               // banded coupling reach of the coarse levels
   int zfinal;          // V-cycle buffer (0: z, 1: z2) holding M^-1 r
   long long n, nu, nv, nw, ncol;
   double h;
@@ -54,7 +60,6 @@ struct TallRec {
   TallCol* cols;       // (nx, nz)
   fp_f32x4* blk;       // per column: inverse (bottom, top) block bb, bt, tt and 0
   float* dinv;         // per sample: inverse diagonal of band unknowns
-  fp_f32x4* w0;        // per sample: lattice surrogate (+x, +y, +z couplings, diagonal)
   double* b;           // FP64 right-hand side
   double* x;           // FP64 pressure (resident: warm start)
   float* r;            // FP32 CG residual

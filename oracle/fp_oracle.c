@@ -1382,3 +1382,41 @@ double orc_tall_max_divergence(const orc_tall* t, const double* u, const double*
   free(mask);
   return m;
 }
+
+typedef struct {
+  int64_t* ri;
+  int64_t* ci;
+  double* v;
+  size_t n, cap;
+  const uint8_t* mask;
+} tall_coo_ctx;
+
+static void tall_coo_cb(void* vctx, const tall_face* f) {
+  tall_coo_ctx* c = (tall_coo_ctx*)vctx;
+  if (f->w == 0.0) return;
+  for (int a = 0; a < f->n; ++a) {
+    if (!c->mask[f->idx[a]]) continue;
+    for (int b = 0; b < f->n; ++b) {
+      if (!c->mask[f->idx[b]]) continue;
+      if (c->n < c->cap) {
+        c->ri[c->n] = (int64_t)f->idx[a];
+        c->ci[c->n] = (int64_t)f->idx[b];
+        c->v[c->n] = f->w * f->c[a] * f->c[b];
+      }
+      ++c->n;
+    }
+  }
+}
+
+/* The entries w c_a c_b of every fine face (duplicates to be summed): the
+ * operator orc_tall_apply applies, as coordinates.  Returns the entry count;
+ * only the first `cap` are written. */
+int64_t orc_tall_coo(const orc_tall* t, int64_t* rows, int64_t* cols, double* vals, int64_t cap) {
+  const size_t n = tall_samples(t);
+  uint8_t* mask = (uint8_t*)malloc(n);
+  orc_tall_unknowns(t, mask);
+  tall_coo_ctx c = {rows, cols, vals, 0, (size_t)cap, mask};
+  const int rc = tall_faces(t, tall_coo_cb, &c);
+  free(mask);
+  return rc ? -(int64_t)rc : (int64_t)c.n;
+}

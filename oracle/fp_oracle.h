@@ -164,6 +164,7 @@ int orc_tall_diagonal(const orc_tall* t, double* d);
 int orc_tall_solve(const orc_tall* t, const double* b, double rel_tol, int max_iter, double* p, int* iterations,
                    double* relative_residual);
 double orc_tall_max_divergence(const orc_tall* t, const double* u, const double* v, const double* w);
+int64_t orc_tall_coo(const orc_tall* t, int64_t* rows, int64_t* cols, double* vals, int64_t cap);
 
 #ifdef __cplusplus
 }

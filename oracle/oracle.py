@@ -107,6 +107,8 @@ def lib():
         L.orc_tall_solve.argtypes = [tp, _dp, d, i, _dp, C.POINTER(i), C.POINTER(d)]
         L.orc_tall_max_divergence.restype = d
         L.orc_tall_max_divergence.argtypes = [tp, _dp, _dp, _dp]
+        L.orc_tall_coo.restype = C.c_int64
+        L.orc_tall_coo.argtypes = [tp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -369,6 +371,20 @@ class Tall:
 
     def max_divergence(self, u, v, w):
         return lib().orc_tall_max_divergence(self.ref, _f64(u), _f64(v), _f64(w))
+
+    def matrix(self):
+        """The operator as a scipy CSR matrix over all samples (zero rows and
+        columns off the unknowns)."""
+        import scipy.sparse as sp
+        cnt = lib().orc_tall_coo(self.ref, None, None, None, 0)
+        if cnt < 0:
+            raise ValueError(f"tall-cell oracle error {-cnt}")
+        r = np.zeros(cnt, np.int64)
+        c = np.zeros(cnt, np.int64)
+        v = np.zeros(cnt)
+        lib().orc_tall_coo(self.ref, r.ctypes.data, c.ctypes.data, v.ctypes.data, cnt)
+        n = self.nx * self.P * self.nz
+        return sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
 
     def projection(self, u, v, w, dt, rho, rel_tol=1e-12):
         """FP64 projection: rhs -> Jacobi-PCG -> subtract -> divergence."""

@@ -250,3 +250,9 @@ def test_hydrostatic_columns(oracle):
     liq = t.unknowns()
     assert rel(p[liq], exact[liq]) < 1e-10
     assert np.abs(vo).max() < 1e-9 and np.abs(uo).max() < 1e-9 and np.abs(wo).max() < 1e-9
+
+
+def test_matrix_export_equals_apply(case):
+    sc, t, ex = case
+    A = t.matrix()
+    assert abs(A - ex.A).max() < 1e-12 * abs(ex.A).max()
