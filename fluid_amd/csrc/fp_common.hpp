@@ -230,18 +230,8 @@ __device__ __forceinline__ float face_weight_fast(float pc, float pn, float g) {
   return (lc || ln) ? w : 0.0f;
 }
 
-// Cell-centred linear transfers along one axis (s: 1 when the axis is not
-// coarsened).  1-D taps of the transpose of the prolongation (restriction gather).
-__device__ __forceinline__ int rtaps(int I, int s, int nf, int nc, int* f, float* w) {
-  if (s == 1) { f[0] = I; w[0] = 1.0f; return 1; }
-  int c = 0;
-  if (I >= 1) { f[c] = 2 * I - 1; w[c] = 0.25f; ++c; }
-  f[c] = 2 * I; w[c] = (I - 1 < 0) ? 1.0f : 0.75f; ++c;
-  if (2 * I + 1 < nf) { f[c] = 2 * I + 1; w[c] = (I + 1 >= nc) ? 1.0f : 0.75f; ++c; }
-  if (2 * I + 2 < nf) { f[c] = 2 * I + 2; w[c] = 0.25f; ++c; }
-  return c;
-}
-// 1-D taps of the prolongation for fine index f.
+// 1-D taps of the cell-centred linear prolongation for fine index f along one
+// axis (s: 1 when the axis is not coarsened).
 __device__ __forceinline__ int ptaps(int f, int s, int nc, int* c, float* w) {
   if (s == 1) { c[0] = f; w[0] = 1.0f; return 1; }
   const int I = f >> 1;

@@ -424,39 +424,128 @@ __global__ void k_residual(const Part* parts, int np, int level) {
   }
 }
 
-// r_{l+1} = R t_l on the coarse level's liquid cells (R = P^T / 2^k).
+// r_{l+1} = R t_l on the coarse level's liquid cells (R = P^T / 2^k, the
+// transpose of the cell-centred trilinear prolongation: per axis the fine
+// taps 2I-1 .. 2I+2 with weights 1/4 3/4 3/4 1/4, 1 at a domain edge), as a
+// column-tile x-march over the coarse level: a 256-lane workgroup owns a 32 x 8 (z x y) tile of coarse columns (the coarse
+// level's march geometry) and walks its chunk of coarse planes.  Each fine
+// plane the chunk touches is staged once in LDS (the tile's fine footprint
+// plus the one-cell ring of the 4-tap stencil), filtered in y and z per lane,
+// and added with its x weight into the two coarse planes it feeds; a coarse
+// plane is written once its last fine plane has been seen.  Fine values are
+// read once per tile instead of once per coarse tap.
 template <typename T>
-__global__ void k_restrict(const Part* parts, int np, int level) {
+__global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int np, int level) {
   FP_PART(parts, np);
-  if (G.st->done) return;
-  const LevelRec& fl = G.lv[level];
-  const LevelRec& cl = G.lv[level + 1];
-  const Grid3 f = grid_of(fl), c = grid_of(cl);
-  const T* __restrict__ tf = (const T*)fl.t;
-  T* __restrict__ rc = (T*)cl.r;
-  const float* __restrict__ phic = cl.phi;
+  if (cload(&cload(&Gp->st)->done)) return;
+  const LevelRec fl = cload(&Gp->lv[level]);
+  const LevelRec cl = cload(&Gp->lv[level + 1]);
+  constexpr int TZ = kMarchTZ, TY = 256 / TZ;
+  constexpr int RY = 2 * TY + 2, RZ = 2 * TZ + 2;
+  constexpr int NC = (RY * RZ + 255) / 256;
+  __shared__ T s[2][RY * RZ];
+  const int fnx = fl.d[0], fny = fl.d[1], fnz = fl.d[2];
+  const int cnx = cl.d[0], cny = cl.d[1], cnz = cl.d[2];
   const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
   const int k2 = (sx == 2) + (sy == 2) + (sz == 2);
   const float fac = (1.0f / (float)(1 << k2)) * fl.St / cl.Sr;
-  FP_STRIDE(idx, c.n()) {
-    int I, J, K;
-    c.ijk(idx, I, J, K);
-    float acc = 0.0f;
-    if (phic[idx] < 0.0f) {
-      int fi[4], fj[4], fk[4];
-      float wi[4], wj[4], wk[4];
-      const int ni = rtaps(I, sx, f.nx, c.nx, fi, wi);
-      const int nj = rtaps(J, sy, f.ny, c.ny, fj, wj);
-      const int nk = rtaps(K, sz, f.nz, c.nz, fk, wk);
-      for (int a = 0; a < ni; ++a)
-        for (int b = 0; b < nj; ++b) {
-          const float wab = wi[a] * wj[b];
-          const size_t base = ((size_t)fi[a] * f.ny + fj[b]) * f.nz;
-          for (int q = 0; q < nk; ++q) acc += wab * wk[q] * ldT(tf, base + fk[q]);
-        }
-      acc *= fac;
+  const T* __restrict__ tf = (const T*)fl.t;
+  T* __restrict__ rc = (T*)cl.r;
+  const float* __restrict__ phic = cl.phi;
+  int tzi, tyi, cxi;
+  tile_of_group(blk, cl.mg.tz, cl.mg.ty, nblk, tzi, tyi, cxi);
+  const int cz0 = tzi * TZ, cy0 = tyi * TY;
+  const int X0 = cxi * cl.mg.chunk, X1 = min(X0 + cl.mg.chunk, cnx);
+  const int tid = threadIdx.x;
+  // staged fine window: rows fy0 .. fy0+ry-1, columns fz0 .. fz0+rz-1
+  const int fy0 = sy == 2 ? 2 * cy0 - 1 : cy0, fz0 = sz == 2 ? 2 * cz0 - 1 : cz0;
+  const int ry = sy == 2 ? RY : TY, rz = sz == 2 ? RZ : TZ;
+  int off[NC], lds[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int q = tid + 256 * k;
+    const int a = q / rz, b = q - (q / rz) * rz;
+    const int yy = fy0 + a, zz = fz0 + b;
+    lds[k] = q < ry * rz ? a * RZ + b : -1;
+    off[k] = (q < ry * rz && yy >= 0 && yy < fny && zz >= 0 && zz < fnz) ? yy * fnz + zz : -1;
+  }
+  // this lane's coarse column and its 4 + 4 in-plane taps (local rows /
+  // columns of the window; absent taps weigh 0)
+  const int ly = tid / TZ, lz = tid - (tid / TZ) * TZ;
+  const int J = cy0 + ly, K = cz0 + lz;
+  int ty_[4], tz_[4];
+  float wy[4], wz[4];
+  auto taps = [&](int I, int s_, int nf, int nc, int l, int* t, float* w) {
+    if (s_ == 1) {
+      t[0] = t[1] = t[2] = t[3] = l;
+      w[0] = 1.0f;
+      w[1] = w[2] = w[3] = 0.0f;
+      return;
     }
-    stT(rc, idx, acc);
+    for (int q = 0; q < 4; ++q) t[q] = 2 * l + q;  // fine 2I-1+q
+    w[0] = I >= 1 ? 0.25f : 0.0f;
+    w[1] = I >= 1 ? 0.75f : 1.0f;
+    w[2] = 2 * I + 1 < nf ? (I + 1 >= nc ? 1.0f : 0.75f) : 0.0f;
+    w[3] = 2 * I + 2 < nf ? 0.25f : 0.0f;
+  };
+  taps(J, sy, fny, cny, ly, ty_, wy);
+  taps(K, sz, fnz, cnz, lz, tz_, wz);
+  const bool mine = J < cny && K < cnz;
+  const long long fplane = (long long)fny * fnz, cplane = (long long)cny * cnz;
+  const size_t cidx = (size_t)J * cnz + K;
+
+  const int p0 = sx == 2 ? max(2 * X0 - 1, 0) : X0;
+  const int p1 = sx == 2 ? min(2 * X1, fnx - 1) : X1 - 1;  // last fine plane, inclusive
+  T pre[NC];
+  auto fetch = [&](int p) {
+    const T* src = tf + (long long)p * fplane;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) pre[k] = off[k] >= 0 ? ldg(src, (size_t)off[k]) : to_store<T>(0.0f);
+  };
+  fetch(p0);
+  int cur = X0;
+  float acc0 = 0.0f, acc1 = 0.0f;
+  for (int p = p0; p <= p1; ++p) {
+    T* sl = s[p & 1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (lds[k] >= 0) sl[lds[k]] = pre[k];
+    lds_barrier();
+    if (p < p1) fetch(p + 1);
+    float v = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const T* row = sl + ty_[b] * RZ;
+      const float rowsum = wz[0] * Store<T>::ld(row, tz_[0]) + wz[1] * Store<T>::ld(row, tz_[1]) +
+                           wz[2] * Store<T>::ld(row, tz_[2]) + wz[3] * Store<T>::ld(row, tz_[3]);
+      v += wy[b] * rowsum;
+    }
+    // x weights: plane p feeds coarse p>>1 (and its neighbour on p's side)
+    if (sx == 1) {
+      acc0 += v;
+    } else {
+      const int I = p >> 1, Jx = (p & 1) ? I + 1 : I - 1;
+      const bool two = Jx >= 0 && Jx < cnx;
+      const float wa = two ? 0.75f : 1.0f;
+      if (I == cur) acc0 += wa * v;
+      else if (I == cur + 1) acc1 += wa * v;
+      if (two) {
+        if (Jx == cur) acc0 += 0.25f * v;
+        else if (Jx == cur + 1) acc1 += 0.25f * v;
+      }
+    }
+    // coarse planes whose last fine plane has been staged
+    while (cur < X1 && p >= (sx == 2 ? min(2 * cur + 2, fnx - 1) : cur)) {
+      if (mine) {
+        const size_t o = (size_t)((long long)cur * cplane) + cidx;
+        Store<T>::gst(rc, o, ldg(phic, o) < 0.0f ? acc0 * fac : 0.0f);
+      }
+      acc0 = acc1;
+      acc1 = 0.0f;
+      ++cur;
+    }
+    // no second barrier: the next plane goes to the other slot, whose last
+    // reads came before this plane's barrier
   }
 }
 
@@ -1175,7 +1264,7 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
       const LevelRec& lv = R.lv[l];
       if (l >= 1) dn.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
       if (l >= R.L - 1) continue;  // the coarsest level is solved directly
-      rs.push_back({g, Plan::blocks_for((size_t)R.lv[l + 1].n, 1)});
+      rs.push_back({g, R.lv[l + 1].mg.ngroups});
       if (lv.fused) {
         (lv.sweep_tz == 32 ? s32 : s16).push_back({g, lv.sw.ngroups});
         res.push_back({g, lv.mg.ngroups});
@@ -1299,7 +1388,7 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
         PLAUNCH((k_residual<T, T>), T_.smooth, s, l);
       }
     });
-    TIMED(top ? FP_KERNEL_RESTRICT_FINE : -1, PLAUNCH(k_restrict<T>, T_.restrict_, s, l));
+    TIMED(top ? FP_KERNEL_RESTRICT_FINE : -1, PLAUNCH(k_restrict_march<T>, T_.restrict_, s, l));
   }
   FLAUNCH((k_coarse_solve<T, float>), p.coarse_top, s);
   FLAUNCH((k_coarse_solve<T, T>), p.coarse_deep, s);
