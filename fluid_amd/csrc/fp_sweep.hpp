@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   constexpr int NW = (WN + 255) / 256, NE = ZERO ? 0 : (EN + 255) / 256, NR = (CN + 255) / 256;
   constexpr int NP = (PN + 255) / 256;
   constexpr int HF = CZ / 2, HS = TZ / 2;
-  static_assert(CY * HF <= 256 && TY * HS <= 256, "lane maps exceed the workgroup");
+  static_assert(CY * HF <= 256 && TY * HS <= 256 && EY * (EZ / 2) <= 256, "lane maps exceed the workgroup");
   using WB = typename W4Bits<T>::type;
   __shared__ WB s_w[4][WN];
   __shared__ T s_e[ZERO ? 1 : 3 * EN];
@@ -308,6 +308,30 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
     c2st[h] = c2 && c2gy < ny && z0 + c2lz[h] < nz;
   }
 
+  // prolongation lane (PROLONG): a second-colour cell of the staged window
+  // (rows/columns -2..T+1, EZ/2 per row); its column alternates with the plane
+  // parity (index h).  Its y / z taps into the coarse window are fixed for the
+  // whole march, so they are evaluated once here.
+  constexpr int PH = EZ / 2;
+  const int ply = tid / PH, plk = tid - (tid / PH) * PH;
+  const int pgy = y0 + ply - 2;
+  int pq[2] = {0, 0}, pza[2] = {0, 0}, pzb[2] = {0, 0}, pya = 0, pyb = 0;
+  float pwza[2] = {0.0f, 0.0f}, pwzb[2] = {0.0f, 0.0f}, pwya = 0.0f, pwyb = 0.0f;
+  bool pok[2] = {false, false};
+  if constexpr (PROLONG) {
+    const bool prow = tid < EY * PH && pgy >= 0 && pgy < ny;
+    if (prow) window_taps(pgy, P.sy, P.cny, cy0, pya, pwya, pyb, pwyb);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int L = x0 - 2 + h;  // parity of L on steps of parity h
+      const int ez = 2 * plk + ((second + L + pgy) & 1);
+      const int pgz = z0 + ez - 2;
+      pq[h] = ply * EZ + ez;
+      pok[h] = prow && pgz >= 0 && pgz < nz;
+      if (pok[h]) window_taps(pgz, P.sz, P.cnz, cz0, pza[h], pwza[h], pzb[h], pwzb[h]);
+    }
+  }
+
 #pragma unroll
   for (int d = 0; d < D; ++d) fetch(d);
 
@@ -359,28 +383,23 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
 
     if constexpr (PROLONG) {
       // e(L) += fac * P ec at second-colour liquid cells (coarse air holds e = 0)
-      if (pin) {
+      if (pin && pok[h]) {
         int xa, xb;
         float wxa, wxb;
         window_taps(L, P.sx, P.cnx, coarse_plane0(L), xa, wxa, xb, wxb);
-        for (int q = tid; q < EN; q += 256) {
-          const int ey = q / EZ, ez = q - (q / EZ) * EZ;
-          const int gy = y0 + ey - 2, gz = z0 + ez - 2;
-          const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[s0][q]);
-          if (gy >= 0 && gy < ny && gz >= 0 && gz < nz && ((L + gy + gz) & 1) == second && wf(wc.id) > 0.0f) {
-            int ya, yb, za, zb;
-            float wya, wyb, wza, wzb;
-            window_taps(gy, P.sy, P.cny, cy0, ya, wya, yb, wyb);
-            window_taps(gz, P.sz, P.cnz, cz0, za, wza, zb, wzb);
-            const T* pa_ = s_p + xa * EN;
-            const T* pb_ = s_p + xb * EN;
-            const float acc =
-                wxa * (wya * (wza * wf(pa_[ya * EZ + za]) + wzb * wf(pa_[ya * EZ + zb])) +
-                       wyb * (wza * wf(pa_[yb * EZ + za]) + wzb * wf(pa_[yb * EZ + zb]))) +
-                wxb * (wya * (wza * wf(pb_[ya * EZ + za]) + wzb * wf(pb_[ya * EZ + zb])) +
-                       wyb * (wza * wf(pb_[yb * EZ + za]) + wzb * wf(pb_[yb * EZ + zb])));
-            s_e[e0 + q] = to_store<T>(wf(s_e[e0 + q]) + P.fac * acc);
-          }
+        const int q = pq[h];
+        const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[s0][q]);
+        if (wf(wc.id) > 0.0f) {
+          const T* pa_ = s_p + xa * EN;
+          const T* pb_ = s_p + xb * EN;
+          const int za = pza[h], zb = pzb[h];
+          const float wza = pwza[h], wzb = pwzb[h];
+          const float acc =
+              wxa * (pwya * (wza * wf(pa_[pya * EZ + za]) + wzb * wf(pa_[pya * EZ + zb])) +
+                     pwyb * (wza * wf(pa_[pyb * EZ + za]) + wzb * wf(pa_[pyb * EZ + zb]))) +
+              wxb * (pwya * (wza * wf(pb_[pya * EZ + za]) + wzb * wf(pb_[pya * EZ + zb])) +
+                     pwyb * (wza * wf(pb_[pyb * EZ + za]) + wzb * wf(pb_[pyb * EZ + zb])));
+          s_e[e0 + q] = to_store<T>(wf(s_e[e0 + q]) + P.fac * acc);
         }
       }
       lds_barrier();  // A2
