@@ -124,6 +124,13 @@ __device__ __forceinline__ T to_store(float v) {
   return t;
 }
 
+// p[i] = a, p[i+1] = b with one store (i even; p aligned to two elements).
+template <typename T>
+__device__ __forceinline__ void gst_pair(T* p, size_t i, float a, float b) {
+  struct alignas(2 * sizeof(T)) Two { T lo, hi; };
+  stg(reinterpret_cast<Two*>(p), i >> 1, Two{to_store<T>(a), to_store<T>(b)});
+}
+
 // Taps of the cell-centred linear prolongation along one axis, as positions
 // in a staged coarse window starting at coarse index c0: fine f of stride s
 // reads coarse f>>1 with 3/4 and its neighbour on f's side with 1/4, or
@@ -211,6 +218,9 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const int tid = threadIdx.x;
   const long long plane = (long long)ny * nz;
   const int second = 1 - first;
+  // even rows: both colours of a plane leave in one pass of paired stores
+  // (colour 2 writes each (z even, z odd) pair); odd nz keeps single stores
+  const bool pairs = (nz & 1) == 0;
 
   // in-plane offsets of the staged cells (-1: outside the grid)
   int ow[NW], oe[NE > 0 ? NE : 1], orr[NR];
@@ -394,12 +404,13 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
           const T* pb_ = s_p + xb * EN;
           const int za = pza[h], zb = pzb[h];
           const float wza = pwza[h], wzb = pwzb[h];
-          const float acc =
-              wxa * (pwya * (wza * wf(pa_[pya * EZ + za]) + wzb * wf(pa_[pya * EZ + zb])) +
-                     pwyb * (wza * wf(pa_[pyb * EZ + za]) + wzb * wf(pa_[pyb * EZ + zb]))) +
-              wxb * (pwya * (wza * wf(pb_[pya * EZ + za]) + wzb * wf(pb_[pya * EZ + zb])) +
-                     pwyb * (wza * wf(pb_[pyb * EZ + za]) + wzb * wf(pb_[pyb * EZ + zb])));
-          s_e[e0 + q] = to_store<T>(wf(s_e[e0 + q]) + P.fac * acc);
+          const T v0 = pa_[pya * EZ + za], v1 = pa_[pya * EZ + zb], v2 = pa_[pyb * EZ + za], v3 = pa_[pyb * EZ + zb];
+          const T v4 = pb_[pya * EZ + za], v5 = pb_[pya * EZ + zb], v6 = pb_[pyb * EZ + za], v7 = pb_[pyb * EZ + zb];
+          const T eq = s_e[e0 + q];
+          __builtin_amdgcn_sched_barrier(0);
+          const float acc = wxa * (pwya * (wza * wf(v0) + wzb * wf(v1)) + pwyb * (wza * wf(v2) + wzb * wf(v3))) +
+                            wxb * (pwya * (wza * wf(v4) + wzb * wf(v5)) + pwyb * (wza * wf(v6) + wzb * wf(v7)));
+          s_e[e0 + q] = to_store<T>(wf(eq) + P.fac * acc);
         }
       }
       lds_barrier();  // A2
@@ -415,22 +426,39 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
         const int lz = c1lz[h];
         const int wi = (c1y + 1) * WZ + lz + 1;
         const int ci = c1y * CZ + lz;
+        // all LDS reads first, then the arithmetic: one wait instead of a
+        // read-wait-FMA chain per neighbour (the scheduler otherwise
+        // interleaves them to shorten live ranges)
         const W4<T> wc = w0[wi];
-        const float idg = wf(wc.id);
-        float acc = wf(s_r[s1][ci]);
+        const T rv = s_r[s1][ci];
+        T ev[6], wn[3];
         if constexpr (!ZERO) {
           const int ei = (c1y + 1) * EZ + lz + 1;
           const T* eo = s_e + e1;
-          acc += wf(wc.wx) * wf(s_e[e0 + ei]);
-          acc += wf(wm[wi].wx) * wf(s_e[e2 + ei]);
-          acc += wf(wc.wy) * wf(eo[ei + EZ]);
-          acc += wf(w0[wi - WZ].wy) * wf(eo[ei - EZ]);
-          acc += wf(wc.wz) * wf(eo[ei + 1]);
-          acc += wf(w0[wi - 1].wz) * wf(eo[ei - 1]);
+          ev[0] = s_e[e0 + ei];
+          ev[1] = s_e[e2 + ei];
+          ev[2] = eo[ei + EZ];
+          ev[3] = eo[ei - EZ];
+          ev[4] = eo[ei + 1];
+          ev[5] = eo[ei - 1];
+          wn[0] = wm[wi].wx;
+          wn[1] = w0[wi - WZ].wy;
+          wn[2] = w0[wi - 1].wz;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const float idg = wf(wc.id);
+        float acc = wf(rv);
+        if constexpr (!ZERO) {
+          acc += wf(wc.wx) * wf(ev[0]);
+          acc += wf(wn[0]) * wf(ev[1]);
+          acc += wf(wc.wy) * wf(ev[2]);
+          acc += wf(wn[1]) * wf(ev[3]);
+          acc += wf(wc.wz) * wf(ev[4]);
+          acc += wf(wn[2]) * wf(ev[5]);
         }
         const float val = idg > 0.0f ? acc * idg : 0.0f;
         s_c[s1][ci] = to_store<T>(val);
-        if (R >= x0 && R < x1 && c1st[h])
+        if (!pairs && R >= x0 && R < x1 && c1st[h])
           Store<T>::gst(eout + oR, (size_t)(c1gy * nz + z0 + lz - 1), val);
       }
       // colour-2 lanes read the -x coefficient of the next step's plane B - 1
@@ -444,18 +472,33 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       const int wi = (c2y + 2) * WZ + lz + 2;
       const int ci = (c2y + 1) * CZ + lz + 1;
       const W4<T>* w0 = reinterpret_cast<const W4<T>*>(s_w[s2]);
-      const W4<T> wc = w0[wi];
-      const float idg = wf(wc.id);
       const T* cc = s_c[s2];
-      float acc = wf(s_r[s2][ci]);
-      acc += wf(wc.wx) * wf(s_c[s1][ci]);
-      acc += wxm2 * wf(s_c[s3][ci]);
-      acc += wf(wc.wy) * wf(cc[ci + CZ]);
-      acc += wf(w0[wi - WZ].wy) * wf(cc[ci - CZ]);
-      acc += wf(wc.wz) * wf(cc[ci + 1]);
-      acc += wf(w0[wi - 1].wz) * wf(cc[ci - 1]);
+      const W4<T> wc = w0[wi];
+      const T rv = s_r[s2][ci];
+      const T c0 = s_c[s1][ci], c1_ = s_c[s3][ci], c2_ = cc[ci + CZ], c3 = cc[ci - CZ], c4 = cc[ci + 1],
+              c5 = cc[ci - 1];
+      const T wy_ = w0[wi - WZ].wy, wz_ = w0[wi - 1].wz;
+      __builtin_amdgcn_sched_barrier(0);
+      const float idg = wf(wc.id);
+      float acc = wf(rv);
+      acc += wf(wc.wx) * wf(c0);
+      acc += wxm2 * wf(c1_);
+      acc += wf(wc.wy) * wf(c2_);
+      acc += wf(wy_) * wf(c3);
+      acc += wf(wc.wz) * wf(c4);
+      acc += wf(wz_) * wf(c5);
       const float val = idg > 0.0f ? acc * idg : 0.0f;
-      if (c2st[h]) Store<T>::gst(eout + (oR - plane), (size_t)(c2gy * nz + z0 + lz), val);
+      if (c2st[h]) {
+        if (pairs) {
+          // the cell pair (z even, z odd) of this row in one store: the
+          // first-colour partner is the staged value beside this cell
+          const float pv = (lz & 1) ? wf(c5) : wf(c4);
+          gst_pair(eout + (oR - plane), (size_t)(c2gy * nz + z0 + (lz & ~1)), (lz & 1) ? pv : val,
+                   (lz & 1) ? val : pv);
+        } else {
+          Store<T>::gst(eout + (oR - plane), (size_t)(c2gy * nz + z0 + lz), val);
+        }
+      }
     }
     wxm2 = wxm;
     oR += plane;
