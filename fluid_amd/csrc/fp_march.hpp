@@ -9,9 +9,14 @@
 //
 // An operation `Op` supplies
 //   using Cell = ...;                                   LDS record per cell
+//   using Head = ...;                                   what decides the cell's other loads
 //   static Op make(const GridRec&, const MarchLaunch&); binds the grid's arrays
 //   void prepare();                                     reads device scalars once per workgroup
-//   Cell load(int x, int y, int z, bool inside) const;  staging (inside: in the domain)
+//   Head head(int x, int y, int z, bool inside) const;  first loads (inside: in the domain)
+//   Cell load(int x, int y, int z, bool inside, const Head&) const;
+//                                                       the loads that depend on the head
+// Heads are fetched two planes ahead and the dependent loads one plane ahead,
+// so neither waits for the other within a plane step.
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
 //              double* acc, float* mx) const;           nb/has: -x,+x,-y,+y,-z,+z
 //   static constexpr int NV;                            sum-reduction width
@@ -63,21 +68,35 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
   double acc[Op::NV > 0 ? Op::NV : 1];
   for (int k = 0; k < (Op::NV > 0 ? Op::NV : 1); ++k) acc[k] = 0.0;
   constexpr int NC = (RY * RZ + 255) / 256;
+  using Head = typename Op::Head;
+  Head hd[NC];
   Cell pre[NC];
-  auto fetch = [&](int L) {
-    const bool pin = (L >= 0 && L < nx);
+  auto where = [&](int L, int k, int& yy, int& zz) {
+    const int q = tid + 256 * k;
+    const int ry = q / RZ, rz = q - ry * RZ;
+    yy = y0 + ry - 1;
+    zz = z0 + rz - 1;
+    return q < RY * RZ && L >= 0 && L < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
+  };
+  auto fetch_head = [&](int L) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      const int q = tid + 256 * k;
-      if (q < RY * RZ) {
-        const int ry = q / RZ, rz = q - ry * RZ;
-        const int yy = y0 + ry - 1, zz = z0 + rz - 1;
-        const bool inside = pin && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
-        pre[k] = op.load(L, yy, zz, inside);
-      }
+      int yy, zz;
+      const bool inside = where(L, k, yy, zz);
+      hd[k] = op.head(L, yy, zz, inside);
     }
   };
-  fetch(x0 - 1);
+  auto fetch_cell = [&](int L) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      int yy, zz;
+      const bool inside = where(L, k, yy, zz);
+      pre[k] = op.load(L, yy, zz, inside, hd[k]);
+    }
+  };
+  fetch_head(x0 - 1);
+  fetch_cell(x0 - 1);
+  fetch_head(x0);
   for (int L = x0 - 1; L <= x1; ++L) {
     const int slot = (L + 3) % 3;
 #pragma unroll
@@ -89,7 +108,8 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
       }
     }
     lds_barrier();
-    if (L + 1 <= x1) fetch(L + 1);
+    if (L + 1 <= x1) fetch_cell(L + 1);
+    if (L + 2 <= x1) fetch_head(L + 2);
     const int B = L - 1;
     if (B >= x0 && B < x1 && gy < ny && gz < nz) {
       const int sB = (B + 3) % 3, sBm = (B + 2) % 3, sBp = (B + 4) % 3;
@@ -167,12 +187,15 @@ struct OpDirection {
     zm = zmul / cload(&st->rscale);
     beta = (float)cload(&st->beta);
   }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
-    Cell c{0.0f, 1.0f};
-    if (inside) {
+  using Head = float;  // phi (air outside the domain)
+  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
+    return inside ? ldg(phi, ((size_t)x * ny + y) * nz + zz) : 1.0f;
+  }
+  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, Head p) const {
+    Cell c{0.0f, p};
+    if (inside && p < 0.0f) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.p = ldg(phi, i);
-      if (c.p < 0.0f) c.d = Store<T>::gld(z, i) * zm + beta * ldg(dold, i);
+      c.d = Store<T>::gld(z, i) * zm + beta * ldg(dold, i);
     }
     return c;
   }
@@ -228,18 +251,21 @@ struct OpResidualW {
     return o;
   }
   __device__ __forceinline__ void prepare() { rm = dyn ? rmul * cload(&st->rscale) : rmul; }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
+  using Head = W4<T>;  // coefficients (zero outside the domain)
+  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
+    W4<T> w;
+    Store<T>::st(&w.wx, 0, 0.0f);
+    Store<T>::st(&w.wy, 0, 0.0f);
+    Store<T>::st(&w.wz, 0, 0.0f);
+    Store<T>::st(&w.id, 0, 0.0f);
+    if (inside) w = ldg(wts, ((size_t)x * ny + y) * nz + zz);
+    return w;
+  }
+  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, const Head& w) const {
     Cell c;
+    c.w = w;
     c.e = 0.0f;
-    Store<T>::st(&c.w.wx, 0, 0.0f);
-    Store<T>::st(&c.w.wy, 0, 0.0f);
-    Store<T>::st(&c.w.wz, 0, 0.0f);
-    Store<T>::st(&c.w.id, 0, 0.0f);
-    if (inside) {
-      const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.w = ldg(wts, i);
-      if (wf(c.w.id) > 0.0f) c.e = Store<T>::gld(e, i);
-    }
+    if (inside && wf(w.id) > 0.0f) c.e = Store<T>::gld(e, ((size_t)x * ny + y) * nz + zz);
     return c;
   }
   __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool*,
@@ -280,13 +306,13 @@ struct OpTrueResidual {
     return o;
   }
   __device__ __forceinline__ void prepare() {}
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside) const {
-    Cell c{0.0, 1.0f};
-    if (inside) {
-      const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.p = ldg(phi, i);
-      if (c.p < 0.0f) c.x = ldg(xv, i);
-    }
+  using Head = float;  // phi (air outside the domain)
+  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
+    return inside ? ldg(phi, ((size_t)x * ny + y) * nz + zz) : 1.0f;
+  }
+  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, Head p) const {
+    Cell c{0.0, p};
+    if (inside && p < 0.0f) c.x = ldg(xv, ((size_t)x * ny + y) * nz + zz);
     return c;
   }
   __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool* has,
