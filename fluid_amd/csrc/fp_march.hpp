@@ -19,6 +19,7 @@
 // so neither waits for the other within a plane step.
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
 //              double* acc, float* mx) const;           nb/has: -x,+x,-y,+y,-z,+z
+//   void zero(int x, int y, int z) const;               the outputs of an air cell
 //   static constexpr int NV;                            sum-reduction width
 //   static constexpr bool WITH_MAX;                     also reduce a max (stored after the sums)
 // Reductions go to the grid's partials (a finisher kernel completes them).
@@ -36,7 +37,7 @@ struct MarchLaunch {
   int which_e;  // correction buffer (0: e, 1: e2) the residual reads
 };
 
-constexpr int kMarchTZ = 32;
+constexpr int kMarchTZ = kHullTZ;
 
 // honour: 0 always runs, 1 skips grids whose inner solve is done, 2 skips
 // grids whose refinement has converged.
@@ -60,11 +61,21 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
   int tzi, tyi, cxi;
   tile_of_group(blk, lv.mg.tz, lv.mg.ty, nblk, tzi, tyi, cxi);
   const int z0 = tzi * TZ, y0 = tyi * TY;
-  const int x0 = cxi * chunk;
-  const int x1 = min(x0 + chunk, nx);
+  int x0 = cxi * chunk;
+  int x1 = min(x0 + chunk, nx);
   const int tid = threadIdx.x;
   const int ly = tid / TZ, lz = tid - ly * TZ;
   const int gy = y0 + ly, gz = z0 + lz;
+  if (lv.hull) {
+    // planes of the chunk outside the tile's liquid hull: zeros
+    const int* hp = lv.hull + 2 * (tyi * lv.mg.tz + tzi);
+    const int xs = max(x0, cload(hp)), xe = min(x1, cload(hp + 1) + 1);
+    if (gy < ny && gz < nz)
+      for (int x = x0; x < x1; ++x)
+        if (x < xs || x >= xe) op.zero(x, gy, gz);
+    x0 = xs;
+    x1 = xe;
+  }
   double acc[Op::NV > 0 ? Op::NV : 1];
   for (int k = 0; k < (Op::NV > 0 ? Op::NV : 1); ++k) acc[k] = 0.0;
   constexpr int NC = (RY * RZ + 255) / 256;
@@ -94,10 +105,12 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
       pre[k] = op.load(L, yy, zz, inside, hd[k]);
     }
   };
-  fetch_head(x0 - 1);
-  fetch_cell(x0 - 1);
-  fetch_head(x0);
-  for (int L = x0 - 1; L <= x1; ++L) {
+  if (x0 < x1) {
+    fetch_head(x0 - 1);
+    fetch_cell(x0 - 1);
+    fetch_head(x0);
+  }
+  for (int L = x0 - 1; x0 < x1 && L <= x1; ++L) {
     const int slot = (L + 3) % 3;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -144,6 +157,73 @@ inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = 4096)
   m.ty = ty;
   m.ngroups = (int)(tiles * ((nx + c - 1) / c));
   return m;
+}
+
+// ---------------------------------------------------------------------------
+// Liquid hull of the march tiles along x.  A tile's planes outside
+// [hull[0], hull[1]] hold no liquid in its 32 x 8 columns, so every stencil
+// pass on the level (sweeps, residual, CG direction, true residual) yields
+// zeros there: those kernels store zeros for such planes and march only over
+// the hull, which gives the same bits as marching over the whole chunk (their
+// results do not depend on where a chunk starts or ends).
+// ---------------------------------------------------------------------------
+__global__ void k_hull_reset(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  int* hull = G.lv[level].hull;
+  if (!hull) return;
+  const size_t tiles = (size_t)G.lv[level].mg.tz * G.lv[level].mg.ty;
+  FP_STRIDE(t, tiles) {
+    hull[2 * t] = 0x7fffffff;
+    hull[2 * t + 1] = -1;
+  }
+}
+
+// One workgroup per (tile, x chunk) of the march geometry: the chunk's first
+// and last liquid planes, merged into the tile's hull with one atomic min and
+// max.
+__global__ __launch_bounds__(256) void k_tile_hull(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  const LevelRec lv = cload(&Gp->lv[level]);
+  if (!lv.hull) return;
+  constexpr int TZ = kMarchTZ, TY = 256 / TZ;
+  const int nx = lv.d[0], ny = lv.d[1], nz = lv.d[2];
+  int tzi, tyi, cxi;
+  tile_of_group(blk, lv.mg.tz, lv.mg.ty, nblk, tzi, tyi, cxi);
+  const int x0 = cxi * lv.mg.chunk, x1 = min(x0 + lv.mg.chunk, nx);
+  const int tid = threadIdx.x;
+  const int gy = tyi * TY + tid / TZ, gz = tzi * TZ + tid % TZ;
+  const bool inside = gy < ny && gz < nz;
+  const size_t col = (size_t)gy * nz + gz;
+  const long long plane = (long long)ny * nz;
+  int lo = 0x7fffffff, hi = -1;
+  for (int xb = x0; xb < x1; xb += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (inside && xb + k < x1) ? ldg(lv.phi, (size_t)((xb + k) * plane) + col) : 1.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (__any(v[k] < 0.0f)) {
+        lo = min(lo, xb + k);
+        hi = max(hi, xb + k);
+      }
+  }
+  __shared__ int s_lo[4], s_hi[4];
+  if ((tid & 63) == 0) {
+    s_lo[tid >> 6] = lo;
+    s_hi[tid >> 6] = hi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; ++w) {
+      lo = min(lo, s_lo[w]);
+      hi = max(hi, s_hi[w]);
+    }
+    if (lo <= hi) {
+      int* h = lv.hull + 2 * (tyi * lv.mg.tz + tzi);
+      atomicMin(h, lo);
+      atomicMax(h + 1, hi);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -213,6 +293,11 @@ struct OpDirection {
     stg(dnew, i, c.d);
     stg(q, i, qc);
   }
+  __device__ __forceinline__ void zero(int x, int y, int zz) const {
+    const size_t i = ((size_t)x * ny + y) * nz + zz;
+    stg(dnew, i, 0.0f);
+    stg(q, i, 0.0f);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -280,6 +365,9 @@ struct OpResidualW {
     }
     Store<T>::gst(t, i, res);
   }
+  __device__ __forceinline__ void zero(int x, int y, int zz) const {
+    Store<T>::gst(t, ((size_t)x * ny + y) * nz + zz, 0.0f);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -329,6 +417,9 @@ struct OpTrueResidual {
       *mx = fmaxf(*mx, (float)fabs(res));
     }
     stg(r, i, (float)res);
+  }
+  __device__ __forceinline__ void zero(int x, int y, int zz) const {
+    stg(r, ((size_t)x * ny + y) * nz + zz, 0.0f);
   }
 };
 

@@ -66,28 +66,36 @@ __global__ void k_build_weights(int nx, int ny, int nz, float gx, float gy, floa
   }
 }
 
+// Tile width along z of the levels' liquid hulls (the 32 x 8 march tiles).
+constexpr int kHullTZ = 32;
+
 // Workgroups are dispatched round-robin over the 8 XCDs of the device, each
 // with its own L2: group g runs on XCD g % 8.  Logical tiles run z fastest,
-// then y, then the x chunk.  Mode 2 keeps the x chunks in lock step across
-// the device and gives every XCD one y slab of each chunk (tiles % 8 == 0):
-// neighbouring tiles, which read each other's halo rows, share an L2 while the
-// device streams the same few x planes.  Mode 1 gives every XCD a contiguous
-// run of tiles; mode 0 is the identity.
-#ifndef FP_XCD_REMAP
-#define FP_XCD_REMAP 2
-#endif
+// then y, then the x chunk.  The x chunks stay in lock step across the device
+// and every XCD takes slabs of two tile rows (all z tiles of two y rows) of
+// each chunk, slab b going to XCD b % 8: neighbouring tiles in a slab, which
+// read each other's halo rows, share an L2, and liquid that fills only part
+// of the grid's height still reaches all eight XCDs (the hull-clamped passes
+// do work only where liquid is).  Grids whose tile rows do not split into
+// 2-row slabs for 8 XCDs take one contiguous slab per XCD (tiles % 8 == 0),
+// else the identity.
 __device__ __forceinline__ void tile_of_group(int g, int tiles_z, int tiles_y, int ngroups, int& tzi, int& tyi,
                                               int& cxi) {
+  (void)ngroups;
   const int tiles = tiles_z * tiles_y;
   int l = g;
-  if (FP_XCD_REMAP == 2 && (tiles & 7) == 0) {
+  if ((tiles_y & 15) == 0) {
+    const int S = tiles >> 3;          // tiles per XCD per chunk
+    const int Bt = 2 * tiles_z;        // tiles per slab
+    const int i = g >> 3;
+    const int chunk = i / S, r = i - chunk * S;
+    const int bx = r / Bt;
+    l = chunk * tiles + (bx * 8 + (g & 7)) * Bt + (r - bx * Bt);
+  } else if ((tiles & 7) == 0) {
     const int S = tiles >> 3;
     const int i = g >> 3;
     const int chunk = i / S;
     l = chunk * tiles + (g & 7) * S + (i - chunk * S);
-  } else if (FP_XCD_REMAP == 1) {
-    const int per = ngroups >> 3;
-    if (g < (per << 3)) l = (g & 7) * per + (g >> 3);
   }
   tzi = l % tiles_z;
   const int t = l / tiles_z;
@@ -213,14 +221,41 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const int nx = lvl.d[0], ny = lvl.d[1], nz = lvl.d[2];
   int tzi, tyi, cxi;
   tile_of_group(blk, tiles_z, tiles_y, nblk, tzi, tyi, cxi);
-  const int z0 = tzi * TZ, y0 = tyi * TY, x0 = cxi * chunk;
-  const int x1 = min(x0 + chunk, nx);
+  const int z0 = tzi * TZ, y0 = tyi * TY;
+  int x0 = cxi * chunk;
+  int x1 = min(x0 + chunk, nx);
   const int tid = threadIdx.x;
   const long long plane = (long long)ny * nz;
   const int second = 1 - first;
   // even rows: both colours of a plane leave in one pass of paired stores
   // (colour 2 writes each (z even, z odd) pair); odd nz keeps single stores
   const bool pairs = (nz & 1) == 0;
+  if constexpr (TZ == kHullTZ) {
+    // the level's 32 x 8 tiles carry liquid hulls (k_tile_hull): planes of
+    // the chunk outside the hull are all air in this tile and sweep to zero
+    if (lvl.hull) {
+      const int* hp = lvl.hull + 2 * (tyi * tiles_z + tzi);
+      const int xs = max(x0, cload(hp)), xe = min(x1, cload(hp + 1) + 1);
+      if (xs > x0 || xe < x1) {
+        const int zy = tid / (TZ / 2), zk = tid - (tid / (TZ / 2)) * (TZ / 2);
+        const int gy = y0 + zy, gz = z0 + 2 * zk;
+        if (tid < TY * (TZ / 2) && gy < ny && gz < nz)
+          for (int x = x0; x < x1; ++x) {
+            if (x >= xs && x < xe) continue;
+            const size_t o = (size_t)((long long)x * plane) + (size_t)gy * nz + gz;
+            if (pairs) {
+              gst_pair(eout, o, 0.0f, 0.0f);
+            } else {
+              Store<T>::gst(eout, o, 0.0f);
+              if (gz + 1 < nz) Store<T>::gst(eout, o + 1, 0.0f);
+            }
+          }
+      }
+      x0 = xs;
+      x1 = xe;
+      if (x0 >= x1) return;
+    }
+  }
 
   // in-plane offsets of the staged cells (-1: outside the grid)
   int ow[NW], oe[NE > 0 ? NE : 1], orr[NR];

@@ -1037,6 +1037,8 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
     lv.sweep_tz = lv.d[2] >= 32 ? 32 : 16;
     lv.sw = sweep_geometry(lv.d, lv.sweep_tz);
     lv.mg = march_geometry(lv.d[0], lv.d[1], lv.d[2]);
+    lv.hull = nullptr;
+    if (lv.fused && (rc = alloc((void**)&lv.hull, (size_t)lv.mg.tz * lv.mg.ty * 2 * sizeof(int)))) return rc;
   }
   const LevelRec& c = R.lv[L - 1];
   R.max_coarse = (int)std::min<long long>(c.n, 512);
@@ -1493,6 +1495,10 @@ int Projector::setup_tables(hipStream_t s, Plan& p, int preserve) {
   if (p.precision == FP_PRECISION_MIXED_FP16) build_weights_t<__half>(s, this, p.maxL, wt);
   else if (p.precision == FP_PRECISION_MIXED_BF16) build_weights_t<__hip_bfloat16>(s, this, p.maxL, wt);
   else build_weights_t<float>(s, this, p.maxL, wt);
+  for (int l = 0; l + 1 < p.maxL; ++l) {
+    PLAUNCH(k_hull_reset, p.lv[l].fusedcells, s, l);
+    PLAUNCH(k_tile_hull, p.lv[l].res, s, l);
+  }
   return FP_OK;
 }
 
