@@ -161,7 +161,7 @@ inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = 4096)
 
 // ---------------------------------------------------------------------------
 // Liquid hull of the march tiles along x.  A tile's planes outside
-// [hull[0], hull[1]] hold no liquid in its 32 x 8 columns, so every stencil
+// [hull[0], hull[1]] ([nx, -1] when it has none) hold no liquid in its 32 x 8 columns, so every stencil
 // pass on the level (sweeps, residual, CG direction, true residual) yields
 // zeros there: those kernels store zeros for such planes and march only over
 // the hull, which gives the same bits as marching over the whole chunk (their
@@ -172,8 +172,9 @@ __global__ void k_hull_reset(const Part* parts, int np, int level) {
   int* hull = G.lv[level].hull;
   if (!hull) return;
   const size_t tiles = (size_t)G.lv[level].mg.tz * G.lv[level].mg.ty;
-  FP_STRIDE(t, tiles) {
-    hull[2 * t] = 0x7fffffff;
+  const int nx = G.lv[level].d[0];
+  FP_STRIDE(t, tiles) {  // empty: [nx, -1]
+    hull[2 * t] = nx;
     hull[2 * t + 1] = -1;
   }
 }
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void k_tile_hull(const Part* parts, int np, in
   const bool inside = gy < ny && gz < nz;
   const size_t col = (size_t)gy * nz + gz;
   const long long plane = (long long)ny * nz;
-  int lo = 0x7fffffff, hi = -1;
+  int lo = nx, hi = -1;
   for (int xb = x0; xb < x1; xb += 8) {
     float v[8];
 #pragma unroll
@@ -224,6 +225,33 @@ __global__ __launch_bounds__(256) void k_tile_hull(const Part* parts, int np, in
       atomicMax(h + 1, hi);
     }
   }
+}
+
+// Planes of a march-geometry workgroup's chunk that can hold liquid in its
+// tile (the chunk clamped to the tile's hull; the whole chunk on levels
+// without hulls) and the lane's cell column.
+struct TileSpan {
+  int x0, x1;
+  size_t col;    // y * nz + z of the lane's column
+  bool inside;   // the column lies in the grid
+};
+__device__ __forceinline__ TileSpan tile_span(const LevelRec& lv, int blk, int nblk) {
+  constexpr int TZ = kMarchTZ, TY = 256 / TZ;
+  int tzi, tyi, cxi;
+  tile_of_group(blk, lv.mg.tz, lv.mg.ty, nblk, tzi, tyi, cxi);
+  TileSpan t;
+  t.x0 = cxi * lv.mg.chunk;
+  t.x1 = min(t.x0 + lv.mg.chunk, lv.d[0]);
+  if (lv.hull) {
+    const int* hp = lv.hull + 2 * (tyi * lv.mg.tz + tzi);
+    t.x0 = max(t.x0, cload(hp));
+    t.x1 = min(t.x1, cload(hp + 1) + 1);
+    if (t.x0 > t.x1) t.x0 = t.x1;  // no liquid: an empty span
+  }
+  const int gy = tyi * TY + (int)threadIdx.x / TZ, gz = tzi * TZ + (int)threadIdx.x % TZ;
+  t.inside = gy < lv.d[1] && gz < lv.d[2];
+  t.col = (size_t)gy * lv.d[2] + gz;
+  return t;
 }
 
 // ---------------------------------------------------------------------------

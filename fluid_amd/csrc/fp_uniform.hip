@@ -149,28 +149,21 @@ __global__ void k_rhs(const Part* parts, int np) {
 
 // x += e; e = 0 (folds one inner solve into the FP64 iterate).  A broken CG
 // recurrence leaves the FP64 iterate as it was (fail closed).
-__global__ void k_accumulate(const Part* parts, int np) {
+__global__ __launch_bounds__(256) void k_accumulate(const Part* parts, int np) {
   FP_PART(parts, np);
-  const State* st = G.st;
-  if (st->outer_done) return;
-  const bool keep = !(st->flags & 0x7);
-  const size_t n = (size_t)G.n;
-  double* __restrict__ x = G.x;
-  float* __restrict__ e = G.e;
-  const size_t n4 = n / 4;
-  FP_STRIDE(i, n4) {
-    const float4 ev = reinterpret_cast<const float4*>(e)[i];
-    if (keep) {
-      double4 xv = reinterpret_cast<const double4*>(x)[i];
-      xv.x += ev.x; xv.y += ev.y; xv.z += ev.z; xv.w += ev.w;
-      reinterpret_cast<double4*>(x)[i] = xv;
-    }
-    reinterpret_cast<float4*>(e)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-  if (blk == 0 && threadIdx.x < (n & 3)) {
-    const size_t i = n4 * 4 + threadIdx.x;
-    if (keep) x[i] += (double)e[i];
-    e[i] = 0.0f;
+  const State* st = cload(&Gp->st);
+  if (cload(&st->outer_done)) return;
+  const bool keep = !(cload(&st->flags) & 0x7);
+  double* __restrict__ x = cload(&Gp->x);
+  float* __restrict__ e = cload(&Gp->e);
+  const LevelRec l0 = cload(&Gp->lv[0]);
+  const TileSpan sp = tile_span(l0, blk, nblk);
+  const size_t plane = (size_t)l0.d[1] * l0.d[2];
+  if (!sp.inside) return;
+  for (int px = sp.x0; px < sp.x1; ++px) {
+    const size_t i = (size_t)px * plane + sp.col;
+    if (keep) stg(x, i, ldg(x, i) + (double)ldg(e, i));
+    stg(e, i, 0.0f);
   }
 }
 
@@ -678,8 +671,9 @@ __device__ __forceinline__ fp_f32x4 ldg4T(const __hip_bfloat16* p, size_t i4) {
                   __uint_as_float(u.y & 0xffff0000u)};
 }
 
+// r . z over the finest level's liquid hulls (r and z vanish on air cells).
 template <typename T>
-__global__ void k_cg_rz(const Part* parts, int np) {
+__global__ __launch_bounds__(256) void k_cg_rz(const Part* parts, int np) {
   FP_PART(parts, np);
   const State* st = cload(&Gp->st);
   if (cload(&st->done)) return;
@@ -687,25 +681,35 @@ __global__ void k_cg_rz(const Part* parts, int np) {
   const T* __restrict__ z = (const T*)(l0.efinal ? l0.e2 : l0.e);
   const float* __restrict__ r = cload(&Gp->r);
   const float zm = l0.Se / cload(&st->rscale);
-  const size_t n = (size_t)cload(&Gp->n);
+  const TileSpan sp = tile_span(l0, blk, nblk);
+  const size_t plane = (size_t)l0.d[1] * l0.d[2];
   double acc[1] = {0.0};
-  const size_t n4 = n / 4;
-  FP_STRIDE(i, n4) {
-    const fp_f32x4 rv = ldg4(r, i);
-    const fp_f32x4 zv = ldg4T(z, i);
-    acc[0] += ((double)rv.x * (double)(zv.x * zm) + (double)rv.y * (double)(zv.y * zm)) +
-              ((double)rv.z * (double)(zv.z * zm) + (double)rv.w * (double)(zv.w * zm));
-  }
-  if (blk == 0 && threadIdx.x < (n & 3)) {
-    const size_t i = n4 * 4 + threadIdx.x;
-    acc[0] += (double)r[i] * (double)(ldT(z, i) * zm);
+  if (sp.inside) {
+    int x = sp.x0;
+    for (; x + 4 <= sp.x1; x += 4) {
+      float rv[4], zv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)(x + k) * plane + sp.col;
+        rv[k] = ldg(r, i);
+        zv[k] = Store<T>::gld(z, i);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[0] += (double)rv[k] * (double)(zv[k] * zm);
+    }
+    for (; x < sp.x1; ++x) {
+      const size_t i = (size_t)x * plane + sp.col;
+      acc[0] += (double)ldg(r, i) * (double)(Store<T>::gld(z, i) * zm);
+    }
   }
   block_partials<1>(acc, cload(&Gp->partials), blk, nblk);
 }
 
-// e += alpha d ; r -= alpha q ; ||r||^2, max|r| -> convergence and next rscale.
-// d: the direction this iteration wrote (buffer (iter & 1) ^ 1).
-__global__ void k_cg_update(const Part* parts, int np) {
+// e += alpha d ; r -= alpha q ; ||r||^2, max|r| -> convergence and next rscale,
+// over the finest level's liquid hulls (d, q and r vanish on air cells, so the
+// update is the identity there).  d: the direction this iteration wrote
+// (buffer (iter & 1) ^ 1).
+__global__ __launch_bounds__(256) void k_cg_update(const Part* parts, int np) {
   FP_PART(parts, np);
   const State* st = cload(&Gp->st);
   if (cload(&st->done)) return;
@@ -714,29 +718,41 @@ __global__ void k_cg_update(const Part* parts, int np) {
   const float* __restrict__ q = cload(&Gp->q);
   float* __restrict__ e = cload(&Gp->e);
   float* __restrict__ r = cload(&Gp->r);
-  const size_t n = (size_t)cload(&Gp->n);
+  const LevelRec l0 = cload(&Gp->lv[0]);
+  const TileSpan sp = tile_span(l0, blk, nblk);
+  const size_t plane = (size_t)l0.d[1] * l0.d[2];
   double acc[1] = {0.0};
   float rmax = 0.0f;
-  const size_t n4 = n / 4;
-  FP_STRIDE(i, n4) {
-    const fp_f32x4 dv = ldg4(d, i);
-    const fp_f32x4 qv = ldg4(q, i);
-    fp_f32x4 ev = ldg4(e, i);
-    fp_f32x4 rv = ldg4(r, i);
-    ev.x += alpha * dv.x; ev.y += alpha * dv.y; ev.z += alpha * dv.z; ev.w += alpha * dv.w;
-    rv.x -= alpha * qv.x; rv.y -= alpha * qv.y; rv.z -= alpha * qv.z; rv.w -= alpha * qv.w;
-    stg4(e, i, ev);
-    stg4(r, i, rv);
-    acc[0] += ((double)rv.x * rv.x + (double)rv.y * rv.y) + ((double)rv.z * rv.z + (double)rv.w * rv.w);
-    rmax = fmaxf(rmax, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
-  }
-  if (blk == 0 && threadIdx.x < (n & 3)) {
-    const size_t i = n4 * 4 + threadIdx.x;
-    e[i] += alpha * d[i];
-    const float rn = r[i] - alpha * q[i];
-    r[i] = rn;
-    acc[0] += (double)rn * (double)rn;
-    rmax = fmaxf(rmax, fabsf(rn));
+  if (sp.inside) {
+    int x = sp.x0;
+    for (; x + 4 <= sp.x1; x += 4) {
+      float dv[4], qv[4], ev[4], rv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)(x + k) * plane + sp.col;
+        dv[k] = ldg(d, i);
+        qv[k] = ldg(q, i);
+        ev[k] = ldg(e, i);
+        rv[k] = ldg(r, i);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)(x + k) * plane + sp.col;
+        const float en = ev[k] + alpha * dv[k], rn = rv[k] - alpha * qv[k];
+        stg(e, i, en);
+        stg(r, i, rn);
+        acc[0] += (double)rn * (double)rn;
+        rmax = fmaxf(rmax, fabsf(rn));
+      }
+    }
+    for (; x < sp.x1; ++x) {
+      const size_t i = (size_t)x * plane + sp.col;
+      const float en = ldg(e, i) + alpha * ldg(d, i), rn = ldg(r, i) - alpha * ldg(q, i);
+      stg(e, i, en);
+      stg(r, i, rn);
+      acc[0] += (double)rn * (double)rn;
+      rmax = fmaxf(rmax, fabsf(rn));
+    }
   }
   block_partials_max<1>(acc, rmax, cload(&Gp->partials), blk, nblk);
 }
@@ -1427,12 +1443,12 @@ int Projector::enqueue_vcycle(hipStream_t s, Plan& p) {
 template <typename T>
 int Projector::iteration_t(hipStream_t s, Plan& p) {
   vcycle_t<T>(s, p);
-  PLAUNCH(k_cg_rz<T>, p.vec4, s);
-  FLAUNCH(k_fin_rz<Part>, p.vec4, s);
+  PLAUNCH(k_cg_rz<T>, p.march0, s);
+  FLAUNCH(k_fin_rz<Part>, p.march0, s);
   TIMED(FP_KERNEL_CG_DIRECTION, PLAUNCH(k_march<OpDirection<T>>, p.march0, s, 1, MarchLaunch{0, 0}));
   FLAUNCH(k_fin_dq<Part>, p.march0, s);
-  TIMED(FP_KERNEL_CG_UPDATE, PLAUNCH(k_cg_update, p.vec4, s));
-  FLAUNCH(k_fin_update<Part>, p.vec4, s);
+  TIMED(FP_KERNEL_CG_UPDATE, PLAUNCH(k_cg_update, p.march0, s));
+  FLAUNCH(k_fin_update<Part>, p.march0, s);
   return FP_OK;
 }
 
@@ -1557,7 +1573,7 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
         }
       }
     }
-    PLAUNCH(k_accumulate, p.vec4, s);
+    PLAUNCH(k_accumulate, p.march0, s);
     PLAUNCH(k_zero, p.vec4, s, 2, 1);
   }
   for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, p.faces[a], s, a);
@@ -1720,7 +1736,7 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
   }
   // ---- finished inner solves fold into the FP64 iterate ----
   for (auto& ap : accum_p) {
-    PLAUNCH(k_accumulate, ap.first->vec4, s);
+    PLAUNCH(k_accumulate, ap.first->march0, s);
     PLAUNCH(k_zero, ap.first->vec4, s, 2, 1);
   }
   // ---- FP64 true residuals (refinement control) ----
