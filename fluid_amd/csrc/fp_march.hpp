@@ -20,6 +20,9 @@
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
 //              double* acc, float* mx) const;           nb/has: -x,+x,-y,+y,-z,+z
 //   void zero(int x, int y, int z) const;               the outputs of an air cell
+//   static constexpr bool ZERO_OUTSIDE;                 store zeros outside the liquid hulls
+//                                                       (false: every reader of the outputs
+//                                                       stays inside the hulls or on liquid)
 //   static constexpr int NV;                            sum-reduction width
 //   static constexpr bool WITH_MAX;                     also reduce a max (stored after the sums)
 // Reductions go to the grid's partials (a finisher kernel completes them).
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
     // planes of the chunk outside the tile's liquid hull: zeros
     const int* hp = lv.hull + 2 * (tyi * lv.mg.tz + tzi);
     const int xs = max(x0, cload(hp)), xe = min(x1, cload(hp + 1) + 1);
-    if (gy < ny && gz < nz)
+    if (Op::ZERO_OUTSIDE && gy < ny && gz < nz)
       for (int x = x0; x < x1; ++x)
         if (x < xs || x >= xe) op.zero(x, gy, gz);
     x0 = xs;
@@ -143,7 +146,10 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
 
 // Tiles of 32 x 8 columns, x chunks of at least 8 planes, about target_blocks
 // workgroups (at most the 16384 partials of the reductions).
-inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = 4096) {
+#ifndef FP_MARCH_GROUPS
+#define FP_MARCH_GROUPS 8192
+#endif
+inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = FP_MARCH_GROUPS) {
   const int TZ = kMarchTZ, TY = 256 / TZ;
   const int tz = (nz + TZ - 1) / TZ, ty = (ny + TY - 1) / TY;
   const long tiles = (long)tz * ty;
@@ -262,6 +268,9 @@ struct OpDirection {
   struct Cell { float d; float p; };
   static constexpr int NV = 1;
   static constexpr bool WITH_MAX = false;
+  // d and q are read on liquid cells (the next direction) and over the hulls
+  // (the CG update), never on the air planes outside them
+  static constexpr bool ZERO_OUTSIDE = false;
   int ny, nz;
   const float* phi;
   const T* z;
@@ -337,6 +346,7 @@ struct OpResidualW {
   struct Cell { W4<T> w; float e; };
   static constexpr int NV = 0;
   static constexpr bool WITH_MAX = false;
+  static constexpr bool ZERO_OUTSIDE = true;  // the restriction reads t everywhere
   int ny, nz;
   const W4<T>* wts;
   const TR* r;
@@ -405,6 +415,9 @@ struct OpTrueResidual {
   struct Cell { double x; float p; };
   static constexpr int NV = 1;
   static constexpr bool WITH_MAX = true;
+  // r is read on liquid cells (the sweeps' own right-hand side) and over the
+  // hulls (CG update, r.z)
+  static constexpr bool ZERO_OUTSIDE = false;
   int ny, nz;
   const float* phi;
   const double* b;

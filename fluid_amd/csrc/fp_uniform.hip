@@ -936,13 +936,18 @@ static bool fused_dims(const int* d, long long n) { return d[2] >= 16 && d[1] >=
 // Sweep geometry: tiles of TZ x 256/TZ columns, about two rounds of 8
 // resident workgroups per CU on 256 CUs, at least 8 planes per workgroup
 // (the march carries 4 planes of start-up).
+// About FP_SWEEP_GROUPS workgroups per sweep of a level (x chunks of at least
+// 8 planes).
+#ifndef FP_SWEEP_GROUPS
+#define FP_SWEEP_GROUPS 8192
+#endif
 static TileGeo sweep_geometry(const int* d, int TZ) {
   const int TY = 256 / TZ;
   TileGeo g;
   g.tz = (d[2] + TZ - 1) / TZ;
   g.ty = (d[1] + TY - 1) / TY;
   const long tiles = (long)g.tz * g.ty;
-  const long cx_target = std::max<long>(1, 4096 / tiles);
+  const long cx_target = std::max<long>(1, FP_SWEEP_GROUPS / tiles);
   int chunk = (int)std::max<long>(8, (d[0] + cx_target - 1) / cx_target);
   chunk = std::min(chunk, d[0]);
   g.chunk = chunk;
