@@ -44,6 +44,8 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
     residual_fine: read coefficients (4s), r (4), e (s); write t (s).
     cg_direction: read z (s), d_old (4), phi (4); write d (4), q (4).
     cg_update: read d, q, e, r (16); write e, r (8).
+    Per cell of the liquid hulls (hull_cells); the launch's algorithmic bytes
+    add s per remaining cell for the passes that store zeros there.
     """
     s = 4 if precision == "fp32" else 2
     plain = 4 * s + 4 + 2 * s
@@ -54,6 +56,30 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
         "cg_update": 24.0,
         "residual_fine": 4 * s + 4 + s + s,
     }[kernel]
+
+
+def hull_cells(phi, tz: int = 32, ty: int = 8) -> int:
+    """Cells of the finest level inside the liquid hulls of its 32 x 8 (z x y)
+    column tiles: per tile, the planes from the first to the last x plane in
+    which any of the tile's cells is liquid.  The hull-clamped passes (sweeps,
+    residual, CG direction and update) read only these cells; elsewhere a
+    sweep stores zeros."""
+    import numpy as np
+    liq = np.asarray(phi) < 0
+    nx, ny, nz = liq.shape
+    py, pz = -ny % ty, -nz % tz
+    inside = np.ones((ny + py, nz + pz), bool)
+    if py or pz:
+        liq = np.pad(liq, ((0, 0), (0, py), (0, pz)))
+        inside[ny:, :] = False
+        inside[:, nz:] = False
+    t = liq.reshape(nx, (ny + py) // ty, ty, (nz + pz) // tz, tz).any(axis=(2, 4))
+    has = t.any(axis=0)
+    lo = np.where(has, t.argmax(axis=0), 0)
+    hi = np.where(has, nx - 1 - t[::-1].argmax(axis=0), -1)
+    planes = np.maximum(hi - lo + 1, 0)
+    cols = inside.reshape((ny + py) // ty, ty, (nz + pz) // tz, tz).sum(axis=(1, 3))
+    return int((planes * cols).sum())
 
 
 def traffic_bytes(key):
@@ -408,7 +434,12 @@ def main():
     kl = sum(x["timed_kernel_launches"] for x in reports)
     avg_ms = kms / max(kl, 1)
     bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps)
-    achieved = bpc * cells / (avg_ms * 1e-3) / 1e9
+    # the timed passes touch the cells of the liquid hulls; a sweep stores a
+    # zero correction (s bytes) on every other cell
+    active = hull_cells(s.phi)
+    zero_b = 0.0 if args.timed_kernel in ("cg_direction", "cg_update") else (4 if args.precision == "fp32" else 2)
+    alg_bytes = bpc * active + zero_b * (cells - active)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # error against a tight FP64-refined solve of the same system (1e-12)
     err = None
     if not args.no_error_solve:
@@ -441,8 +472,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps)),
-                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc,
-                     "algorithmic_bytes": round(bpc * cells), "launches": kl},
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc, "hull_cells": active,
+                     "algorithmic_bytes": round(alg_bytes), "launches": kl},
     }
     g.destroy()
     if not args.no_cpu_baseline:
