@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 1246.1,
-    ("tall", 512, "fp32", "smooth_fine", 2): 381.8,
+    ("uniform", 384, "fp16", "smooth_fine", 2): 577.3,
+    ("tall", 512, "fp32", "smooth_fine", 2): 385.5,
 }
 
 
