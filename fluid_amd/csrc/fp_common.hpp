@@ -174,8 +174,20 @@ __device__ __forceinline__ void sum_partials(const double* partials, unsigned G,
   double s[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    s[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s[k] += partials[(size_t)k * G + b];
+    // four independent chains keep several loads in flight per thread; the
+    // order of the additions is fixed, so the sum stays deterministic
+    const double* p = partials + (size_t)k * G;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const unsigned step = blockDim.x;
+    unsigned b = threadIdx.x;
+    for (; b + 3 * step < G; b += 4 * step) {
+      a0 += p[b];
+      a1 += p[b + step];
+      a2 += p[b + 2 * step];
+      a3 += p[b + 3 * step];
+    }
+    for (; b < G; b += step) a0 += p[b];
+    s[k] = (a0 + a1) + (a2 + a3);
   }
   block_sum<NV>(s);
 #pragma unroll
@@ -212,9 +224,16 @@ __device__ __forceinline__ void block_partials_max(double (&v)[NV], float m, dou
 template <int NV>
 __device__ __forceinline__ void sum_partials_max(const double* partials, unsigned G, double (&out)[NV], float& mx) {
   sum_partials<NV>(partials, G, out);
-  float m = 0.0f;
-  for (unsigned b = threadIdx.x; b < G; b += blockDim.x) m = fmaxf(m, (float)partials[(size_t)NV * G + b]);
-  mx = block_max(m);
+  float m0 = 0.0f, m1 = 0.0f;
+  const double* p = partials + (size_t)NV * G;
+  const unsigned step = blockDim.x;
+  unsigned b = threadIdx.x;
+  for (; b + step < G; b += 2 * step) {
+    m0 = fmaxf(m0, (float)p[b]);
+    m1 = fmaxf(m1, (float)p[b + step]);
+  }
+  for (; b < G; b += step) m0 = fmaxf(m0, (float)p[b]);
+  mx = block_max(fmaxf(m0, m1));
 }
 
 // Fast h^2-scaled face coefficient for the FP32 solve and the multigrid:

@@ -426,7 +426,9 @@ __global__ void k_residual(const Part* parts, int np, int level) {
 // plus the one-cell ring of the 4-tap stencil), filtered in y and z per lane,
 // and added with its x weight into the two coarse planes it feeds; a coarse
 // plane is written once its last fine plane has been seen.  Fine values are
-// read once per tile instead of once per coarse tap.
+// read once per tile instead of once per coarse tap.  Every reader of r uses
+// it on liquid cells only, so on levels with liquid hulls the march covers the
+// coarse tile's hull and leaves the other planes unwritten.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int np, int level) {
   FP_PART(parts, np);
@@ -448,7 +450,15 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
   int tzi, tyi, cxi;
   tile_of_group(blk, cl.mg.tz, cl.mg.ty, nblk, tzi, tyi, cxi);
   const int cz0 = tzi * TZ, cy0 = tyi * TY;
-  const int X0 = cxi * cl.mg.chunk, X1 = min(X0 + cl.mg.chunk, cnx);
+  int X0 = cxi * cl.mg.chunk, X1 = min(X0 + cl.mg.chunk, cnx);
+  if (cl.hull) {
+    // r is read on the coarse level's liquid cells only: march over the
+    // coarse tile's liquid hull
+    const int* hp = cl.hull + 2 * (tyi * cl.mg.tz + tzi);
+    X0 = max(X0, cload(hp));
+    X1 = min(X1, cload(hp + 1) + 1);
+    if (X0 >= X1) return;
+  }
   const int tid = threadIdx.x;
   // staged fine window: rows fy0 .. fy0+ry-1, columns fz0 .. fz0+rz-1
   const int fy0 = sy == 2 ? 2 * cy0 - 1 : cy0, fz0 = sz == 2 ? 2 * cz0 - 1 : cz0;
