@@ -1002,6 +1002,7 @@ size_t TallGrid::estimate_bytes(int nx, int ny, int nz, int band, int max_delta)
     add(nl * 4 * (5 * K + 3));     // banded couplings
     add(nl * 4 * 5);               // dinv r e e2 t
   }
+  add(2 * kTallMaxLevels * sizeof(int));  // reach
   add(kTallMaxCoarse * 4 + 16);
   add(16);
   add((size_t)kTallMaxCoarse * kTallMaxCoarse * 8);
@@ -1089,6 +1090,7 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   R.lv[0].n = R.n;
   R.lv[0].cy = 1;
   R.max_coarse = kTallMaxCoarse;
+  if ((rc = alloc((void**)&R.reach, 2 * kTallMaxLevels * sizeof(int)))) return rc;
   if ((rc = alloc((void**)&R.cmap, (size_t)kTallMaxCoarse * 4 + 16))) return rc;
   if ((rc = alloc((void**)&R.ncoarse, 16))) return rc;
   if ((rc = alloc((void**)&R.minv, (size_t)kTallMaxCoarse * kTallMaxCoarse * 8))) return rc;
@@ -1330,6 +1332,7 @@ void TallProjector::timer_end(hipStream_t s) {
 int TallProjector::setup(hipStream_t s, Plan& p) {
   PLAUNCH(k_tall_columns, p.columns, s);
   PLAUNCH(k_tall_diag, p.samples, s);
+  FLAUNCH(k_tall_reach_reset, p.one, s);
   PLAUNCH(k_tall_galerkin1, p.lv[0].coarser, s);
   PLAUNCH(k_tall_l1diag, p.lv[0].coarser, s, 1);
   for (int l = 1; l + 1 < p.maxL; ++l) {

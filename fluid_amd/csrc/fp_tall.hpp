@@ -70,6 +70,8 @@ struct TallRec {
   float* z2;
   float* t;            // sample-level residual
   TallLevel lv[kTallMaxLevels];
+  int* reach;          // per lattice level l: [2l] the largest row offset of a nonzero x/z
+                       // coupling, [2l+1] of a same-column one (k_tall_l1diag)
   int* cmap;
   int* ncoarse;
   double* minv;

@@ -202,14 +202,17 @@ __global__ void k_tall_galerkin1(const TallPart* parts, int np) {
 
 // ---- level l+1 from level l (both banded) -----------------------------------
 // Every coupling of a level-l node i, emitted as (neighbour X, Y, Z, a).
+// kh / kv bound the row offsets visited for the x/z and the same-column
+// couplings (K, K visit the whole band; a level's measured reach skips slots
+// that are zero on every node).
 template <class Emit>
-__device__ void lattice_row_entries(const TallLevel& L, int K, int x, int y, int z, Emit& emit) {
+__device__ void lattice_row_entries(const TallLevel& L, int K, int kh, int kv, int x, int y, int z, Emit& emit) {
   const size_t n = (size_t)L.n;
   const float* __restrict__ w = L.w;
   auto at = [&](int xx, int yy, int zz) { return ((size_t)xx * L.ny + yy) * L.nz + zz; };
   const size_t i = at(x, y, z);
   emit(x, y, z, w[(size_t)slot_d(K) * n + i]);
-  for (int dy = -K; dy <= K; ++dy) {
+  for (int dy = -kh; dy <= kh; ++dy) {
     const int yy = y + dy;
     if (yy < 0 || yy >= L.ny) continue;
     if (x + 1 < L.nx) {
@@ -229,7 +232,7 @@ __device__ void lattice_row_entries(const TallLevel& L, int K, int x, int y, int
       if (v != 0.0f) emit(x, yy, z - 1, v);
     }
   }
-  for (int dy = 1; dy <= K; ++dy) {
+  for (int dy = 1; dy <= kv; ++dy) {
     if (y + dy < L.ny) {
       const float v = w[(size_t)slot_y(K, dy) * n + i];
       if (v != 0.0f) emit(x, y + dy, z, v);
@@ -269,7 +272,7 @@ __global__ void k_tall_galerkin(const TallPart* parts, int np, int level) {
         for (int z = 2 * Z; z < min(2 * Z + 2, F.nz); ++z) {
           const size_t fi = ((size_t)x * F.ny + y) * F.nz + z;
           if (!(F.w[(size_t)slot_d(K) * nf + fi] > 0.0f)) continue;
-          lattice_row_entries(F, K, x, y, z, emit);
+          lattice_row_entries(F, K, K, K, x, y, z, emit);
         }
     const int NS = 5 * K + 3;
     for (int s = 0; s < NS; ++s) C.w[(size_t)s * n + ci] = kTallGalerkinScale * acc.a[s];
@@ -282,6 +285,7 @@ __global__ void k_tall_l1diag(const TallPart* parts, int np, int level) {
   const TallLevel& L = G.lv[level];
   const int K = G.K;
   const size_t n = (size_t)L.n;
+  int kh = 0, kv = 0;
   FP_STRIDE(i, n) {
     const int z = (int)(i % (size_t)L.nz);
     const size_t tt = i / (size_t)L.nz;
@@ -289,20 +293,42 @@ __global__ void k_tall_l1diag(const TallPart* parts, int np, int level) {
     const int x = (int)(tt / (size_t)L.ny);
     float s = 0.0f, d = 0.0f;
     auto emit = [&](int xj, int yj, int zj, float v) {
-      if (xj == x && yj == y && zj == z) d = v;
-      else s += fabsf(v);
+      if (xj == x && yj == y && zj == z) {
+        d = v;
+        return;
+      }
+      s += fabsf(v);
+      const int dy = yj > y ? yj - y : y - yj;
+      if (xj == x && zj == z) kv = max(kv, dy);
+      else kh = max(kh, dy);
     };
-    lattice_row_entries(L, K, x, y, z, emit);
+    lattice_row_entries(L, K, K, K, x, y, z, emit);
     L.dinv[i] = d > 0.0f ? kTallL1Theta / (d + s) : 0.0f;
+  }
+  // the level's reach: the largest row offset of a nonzero coupling
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    kh = max(kh, __shfl_xor(kh, o, 64));
+    kv = max(kv, __shfl_xor(kv, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(G.reach + 2 * level, kh);
+    atomicMax(G.reach + 2 * level + 1, kv);
   }
 }
 
+// Zero reach before the levels' l1 diagonals measure it (one workgroup per grid).
+__global__ void k_tall_reach_reset(const TallPart* parts) {
+  const TallRec& G = *parts[blockIdx.x].g;
+  for (int i = threadIdx.x; i < 2 * kTallMaxLevels; i += blockDim.x) G.reach[i] = 0;
+}
+
 // (A e)_i on a lattice level.
-__device__ __forceinline__ float lattice_apply(const TallLevel& L, int K, const float* __restrict__ e, int x, int y,
-                                               int z) {
+__device__ __forceinline__ float lattice_apply(const TallLevel& L, int K, int kh, int kv, const float* __restrict__ e,
+                                               int x, int y, int z) {
   float acc = 0.0f;
   auto emit = [&](int xj, int yj, int zj, float v) { acc += v * e[((size_t)xj * L.ny + yj) * L.nz + zj]; };
-  lattice_row_entries(L, K, x, y, z, emit);
+  lattice_row_entries(L, K, kh, kv, x, y, z, emit);
   return acc;
 }
 
@@ -312,7 +338,7 @@ __global__ void k_lat_jacobi(const TallPart* parts, int np, int level, int sel, 
   FP_PART(parts, np);
   if (G.st->done) return;
   const TallLevel& L = G.lv[level];
-  const int K = G.K;
+  const int K = G.K, kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
   const float* __restrict__ ein = sel ? L.e2 : L.e;
   float* __restrict__ eout = first ? L.e : (sel ? L.e : L.e2);
   FP_STRIDE(i, (size_t)L.n) {
@@ -326,7 +352,7 @@ __global__ void k_lat_jacobi(const TallPart* parts, int np, int level, int sel, 
         const size_t tt = i / (size_t)L.nz;
         const int y = (int)(tt % (size_t)L.ny);
         const int x = (int)(tt / (size_t)L.ny);
-        out = ein[i] + (L.r[i] - lattice_apply(L, K, ein, x, y, z)) * di;
+        out = ein[i] + (L.r[i] - lattice_apply(L, K, kh, kv, ein, x, y, z)) * di;
       }
     }
     eout[i] = out;
@@ -338,7 +364,7 @@ __global__ void k_lat_residual(const TallPart* parts, int np, int level, int sel
   FP_PART(parts, np);
   if (G.st->done) return;
   const TallLevel& L = G.lv[level];
-  const int K = G.K;
+  const int K = G.K, kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
   const float* __restrict__ e = sel ? L.e2 : L.e;
   FP_STRIDE(i, (size_t)L.n) {
     float res = 0.0f;
@@ -347,7 +373,7 @@ __global__ void k_lat_residual(const TallPart* parts, int np, int level, int sel
       const size_t tt = i / (size_t)L.nz;
       const int y = (int)(tt % (size_t)L.ny);
       const int x = (int)(tt / (size_t)L.ny);
-      res = L.r[i] - lattice_apply(L, K, e, x, y, z);
+      res = L.r[i] - lattice_apply(L, K, kh, kv, e, x, y, z);
     }
     L.t[i] = res;
   }
@@ -446,7 +472,7 @@ __global__ void k_tall_coarse_setup(const TallPart* parts) {
           break;
         }
     };
-    lattice_row_entries(L, K, x, y, z, emit);
+    lattice_row_entries(L, K, K, K, x, y, z, emit);
   }
   __syncthreads();
   for (int p = 0; p < m; ++p) {
