@@ -277,6 +277,108 @@ struct VecDir {
   if (z >= (g).nz) continue;                              \
   const size_t idx = ((size_t)x * (g).P + py) * (g).nz + z;
 
+// FP32 band segments.  The FP32 passes of the solve (block Jacobi, V-cycle
+// residual, CG direction) take a column's band rows kSegRows at a time: a
+// lane loads its column's base, the four neighbour bases and the
+// neighbours' tall bottom / top samples once for the segment, and fetches
+// every neighbour's value at the segment's fine rows without branches (the
+// band candidate at a clamped packed row, the tall interpolation and air are
+// evaluated and selected), so all loads of a segment are in flight together
+// instead of one dependent chain per sample.  The rows' (A p) and diagonal
+// follow band_row<float> term by term.
+constexpr int kSegRows = 4;
+
+template <class V>
+__device__ __forceinline__ void band_segment(const TallGeo& g, const int* __restrict__ bases,
+                                             const float* __restrict__ phi, int x, int py0, int z, int b,
+                                             const V& vec, float (&ap)[kSegRows], float (&dg)[kSegRows]) {
+  const float phi_air = (float)(5.0 * g.h);
+  float pc[kSegRows + 2], vc[kSegRows + 2];  // rows py0-1 .. py0+kSegRows
+#pragma unroll
+  for (int j = 0; j < kSegRows + 2; ++j) {
+    const int py = min(py0 - 1 + j, g.P - 1);
+    const size_t i = g.sidx(x, py, z);
+    pc[j] = phi[i];
+    vc[j] = vec(i);
+  }
+  float d[kSegRows], acc[kSegRows];
+#pragma unroll
+  for (int j = 0; j < kSegRows; ++j) {
+    const int py = py0 + j, y = b + py - 2;
+    const float p0 = pc[j + 1];
+    // below: the previous band row or the tall top (always a sample)
+    const float wd = fw<float>(p0, pc[j]);
+    d[j] = wd;
+    acc[j] = wd * vc[j];
+    // above: the next band row, air past the band, nothing past the grid
+    if (y + 1 < g.ny) {
+      if (py + 1 < g.P) {
+        const float wu = fw<float>(p0, pc[j + 2]);
+        d[j] += wu;
+        acc[j] += wu * vc[j + 2];
+      } else {
+        d[j] += fwr<float>(p0, phi_air);
+      }
+    }
+  }
+#pragma unroll
+  for (int dir = 0; dir < 4; ++dir) {
+    int xn, zn;
+    if (!neighbour(g, x, z, dir, xn, zn)) continue;
+    const int bn = bases[g.col(xn, zn)];
+    const size_t ib = g.sidx(xn, 0, zn), it = g.sidx(xn, 1, zn);
+    const float pb = phi[ib], pt = phi[it];
+    const float vb = vec(ib), vt = vec(it);
+    float pband[kSegRows], vband[kSegRows];
+#pragma unroll
+    for (int j = 0; j < kSegRows; ++j) {
+      const int y = b + py0 + j - 2;
+      const int pr = min(max(y - bn + 2, 2), g.P - 1);  // in bounds whichever case applies
+      const size_t i = g.sidx(xn, pr, zn);
+      pband[j] = phi[i];
+      vband[j] = vec(i);
+    }
+#pragma unroll
+    for (int j = 0; j < kSegRows; ++j) {
+      const int y = b + py0 + j - 2;
+      float pn, vn;
+      if (y < bn) {  // inside the neighbour's tall cell (row_rep: rows 0, bn-1 exact)
+        const float k = (float)y / (float)(bn - 1);
+        pn = y == 0 ? pb : (y == bn - 1 ? pt : pb + (pt - pb) * k);
+        vn = y == 0 ? vb : (y == bn - 1 ? vt : (1.0f - k) * vb + k * vt);
+      } else if (y - bn < g.B && y < g.ny) {
+        pn = pband[j];
+        vn = vband[j];
+      } else {
+        pn = phi_air;
+        vn = 0.0f;
+      }
+      const float w = fwr<float>(pc[j + 1], pn);
+      d[j] += w;
+      acc[j] += w * vn;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kSegRows; ++j) {
+    ap[j] = d[j] * vc[j + 1] - acc[j];
+    dg[j] = d[j];
+  }
+}
+
+// Work items of the segment passes: per (x, z chunk of FP_BLOCK), item 0 is
+// the column's (bottom, top) pair and items 1.. its band segments (py0 =
+// 2 + kSegRows (k-1)); the item index is uniform across the workgroup and
+// each lane takes one z.
+#define TALL_ITEMS(g) for (int item_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK, \
+                           ni_ = 1 + ((g).B + kSegRows - 1) / kSegRows;                \
+                           item_ < (g).nx * ni_ * zch_; item_ += nblk)
+#define TALL_ITEM(g, x, k, z)                                   \
+  const int xk_ = item_ / zch_;                                 \
+  const int z = (item_ - xk_ * zch_) * FP_BLOCK + (int)threadIdx.x; \
+  const int k = xk_ % ni_;                                      \
+  const int x = xk_ / ni_;                                      \
+  if (z >= (g).nz) continue;
+
 // --------------------------------------------------------------------------
 // set-up
 // --------------------------------------------------------------------------
@@ -582,24 +684,24 @@ __global__ void k_tall_jacobi(const TallPart* parts, int np, int sel, int first)
   float* __restrict__ zout = first ? G.z : (sel ? G.z : G.z2);
   const float om = G.omega;
   const VecF zv{zin};
-  TALL_LINES(g) {
-    TALL_LINE(g, idx, x, py, z);
-    if (py == 1) continue;
+  TALL_ITEMS(g) {
+    TALL_ITEM(g, x, k, z);
     const int b = bases[g.col(x, z)];
-    if (py >= 2) {
-      float out = 0.0f;
-      const float di = G.dinv[idx];
-      if (di != 0.0f) {
-        if (first) {
-          out = om * di * r[idx];
-        } else {
-          float ap, dg;
-          band_row<float>(g, bases, phi, x, py, z, b, zv, ap, dg);
-          out = zin[idx] + om * di * (r[idx] - ap);
-        }
+    if (k > 0) {
+      const int py0 = 2 + kSegRows * (k - 1);
+      float ap[kSegRows], dg[kSegRows];
+      if (!first) band_segment(g, bases, phi, x, py0, z, b, zv, ap, dg);
+#pragma unroll
+      for (int j = 0; j < kSegRows; ++j) {
+        if (py0 + j >= g.P) break;
+        const size_t idx = g.sidx(x, py0 + j, z);
+        const float di = G.dinv[idx];
+        float out = 0.0f;
+        if (di != 0.0f) out = first ? om * di * r[idx] : zin[idx] + om * di * (r[idx] - ap[j]);
+        zout[idx] = out;
       }
-      zout[idx] = out;
     } else {
+      const size_t idx = g.sidx(x, 0, z);
       const fp_f32x4 inv = G.blk[g.col(x, z)];
       const size_t it = g.sidx(x, 1, z);
       float rb = r[idx], rt = r[it];
@@ -627,19 +729,21 @@ __global__ void k_tall_residual(const TallPart* parts, int np, int sel) {
   const float* __restrict__ r = G.r;
   const VecF zv{sel ? G.z2 : G.z};
   float* __restrict__ t = G.t;
-  TALL_LINES(g) {
-    TALL_LINE(g, idx, x, py, z);
-    if (py == 1) continue;
+  TALL_ITEMS(g) {
+    TALL_ITEM(g, x, k, z);
     const int b = bases[g.col(x, z)];
-    if (py >= 2) {
-      float res = 0.0f;
-      if (is_unknown(g, phi, b, x, py, z)) {
-        float ap, dg;
-        band_row<float>(g, bases, phi, x, py, z, b, zv, ap, dg);
-        res = r[idx] - ap;
+    if (k > 0) {
+      const int py0 = 2 + kSegRows * (k - 1);
+      float ap[kSegRows], dg[kSegRows];
+      band_segment(g, bases, phi, x, py0, z, b, zv, ap, dg);
+#pragma unroll
+      for (int j = 0; j < kSegRows; ++j) {
+        if (py0 + j >= g.P) break;
+        const size_t idx = g.sidx(x, py0 + j, z);
+        t[idx] = is_unknown(g, phi, b, x, py0 + j, z) ? r[idx] - ap[j] : 0.0f;
       }
-      t[idx] = res;
     } else {
+      const size_t idx = g.sidx(x, 0, z);
       const PairOut<float> o = tall_pair<float>(g, bases, phi, G.cols, x, z, b, zv);
       const size_t it = g.sidx(x, 1, z);
       t[idx] = is_unknown(g, phi, b, x, 0, z) ? r[idx] - o.ab : 0.0f;
@@ -663,21 +767,25 @@ __global__ void k_tall_direction(const TallPart* parts, int np) {
   float* __restrict__ q = G.q;
   const VecDir dv{zz, dold, (float)st->beta};
   double acc[1] = {0.0};
-  TALL_LINES(g) {
-    TALL_LINE(g, idx, x, py, z);
-    if (py == 1) continue;
+  TALL_ITEMS(g) {
+    TALL_ITEM(g, x, k, z);
     const int b = bases[g.col(x, z)];
-    if (py >= 2) {
-      float qa = 0.0f, dd = 0.0f;
-      if (is_unknown(g, phi, b, x, py, z)) {
-        float dg;
-        band_row<float>(g, bases, phi, x, py, z, b, dv, qa, dg);
-        dd = dv(idx);
+    if (k > 0) {
+      const int py0 = 2 + kSegRows * (k - 1);
+      float ap[kSegRows], dg[kSegRows];
+      band_segment(g, bases, phi, x, py0, z, b, dv, ap, dg);
+#pragma unroll
+      for (int j = 0; j < kSegRows; ++j) {
+        if (py0 + j >= g.P) break;
+        const size_t idx = g.sidx(x, py0 + j, z);
+        const bool u = is_unknown(g, phi, b, x, py0 + j, z);
+        const float dd = u ? dv(idx) : 0.0f, qa = u ? ap[j] : 0.0f;
+        dnew[idx] = dd;
+        q[idx] = qa;
+        acc[0] += (double)dd * (double)qa;
       }
-      dnew[idx] = dd;
-      q[idx] = qa;
-      acc[0] += (double)dd * (double)qa;
     } else {
+      const size_t idx = g.sidx(x, 0, z);
       const PairOut<float> o = tall_pair<float>(g, bases, phi, G.cols, x, z, b, dv);
       const size_t it = g.sidx(x, 1, z);
       const bool ub = is_unknown(g, phi, b, x, 0, z), ut = is_unknown(g, phi, b, x, 1, z);
