@@ -323,12 +323,41 @@ __global__ void k_tall_reach_reset(const TallPart* parts) {
   for (int i = threadIdx.x; i < 2 * kTallMaxLevels; i += blockDim.x) G.reach[i] = 0;
 }
 
-// (A e)_i on a lattice level.
+// (A e)_i on a lattice level within the level's reach (kh, kv).  Every weight
+// and value load is independent of the others: a zero coupling contributes
+// zero, and rows past the level's edge are clamped and masked, so the loads of
+// a node are all in flight together.
 __device__ __forceinline__ float lattice_apply(const TallLevel& L, int K, int kh, int kv, const float* __restrict__ e,
                                                int x, int y, int z) {
-  float acc = 0.0f;
-  auto emit = [&](int xj, int yj, int zj, float v) { acc += v * e[((size_t)xj * L.ny + yj) * L.nz + zj]; };
-  lattice_row_entries(L, K, kh, kv, x, y, z, emit);
+  const size_t n = (size_t)L.n;
+  const float* __restrict__ w = L.w;
+  const size_t sy = (size_t)L.nz, sx = (size_t)L.ny * L.nz;
+  const size_t i = (size_t)x * sx + (size_t)y * sy + z;
+  float acc = w[(size_t)slot_d(K) * n + i] * e[i];
+  const bool xp = x + 1 < L.nx, xm = x > 0, zp = z + 1 < L.nz, zm = z > 0;
+  for (int dy = -kh; dy <= kh; ++dy) {
+    const int yy = y + dy;
+    const bool row = yy >= 0 && yy < L.ny;
+    const size_t j = i + (size_t)((long long)(row ? dy : 0) * (long long)sy);  // (x, yy, z), clamped
+    const float a0 = xp ? w[(size_t)slot_x(K, dy) * n + i] : 0.0f;
+    const float e0 = xp ? e[j + sx] : 0.0f;
+    const float a1 = xm ? w[(size_t)slot_x(K, -dy) * n + j - sx] : 0.0f;
+    const float e1 = xm ? e[j - sx] : 0.0f;
+    const float a2 = zp ? w[(size_t)slot_z(K, dy) * n + i] : 0.0f;
+    const float e2 = zp ? e[j + 1] : 0.0f;
+    const float a3 = zm ? w[(size_t)slot_z(K, -dy) * n + j - 1] : 0.0f;
+    const float e3 = zm ? e[j - 1] : 0.0f;
+    const float s = (a0 * e0 + a1 * e1) + (a2 * e2 + a3 * e3);
+    acc += row ? s : 0.0f;
+  }
+  for (int dy = 1; dy <= kv; ++dy) {
+    const bool up = y + dy < L.ny, dn = y - dy >= 0;
+    const float a0 = up ? w[(size_t)slot_y(K, dy) * n + i] : 0.0f;
+    const float e0 = up ? e[i + (size_t)dy * sy] : 0.0f;
+    const float a1 = dn ? w[(size_t)slot_y(K, dy) * n + i - (size_t)dy * sy] : 0.0f;
+    const float e1 = dn ? e[i - (size_t)dy * sy] : 0.0f;
+    acc += a0 * e0 + a1 * e1;
+  }
   return acc;
 }
 
