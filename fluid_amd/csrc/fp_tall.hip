@@ -1156,6 +1156,12 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   R.nv = R.n;
   R.nw = (long long)nx * R.S * (nz + 1);
   R.omega = 0.8f;
+  R.lb = L - 1;
+  for (int l = 1; l < L; ++l)
+    if ((long long)dims[l][0] * dims[l][1] * dims[l][2] <= kTallBottomNodes) {
+      R.lb = l;
+      break;
+    }
   R.K = (max_delta + 5) / 2;
   if (R.K > kTallMaxK) {
     set_error("tall-cell max neighbour delta %d exceeds %d", max_delta, 2 * kTallMaxK - 5);
@@ -1286,9 +1292,13 @@ struct TallProjector::Plan {
   // grids for which lattice level l is not the coarsest: level-l nodes
   // (smoothing, residual, prolongation into l) and level-(l+1) nodes
   // (restriction and coarsening into l+1)
+  // cnodes / ccoarser: the same for the levels the V-cycle smooths with grid
+  // kernels (below the grid's bottom level lb)
   struct Level {
     TallTable nodes;
     TallTable coarser;
+    TallTable cnodes;
+    TallTable ccoarser;
   } lv[kTallMaxLevels];
   TallTable coarse;      // one workgroup per grid (coarsest lattice)
   hipGraphExec_t graph = nullptr;
@@ -1371,16 +1381,22 @@ void TallProjector::fill_tables(Plan* p, const std::vector<TallGrid*>& grids, in
   for (int a = 0; a < 3; ++a) p->add(p->faces[a], faces[a]);
   p->add(p->coarse, one);
   for (int l = 1; l < p->maxL; ++l) {
-    std::vector<std::pair<TallGrid*, int>> nodes, coarser;
+    std::vector<std::pair<TallGrid*, int>> nodes, coarser, cnodes, ccoarser;
     for (TallGrid* g : grids) {
       const TallRec& R = *g->host_rec();
       if (l + 1 < R.L) {
         nodes.push_back({g, Plan::blocks_for((size_t)R.lv[l].n)});
         coarser.push_back({g, Plan::blocks_for((size_t)R.lv[l + 1].n)});
+        if (l < R.lb) {
+          cnodes.push_back(nodes.back());
+          ccoarser.push_back(coarser.back());
+        }
       }
     }
     p->add(p->lv[l].nodes, nodes);
     p->add(p->lv[l].coarser, coarser);
+    p->add(p->lv[l].cnodes, cnodes);
+    p->add(p->lv[l].ccoarser, ccoarser);
   }
   std::vector<std::pair<TallGrid*, int>> coarser;
   for (TallGrid* g : grids) coarser.push_back({g, Plan::blocks_for((size_t)g->host_rec()->lv[1].n)});
@@ -1467,16 +1483,19 @@ int TallProjector::enqueue_vcycle(hipStream_t s, Plan& p) {
   // correction of a smoothed level always ends in e2 (final_e); the coarsest
   // solve writes e
   const int cpre = (kTallCoarseSweeps - 1) & 1;  // buffer of e after the pre-sweeps
+  // (levels from each grid's lb on run inside k_lat_bottom, one workgroup per
+  // grid, with the coarsest solve)
   for (int l = 1; l + 1 < p.maxL; ++l) {
-    PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, 0, 1);
-    for (int sw = 1, c = 0; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, c, 0);
-    PLAUNCH(k_lat_residual, p.lv[l].nodes, s, l, cpre);
-    PLAUNCH(k_lat_restrict, p.lv[l].coarser, s, l);
+    PLAUNCH(k_lat_jacobi, p.lv[l].cnodes, s, l, 0, 1);
+    for (int sw = 1, c = 0; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].cnodes, s, l, c, 0);
+    PLAUNCH(k_lat_residual, p.lv[l].cnodes, s, l, cpre);
+    PLAUNCH(k_lat_restrict, p.lv[l].ccoarser, s, l);
   }
-  FLAUNCH(k_tall_coarse_solve, p.coarse, s);
+  if (p.coarse.n > 0)
+    hipLaunchKernelGGL(k_lat_bottom, dim3(p.coarse.n), dim3(kTallBottomThreads), 0, s, p.coarse.dev);
   for (int l = p.maxL - 2; l >= 1; --l) {
-    PLAUNCH(k_lat_prolong, p.lv[l].nodes, s, l, cpre);
-    for (int sw = 0, c = cpre; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].nodes, s, l, c, 0);
+    PLAUNCH(k_lat_prolong, p.lv[l].cnodes, s, l, cpre);
+    for (int sw = 0, c = cpre; sw < kTallCoarseSweeps; ++sw, c ^= 1) PLAUNCH(k_lat_jacobi, p.lv[l].cnodes, s, l, c, 0);
   }
   TTIMED(FP_KERNEL_PROLONG_FINE, PLAUNCH(k_tall_prolong0, p.samples, s, cur));
   for (int sw = 0; sw < post; ++sw) {

@@ -18,6 +18,7 @@ namespace fp {
 constexpr int kTallMaxLevels = 14;
 constexpr int kTallMaxCoarse = 64;  // nodes of the coarsest lattice (dense FP64 inverse)
 constexpr int kTallMaxK = 8;        // banded coupling reach of the coarse levels (max neighbour delta 11)
+constexpr long long kTallBottomNodes = 1024;  // levels this small run inside the one-workgroup bottom of the V-cycle
 constexpr int kTallSlotsMax = 5 * kTallMaxK + 3;
 
 // Per-column FP64 face data: the weight moments sum w, sum w t, sum w t^2
@@ -52,6 +53,7 @@ struct TallRec {
   int band, P, S, D;   // band rows, packed rows, face slots, max neighbour base delta
   int L;               // levels including the sample level
   int K;               // banded coupling reach of the coarse levels
+  int lb;              // first lattice level of the V-cycle's one-workgroup bottom (L-1: the coarsest only)
   int zfinal;          // V-cycle buffer (0: z, 1: z2) holding M^-1 r
   long long n, nu, nv, nw, ncol;
   double h;

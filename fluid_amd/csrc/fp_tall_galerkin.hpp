@@ -36,6 +36,7 @@ namespace fp {
 constexpr float kTallGalerkinScale = 0.6f;
 constexpr float kTallL1Theta = 1.6f;
 constexpr int kTallCoarseSweeps = 3;
+constexpr int kTallBottomThreads = 1024;
 
 __device__ __forceinline__ int slot_x(int K, int dy) { return dy + K; }
 __device__ __forceinline__ int slot_z(int K, int dy) { return 2 * K + 1 + dy + K; }
@@ -361,84 +362,74 @@ __device__ __forceinline__ float lattice_apply(const TallLevel& L, int K, int kh
   return acc;
 }
 
-// Scaled l1-Jacobi sweep on level l: out = in + dinv (r - A in) (first: in = 0).
-// sel 0: reads e writes e2; 1: reads e2 writes e.
-__global__ void k_lat_jacobi(const TallPart* parts, int np, int level, int sel, int first) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
+// ---- per-node steps of the lattice V-cycle (grid kernels below, and the
+// one-workgroup bottom of the cycle, k_lat_bottom) ----
+__device__ __forceinline__ void node_xyz(const TallLevel& L, size_t i, int& x, int& y, int& z) {
+  z = (int)(i % (size_t)L.nz);
+  const size_t tt = i / (size_t)L.nz;
+  y = (int)(tt % (size_t)L.ny);
+  x = (int)(tt / (size_t)L.ny);
+}
+
+// Scaled l1-Jacobi: out = in + dinv (r - A in) (first: in = 0).  sel 0 reads e
+// writes e2; 1 reads e2 writes e.
+__device__ __forceinline__ void lat_jacobi_node(const TallRec& G, int level, int sel, int first, int kh, int kv,
+                                                size_t i) {
   const TallLevel& L = G.lv[level];
-  const int K = G.K, kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
   const float* __restrict__ ein = sel ? L.e2 : L.e;
   float* __restrict__ eout = first ? L.e : (sel ? L.e : L.e2);
-  FP_STRIDE(i, (size_t)L.n) {
-    const float di = L.dinv[i];
-    float out = 0.0f;
-    if (di != 0.0f) {
-      if (first) {
-        out = L.r[i] * di;
-      } else {
-        const int z = (int)(i % (size_t)L.nz);
-        const size_t tt = i / (size_t)L.nz;
-        const int y = (int)(tt % (size_t)L.ny);
-        const int x = (int)(tt / (size_t)L.ny);
-        out = ein[i] + (L.r[i] - lattice_apply(L, K, kh, kv, ein, x, y, z)) * di;
-      }
+  const float di = L.dinv[i];
+  float out = 0.0f;
+  if (di != 0.0f) {
+    if (first) {
+      out = L.r[i] * di;
+    } else {
+      int x, y, z;
+      node_xyz(L, i, x, y, z);
+      out = ein[i] + (L.r[i] - lattice_apply(L, G.K, kh, kv, ein, x, y, z)) * di;
     }
-    eout[i] = out;
   }
+  eout[i] = out;
 }
 
-// t = r - A e on level l (sel: which buffer holds e).
-__global__ void k_lat_residual(const TallPart* parts, int np, int level, int sel) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
+// t = r - A e (sel: which buffer holds e).
+__device__ __forceinline__ void lat_residual_node(const TallRec& G, int level, int sel, int kh, int kv, size_t i) {
   const TallLevel& L = G.lv[level];
-  const int K = G.K, kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
   const float* __restrict__ e = sel ? L.e2 : L.e;
-  FP_STRIDE(i, (size_t)L.n) {
-    float res = 0.0f;
-    if (L.dinv[i] != 0.0f) {
-      const int z = (int)(i % (size_t)L.nz);
-      const size_t tt = i / (size_t)L.nz;
-      const int y = (int)(tt % (size_t)L.ny);
-      const int x = (int)(tt / (size_t)L.ny);
-      res = L.r[i] - lattice_apply(L, K, kh, kv, e, x, y, z);
-    }
-    L.t[i] = res;
+  float res = 0.0f;
+  if (L.dinv[i] != 0.0f) {
+    int x, y, z;
+    node_xyz(L, i, x, y, z);
+    res = L.r[i] - lattice_apply(L, G.K, kh, kv, e, x, y, z);
   }
+  L.t[i] = res;
 }
 
-// r_{l+1} = P^T t_l (level 0: the sample residual).
-__global__ void k_lat_restrict(const TallPart* parts, int np, int level) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
+// r_{l+1} = P^T t_l at coarse node ci (level 0: the sample residual).
+__device__ __forceinline__ void lat_restrict_node(const TallRec& G, int level, size_t ci) {
   const TallLevel& C = G.lv[level + 1];
   const int fnx = level == 0 ? G.nx : G.lv[level].nx;
   const int fny = level == 0 ? G.P : G.lv[level].ny;
   const int fnz = level == 0 ? G.nz : G.lv[level].nz;
   const float* __restrict__ t = level == 0 ? G.t : G.lv[level].t;
   const int cy = C.cy;
-  FP_STRIDE(ci, (size_t)C.n) {
-    const int Z = (int)(ci % (size_t)C.nz);
-    const size_t tt = ci / (size_t)C.nz;
-    const int Y = (int)(tt % (size_t)C.ny);
-    const int X = (int)(tt / (size_t)C.ny);
-    float s = 0.0f;
-    if (C.dinv[ci] != 0.0f) {
-      int y0, y1;
-      if (Y < 2) {
-        y0 = Y;
-        y1 = Y + 1;
-      } else {
-        y0 = 2 + cy * (Y - 2);
-        y1 = min(y0 + cy, fny);
-      }
-      for (int x = 2 * X; x < min(2 * X + 2, fnx); ++x)
-        for (int y = y0; y < y1; ++y)
-          for (int z = 2 * Z; z < min(2 * Z + 2, fnz); ++z) s += t[((size_t)x * fny + y) * fnz + z];
+  int X, Y, Z;
+  node_xyz(C, ci, X, Y, Z);
+  float s = 0.0f;
+  if (C.dinv[ci] != 0.0f) {
+    int y0, y1;
+    if (Y < 2) {
+      y0 = Y;
+      y1 = Y + 1;
+    } else {
+      y0 = 2 + cy * (Y - 2);
+      y1 = min(y0 + cy, fny);
     }
-    C.r[ci] = s;
+    for (int x = 2 * X; x < min(2 * X + 2, fnx); ++x)
+      for (int y = y0; y < y1; ++y)
+        for (int z = 2 * Z; z < min(2 * Z + 2, fnz); ++z) s += t[((size_t)x * fny + y) * fnz + z];
   }
+  C.r[ci] = s;
 }
 
 // Buffer holding the final correction of lattice level l: the coarsest
@@ -447,22 +438,42 @@ __device__ __forceinline__ const float* final_e(const TallRec& G, int l) {
   return l == G.L - 1 ? G.lv[l].e : G.lv[l].e2;
 }
 
-// e_l += P e_{l+1} on active level-l nodes (l >= 1; sel: e_l buffer).
+// e_l += P e_{l+1} on active level-l node i (l >= 1; sel: e_l buffer).
+__device__ __forceinline__ void lat_prolong_node(const TallRec& G, int level, int sel, size_t i) {
+  const TallLevel& F = G.lv[level];
+  const TallLevel& C = G.lv[level + 1];
+  if (F.dinv[i] == 0.0f) return;
+  float* __restrict__ ef = sel ? F.e2 : F.e;
+  const float* __restrict__ ec = final_e(G, level + 1);
+  int x, y, z;
+  node_xyz(F, i, x, y, z);
+  ef[i] += ec[((size_t)(x >> 1) * C.ny + coarse_row(y, C.cy)) * C.nz + (z >> 1)];
+}
+
+__global__ void k_lat_jacobi(const TallPart* parts, int np, int level, int sel, int first) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  const int kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
+  FP_STRIDE(i, (size_t)G.lv[level].n) lat_jacobi_node(G, level, sel, first, kh, kv, i);
+}
+
+__global__ void k_lat_residual(const TallPart* parts, int np, int level, int sel) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  const int kh = cload(G.reach + 2 * level), kv = cload(G.reach + 2 * level + 1);
+  FP_STRIDE(i, (size_t)G.lv[level].n) lat_residual_node(G, level, sel, kh, kv, i);
+}
+
+__global__ void k_lat_restrict(const TallPart* parts, int np, int level) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  FP_STRIDE(ci, (size_t)G.lv[level + 1].n) lat_restrict_node(G, level, ci);
+}
+
 __global__ void k_lat_prolong(const TallPart* parts, int np, int level, int sel) {
   FP_PART(parts, np);
   if (G.st->done) return;
-  const TallLevel& F = G.lv[level];
-  const TallLevel& C = G.lv[level + 1];
-  float* __restrict__ ef = sel ? F.e2 : F.e;
-  const float* __restrict__ ec = final_e(G, level + 1);
-  FP_STRIDE(i, (size_t)F.n) {
-    if (F.dinv[i] == 0.0f) continue;
-    const int z = (int)(i % (size_t)F.nz);
-    const size_t tt = i / (size_t)F.nz;
-    const int y = (int)(tt % (size_t)F.ny);
-    const int x = (int)(tt / (size_t)F.ny);
-    ef[i] += ec[((size_t)(x >> 1) * C.ny + coarse_row(y, C.cy)) * C.nz + (z >> 1)];
-  }
+  FP_STRIDE(i, (size_t)G.lv[level].n) lat_prolong_node(G, level, sel, i);
 }
 
 // Dense FP64 inverse of the coarsest lattice (one workgroup per grid).
@@ -534,9 +545,8 @@ __global__ void k_tall_coarse_setup(const TallPart* parts) {
   }
 }
 
-__global__ void k_tall_coarse_solve(const TallPart* parts) {
-  const TallRec& G = *parts[blockIdx.x].g;
-  if (G.st->done) return;
+// e = Minv r on the coarsest level's active nodes (the calling workgroup).
+__device__ __forceinline__ void coarse_solve_block(const TallRec& G) {
   __shared__ double rs[kTallMaxCoarse];
   const TallLevel& L = G.lv[G.L - 1];
   const int m = *G.ncoarse;
@@ -547,6 +557,43 @@ __global__ void k_tall_coarse_solve(const TallPart* parts) {
     double s = 0.0;
     for (int q = 0; q < m; ++q) s += G.minv[(size_t)row * m + q] * rs[q];
     L.e[G.cmap[row]] = (float)s;
+  }
+}
+
+// The bottom of the V-cycle in one workgroup per grid: levels lb .. L-2 (at
+// most kTallBottomNodes nodes each) are smoothed, restricted, solved on the
+// coarsest level and prolonged with the same steps and buffer conventions as
+// the grid kernels, separated by workgroup barriers instead of launches.
+__global__ __launch_bounds__(kTallBottomThreads) void k_lat_bottom(const TallPart* parts) {
+  const TallRec& G = *parts[blockIdx.x].g;
+  if (G.st->done) return;
+  const int lb = G.lb, lc = G.L - 1;
+  const int cpre = (kTallCoarseSweeps - 1) & 1;
+  for (int l = lb; l < lc; ++l) {
+    const int kh = G.reach[2 * l], kv = G.reach[2 * l + 1];
+    const size_t n = (size_t)G.lv[l].n;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) lat_jacobi_node(G, l, 0, 1, kh, kv, i);
+    __syncthreads();
+    for (int sw = 1, c = 0; sw < kTallCoarseSweeps; ++sw, c ^= 1) {
+      for (size_t i = threadIdx.x; i < n; i += blockDim.x) lat_jacobi_node(G, l, c, 0, kh, kv, i);
+      __syncthreads();
+    }
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) lat_residual_node(G, l, cpre, kh, kv, i);
+    __syncthreads();
+    for (size_t ci = threadIdx.x; ci < (size_t)G.lv[l + 1].n; ci += blockDim.x) lat_restrict_node(G, l, ci);
+    __syncthreads();
+  }
+  coarse_solve_block(G);
+  __syncthreads();
+  for (int l = lc - 1; l >= lb; --l) {
+    const int kh = G.reach[2 * l], kv = G.reach[2 * l + 1];
+    const size_t n = (size_t)G.lv[l].n;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) lat_prolong_node(G, l, cpre, i);
+    __syncthreads();
+    for (int sw = 0, c = cpre; sw < kTallCoarseSweeps; ++sw, c ^= 1) {
+      for (size_t i = threadIdx.x; i < n; i += blockDim.x) lat_jacobi_node(G, l, c, 0, kh, kv, i);
+      __syncthreads();
+    }
   }
 }
 
