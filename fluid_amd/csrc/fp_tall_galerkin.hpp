@@ -47,27 +47,31 @@ __device__ __forceinline__ int slot_d(int K) { return 5 * K + 2; }
 __device__ __forceinline__ int coarse_row(int y, int cy) { return y < 2 ? y : 2 + (y - 2) / cy; }
 
 // Accumulates entries a_ij of the finer operator, j in coarse node C(j), into
-// the couplings of coarse node C = (X, Y, Z).
+// the couplings of coarse node C = (X, Y, Z).  The slots live in LDS (column
+// threadIdx.x of a slot-major block array): dynamically indexed private
+// arrays would go to scratch memory.
 struct CoarseAcc {
-  float a[kTallSlotsMax];
+  float* a;  // slot s at a[s * FP_BLOCK]
   int K, X, Y, Z;
-  __device__ void init(int K_, int X_, int Y_, int Z_) {
+  __device__ float& at(int s) { return a[s * FP_BLOCK]; }
+  __device__ void init(float* col, int K_, int X_, int Y_, int Z_) {
+    a = col;
     K = K_; X = X_; Y = Y_; Z = Z_;
-    for (int s = 0; s < kTallSlotsMax; ++s) a[s] = 0.0f;
+    for (int s = 0; s < 5 * K + 3; ++s) at(s) = 0.0f;
   }
   __device__ void add(int Xj, int Yj, int Zj, float v) {
     const int dX = Xj - X, dY = Yj - Y, dZ = Zj - Z;
     if (dX == 0 && dY == 0 && dZ == 0) {
-      a[slot_d(K)] += v;
+      at(slot_d(K)) += v;
       return;
     }
     if (dY > K || dY < -K) {
-      a[slot_d(K)] += fabsf(v);
+      at(slot_d(K)) += fabsf(v);
       return;
     }
-    if (dX == 1 && dZ == 0) a[slot_x(K, dY)] += v;
-    else if (dZ == 1 && dX == 0) a[slot_z(K, dY)] += v;
-    else if (dX == 0 && dZ == 0 && dY > 0) a[slot_y(K, dY)] += v;
+    if (dX == 1 && dZ == 0) at(slot_x(K, dY)) += v;
+    else if (dZ == 1 && dX == 0) at(slot_z(K, dY)) += v;
+    else if (dX == 0 && dZ == 0 && dY > 0) at(slot_y(K, dY)) += v;
     // the -x, -z and below couplings are stored by the neighbour
   }
 };
@@ -163,7 +167,8 @@ __device__ void tall_row_entries(const TallGeo& g, const int* __restrict__ bases
   if (which == 0 ? ut : ub) emit(x, 1 - which, z, other);
 }
 
-__global__ void k_tall_galerkin1(const TallPart* parts, int np) {
+__global__ __launch_bounds__(FP_BLOCK) void k_tall_galerkin1(const TallPart* parts, int np) {
+  __shared__ float s_acc[kTallSlotsMax * FP_BLOCK];
   FP_PART(parts, np);
   const TallGeo g = geo_of(G);
   const int* __restrict__ bases = G.bases;
@@ -177,7 +182,7 @@ __global__ void k_tall_galerkin1(const TallPart* parts, int np) {
     const int Y = (int)(tt % (size_t)C.ny);
     const int X = (int)(tt / (size_t)C.ny);
     CoarseAcc acc;
-    acc.init(K, X, Y, Z);
+    acc.init(s_acc + threadIdx.x, K, X, Y, Z);
     auto emit = [&](int xj, int pyj, int zj, float v) { acc.add(xj >> 1, coarse_row(pyj, cy), zj >> 1, v); };
     int y0, y1;
     if (Y < 2) {
@@ -197,7 +202,7 @@ __global__ void k_tall_galerkin1(const TallPart* parts, int np) {
         }
       }
     const int NS = 5 * K + 3;
-    for (int s = 0; s < NS; ++s) C.w[(size_t)s * n + ci] = kTallGalerkinScale * acc.a[s];
+    for (int s = 0; s < NS; ++s) C.w[(size_t)s * n + ci] = kTallGalerkinScale * acc.at(s);
   }
 }
 
@@ -245,7 +250,8 @@ __device__ void lattice_row_entries(const TallLevel& L, int K, int kh, int kv, i
   }
 }
 
-__global__ void k_tall_galerkin(const TallPart* parts, int np, int level) {
+__global__ __launch_bounds__(FP_BLOCK) void k_tall_galerkin(const TallPart* parts, int np, int level) {
+  __shared__ float s_acc[kTallSlotsMax * FP_BLOCK];
   FP_PART(parts, np);
   const TallLevel& F = G.lv[level];
   const TallLevel& C = G.lv[level + 1];
@@ -258,7 +264,7 @@ __global__ void k_tall_galerkin(const TallPart* parts, int np, int level) {
     const int Y = (int)(tt % (size_t)C.ny);
     const int X = (int)(tt / (size_t)C.ny);
     CoarseAcc acc;
-    acc.init(K, X, Y, Z);
+    acc.init(s_acc + threadIdx.x, K, X, Y, Z);
     auto emit = [&](int xj, int yj, int zj, float v) { acc.add(xj >> 1, coarse_row(yj, cy), zj >> 1, v); };
     int y0, y1;
     if (Y < 2) {
@@ -276,7 +282,7 @@ __global__ void k_tall_galerkin(const TallPart* parts, int np, int level) {
           lattice_row_entries(F, K, K, K, x, y, z, emit);
         }
     const int NS = 5 * K + 3;
-    for (int s = 0; s < NS; ++s) C.w[(size_t)s * n + ci] = kTallGalerkinScale * acc.a[s];
+    for (int s = 0; s < NS; ++s) C.w[(size_t)s * n + ci] = kTallGalerkinScale * acc.at(s);
   }
 }
 
