@@ -90,20 +90,24 @@ __device__ __forceinline__ void stT(T* p, size_t i, float v) { Store<T>::st(p, i
 
 // which: 0 x (FP64), 1 e, 2 d[0].  skip_converged: leave grids whose
 // refinement has converged.
-__global__ void k_zero(const Part* parts, int np, int which, int skip_converged) {
+// which: 0 x (unless warm), 1 e, 2 d[0]; over the finest level's liquid hulls
+// (every reader of these vectors stays on liquid cells or inside the hulls).
+__global__ __launch_bounds__(256) void k_zero(const Part* parts, int np, int which, int skip_converged) {
   FP_PART(parts, np);
-  if (skip_converged && G.st->outer_done) return;
-  const size_t n = (size_t)G.n;
+  const State* st = cload(&Gp->st);
+  if (skip_converged && cload(&st->outer_done)) return;
+  if (which == 0 && cload(&Gp->warm)) return;
+  const LevelRec l0 = cload(&Gp->lv[0]);
+  const TileSpan sp = tile_span(l0, blk, nblk);
+  if (!sp.inside) return;
+  const size_t plane = (size_t)l0.d[1] * l0.d[2];
   if (which == 0) {
-    if (G.warm) return;
-    double* p = G.x;
-    FP_STRIDE(i, n) p[i] = 0.0;
+    double* __restrict__ x = cload(&Gp->x);
+    for (int px = sp.x0; px < sp.x1; ++px) stg(x, (size_t)px * plane + sp.col, 0.0);
     return;
   }
-  float* p = which == 1 ? G.e : G.d[0];
-  const size_t n4 = n / 4;
-  FP_STRIDE(i, n4) reinterpret_cast<float4*>(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (blk == 0 && threadIdx.x < (n & 3)) p[n4 * 4 + threadIdx.x] = 0.0f;
+  float* __restrict__ p = which == 1 ? cload(&Gp->e) : cload(&Gp->d[0]);
+  for (int px = sp.x0; px < sp.x1; ++px) stg(p, (size_t)px * plane + sp.col, 0.0f);
 }
 
 // --------------------------------------------------------------------------
@@ -1187,7 +1191,6 @@ struct Projector::Plan {
   };
   std::vector<Pending> pending;
   PartTable one;        // one workgroup per grid
-  PartTable vec4;       // finest-level vectors, 4 per lane
   PartTable cells;      // finest-level cells, 1 per lane
   PartTable faces[3];   // face arrays
   PartTable march0;     // finest-level column-tile march
@@ -1267,13 +1270,12 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
   p->host.clear();
   p->pending.clear();
   using Items = std::vector<std::pair<UniformGrid*, int>>;
-  Items one, vec4, cells, faces[3], march0, ctop, cdeep;
+  Items one, cells, faces[3], march0, ctop, cdeep;
   int maxL = 1;
   for (UniformGrid* g : grids) {
     const GridRec& R = *g->host_rec();
     maxL = std::max(maxL, R.L);
     one.push_back({g, 1});
-    vec4.push_back({g, Plan::blocks_for((size_t)R.n, 4)});
     cells.push_back({g, Plan::blocks_for((size_t)R.n, 1)});
     faces[0].push_back({g, Plan::blocks_for((size_t)R.nu, 1)});
     faces[1].push_back({g, Plan::blocks_for((size_t)R.nv, 1)});
@@ -1283,7 +1285,6 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
   }
   p->maxL = maxL;
   p->add(p->one, one);
-  p->add(p->vec4, vec4);
   p->add(p->cells, cells);
   for (int a = 0; a < 3; ++a) p->add(p->faces[a], faces[a]);
   p->add(p->march0, march0);
@@ -1553,9 +1554,9 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   setup_tables(s, p, opt->phi_preserve_levels);
   PLAUNCH(k_rhs, p.cells, s);
   FLAUNCH(k_fin_rhs<Part>, p.cells, s);
-  PLAUNCH(k_zero, p.vec4, s, 0, 0);  // x, except on warm-started grids
-  PLAUNCH(k_zero, p.vec4, s, 1, 0);
-  PLAUNCH(k_zero, p.vec4, s, 2, 0);
+  PLAUNCH(k_zero, p.march0, s, 0, 0);  // x, except on warm-started grids
+  PLAUNCH(k_zero, p.march0, s, 1, 0);
+  PLAUNCH(k_zero, p.march0, s, 2, 0);
   const int max_outer = std::max(1, opt->max_outer);
   const int max_iter = std::max(1, opt->max_iterations);
   const int check = std::max(1, opt->check_every);
@@ -1589,7 +1590,7 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
       }
     }
     PLAUNCH(k_accumulate, p.march0, s);
-    PLAUNCH(k_zero, p.vec4, s, 2, 1);
+    PLAUNCH(k_zero, p.march0, s, 2, 1);
   }
   for (int a = 0; a < 3; ++a) PLAUNCH(k_subtract, p.faces[a], s, a);
   PLAUNCH(k_div_after, p.cells, s);
@@ -1745,14 +1746,14 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
     setup_tables(s, p, sp.second->opt.phi_preserve_levels);
     PLAUNCH(k_rhs, p.cells, s);
     FLAUNCH(k_fin_rhs<Part>, p.cells, s);
-    PLAUNCH(k_zero, p.vec4, s, 0, 0);
-    PLAUNCH(k_zero, p.vec4, s, 1, 0);
-    PLAUNCH(k_zero, p.vec4, s, 2, 0);
+    PLAUNCH(k_zero, p.march0, s, 0, 0);
+    PLAUNCH(k_zero, p.march0, s, 1, 0);
+    PLAUNCH(k_zero, p.march0, s, 2, 0);
   }
   // ---- finished inner solves fold into the FP64 iterate ----
   for (auto& ap : accum_p) {
     PLAUNCH(k_accumulate, ap.first->march0, s);
-    PLAUNCH(k_zero, ap.first->vec4, s, 2, 1);
+    PLAUNCH(k_zero, ap.first->march0, s, 2, 1);
   }
   // ---- FP64 true residuals (refinement control) ----
   for (int np = 0; np < 2; ++np)
