@@ -439,6 +439,13 @@ __global__ void k_tall_columns(const TallPart* parts, int np) {
       out.wint = s;
     }
     cols[c] = out;
+    int tp = 1;
+    for (int py = g.P - 1; py >= 2; --py)
+      if (b + py - 2 < g.ny && phi[g.sidx(x, py, z)] < 0.0f) {
+        tp = py;
+        break;
+      }
+    G.top[c] = tp;
   }
 }
 
@@ -689,6 +696,12 @@ __global__ void k_tall_jacobi(const TallPart* parts, int np, int sel, int first)
     const int b = bases[g.col(x, z)];
     if (k > 0) {
       const int py0 = 2 + kSegRows * (k - 1);
+      if (py0 > G.top[g.col(x, z)]) {  // all air: the zero correction
+#pragma unroll
+        for (int j = 0; j < kSegRows; ++j)
+          if (py0 + j < g.P) zout[g.sidx(x, py0 + j, z)] = 0.0f;
+        continue;
+      }
       float ap[kSegRows], dg[kSegRows];
       if (!first) band_segment(g, bases, phi, x, py0, z, b, zv, ap, dg);
 #pragma unroll
@@ -734,6 +747,12 @@ __global__ void k_tall_residual(const TallPart* parts, int np, int sel) {
     const int b = bases[g.col(x, z)];
     if (k > 0) {
       const int py0 = 2 + kSegRows * (k - 1);
+      if (py0 > G.top[g.col(x, z)]) {  // all air: no residual
+#pragma unroll
+        for (int j = 0; j < kSegRows; ++j)
+          if (py0 + j < g.P) t[g.sidx(x, py0 + j, z)] = 0.0f;
+        continue;
+      }
       float ap[kSegRows], dg[kSegRows];
       band_segment(g, bases, phi, x, py0, z, b, zv, ap, dg);
 #pragma unroll
@@ -772,6 +791,15 @@ __global__ void k_tall_direction(const TallPart* parts, int np) {
     const int b = bases[g.col(x, z)];
     if (k > 0) {
       const int py0 = 2 + kSegRows * (k - 1);
+      if (py0 > G.top[g.col(x, z)]) {  // all air: d and q vanish
+#pragma unroll
+        for (int j = 0; j < kSegRows; ++j)
+          if (py0 + j < g.P) {
+            dnew[g.sidx(x, py0 + j, z)] = 0.0f;
+            q[g.sidx(x, py0 + j, z)] = 0.0f;
+          }
+        continue;
+      }
       float ap[kSegRows], dg[kSegRows];
       band_segment(g, bases, phi, x, py0, z, b, dv, ap, dg);
 #pragma unroll
@@ -1096,6 +1124,7 @@ size_t TallGrid::estimate_bytes(int nx, int ny, int nz, int band, int max_delta)
   add(ncol * 4);                 // bases
   add(n * 4);                    // phi
   add(ncol * sizeof(TallCol));
+  add(ncol * sizeof(int));        // top
   add(ncol * 16);                // blk
   add(n * 4);                    // dinv
   add(n * 8);                    // b
@@ -1175,6 +1204,7 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   HIPCHK(hipMemcpy(bases, bases_host, ncol * 4, hipMemcpyHostToDevice));
   if ((rc = alloc((void**)&R.phi, n * 4))) return rc;
   if ((rc = alloc((void**)&R.cols, ncol * sizeof(TallCol)))) return rc;
+  if ((rc = alloc((void**)&R.top, (size_t)R.ncol * sizeof(int)))) return rc;
   if ((rc = alloc((void**)&R.blk, ncol * 16))) return rc;
   if ((rc = alloc((void**)&R.dinv, n * 4))) return rc;
   if ((rc = alloc((void**)&R.b, n * 8))) return rc;

@@ -60,6 +60,7 @@ struct TallRec {
   const int* bases;    // (nx, nz)
   float* phi;          // samples (private copy)
   TallCol* cols;       // (nx, nz)
+  int* top;            // (nx, nz): highest packed band row holding liquid (1: none)
   fp_f32x4* blk;       // per column: inverse (bottom, top) block bb, bt, tt and 0
   float* dinv;         // per sample: inverse diagonal of band unknowns
   double* b;           // FP64 right-hand side
