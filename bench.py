@@ -100,6 +100,20 @@ def copy_probe(mib: int, reps: int) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}))
 
 
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: processor model and logical CPUs."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(seconds_budget: float = 20.0):
     """The reference's CPU MAC/PCG algorithm (sparse-atlas Jacobi-PCG, restated
     in C: oracle/fp_oracle.c) on a bounded sample of the same workload."""
@@ -125,6 +139,7 @@ def cpu_baseline(seconds_budget: float = 20.0):
                   f"(sparse-atlas-composite-projection.ts:1346-1457 restated in C, FP64, OpenMP stencils) "
                   f"to 1e-6 relative residual: {rep['iterations']} iterations in {dt:.2f} s",
         "relative_residual": rep["relative_residual"],
+        "host": host_cpu(),
     }
 
 
@@ -150,6 +165,7 @@ def tall_cpu_baseline(n: int = 96):
         "sample": f"wave-surface tall-cell scene at {n}^2 columns x {n} rows, band 32 (seed 0), one projection, "
                   f"Jacobi-PCG to 1e-6 on the oracle's fine-face walk (scalar C): {rep['iterations']} "
                   f"iterations in {dt:.2f} s",
+        "host": host_cpu(),
     }
 
 
