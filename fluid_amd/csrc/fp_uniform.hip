@@ -452,9 +452,9 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
   T* __restrict__ rc = (T*)cl.r;
   const float* __restrict__ phic = cl.phi;
   int tzi, tyi, cxi;
-  tile_of_group(blk, cl.mg.tz, cl.mg.ty, nblk, tzi, tyi, cxi);
+  tile_of_group(blk, cl.rg.tz, cl.rg.ty, nblk, tzi, tyi, cxi);
   const int cz0 = tzi * TZ, cy0 = tyi * TY;
-  int X0 = cxi * cl.mg.chunk, X1 = min(X0 + cl.mg.chunk, cnx);
+  int X0 = cxi * cl.rg.chunk, X1 = min(X0 + cl.rg.chunk, cnx);
   if (cl.hull) {
     // r is read on the coarse level's liquid cells only: march over the
     // coarse tile's liquid hull
@@ -1072,6 +1072,17 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
     lv.sweep_tz = lv.d[2] >= 32 ? 32 : 16;
     lv.sw = sweep_geometry(lv.d, lv.sweep_tz);
     lv.mg = march_geometry(lv.d[0], lv.d[1], lv.d[2]);
+    // the restriction walks two fine planes per coarse plane, one barrier
+    // each, and re-reads two fine planes per chunk: about 4096 workgroups,
+    // chunks of 2 to 8 coarse planes (short chains on small levels, few
+    // re-reads on large ones)
+    lv.rg = lv.mg;
+    {
+      const long tiles = (long)lv.rg.tz * lv.rg.ty;
+      const long c = ((long)lv.d[0] * tiles + 4095) / 4096;
+      lv.rg.chunk = (int)std::min<long>(lv.d[0], std::max<long>(2, std::min<long>(8, c)));
+    }
+    lv.rg.ngroups = lv.rg.tz * lv.rg.ty * ((lv.d[0] + lv.rg.chunk - 1) / lv.rg.chunk);
     lv.hull = nullptr;
     if (lv.fused && (rc = alloc((void**)&lv.hull, (size_t)lv.mg.tz * lv.mg.ty * 2 * sizeof(int)))) return rc;
   }
@@ -1298,7 +1309,7 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
       const LevelRec& lv = R.lv[l];
       if (l >= 1) dn.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
       if (l >= R.L - 1) continue;  // the coarsest level is solved directly
-      rs.push_back({g, R.lv[l + 1].mg.ngroups});
+      rs.push_back({g, R.lv[l + 1].rg.ngroups});
       if (lv.fused) {
         (lv.sweep_tz == 32 ? s32 : s16).push_back({g, lv.sw.ngroups});
         res.push_back({g, lv.mg.ngroups});

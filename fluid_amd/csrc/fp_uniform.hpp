@@ -71,6 +71,7 @@ struct LevelRec {
   int efinal;         // buffer (0: e, 1: e2) holding e after the post-smoothing
   TileGeo sw;         // sweep geometry
   TileGeo mg;         // march geometry (tile 32 x 8)
+  TileGeo rg;         // restriction into this level: the march tiles, short x chunks
   int* hull;          // per march tile: first and last plane holding liquid
                       // (fused levels, k_tile_hull; null otherwise)
 };
