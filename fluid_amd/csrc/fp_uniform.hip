@@ -329,35 +329,30 @@ __device__ __forceinline__ const TR* level_rhs(const GridRec& G, int level) {
   return (const TR*)(level == 0 ? (const void*)G.r : G.lv[level].r);
 }
 
-// One colour of red-black Gauss-Seidel (colour = (i+j+k)&1) on levels too
-// small for the fused sweep.  zero_init: the first pass from a zero guess,
-// which also clears the other colour.
+// One colour of red-black Gauss-Seidel (colour = (i+j+k)&1) at cell idx of a
+// level too small for the fused sweep.  zero_init: the first pass from a zero
+// guess, which also clears the other colour.  rm: stored-rhs to stored-e scale.
 template <typename TR, typename T>
-__global__ void k_smooth(const Part* parts, int np, int level, int colour, int zero_init) {
-  FP_PART(parts, np);
-  const State* st = G.st;
-  if (st->done) return;
+__device__ __forceinline__ void smooth_cell(const GridRec& G, int level, int colour, int zero_init, float rm,
+                                            size_t idx) {
   const LevelRec& lv = G.lv[level];
   const LevelArgs L = largs_of(lv);
   const TR* __restrict__ r = level_rhs<TR>(G, level);
   T* __restrict__ e = (T*)lv.e;
-  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.Se;
-  const float rm = level == 0 ? rmul * st->rscale : rmul;
   const Grid3 g = L.g;
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
-  FP_STRIDE(idx, g.n()) {
-    int i, j, k;
-    g.ijk(idx, i, j, k);
-    if (((i + j + k) & 1) != colour) {
-      if (zero_init) stT(e, idx, 0.0f);
-      continue;
-    }
-    const float pc = L.phi[idx];
-    if (!(pc < 0.0f)) {
-      stT(e, idx, 0.0f);
-      continue;
-    }
-    float diag = 0.0f, acc = ldT(r, idx) * rm;
+  int i, j, k;
+  g.ijk(idx, i, j, k);
+  if (((i + j + k) & 1) != colour) {
+    if (zero_init) stT(e, idx, 0.0f);
+    return;
+  }
+  const float pc = L.phi[idx];
+  if (!(pc < 0.0f)) {
+    stT(e, idx, 0.0f);
+    return;
+  }
+  float diag = 0.0f, acc = ldT(r, idx) * rm;
 #define SNB(cond, off, gg)                                      \
   if (cond) {                                                   \
     const size_t m = idx + (off);                               \
@@ -366,41 +361,46 @@ __global__ void k_smooth(const Part* parts, int np, int level, int colour, int z
     diag += wf;                                                 \
     if (!zero_init && pn < 0.0f) acc += wf * ldT(e, m);         \
   }
-    SNB(i > 0, -(ptrdiff_t)sx, L.gx)
-    SNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
-    SNB(j > 0, -(ptrdiff_t)sy, L.gy)
-    SNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
-    SNB(k > 0, -1, L.gz)
-    SNB(k + 1 < g.nz, 1, L.gz)
+  SNB(i > 0, -(ptrdiff_t)sx, L.gx)
+  SNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
+  SNB(j > 0, -(ptrdiff_t)sy, L.gy)
+  SNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
+  SNB(k > 0, -1, L.gz)
+  SNB(k + 1 < g.nz, 1, L.gz)
 #undef SNB
-    stT(e, idx, diag > 0.0f ? acc / diag : 0.0f);
-  }
+  stT(e, idx, diag > 0.0f ? acc / diag : 0.0f);
 }
 
-// t = r - A e on levels too small for the fused march (stored with St).
 template <typename TR, typename T>
-__global__ void k_residual(const Part* parts, int np, int level) {
+__global__ void k_smooth(const Part* parts, int np, int level, int colour, int zero_init) {
   FP_PART(parts, np);
   const State* st = G.st;
   if (st->done) return;
+  const LevelRec& lv = G.lv[level];
+  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.Se;
+  const float rm = level == 0 ? rmul * st->rscale : rmul;
+  FP_STRIDE(idx, (size_t)lv.n) smooth_cell<TR, T>(G, level, colour, zero_init, rm, idx);
+}
+
+// t = r - A e at cell idx of a level too small for the fused march (stored
+// with St); rm: stored-rhs to stored-t scale.
+template <typename TR, typename T>
+__device__ __forceinline__ void residual_cell(const GridRec& G, int level, float rm, size_t idx) {
   const LevelRec& lv = G.lv[level];
   const LevelArgs L = largs_of(lv);
   const TR* __restrict__ r = level_rhs<TR>(G, level);
   const T* __restrict__ e = (const T*)lv.e;
   T* __restrict__ t = (T*)lv.t;
-  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.St;
-  const float rm = level == 0 ? rmul * st->rscale : rmul;
   const float emul = lv.Se / lv.St;
   const Grid3 g = L.g;
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
-  FP_STRIDE(idx, g.n()) {
-    int i, j, k;
-    g.ijk(idx, i, j, k);
-    const float pc = L.phi[idx];
-    float res = 0.0f;
-    if (pc < 0.0f) {
-      const float ec = ldT(e, idx);
-      float ap = 0.0f;
+  int i, j, k;
+  g.ijk(idx, i, j, k);
+  const float pc = L.phi[idx];
+  float res = 0.0f;
+  if (pc < 0.0f) {
+    const float ec = ldT(e, idx);
+    float ap = 0.0f;
 #define RNB(cond, off, gg)                                          \
   if (cond) {                                                       \
     const size_t m = idx + (off);                                   \
@@ -408,17 +408,74 @@ __global__ void k_residual(const Part* parts, int np, int level) {
     const float wf = face_weight(pc, pn, gg);                       \
     ap += wf * (ec - (pn < 0.0f ? ldT(e, m) : 0.0f));               \
   }
-      RNB(i > 0, -(ptrdiff_t)sx, L.gx)
-      RNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
-      RNB(j > 0, -(ptrdiff_t)sy, L.gy)
-      RNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
-      RNB(k > 0, -1, L.gz)
-      RNB(k + 1 < g.nz, 1, L.gz)
+    RNB(i > 0, -(ptrdiff_t)sx, L.gx)
+    RNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
+    RNB(j > 0, -(ptrdiff_t)sy, L.gy)
+    RNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
+    RNB(k > 0, -1, L.gz)
+    RNB(k + 1 < g.nz, 1, L.gz)
 #undef RNB
-      res = ldT(r, idx) * rm - emul * ap;
-    }
-    stT(t, idx, res);
+    res = ldT(r, idx) * rm - emul * ap;
   }
+  stT(t, idx, res);
+}
+
+template <typename TR, typename T>
+__global__ void k_residual(const Part* parts, int np, int level) {
+  FP_PART(parts, np);
+  const State* st = G.st;
+  if (st->done) return;
+  const LevelRec& lv = G.lv[level];
+  const float rmul = (level == 0 ? 1.0f : lv.Sr) / lv.St;
+  const float rm = level == 0 ? rmul * st->rscale : rmul;
+  FP_STRIDE(idx, (size_t)lv.n) residual_cell<TR, T>(G, level, rm, idx);
+}
+
+// The restriction at one coarse cell (the gather form of k_restrict_march, for
+// the one-workgroup bottom of the cycle): per axis the fine taps 2I-1 .. 2I+2
+// with weights 1/4 3/4 3/4 1/4 (1 at a domain edge), or the cell itself on an
+// axis that is not coarsened.
+__device__ __forceinline__ int restrict_taps(int I, int s, int nf, int nc, int* f, float* w) {
+  if (s == 1) {
+    f[0] = I;
+    w[0] = 1.0f;
+    return 1;
+  }
+  int c = 0;
+  if (I >= 1) { f[c] = 2 * I - 1; w[c] = 0.25f; ++c; }
+  f[c] = 2 * I; w[c] = I >= 1 ? 0.75f : 1.0f; ++c;
+  if (2 * I + 1 < nf) { f[c] = 2 * I + 1; w[c] = I + 1 >= nc ? 1.0f : 0.75f; ++c; }
+  if (2 * I + 2 < nf) { f[c] = 2 * I + 2; w[c] = 0.25f; ++c; }
+  return c;
+}
+
+template <typename T>
+__device__ __forceinline__ void restrict_cell(const GridRec& G, int level, size_t ci) {
+  const LevelRec& fl = G.lv[level];
+  const LevelRec& cl = G.lv[level + 1];
+  const Grid3 f = grid_of(fl), c = grid_of(cl);
+  const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
+  const int k2 = (sx == 2) + (sy == 2) + (sz == 2);
+  const float fac = (1.0f / (float)(1 << k2)) * fl.St / cl.Sr;
+  const T* __restrict__ tf = (const T*)fl.t;
+  float acc = 0.0f;
+  if (cl.phi[ci] < 0.0f) {
+    int I, J, K;
+    c.ijk(ci, I, J, K);
+    int fi[4], fj[4], fk[4];
+    float wi[4], wj[4], wk[4];
+    const int ni = restrict_taps(I, sx, f.nx, c.nx, fi, wi);
+    const int nj = restrict_taps(J, sy, f.ny, c.ny, fj, wj);
+    const int nk = restrict_taps(K, sz, f.nz, c.nz, fk, wk);
+    for (int a = 0; a < ni; ++a)
+      for (int b = 0; b < nj; ++b) {
+        const float wab = wi[a] * wj[b];
+        const size_t base = ((size_t)fi[a] * f.ny + fj[b]) * f.nz;
+        for (int q = 0; q < nk; ++q) acc += wab * wk[q] * ldT(tf, base + fk[q]);
+      }
+    acc *= fac;
+  }
+  stT((T*)cl.r, ci, acc);
 }
 
 // r_{l+1} = R t_l on the coarse level's liquid cells (R = P^T / 2^k, the
@@ -558,39 +615,41 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
 
 // e_l += P e_{l+1} on the fine level's liquid cells; which: the fine buffer.
 template <typename T>
-__global__ void k_prolong_add(const Part* parts, int np, int level, int which) {
-  FP_PART(parts, np);
-  if (G.st->done) return;
+__device__ __forceinline__ void prolong_cell(const GridRec& G, int level, int which, size_t idx) {
   const LevelRec& fl = G.lv[level];
   const LevelRec& cl = G.lv[level + 1];
+  if (!(fl.phi[idx] < 0.0f)) return;
   const Grid3 f = grid_of(fl), c = grid_of(cl);
   T* __restrict__ ef = (T*)(which ? fl.e2 : fl.e);
   const T* __restrict__ ec = (const T*)(cl.efinal ? cl.e2 : cl.e);
-  const float* __restrict__ phif = fl.phi;
   const float* __restrict__ phic = cl.phi;
   const int sx = cl.s[0], sy = cl.s[1], sz = cl.s[2];
   const float fac = cl.Se / fl.Se;
-  FP_STRIDE(idx, f.n()) {
-    if (!(phif[idx] < 0.0f)) continue;
-    int i, j, k;
-    f.ijk(idx, i, j, k);
-    int ci[2], cj[2], ck[2];
-    float wi[2], wj[2], wk[2];
-    const int ni = ptaps(i, sx, c.nx, ci, wi);
-    const int nj = ptaps(j, sy, c.ny, cj, wj);
-    const int nk = ptaps(k, sz, c.nz, ck, wk);
-    float acc = 0.0f;
-    for (int a = 0; a < ni; ++a)
-      for (int b = 0; b < nj; ++b) {
-        const float wab = wi[a] * wj[b];
-        const size_t base = ((size_t)ci[a] * c.ny + cj[b]) * c.nz;
-        for (int q = 0; q < nk; ++q) {
-          const size_t m = base + ck[q];
-          if (phic[m] < 0.0f) acc += wab * wk[q] * ldT(ec, m);
-        }
+  int i, j, k;
+  f.ijk(idx, i, j, k);
+  int ci[2], cj[2], ck[2];
+  float wi[2], wj[2], wk[2];
+  const int ni = ptaps(i, sx, c.nx, ci, wi);
+  const int nj = ptaps(j, sy, c.ny, cj, wj);
+  const int nk = ptaps(k, sz, c.nz, ck, wk);
+  float acc = 0.0f;
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b) {
+      const float wab = wi[a] * wj[b];
+      const size_t base = ((size_t)ci[a] * c.ny + cj[b]) * c.nz;
+      for (int q = 0; q < nk; ++q) {
+        const size_t m = base + ck[q];
+        if (phic[m] < 0.0f) acc += wab * wk[q] * ldT(ec, m);
       }
-    stT(ef, idx, ldT(ef, idx) + fac * acc);
-  }
+    }
+  stT(ef, idx, ldT(ef, idx) + fac * acc);
+}
+
+template <typename T>
+__global__ void k_prolong_add(const Part* parts, int np, int level, int which) {
+  FP_PART(parts, np);
+  if (G.st->done) return;
+  FP_STRIDE(idx, (size_t)G.lv[level].n) prolong_cell<T>(G, level, which, idx);
 }
 
 // Clears a level's correction buffer (pre-smoothing with zero sweeps).
@@ -607,10 +666,8 @@ __global__ void k_zero_level(const Part* parts, int np, int level) {
 // grid).  TR: storage of that level's right-hand side (FP32 when the coarsest
 // level is the finest).
 template <typename T, typename TR>
-__global__ void k_coarse_solve(const Part* parts) {
-  const GridRec& G = *parts[blockIdx.x].g;
+__device__ __forceinline__ void coarse_solve_block(const GridRec& G) {
   const State* st = G.st;
-  if (st->done) return;
   __shared__ double rs[512];
   const int lc = G.L - 1;
   const LevelRec& lv = G.lv[lc];
@@ -629,6 +686,59 @@ __global__ void k_coarse_solve(const Part* parts) {
     double s = 0.0;
     for (int q = 0; q < n; ++q) s += minv[(size_t)row * n + q] * rs[q];
     stT(e, (size_t)cmap[row], (float)(s * fac));
+  }
+}
+
+template <typename T, typename TR>
+__global__ void k_coarse_solve(const Part* parts) {
+  const GridRec& G = *parts[blockIdx.x].g;
+  if (G.st->done) return;
+  coarse_solve_block<T, TR>(G);
+}
+
+// The bottom of the V-cycle in one workgroup per grid: the levels from lb on
+// (each under kBottomCells cells, so never fused and never the finest) are
+// smoothed, residual-restricted, solved on the coarsest level and prolonged
+// with the same per-cell steps as k_smooth / k_residual / k_prolong_add and
+// the restriction's weights, separated by workgroup barriers instead of
+// launches.
+template <typename T>
+__global__ __launch_bounds__(kBottomThreads) void k_bottom(const Part* parts, int pre, int post) {
+  const GridRec& G = *parts[blockIdx.x].g;
+  if (G.st->done) return;
+  const int lb = G.lb, lc = G.L - 1;
+  for (int l = lb; l < lc; ++l) {
+    const LevelRec& lv = G.lv[l];
+    const size_t n = (size_t)lv.n;
+    const float rs = lv.Sr / lv.Se, rt = lv.Sr / lv.St;
+    if (pre == 0) {
+      for (size_t i = threadIdx.x; i < n; i += blockDim.x) stT((T*)lv.e, i, 0.0f);
+      __syncthreads();
+    }
+    for (int sw = 0; sw < pre; ++sw)
+      for (int colour = 0; colour < 2; ++colour) {
+        const int zi = (sw == 0 && colour == 0) ? 1 : 0;
+        for (size_t i = threadIdx.x; i < n; i += blockDim.x) smooth_cell<T, T>(G, l, colour, zi, rs, i);
+        __syncthreads();
+      }
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) residual_cell<T, T>(G, l, rt, i);
+    __syncthreads();
+    for (size_t ci = threadIdx.x; ci < (size_t)G.lv[l + 1].n; ci += blockDim.x) restrict_cell<T>(G, l, ci);
+    __syncthreads();
+  }
+  coarse_solve_block<T, T>(G);
+  __syncthreads();
+  for (int l = lc - 1; l >= lb; --l) {
+    const LevelRec& lv = G.lv[l];
+    const size_t n = (size_t)lv.n;
+    const float rs = lv.Sr / lv.Se;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) prolong_cell<T>(G, l, 0, i);
+    __syncthreads();
+    for (int sw = 0; sw < post; ++sw)
+      for (int cc = 0; cc < 2; ++cc) {
+        for (size_t i = threadIdx.x; i < n; i += blockDim.x) smooth_cell<T, T>(G, l, 1 - cc, 0, rs, i);
+        __syncthreads();
+      }
   }
 }
 
@@ -1086,6 +1196,12 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
     lv.hull = nullptr;
     if (lv.fused && (rc = alloc((void**)&lv.hull, (size_t)lv.mg.tz * lv.mg.ty * 2 * sizeof(int)))) return rc;
   }
+  R.lb = L - 1;
+  for (int l = 1; l < L; ++l)
+    if (R.lv[l].n < kBottomCells) {
+      R.lb = l;
+      break;
+    }
   const LevelRec& c = R.lv[L - 1];
   R.max_coarse = (int)std::min<long long>(c.n, 512);
   if ((rc = alloc((void**)&R.cmap, (size_t)R.max_coarse * 4 + 16))) return rc;
@@ -1309,12 +1425,12 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
       const LevelRec& lv = R.lv[l];
       if (l >= 1) dn.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
       if (l >= R.L - 1) continue;  // the coarsest level is solved directly
-      rs.push_back({g, R.lv[l + 1].rg.ngroups});
+      if (l < R.lb) rs.push_back({g, R.lv[l + 1].rg.ngroups});
       if (lv.fused) {
         (lv.sweep_tz == 32 ? s32 : s16).push_back({g, lv.sw.ngroups});
         res.push_back({g, lv.mg.ngroups});
         fc.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
-      } else {
+      } else if (l < R.lb) {
         sm.push_back({g, Plan::blocks_for((size_t)lv.n, 1)});
       }
     }
@@ -1436,7 +1552,10 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
     TIMED(top ? FP_KERNEL_RESTRICT_FINE : -1, PLAUNCH(k_restrict_march<T>, T_.restrict_, s, l));
   }
   FLAUNCH((k_coarse_solve<T, float>), p.coarse_top, s);
-  FLAUNCH((k_coarse_solve<T, T>), p.coarse_deep, s);
+  // grids with a coarser level: the levels from lb on and the coarsest solve
+  // in one workgroup per grid
+  if (p.coarse_deep.n > 0)
+    hipLaunchKernelGGL((k_bottom<T>), dim3(p.coarse_deep.n), dim3(kBottomThreads), 0, s, p.coarse_deep.dev, pre, post);
   for (int l = p.maxL - 2; l >= 0; --l) {
     Plan::Level& T_ = p.lv[l];
     const bool top = l == 0;

@@ -24,6 +24,8 @@ namespace fp {
 
 constexpr int kMaxLevels = 16;
 constexpr unsigned kMaxBlocks = 16384;
+constexpr long long kBottomCells = 4096;  // levels below this size run in the one-workgroup bottom of the V-cycle
+constexpr int kBottomThreads = 1024;
 
 void set_error(const char* fmt, ...);
 
@@ -78,6 +80,7 @@ struct LevelRec {
 
 struct GridRec {
   int nx, ny, nz, L;
+  int lb;             // first level of the V-cycle's one-workgroup bottom (levels >= 1 under kBottomCells)
   long long n, nu, nv, nw;
   double h;
   LevelRec lv[kMaxLevels];
