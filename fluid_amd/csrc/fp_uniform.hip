@@ -343,31 +343,35 @@ __device__ __forceinline__ void smooth_cell(const GridRec& G, int level, int col
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
   int i, j, k;
   g.ijk(idx, i, j, k);
+  // every load first (neighbours past the edge clamped to the cell itself),
+  // then the arithmetic: one round trip per pass
+  const bool has[6] = {i > 0, i + 1 < g.nx, j > 0, j + 1 < g.ny, k > 0, k + 1 < g.nz};
+  const ptrdiff_t off[6] = {-(ptrdiff_t)sx, (ptrdiff_t)sx, -(ptrdiff_t)sy, (ptrdiff_t)sy, -1, 1};
+  const float gw[3] = {L.gx, L.gy, L.gz};
+  const float pc = L.phi[idx];
+  const float rv = ldT(r, idx);
+  float pn[6], en[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const size_t m = has[q] ? (size_t)((ptrdiff_t)idx + off[q]) : idx;
+    pn[q] = L.phi[m];
+    en[q] = ldT(e, m);
+  }
   if (((i + j + k) & 1) != colour) {
     if (zero_init) stT(e, idx, 0.0f);
     return;
   }
-  const float pc = L.phi[idx];
   if (!(pc < 0.0f)) {
     stT(e, idx, 0.0f);
     return;
   }
-  float diag = 0.0f, acc = ldT(r, idx) * rm;
-#define SNB(cond, off, gg)                                      \
-  if (cond) {                                                   \
-    const size_t m = idx + (off);                               \
-    const float pn = L.phi[m];                                  \
-    const float wf = face_weight(pc, pn, gg);                   \
-    diag += wf;                                                 \
-    if (!zero_init && pn < 0.0f) acc += wf * ldT(e, m);         \
+  float diag = 0.0f, acc = rv * rm;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const float wf = has[q] ? face_weight(pc, pn[q], gw[q >> 1]) : 0.0f;
+    diag += wf;
+    acc += (!zero_init && has[q] && pn[q] < 0.0f) ? wf * en[q] : 0.0f;
   }
-  SNB(i > 0, -(ptrdiff_t)sx, L.gx)
-  SNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
-  SNB(j > 0, -(ptrdiff_t)sy, L.gy)
-  SNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
-  SNB(k > 0, -1, L.gz)
-  SNB(k + 1 < g.nz, 1, L.gz)
-#undef SNB
   stT(e, idx, diag > 0.0f ? acc / diag : 0.0f);
 }
 
@@ -396,26 +400,27 @@ __device__ __forceinline__ void residual_cell(const GridRec& G, int level, float
   const size_t sx = (size_t)g.ny * g.nz, sy = (size_t)g.nz;
   int i, j, k;
   g.ijk(idx, i, j, k);
+  const bool has[6] = {i > 0, i + 1 < g.nx, j > 0, j + 1 < g.ny, k > 0, k + 1 < g.nz};
+  const ptrdiff_t off[6] = {-(ptrdiff_t)sx, (ptrdiff_t)sx, -(ptrdiff_t)sy, (ptrdiff_t)sy, -1, 1};
+  const float gw[3] = {L.gx, L.gy, L.gz};
   const float pc = L.phi[idx];
+  const float ec = ldT(e, idx), rv = ldT(r, idx);
+  float pn[6], en[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const size_t m = has[q] ? (size_t)((ptrdiff_t)idx + off[q]) : idx;
+    pn[q] = L.phi[m];
+    en[q] = ldT(e, m);
+  }
   float res = 0.0f;
   if (pc < 0.0f) {
-    const float ec = ldT(e, idx);
     float ap = 0.0f;
-#define RNB(cond, off, gg)                                          \
-  if (cond) {                                                       \
-    const size_t m = idx + (off);                                   \
-    const float pn = L.phi[m];                                      \
-    const float wf = face_weight(pc, pn, gg);                       \
-    ap += wf * (ec - (pn < 0.0f ? ldT(e, m) : 0.0f));               \
-  }
-    RNB(i > 0, -(ptrdiff_t)sx, L.gx)
-    RNB(i + 1 < g.nx, (ptrdiff_t)sx, L.gx)
-    RNB(j > 0, -(ptrdiff_t)sy, L.gy)
-    RNB(j + 1 < g.ny, (ptrdiff_t)sy, L.gy)
-    RNB(k > 0, -1, L.gz)
-    RNB(k + 1 < g.nz, 1, L.gz)
-#undef RNB
-    res = ldT(r, idx) * rm - emul * ap;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const float wf = has[q] ? face_weight(pc, pn[q], gw[q >> 1]) : 0.0f;
+      ap += wf * (ec - (pn[q] < 0.0f ? en[q] : 0.0f));
+    }
+    res = rv * rm - emul * ap;
   }
   stT(t, idx, res);
 }
