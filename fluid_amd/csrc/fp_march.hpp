@@ -144,17 +144,30 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
   }
 }
 
-// Tiles of 32 x 8 columns, x chunks of at least 8 planes, about target_blocks
-// workgroups (at most the 16384 partials of the reductions).
+// Coarse levels (level >= 2) of at most this many cells stay in the caches:
+// their marches are bound by the latency of the plane chain, not by HBM, so
+// they take short x chunks (more, shorter chains).  The other levels take
+// chunks of at least 8 planes (a march carries up to 4 planes of start-up).
+// A batched launch covers the level of every grid of the batch; from level 2
+// on that is at most 1/64 of the finest levels' cells, so the launch stays
+// cache-sized too (level 1 of a large batch is not).
+constexpr long long kSmallLevelCells = 1ll << 21;
+inline long min_march_chunk(long long cells, int level) {
+  return (level >= 2 && cells <= kSmallLevelCells) ? 2 : 8;
+}
+
+// Tiles of 32 x 8 columns, about target_blocks workgroups (at most the 16384
+// partials of the reductions).
 #ifndef FP_MARCH_GROUPS
 #define FP_MARCH_GROUPS 8192
 #endif
-inline TileGeo march_geometry(int nx, int ny, int nz, long target_blocks = FP_MARCH_GROUPS) {
+inline TileGeo march_geometry(int nx, int ny, int nz, int level, long target_blocks = FP_MARCH_GROUPS) {
   const int TZ = kMarchTZ, TY = 256 / TZ;
   const int tz = (nz + TZ - 1) / TZ, ty = (ny + TY - 1) / TY;
   const long tiles = (long)tz * ty;
   long c = ((long)nx * tiles + target_blocks - 1) / target_blocks;
-  if (c < 8) c = 8;
+  const long cmin = min_march_chunk((long long)nx * ny * nz, level);
+  if (c < cmin) c = cmin;
   if (c > nx) c = nx;
   while ((long)tiles * ((nx + c - 1) / c) > (long)kMaxBlocks) ++c;
   TileGeo m;

@@ -205,9 +205,9 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   constexpr int WZ = PROLONG ? EZ : TZ + 3;
   constexpr int CY = TY + 2, CZ = TZ + 2;    // r, first-colour values: -1 .. T
   constexpr int WN = WY * WZ, EN = EY * EZ, CN = CY * CZ;
-  constexpr int PN = PROLONG ? 2 * EN : 0;   // two coarse planes, window <= EY x EZ
+  constexpr int PN = PROLONG ? 2 * EN : 0;   // two coarse plane slots, window <= EY x EZ
   constexpr int NW = (WN + 255) / 256, NE = ZERO ? 0 : (EN + 255) / 256, NR = (CN + 255) / 256;
-  constexpr int NP = (PN + 255) / 256;
+  constexpr int NP = PROLONG ? (EN + 255) / 256 : 0;  // one coarse plane per staging
   constexpr int HF = CZ / 2, HS = TZ / 2;
   static_assert(CY * HF <= 256 && TY * HS <= 256 && EY * (EZ / 2) <= 256, "lane maps exceed the workgroup");
   using WB = typename W4Bits<T>::type;
@@ -277,20 +277,56 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
     const int yy = y0 + q / CZ - 1, zz = z0 + q % CZ - 1;
     orr[k] = (q < CN && yy >= 0 && yy < ny && zz >= 0 && zz < nz) ? yy * nz + zz : -1;
   }
-  // coarse window (PROLONG): rows/columns from cy0/cz0, planes pa(L), pa(L)+1
+  // coarse window (PROLONG): rows/columns from cy0/cz0, CWY x CWZ of them
+  // (TY/2 + 4 x TZ/2 + 4 on an axis of stride 2, EY / EZ on one of stride 1),
+  // stored at row pitch EZ.  Coarse plane c lives in slot c & 1.  Fine plane L
+  // interpolates coarse planes (L-1)>>1 and the next one (stride 2; fine
+  // planes 2c+1 and 2c+2 share the pair c, c+1) or plane L (stride 1), so a
+  // step stages at most one new coarse plane (cplane_new) and the other is
+  // already in its slot.
   const int cy0 = PROLONG ? (P.sy == 2 ? (y0 >> 1) - 2 : y0 - 2) : 0;
   const int cz0 = PROLONG ? (P.sz == 2 ? (z0 >> 1) - 2 : z0 - 2) : 0;
   const long long cplane = PROLONG ? (long long)P.cny * P.cnz : 0;
-  int op[NP > 0 ? NP : 1], opl[NP > 0 ? NP : 1];
-#pragma unroll
-  for (int k = 0; k < NP; ++k) {
+  // window entry k of this lane: its LDS position (-1: none) and its offset
+  // in a coarse plane (-1: outside the coarse grid); recomputed where used,
+  // which is cheaper than holding it in registers for the whole march
+  auto coarse_entry = [&](int k, int& lq) {
     const int q = tid + 256 * k;
-    const int qq = q % EN;
-    const int yy = cy0 + qq / EZ, zz = cz0 + qq % EZ;
-    opl[k] = q / EN;
-    op[k] = (q < PN && yy >= 0 && yy < P.cny && zz >= 0 && zz < P.cnz) ? yy * P.cnz + zz : -1;
+    int row, col, cwn;
+    if (P.sz == 2) {
+      row = q / (TZ / 2 + 4);
+      col = q - row * (TZ / 2 + 4);
+    } else {
+      row = q / EZ;
+      col = q - row * EZ;
+    }
+    cwn = (P.sy == 2 ? TY / 2 + 4 : EY) * (P.sz == 2 ? TZ / 2 + 4 : EZ);
+    const int yy = cy0 + row, zz = cz0 + col;
+    lq = q < cwn ? row * EZ + col : -1;
+    return (lq >= 0 && yy >= 0 && yy < P.cny && zz >= 0 && zz < P.cnz) ? yy * P.cnz + zz : -1;
+  };
+  constexpr int kNoPlane = -0x40000000;
+  auto cplane_new = [&](int L) {
+    if (P.sx != 2) return L;
+    return ((L & 1) || L == x0 - 2) ? ((L - 1) >> 1) + 1 : kNoPlane;
+  };
+  if constexpr (PROLONG) {
+    // the first step's pair also needs its lower plane: staged here (the
+    // first step's barrier orders it before any use)
+    if (P.sx == 2) {
+      const int c = (x0 - 3) >> 1;
+      const T* ec = (const T*)P.ec;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        int lq;
+        const int o = coarse_entry(k, lq);
+        if (lq >= 0) {
+          const bool ok = c >= 0 && c < P.cnx && o >= 0;
+          s_p[(c & 1) * EN + lq] = ok ? ldg(ec, (size_t)((long long)c * cplane + o)) : value_unless(T(), false);
+        }
+      }
+    }
   }
-  auto coarse_plane0 = [&](int L) { return P.sx == 2 ? ((L - 1) >> 1) : L; };
 
   // D register sets of raw prefetched planes
   WB pw[D][NW];
@@ -300,6 +336,20 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   int lf = x0 - 2;                 // next plane to fetch
   long long fo = (long long)lf * plane;
   auto fetch = [&](int d) {
+    if constexpr (PROLONG) {
+      // coarse planes are staged also for fine planes outside the grid: the
+      // later steps of the pair read them
+      const int c = cplane_new(lf);
+      if (c >= 0 && c < P.cnx) {
+        const T* ec = (const T*)P.ec;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          int lq;
+          const int o = coarse_entry(k, lq);
+          pp[d][k] = ldg(ec, (size_t)((long long)c * cplane + max(o, 0)));
+        }
+      }
+    }
     if (lf >= 0 && lf < nx) {
       const W4<T>* wp = wts + fo;
       const TR* rp = r + fo;
@@ -312,15 +362,6 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       }
 #pragma unroll
       for (int k = 0; k < NR; ++k) pr[d][k] = ldg(rp, (size_t)max(orr[k], 0));
-      if constexpr (PROLONG) {
-        const int pa = coarse_plane0(lf);
-        const T* ec = (const T*)P.ec;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          const int c = min(max(pa + opl[k], 0), P.cnx - 1);
-          pp[d][k] = ldg(ec, (size_t)((long long)c * cplane + max(op[k], 0)));
-        }
-      }
     }
     ++lf;
     fo += plane;
@@ -355,27 +396,12 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
 
   // prolongation lane (PROLONG): a second-colour cell of the staged window
   // (rows/columns -2..T+1, EZ/2 per row); its column alternates with the plane
-  // parity (index h).  Its y / z taps into the coarse window are fixed for the
-  // whole march, so they are evaluated once here.
+  // parity.  Its taps into the coarse window are evaluated per plane (a few
+  // integer operations) rather than held across the march.
   constexpr int PH = EZ / 2;
   const int ply = tid / PH, plk = tid - (tid / PH) * PH;
   const int pgy = y0 + ply - 2;
-  int pq[2] = {0, 0}, pza[2] = {0, 0}, pzb[2] = {0, 0}, pya = 0, pyb = 0;
-  float pwza[2] = {0.0f, 0.0f}, pwzb[2] = {0.0f, 0.0f}, pwya = 0.0f, pwyb = 0.0f;
-  bool pok[2] = {false, false};
-  if constexpr (PROLONG) {
-    const bool prow = tid < EY * PH && pgy >= 0 && pgy < ny;
-    if (prow) window_taps(pgy, P.sy, P.cny, cy0, pya, pwya, pyb, pwyb);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int L = x0 - 2 + h;  // parity of L on steps of parity h
-      const int ez = 2 * plk + ((second + L + pgy) & 1);
-      const int pgz = z0 + ez - 2;
-      pq[h] = ply * EZ + ez;
-      pok[h] = prow && pgz >= 0 && pgz < nz;
-      if (pok[h]) window_taps(pgz, P.sz, P.cnz, cz0, pza[h], pwza[h], pzb[h], pwzb[h]);
-    }
-  }
+  const bool prow = PROLONG && tid < EY * PH && pgy >= 0 && pgy < ny;
 
 #pragma unroll
   for (int d = 0; d < D; ++d) fetch(d);
@@ -415,12 +441,14 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       if (k + 1 < NR || q < CN) s_r[s0][q] = to_store<T>((pin && orr[k] >= 0) ? v : 0.0f);
     }
     if constexpr (PROLONG) {
-      const int pa = coarse_plane0(L);
+      const int c = cplane_new(L);
+      if (c != kNoPlane) {
 #pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const int q = tid + 256 * k;
-        const int c = pa + opl[k];
-        if (k + 1 < NP || q < PN) s_p[q] = value_unless(pp[rs][k], c >= 0 && c < P.cnx && op[k] >= 0);
+        for (int k = 0; k < NP; ++k) {
+          int lq;
+          const int o = coarse_entry(k, lq);
+          if (lq >= 0) s_p[(c & 1) * EN + lq] = value_unless(pp[rs][k], c >= 0 && c < P.cnx && o >= 0);
+        }
       }
     }
     lds_barrier();  // A
@@ -428,19 +456,21 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
 
     if constexpr (PROLONG) {
       // e(L) += fac * P ec at second-colour liquid cells (coarse air holds e = 0)
-      if (pin && pok[h]) {
-        int xa, xb;
-        float wxa, wxb;
-        window_taps(L, P.sx, P.cnx, coarse_plane0(L), xa, wxa, xb, wxb);
-        const int q = pq[h];
+      const int ez = 2 * plk + ((second + L + pgy) & 1);
+      const int pgz = z0 + ez - 2;
+      if (pin && prow && pgz >= 0 && pgz < nz) {
+        int xa, xb, ya, yb, za, zb;
+        float wxa, wxb, pwya, pwyb, wza, wzb;
+        window_taps(L, P.sx, P.cnx, 0, xa, wxa, xb, wxb);
+        const int q = ply * EZ + ez;
         const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[s0][q]);
         if (wf(wc.id) > 0.0f) {
-          const T* pa_ = s_p + xa * EN;
-          const T* pb_ = s_p + xb * EN;
-          const int za = pza[h], zb = pzb[h];
-          const float wza = pwza[h], wzb = pwzb[h];
-          const T v0 = pa_[pya * EZ + za], v1 = pa_[pya * EZ + zb], v2 = pa_[pyb * EZ + za], v3 = pa_[pyb * EZ + zb];
-          const T v4 = pb_[pya * EZ + za], v5 = pb_[pya * EZ + zb], v6 = pb_[pyb * EZ + za], v7 = pb_[pyb * EZ + zb];
+          window_taps(pgy, P.sy, P.cny, cy0, ya, pwya, yb, pwyb);
+          window_taps(pgz, P.sz, P.cnz, cz0, za, wza, zb, wzb);
+          const T* pa_ = s_p + (xa & 1) * EN;
+          const T* pb_ = s_p + (xb & 1) * EN;
+          const T v0 = pa_[ya * EZ + za], v1 = pa_[ya * EZ + zb], v2 = pa_[yb * EZ + za], v3 = pa_[yb * EZ + zb];
+          const T v4 = pb_[ya * EZ + za], v5 = pb_[ya * EZ + zb], v6 = pb_[yb * EZ + za], v7 = pb_[yb * EZ + zb];
           const T eq = s_e[e0 + q];
           __builtin_amdgcn_sched_barrier(0);
           const float acc = wxa * (pwya * (wza * wf(v0) + wzb * wf(v1)) + pwyb * (wza * wf(v2) + wzb * wf(v3))) +

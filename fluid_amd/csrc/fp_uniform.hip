@@ -1062,22 +1062,22 @@ constexpr int kSweepDepth = FP_SWEEP_DEPTH;  // planes of loads in flight per wo
 // Fused sweeps run on levels large enough to fill a 256-lane column tile.
 static bool fused_dims(const int* d, long long n) { return d[2] >= 16 && d[1] >= 4 && d[0] >= 4 && n >= 4096; }
 
-// Sweep geometry: tiles of TZ x 256/TZ columns, about two rounds of 8
-// resident workgroups per CU on 256 CUs, at least 8 planes per workgroup
-// (the march carries 4 planes of start-up).
-// About FP_SWEEP_GROUPS workgroups per sweep of a level (x chunks of at least
-// 8 planes).
+// Sweep geometry: tiles of TZ x 256/TZ columns, about FP_SWEEP_GROUPS
+// workgroups per sweep of a level, x chunks of min_march_chunk planes or more
+// (short on cache-resident coarse levels, where a sweep is bound by the
+// latency of its chain of plane steps rather than by HBM).
 #ifndef FP_SWEEP_GROUPS
 #define FP_SWEEP_GROUPS 8192
 #endif
-static TileGeo sweep_geometry(const int* d, int TZ) {
+static TileGeo sweep_geometry(const int* d, int TZ, int level) {
   const int TY = 256 / TZ;
   TileGeo g;
   g.tz = (d[2] + TZ - 1) / TZ;
   g.ty = (d[1] + TY - 1) / TY;
   const long tiles = (long)g.tz * g.ty;
   const long cx_target = std::max<long>(1, FP_SWEEP_GROUPS / tiles);
-  int chunk = (int)std::max<long>(8, (d[0] + cx_target - 1) / cx_target);
+  const long min_chunk = min_march_chunk((long long)d[0] * d[1] * d[2], level);
+  int chunk = (int)std::max<long>(min_chunk, (d[0] + cx_target - 1) / cx_target);
   chunk = std::min(chunk, d[0]);
   g.chunk = chunk;
   g.ngroups = (int)(tiles * ((d[0] + chunk - 1) / chunk));
@@ -1185,8 +1185,8 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
     lv.Sr = lv.Se = lv.St = 1.0f;
     lv.fused = (l < L - 1) && fused_dims(lv.d, lv.n);
     lv.sweep_tz = lv.d[2] >= 32 ? 32 : 16;
-    lv.sw = sweep_geometry(lv.d, lv.sweep_tz);
-    lv.mg = march_geometry(lv.d[0], lv.d[1], lv.d[2]);
+    lv.sw = sweep_geometry(lv.d, lv.sweep_tz, l);
+    lv.mg = march_geometry(lv.d[0], lv.d[1], lv.d[2], l);
     // the restriction walks two fine planes per coarse plane, one barrier
     // each, and re-reads two fine planes per chunk: about 4096 workgroups,
     // chunks of 2 to 8 coarse planes (short chains on small levels, few
