@@ -17,8 +17,10 @@
 //                                                       the loads that depend on the head
 // Heads are fetched two planes ahead and the dependent loads one plane ahead,
 // so neither waits for the other within a plane step.
+//   using Own = ...;                                    the lane's own-cell loads (not staged)
+//   Own own(int x, int y, int z, bool inside) const;    fetched one plane step ahead
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
-//              double* acc, float* mx) const;           nb/has: -x,+x,-y,+y,-z,+z
+//              const Own& o, double* acc, float* mx) const;  nb/has: -x,+x,-y,+y,-z,+z
 //   void zero(int x, int y, int z) const;               the outputs of an air cell
 //   static constexpr bool ZERO_OUTSIDE;                 store zeros outside the liquid hulls
 //                                                       (false: every reader of the outputs
@@ -81,6 +83,10 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
   }
   double acc[Op::NV > 0 ? Op::NV : 1];
   for (int k = 0; k < (Op::NV > 0 ? Op::NV : 1); ++k) acc[k] = 0.0;
+  // own-cell loads: plane L's are issued at step L and used at step L + 1
+  using Own = typename Op::Own;
+  const bool mine = gy < ny && gz < nz;
+  Own own_cur = Own(), own_next = Own();
   constexpr int NC = (RY * RZ + 255) / 256;
   using Head = typename Op::Head;
   Head hd[NC];
@@ -126,15 +132,17 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
     lds_barrier();
     if (L + 1 <= x1) fetch_cell(L + 1);
     if (L + 2 <= x1) fetch_head(L + 2);
+    if (L >= x0 && L < x1) own_next = op.own(L, gy, gz, mine);
     const int B = L - 1;
-    if (B >= x0 && B < x1 && gy < ny && gz < nz) {
+    if (B >= x0 && B < x1 && mine) {
       const int sB = (B + 3) % 3, sBm = (B + 2) % 3, sBp = (B + 4) % 3;
       const int cy = ly + 1, cz = lz + 1;
       Cell nb[6] = {s[sBm][cy][cz], s[sBp][cy][cz], s[sB][cy - 1][cz], s[sB][cy + 1][cz], s[sB][cy][cz - 1],
                     s[sB][cy][cz + 1]};
       const bool has[6] = {B > 0, B + 1 < nx, gy > 0, gy + 1 < ny, gz > 0, gz + 1 < nz};
-      op.apply(B, gy, gz, s[sB][cy][cz], nb, has, acc, &mx);
+      op.apply(B, gy, gz, s[sB][cy][cz], nb, has, own_cur, acc, &mx);
     }
+    own_cur = own_next;
     lds_barrier();
   }
   if constexpr (Op::NV > 0) {
@@ -329,8 +337,10 @@ struct OpDirection {
     }
     return c;
   }
+  struct Own {};
+  __device__ __forceinline__ Own own(int, int, int, bool) const { return Own(); }
   __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool* has,
-                                        double* acc, float*) const {
+                                        const Own&, double* acc, float*) const {
     const size_t i = ((size_t)x * ny + y) * nz + zz;
     float qc = 0.0f;
     if (c.p < 0.0f) {
@@ -404,15 +414,19 @@ struct OpResidualW {
     if (inside && wf(w.id) > 0.0f) c.e = Store<T>::gld(e, ((size_t)x * ny + y) * nz + zz);
     return c;
   }
+  using Own = TR;  // right-hand side of the lane's cell
+  __device__ __forceinline__ Own own(int x, int y, int zz, bool inside) const {
+    return inside ? ldg(r, ((size_t)x * ny + y) * nz + zz) : to_store<TR>(0.0f);
+  }
   __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool*,
-                                        double*, float*) const {
+                                        const Own& rv, double*, float*) const {
     const size_t i = ((size_t)x * ny + y) * nz + zz;
     float res = 0.0f;
     if (wf(c.w.id) > 0.0f) {
       const float ap = wf(c.w.wx) * (c.e - nb[1].e) + wf(nb[0].w.wx) * (c.e - nb[0].e) +
                        wf(c.w.wy) * (c.e - nb[3].e) + wf(nb[2].w.wy) * (c.e - nb[2].e) +
                        wf(c.w.wz) * (c.e - nb[5].e) + wf(nb[4].w.wz) * (c.e - nb[4].e);
-      res = (Store<TR>::gld(r, i) * rm - emul * ap) * tinv;
+      res = (Store<TR>::ld(&rv, 0) * rm - emul * ap) * tinv;
     }
     Store<T>::gst(t, i, res);
   }
@@ -457,8 +471,12 @@ struct OpTrueResidual {
     if (inside && p < 0.0f) c.x = ldg(xv, ((size_t)x * ny + y) * nz + zz);
     return c;
   }
+  using Own = double;  // b of the lane's cell
+  __device__ __forceinline__ Own own(int x, int y, int zz, bool inside) const {
+    return inside ? ldg(b, ((size_t)x * ny + y) * nz + zz) : 0.0;
+  }
   __device__ __forceinline__ void apply(int x, int y, int zz, const Cell& c, const Cell* nb, const bool* has,
-                                        double* acc, float* mx) const {
+                                        const Own& bv, double* acc, float* mx) const {
     const size_t i = ((size_t)x * ny + y) * nz + zz;
     double res = 0.0;
     if (c.p < 0.0f) {
@@ -466,7 +484,7 @@ struct OpTrueResidual {
 #pragma unroll
       for (int f = 0; f < 6; ++f)
         if (has[f]) ap += face_weight_d(c.p, nb[f].p) * (c.x - nb[f].x);
-      res = ldg(b, i) - ap;
+      res = bv - ap;
       acc[0] += res * res;
       *mx = fmaxf(*mx, (float)fabs(res));
     }
