@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 577.3,
-    ("tall", 512, "fp32", "smooth_fine", 2): 481.3,
+    ("uniform", 384, "fp16", "smooth_fine", 2): 575.5,
+    ("tall", 512, "fp32", "smooth_fine", 2): 454.8,
 }
 
 
