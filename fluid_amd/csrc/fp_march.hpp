@@ -9,14 +9,14 @@
 //
 // An operation `Op` supplies
 //   using Cell = ...;                                   LDS record per cell
-//   using Head = ...;                                   what decides the cell's other loads
+//   using Raw = ...;                                    a cell's loads as fetched
 //   static Op make(const GridRec&, const MarchLaunch&); binds the grid's arrays
 //   void prepare();                                     reads device scalars once per workgroup
-//   Head head(int x, int y, int z, bool inside) const;  first loads (inside: in the domain)
-//   Cell load(int x, int y, int z, bool inside, const Head&) const;
-//                                                       the loads that depend on the head
-// Heads are fetched two planes ahead and the dependent loads one plane ahead,
-// so neither waits for the other within a plane step.
+//   Raw fetch(int x, int y, int z, bool inside) const;  every load of a cell (inside: in the
+//                                                       domain), none depending on another
+//   Cell finish(const Raw&) const;                      masks and scaling, at the LDS commit
+// A plane's loads are issued one plane step ahead, all at once, and masked
+// only when they are committed, so no load waits for another.
 //   using Own = ...;                                    the lane's own-cell loads (not staged)
 //   Own own(int x, int y, int z, bool inside) const;    fetched one plane step ahead
 //   void apply(int x, int y, int z, const Cell& c, const Cell* nb, const bool* has,
@@ -88,9 +88,8 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
   const bool mine = gy < ny && gz < nz;
   Own own_cur = Own(), own_next = Own();
   constexpr int NC = (RY * RZ + 255) / 256;
-  using Head = typename Op::Head;
-  Head hd[NC];
-  Cell pre[NC];
+  using Raw = typename Op::Raw;
+  Raw pre[NC];
   auto where = [&](int L, int k, int& yy, int& zz) {
     const int q = tid + 256 * k;
     const int ry = q / RZ, rz = q - ry * RZ;
@@ -98,27 +97,15 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
     zz = z0 + rz - 1;
     return q < RY * RZ && L >= 0 && L < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
   };
-  auto fetch_head = [&](int L) {
+  auto fetch = [&](int L) {
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       int yy, zz;
       const bool inside = where(L, k, yy, zz);
-      hd[k] = op.head(L, yy, zz, inside);
+      pre[k] = op.fetch(L, yy, zz, inside);
     }
   };
-  auto fetch_cell = [&](int L) {
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      int yy, zz;
-      const bool inside = where(L, k, yy, zz);
-      pre[k] = op.load(L, yy, zz, inside, hd[k]);
-    }
-  };
-  if (x0 < x1) {
-    fetch_head(x0 - 1);
-    fetch_cell(x0 - 1);
-    fetch_head(x0);
-  }
+  if (x0 < x1) fetch(x0 - 1);
   for (int L = x0 - 1; x0 < x1 && L <= x1; ++L) {
     const int slot = (L + 3) % 3;
 #pragma unroll
@@ -126,12 +113,11 @@ __global__ __launch_bounds__(256) void k_march(const Part* parts, int np, int ho
       const int q = tid + 256 * k;
       if (q < RY * RZ) {
         const int ry = q / RZ, rz = q - ry * RZ;
-        s[slot][ry][rz] = pre[k];
+        s[slot][ry][rz] = op.finish(pre[k]);
       }
     }
     lds_barrier();
-    if (L + 1 <= x1) fetch_cell(L + 1);
-    if (L + 2 <= x1) fetch_head(L + 2);
+    if (L + 1 <= x1) fetch(L + 1);
     if (L >= x0 && L < x1) own_next = op.own(L, gy, gz, mine);
     const int B = L - 1;
     if (B >= x0 && B < x1 && mine) {
@@ -325,17 +311,19 @@ struct OpDirection {
     zm = zmul / cload(&st->rscale);
     beta = (float)cload(&st->beta);
   }
-  using Head = float;  // phi (air outside the domain)
-  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
-    return inside ? ldg(phi, ((size_t)x * ny + y) * nz + zz) : 1.0f;
-  }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, Head p) const {
-    Cell c{0.0f, p};
-    if (inside && p < 0.0f) {
+  struct Raw { float p; T z; float d; };  // phi (air outside the domain), z, d_old
+  __device__ __forceinline__ Raw fetch(int x, int y, int zz, bool inside) const {
+    Raw w{1.0f, to_store<T>(0.0f), 0.0f};
+    if (inside) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
-      c.d = Store<T>::gld(z, i) * zm + beta * ldg(dold, i);
+      w.p = ldg(phi, i);
+      w.z = ldg(z, i);
+      w.d = ldg(dold, i);
     }
-    return c;
+    return w;
+  }
+  __device__ __forceinline__ Cell finish(const Raw& w) const {
+    return Cell{w.p < 0.0f ? Store<T>::ld(&w.z, 0) * zm + beta * w.d : 0.0f, w.p};
   }
   struct Own {};
   __device__ __forceinline__ Own own(int, int, int, bool) const { return Own(); }
@@ -397,21 +385,25 @@ struct OpResidualW {
     return o;
   }
   __device__ __forceinline__ void prepare() { rm = dyn ? rmul * cload(&st->rscale) : rmul; }
-  using Head = W4<T>;  // coefficients (zero outside the domain)
-  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
-    W4<T> w;
-    Store<T>::st(&w.wx, 0, 0.0f);
-    Store<T>::st(&w.wy, 0, 0.0f);
-    Store<T>::st(&w.wz, 0, 0.0f);
-    Store<T>::st(&w.id, 0, 0.0f);
-    if (inside) w = ldg(wts, ((size_t)x * ny + y) * nz + zz);
-    return w;
+  struct Raw { W4<T> w; T e; };  // coefficients (zero outside the domain), correction
+  __device__ __forceinline__ Raw fetch(int x, int y, int zz, bool inside) const {
+    Raw r;
+    Store<T>::st(&r.w.wx, 0, 0.0f);
+    Store<T>::st(&r.w.wy, 0, 0.0f);
+    Store<T>::st(&r.w.wz, 0, 0.0f);
+    Store<T>::st(&r.w.id, 0, 0.0f);
+    r.e = to_store<T>(0.0f);
+    if (inside) {
+      const size_t i = ((size_t)x * ny + y) * nz + zz;
+      r.w = ldg(wts, i);
+      r.e = ldg(e, i);
+    }
+    return r;
   }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, const Head& w) const {
+  __device__ __forceinline__ Cell finish(const Raw& r) const {
     Cell c;
-    c.w = w;
-    c.e = 0.0f;
-    if (inside && wf(w.id) > 0.0f) c.e = Store<T>::gld(e, ((size_t)x * ny + y) * nz + zz);
+    c.w = r.w;
+    c.e = wf(r.w.id) > 0.0f ? Store<T>::ld(&r.e, 0) : 0.0f;
     return c;
   }
   using Own = TR;  // right-hand side of the lane's cell
@@ -462,15 +454,18 @@ struct OpTrueResidual {
     return o;
   }
   __device__ __forceinline__ void prepare() {}
-  using Head = float;  // phi (air outside the domain)
-  __device__ __forceinline__ Head head(int x, int y, int zz, bool inside) const {
-    return inside ? ldg(phi, ((size_t)x * ny + y) * nz + zz) : 1.0f;
-  }
-  __device__ __forceinline__ Cell load(int x, int y, int zz, bool inside, Head p) const {
-    Cell c{0.0, p};
-    if (inside && p < 0.0f) c.x = ldg(xv, ((size_t)x * ny + y) * nz + zz);
+  using Raw = Cell;  // phi (air outside the domain) and x as loaded
+  __device__ __forceinline__ Raw fetch(int x, int y, int zz, bool inside) const {
+    Raw c{0.0, 1.0f};
+    if (inside) {
+      const size_t i = ((size_t)x * ny + y) * nz + zz;
+      c.p = ldg(phi, i);
+      c.x = ldg(xv, i);
+    }
     return c;
   }
+  // x of an air cell (stale after the liquid moved) is the zero ghost value
+  __device__ __forceinline__ Cell finish(const Raw& c) const { return Cell{c.p < 0.0f ? c.x : 0.0, c.p}; }
   using Own = double;  // b of the lane's cell
   __device__ __forceinline__ Own own(int x, int y, int zz, bool inside) const {
     return inside ? ldg(b, ((size_t)x * ny + y) * nz + zz) : 0.0;
