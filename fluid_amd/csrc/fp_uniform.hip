@@ -574,6 +574,10 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
   fetch(p0);
   int cur = X0;
   float acc0 = 0.0f, acc1 = 0.0f;
+  // level set of the coarse cell being accumulated, loaded when its plane
+  // starts rather than when it is written
+  auto coarse_phi = [&](int c) { return (mine && c < X1) ? ldg(phic, (size_t)((long long)c * cplane) + cidx) : 1.0f; };
+  float phc = coarse_phi(cur);
   for (int p = p0; p <= p1; ++p) {
     T* sl = s[p & 1];
 #pragma unroll
@@ -607,11 +611,12 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
     while (cur < X1 && p >= (sx == 2 ? min(2 * cur + 2, fnx - 1) : cur)) {
       if (mine) {
         const size_t o = (size_t)((long long)cur * cplane) + cidx;
-        Store<T>::gst(rc, o, ldg(phic, o) < 0.0f ? acc0 * fac : 0.0f);
+        Store<T>::gst(rc, o, phc < 0.0f ? acc0 * fac : 0.0f);
       }
       acc0 = acc1;
       acc1 = 0.0f;
       ++cur;
+      phc = coarse_phi(cur);
     }
     // no second barrier: the next plane goes to the other slot, whose last
     // reads came before this plane's barrier
