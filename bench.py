@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 575.5,
+    ("uniform", 384, "fp16", "smooth_fine", 2): 580.2,
     ("tall", 512, "fp32", "smooth_fine", 2): 454.8,
 }
 
@@ -39,8 +39,9 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
     over the launches one V-cycle times (per side `sweeps`): each reads the
     packed face coefficients (4 values of the storage type, 4s B) and the FP32
     right-hand side (4 B) and writes the new correction (s B); all but the
-    first pre-smoothing sweep (zero guess) read the old correction (s B); the
-    first post-smoothing sweep also reads the coarse correction (s/8 B).
+    first pre-smoothing sweep (zero guess) read the old correction (s B).  The
+    coarse correction is added by its own pass before the post-smoothing
+    sweeps.
     residual_fine: read coefficients (4s), r (4), e (s); write t (s).
     cg_direction: read z (s), d_old (4), phi (4); write d (4), q (4).
     cg_update: read d, q, e, r (16); write e, r (8).
@@ -49,7 +50,7 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
     """
     s = 4 if precision == "fp32" else 2
     plain = 4 * s + 4 + 2 * s
-    smooth = (2 * sweeps * plain - s + s / 8) / (2 * sweeps) if sweeps > 0 else plain
+    smooth = (2 * sweeps * plain - s) / (2 * sweeps) if sweeps > 0 else plain
     return {
         "smooth_fine": smooth,
         "cg_direction": s + 4 + 4 + 4 + 4,

@@ -11,23 +11,14 @@
 // different buffers).  One read of the coefficients, r and e and one write of
 // e per cell replace the two passes of separate colour kernels.
 //
-// Variants (template flags):
+// Variant (template flag):
 //   ZERO     the sweep starts from e = 0 (first pre-smoothing sweep; e_in unread)
-//   PROLONG  e_in is corrected on load by the trilinear prolongation of the
-//            coarse correction (first post-smoothing sweep)
 #pragma once
 
 #include "fp_common.hpp"
 #include "fp_uniform.hpp"
 
 namespace fp {
-
-struct ProlongArgs {
-  const void* ec;      // coarse correction (storage T; zero on coarse air cells), or nullptr
-  int cnx, cny, cnz;
-  int sx, sy, sz;      // strides fine/coarse per axis (1 or 2)
-  float fac;           // Se_c / Se_f
-};
 
 // Packed per-cell coefficients of one level: the three positive-face
 // weights (h^2-scaled, 0 on walls and between air cells) and the inverse
@@ -139,37 +130,14 @@ __device__ __forceinline__ void gst_pair(T* p, size_t i, float a, float b) {
   stg(reinterpret_cast<Two*>(p), i >> 1, Two{to_store<T>(a), to_store<T>(b)});
 }
 
-// Taps of the cell-centred linear prolongation along one axis, as positions
-// in a staged coarse window starting at coarse index c0: fine f of stride s
-// reads coarse f>>1 with 3/4 and its neighbour on f's side with 1/4, or
-// coarse f>>1 alone at the domain edge; s = 1 copies.
-__device__ __forceinline__ void window_taps(int f, int s, int nc, int c0, int& a, float& wa, int& b, float& wb) {
-  if (s == 1) {
-    a = f - c0;
-    wa = 1.0f;
-    b = a;
-    wb = 0.0f;
-    return;
-  }
-  const int I = f >> 1;
-  const int J = (f & 1) ? I + 1 : I - 1;
-  const bool two = (J >= 0 && J < nc);
-  a = I - c0;
-  wa = two ? 0.75f : 1.0f;
-  b = two ? J - c0 : a;
-  wb = two ? 0.25f : 0.0f;
-}
-
 // One red-black sweep.  TZ x TY = 256 lanes; TR: storage of the right-hand
 // side, T: of e, of the packed coefficients and of the LDS staging.
 //
-// LDS per plane slot: coefficients on rows/columns -2..T of the tile (-2..T+1
-// with PROLONG), the old correction on -2..T+1, the scaled right-hand side and
-// the first-colour values on -1..T.  Four slots of coefficients, r and
-// first-colour values, three of the old correction.  Per plane step L:
+// LDS per plane slot: coefficients on rows/columns -2..T of the tile, the old
+// correction on -2..T+1, the scaled right-hand side and the first-colour
+// values on -1..T.  Four slots of coefficients, r and first-colour values,
+// three of the old correction.  Per plane step L:
 //   commit plane L | A | prefetch L+D, colour 1 at R = L-1 | B | colour 2 at L-2
-// (PROLONG adds the coarse correction to plane L's second-colour liquid cells
-// between A and a third barrier).
 //
 // The plane loop is unrolled by 4 from x0-2, so the four-slot indices and the
 // register set of a plane (D sets of raw prefetched bits, D | 4) are
@@ -179,7 +147,7 @@ __device__ __forceinline__ void window_taps(int f, int s, int nc, int c0, int& a
 // loaded at all (the commit stages zeros for them).  Colour-1 lanes cover the
 // first-colour cells of the ring-extended tile (CY rows of CZ/2), colour-2
 // lanes the second-colour cells of the tile (TY rows of TZ/2).
-template <int TZ, int D, typename TR, typename T, bool PROLONG, bool ZERO>
+template <int TZ, int D, typename TR, typename T, bool ZERO>
 __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int level, int which_in, int first) {
   static_assert(D == 1 || D == 2 || D == 4, "the prefetch depth must divide the unroll of 4");
   FP_PART(parts, np);
@@ -193,29 +161,20 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const int dyn = top;
   const T* __restrict__ ein = (const T*)(which_in ? lvl.e2 : lvl.e);
   T* __restrict__ eout = (T*)(which_in ? lvl.e : lvl.e2);
-  ProlongArgs P{nullptr, 0, 0, 0, 1, 1, 1, 1.0f};
-  if constexpr (PROLONG) {
-    const LevelRec c = cload(&Gp->lv[level + 1]);
-    P = ProlongArgs{c.efinal ? c.e2 : c.e, c.d[0], c.d[1], c.d[2], c.s[0], c.s[1], c.s[2], c.Se / lvl.Se};
-  }
   const int chunk = lvl.sw.chunk, tiles_z = lvl.sw.tz, tiles_y = lvl.sw.ty;
   constexpr int TY = 256 / TZ;
   constexpr int EY = TY + 4, EZ = TZ + 4;    // old correction: -2 .. T+1
-  constexpr int WY = PROLONG ? EY : TY + 3;  // coefficients: -2 .. T (T+1)
-  constexpr int WZ = PROLONG ? EZ : TZ + 3;
+  constexpr int WY = TY + 3, WZ = TZ + 3;    // coefficients: -2 .. T
   constexpr int CY = TY + 2, CZ = TZ + 2;    // r, first-colour values: -1 .. T
   constexpr int WN = WY * WZ, EN = EY * EZ, CN = CY * CZ;
-  constexpr int PN = PROLONG ? 2 * EN : 0;   // two coarse plane slots, window <= EY x EZ
   constexpr int NW = (WN + 255) / 256, NE = ZERO ? 0 : (EN + 255) / 256, NR = (CN + 255) / 256;
-  constexpr int NP = PROLONG ? (EN + 255) / 256 : 0;  // one coarse plane per staging
   constexpr int HF = CZ / 2, HS = TZ / 2;
-  static_assert(CY * HF <= 256 && TY * HS <= 256 && EY * (EZ / 2) <= 256, "lane maps exceed the workgroup");
+  static_assert(CY * HF <= 256 && TY * HS <= 256, "lane maps exceed the workgroup");
   using WB = typename W4Bits<T>::type;
   __shared__ WB s_w[4][WN];
   __shared__ T s_e[ZERO ? 1 : 3 * EN];
   __shared__ T s_r[4][CN];
   __shared__ T s_c[4][CN];
-  __shared__ T s_p[PROLONG ? PN : 1];
 
   const float rm = dyn ? rmul * cload(&st->rscale) : rmul;
   const int nx = lvl.d[0], ny = lvl.d[1], nz = lvl.d[2];
@@ -277,79 +236,13 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
     const int yy = y0 + q / CZ - 1, zz = z0 + q % CZ - 1;
     orr[k] = (q < CN && yy >= 0 && yy < ny && zz >= 0 && zz < nz) ? yy * nz + zz : -1;
   }
-  // coarse window (PROLONG): rows/columns from cy0/cz0, CWY x CWZ of them
-  // (TY/2 + 4 x TZ/2 + 4 on an axis of stride 2, EY / EZ on one of stride 1),
-  // stored at row pitch EZ.  Coarse plane c lives in slot c & 1.  Fine plane L
-  // interpolates coarse planes (L-1)>>1 and the next one (stride 2; fine
-  // planes 2c+1 and 2c+2 share the pair c, c+1) or plane L (stride 1), so a
-  // step stages at most one new coarse plane (cplane_new) and the other is
-  // already in its slot.
-  const int cy0 = PROLONG ? (P.sy == 2 ? (y0 >> 1) - 2 : y0 - 2) : 0;
-  const int cz0 = PROLONG ? (P.sz == 2 ? (z0 >> 1) - 2 : z0 - 2) : 0;
-  const long long cplane = PROLONG ? (long long)P.cny * P.cnz : 0;
-  // window entry k of this lane: its LDS position (-1: none) and its offset
-  // in a coarse plane (-1: outside the coarse grid); recomputed where used,
-  // which is cheaper than holding it in registers for the whole march
-  auto coarse_entry = [&](int k, int& lq) {
-    const int q = tid + 256 * k;
-    int row, col, cwn;
-    if (P.sz == 2) {
-      row = q / (TZ / 2 + 4);
-      col = q - row * (TZ / 2 + 4);
-    } else {
-      row = q / EZ;
-      col = q - row * EZ;
-    }
-    cwn = (P.sy == 2 ? TY / 2 + 4 : EY) * (P.sz == 2 ? TZ / 2 + 4 : EZ);
-    const int yy = cy0 + row, zz = cz0 + col;
-    lq = q < cwn ? row * EZ + col : -1;
-    return (lq >= 0 && yy >= 0 && yy < P.cny && zz >= 0 && zz < P.cnz) ? yy * P.cnz + zz : -1;
-  };
-  constexpr int kNoPlane = -0x40000000;
-  auto cplane_new = [&](int L) {
-    if (P.sx != 2) return L;
-    return ((L & 1) || L == x0 - 2) ? ((L - 1) >> 1) + 1 : kNoPlane;
-  };
-  if constexpr (PROLONG) {
-    // the first step's pair also needs its lower plane: staged here (the
-    // first step's barrier orders it before any use)
-    if (P.sx == 2) {
-      const int c = (x0 - 3) >> 1;
-      const T* ec = (const T*)P.ec;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        int lq;
-        const int o = coarse_entry(k, lq);
-        if (lq >= 0) {
-          const bool ok = c >= 0 && c < P.cnx && o >= 0;
-          s_p[(c & 1) * EN + lq] = ok ? ldg(ec, (size_t)((long long)c * cplane + o)) : value_unless(T(), false);
-        }
-      }
-    }
-  }
-
   // D register sets of raw prefetched planes
   WB pw[D][NW];
   T pe[D][NE > 0 ? NE : 1];
   TR pr[D][NR];
-  T pp[D][NP > 0 ? NP : 1];
   int lf = x0 - 2;                 // next plane to fetch
   long long fo = (long long)lf * plane;
   auto fetch = [&](int d) {
-    if constexpr (PROLONG) {
-      // coarse planes are staged also for fine planes outside the grid: the
-      // later steps of the pair read them
-      const int c = cplane_new(lf);
-      if (c >= 0 && c < P.cnx) {
-        const T* ec = (const T*)P.ec;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          int lq;
-          const int o = coarse_entry(k, lq);
-          pp[d][k] = ldg(ec, (size_t)((long long)c * cplane + max(o, 0)));
-        }
-      }
-    }
     if (lf >= 0 && lf < nx) {
       const W4<T>* wp = wts + fo;
       const TR* rp = r + fo;
@@ -394,15 +287,6 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
     c2st[h] = c2 && c2gy < ny && z0 + c2lz[h] < nz;
   }
 
-  // prolongation lane (PROLONG): a second-colour cell of the staged window
-  // (rows/columns -2..T+1, EZ/2 per row); its column alternates with the plane
-  // parity.  Its taps into the coarse window are evaluated per plane (a few
-  // integer operations) rather than held across the march.
-  constexpr int PH = EZ / 2;
-  const int ply = tid / PH, plk = tid - (tid / PH) * PH;
-  const int pgy = y0 + ply - 2;
-  const bool prow = PROLONG && tid < EY * PH && pgy >= 0 && pgy < ny;
-
 #pragma unroll
   for (int d = 0; d < D; ++d) fetch(d);
 
@@ -440,46 +324,8 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       const float v = Store<TR>::ld(&pr[rs][k], 0) * rm;
       if (k + 1 < NR || q < CN) s_r[s0][q] = to_store<T>((pin && orr[k] >= 0) ? v : 0.0f);
     }
-    if constexpr (PROLONG) {
-      const int c = cplane_new(L);
-      if (c != kNoPlane) {
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          int lq;
-          const int o = coarse_entry(k, lq);
-          if (lq >= 0) s_p[(c & 1) * EN + lq] = value_unless(pp[rs][k], c >= 0 && c < P.cnx && o >= 0);
-        }
-      }
-    }
     lds_barrier();  // A
     if (L + D <= x1 + 1) fetch(rs);
-
-    if constexpr (PROLONG) {
-      // e(L) += fac * P ec at second-colour liquid cells (coarse air holds e = 0)
-      const int ez = 2 * plk + ((second + L + pgy) & 1);
-      const int pgz = z0 + ez - 2;
-      if (pin && prow && pgz >= 0 && pgz < nz) {
-        int xa, xb, ya, yb, za, zb;
-        float wxa, wxb, pwya, pwyb, wza, wzb;
-        window_taps(L, P.sx, P.cnx, 0, xa, wxa, xb, wxb);
-        const int q = ply * EZ + ez;
-        const W4<T> wc = *reinterpret_cast<const W4<T>*>(&s_w[s0][q]);
-        if (wf(wc.id) > 0.0f) {
-          window_taps(pgy, P.sy, P.cny, cy0, ya, pwya, yb, pwyb);
-          window_taps(pgz, P.sz, P.cnz, cz0, za, wza, zb, wzb);
-          const T* pa_ = s_p + (xa & 1) * EN;
-          const T* pb_ = s_p + (xb & 1) * EN;
-          const T v0 = pa_[ya * EZ + za], v1 = pa_[ya * EZ + zb], v2 = pa_[yb * EZ + za], v3 = pa_[yb * EZ + zb];
-          const T v4 = pb_[ya * EZ + za], v5 = pb_[ya * EZ + zb], v6 = pb_[yb * EZ + za], v7 = pb_[yb * EZ + zb];
-          const T eq = s_e[e0 + q];
-          __builtin_amdgcn_sched_barrier(0);
-          const float acc = wxa * (pwya * (wza * wf(v0) + wzb * wf(v1)) + pwyb * (wza * wf(v2) + wzb * wf(v3))) +
-                            wxb * (pwya * (wza * wf(v4) + wzb * wf(v5)) + pwyb * (wza * wf(v6) + wzb * wf(v7)));
-          s_e[e0 + q] = to_store<T>(wf(eq) + P.fac * acc);
-        }
-      }
-      lds_barrier();  // A2
-    }
 
     const int R = L - 1, B = L - 2;
     float wxm = 0.0f;

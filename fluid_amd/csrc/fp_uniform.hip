@@ -655,6 +655,79 @@ __device__ __forceinline__ void prolong_cell(const GridRec& G, int level, int wh
   stT(ef, idx, ldT(ef, idx) + fac * acc);
 }
 
+// e_l += P e_{l+1} on a fused level's liquid cells, over the liquid hulls of
+// its march tiles (the other planes are air and keep e = 0): each lane walks its
+// cell column along x, and every load of four planes (level set, correction,
+// eight coarse taps each, coarse air holding e = 0) is issued before the
+// arithmetic.  which: the fine buffer, updated in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_march(const Part* parts, int np, int level, int which) {
+  FP_PART(parts, np);
+  if (cload(&cload(&Gp->st)->done)) return;
+  const LevelRec fl = cload(&Gp->lv[level]);
+  const LevelRec cl = cload(&Gp->lv[level + 1]);
+  const TileSpan sp = tile_span(fl, blk, nblk);
+  if (!sp.inside || sp.x0 >= sp.x1) return;
+  T* __restrict__ ef = (T*)(which ? fl.e2 : fl.e);
+  const T* __restrict__ ec = (const T*)(cl.efinal ? cl.e2 : cl.e);
+  const float* __restrict__ phi = fl.phi;
+  const float fac = cl.Se / fl.Se;
+  const int nz = fl.d[2];
+  const int gy = (int)(sp.col / (size_t)nz), gz = (int)(sp.col - (size_t)gy * nz);
+  const int cny = cl.d[1], cnz = cl.d[2];
+  // two taps per axis (a missing second tap repeats the first at weight 0)
+  auto taps2 = [](int f, int s_, int nc, int* c, float* w) {
+    const int n = ptaps(f, s_, nc, c, w);
+    if (n == 1) {
+      c[1] = c[0];
+      w[1] = 0.0f;
+    }
+  };
+  int cy[2], cz[2];
+  float wy[2], wz[2];
+  taps2(gy, cl.s[1], cny, cy, wy);
+  taps2(gz, cl.s[2], cnz, cz, wz);
+  const size_t fplane = (size_t)fl.d[1] * nz, cplane = (size_t)cny * cnz;
+  const size_t o00 = (size_t)cy[0] * cnz + cz[0], o01 = (size_t)cy[0] * cnz + cz[1];
+  const size_t o10 = (size_t)cy[1] * cnz + cz[0], o11 = (size_t)cy[1] * cnz + cz[1];
+  constexpr int U = 4;
+  for (int x = sp.x0; x < sp.x1; x += U) {
+    float ph[U], ev[U], acc[U];
+    T cv[U][8];
+    float wx[U][2];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int xx = min(x + u, sp.x1 - 1);
+      const size_t i = (size_t)xx * fplane + sp.col;
+      ph[u] = ldg(phi, i);
+      ev[u] = Store<T>::gld(ef, i);
+      int cx[2];
+      taps2(xx, cl.s[0], cl.d[0], cx, wx[u]);
+      const T* c0 = ec + (size_t)cx[0] * cplane;
+      const T* c1 = ec + (size_t)cx[1] * cplane;
+      cv[u][0] = ldg(c0, o00);
+      cv[u][1] = ldg(c0, o01);
+      cv[u][2] = ldg(c0, o10);
+      cv[u][3] = ldg(c0, o11);
+      cv[u][4] = ldg(c1, o00);
+      cv[u][5] = ldg(c1, o01);
+      cv[u][6] = ldg(c1, o10);
+      cv[u][7] = ldg(c1, o11);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float a0 = wy[0] * (wz[0] * wf(cv[u][0]) + wz[1] * wf(cv[u][1])) +
+                       wy[1] * (wz[0] * wf(cv[u][2]) + wz[1] * wf(cv[u][3]));
+      const float a1 = wy[0] * (wz[0] * wf(cv[u][4]) + wz[1] * wf(cv[u][5])) +
+                       wy[1] * (wz[0] * wf(cv[u][6]) + wz[1] * wf(cv[u][7]));
+      acc[u] = wx[u][0] * a0 + wx[u][1] * a1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (x + u < sp.x1 && ph[u] < 0.0f) Store<T>::gst(ef, (size_t)(x + u) * fplane + sp.col, ev[u] + fac * acc[u]);
+  }
+}
+
 template <typename T>
 __global__ void k_prolong_add(const Part* parts, int np, int level, int which) {
   FP_PART(parts, np);
@@ -1514,16 +1587,13 @@ void Projector::timer_end(hipStream_t s) {
 
 template <typename TR, typename T>
 static void sweep_launch(hipStream_t s, const PartTable& t32, const PartTable& t16, int level, int which_in, int first,
-                         bool zero, bool prolong) {
+                         bool zero) {
   if (zero) {
-    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, false, true>), t32, s, level, which_in, first);
-    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, false, true>), t16, s, level, which_in, first);
-  } else if (prolong) {
-    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, true, false>), t32, s, level, which_in, first);
-    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, true, false>), t16, s, level, which_in, first);
+    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, true>), t32, s, level, which_in, first);
+    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, true>), t16, s, level, which_in, first);
   } else {
-    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, false, false>), t32, s, level, which_in, first);
-    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, false, false>), t16, s, level, which_in, first);
+    PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, false>), t32, s, level, which_in, first);
+    PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, false>), t16, s, level, which_in, first);
   }
 }
 
@@ -1540,8 +1610,8 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
     }
     for (int sw = 0; sw < pre; ++sw) {
       TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, false)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, false)));
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0)));
       for (int colour = 0; colour < 2; ++colour) {
         const int zi = (sw == 0 && colour == 0) ? 1 : 0;
         if (top) PLAUNCH((k_smooth<float, T>), T_.smooth, s, l, colour, zi);
@@ -1569,14 +1639,16 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
   for (int l = p.maxL - 2; l >= 0; --l) {
     Plan::Level& T_ = p.lv[l];
     const bool top = l == 0;
-    // ---- fused levels: prolongation inside the first post-sweep ----
+    // ---- fused levels: prolongation over the liquid hulls, then the sweeps ----
     if (post == 0) {
       PLAUNCH(k_prolong_add<T>, T_.fusedcells, s, l, pre & 1);
+    } else {
+      TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_march<T>, T_.res, s, l, pre & 1));
     }
     for (int sw = 0; sw < post; ++sw) {
       TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, sw == 0)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, sw == 0)));
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false)));
     }
     // ---- unfused levels ----
     TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_add<T>, T_.smooth, s, l, 0));
