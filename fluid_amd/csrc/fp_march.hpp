@@ -374,7 +374,8 @@ struct OpResidualW {
   static constexpr int DEPTH = 1;  // 143 us against 165 at depth 2
   static constexpr int NV = 0;
   static constexpr bool WITH_MAX = false;
-  static constexpr bool ZERO_OUTSIDE = true;  // the restriction reads t everywhere
+  // the restriction reads t only inside the fine hulls (zero elsewhere)
+  static constexpr bool ZERO_OUTSIDE = false;
   int ny, nz;
   const W4<T>* wts;
   const TR* r;

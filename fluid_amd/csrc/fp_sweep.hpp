@@ -57,8 +57,9 @@ __global__ void k_build_weights(int nx, int ny, int nz, float gx, float gy, floa
   }
 }
 
-// Tile width along z of the levels' liquid hulls (the 32 x 8 march tiles).
-constexpr int kHullTZ = 32;
+// Tile width along z and height along y of the levels' liquid hulls (the
+// 32 x 8 march tiles).
+constexpr int kHullTZ = 32, kHullTY = 256 / kHullTZ;
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the device, each
 // with its own L2: group g runs on XCD g % 8.  Logical tiles run z fastest,

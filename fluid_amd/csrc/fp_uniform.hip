@@ -529,7 +529,10 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
   // staged fine window: rows fy0 .. fy0+ry-1, columns fz0 .. fz0+rz-1
   const int fy0 = sy == 2 ? 2 * cy0 - 1 : cy0, fz0 = sz == 2 ? 2 * cz0 - 1 : cz0;
   const int ry = sy == 2 ? RY : TY, rz = sz == 2 ? RZ : TZ;
-  int off[NC], lds[NC];
+  // off: the staged element's offset in a fine plane (-1: outside the
+  // grid); [hlo, hhi]: the planes of the liquid hull of its fine march tile
+  // (the residual march leaves t unwritten outside it, where it is zero)
+  int off[NC], lds[NC], hlo[NC], hhi[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
     const int q = tid + 256 * k;
@@ -537,6 +540,14 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
     const int yy = fy0 + a, zz = fz0 + b;
     lds[k] = q < ry * rz ? a * RZ + b : -1;
     off[k] = (q < ry * rz && yy >= 0 && yy < fny && zz >= 0 && zz < fnz) ? yy * fnz + zz : -1;
+    hlo[k] = 0;
+    hhi[k] = fnx - 1;
+    if (fl.hull && off[k] >= 0) {
+      // per-lane values: vector loads (cload would make them wave-uniform)
+      const size_t h = 2 * (size_t)((yy / kHullTY) * fl.mg.tz + zz / kHullTZ);
+      hlo[k] = ldg((const int*)fl.hull, h);
+      hhi[k] = ldg((const int*)fl.hull, h + 1);
+    }
   }
   // this lane's coarse column and its 4 + 4 in-plane taps (local rows /
   // columns of the window; absent taps weigh 0)
@@ -569,7 +580,8 @@ __global__ __launch_bounds__(256) void k_restrict_march(const Part* parts, int n
   auto fetch = [&](int p) {
     const T* src = tf + (long long)p * fplane;
 #pragma unroll
-    for (int k = 0; k < NC; ++k) pre[k] = off[k] >= 0 ? ldg(src, (size_t)off[k]) : to_store<T>(0.0f);
+    for (int k = 0; k < NC; ++k)
+      pre[k] = (off[k] >= 0 && p >= hlo[k] && p <= hhi[k]) ? ldg(src, (size_t)off[k]) : to_store<T>(0.0f);
   };
   fetch(p0);
   int cur = X0;
