@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 580.2,
+    ("uniform", 384, "fp16", "smooth_fine", 2): 530.7,
     ("tall", 512, "fp32", "smooth_fine", 2): 454.8,
 }
 
@@ -45,8 +45,8 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
     residual_fine: read coefficients (4s), r (4), e (s); write t (s).
     cg_direction: read z (s), d_old (4), phi (4); write d (4), q (4).
     cg_update: read d, q, e, r (16); write e, r (8).
-    Per cell of the liquid hulls (hull_cells); the launch's algorithmic bytes
-    add s per remaining cell for the passes that store zeros there.
+    Per cell of the liquid hulls (hull_cells): the finest level's passes
+    neither read nor write the other cells.
     """
     s = 4 if precision == "fp32" else 2
     plain = 4 * s + 4 + 2 * s
@@ -63,8 +63,7 @@ def hull_cells(phi, tz: int = 32, ty: int = 8) -> int:
     """Cells of the finest level inside the liquid hulls of its 32 x 8 (z x y)
     column tiles: per tile, the planes from the first to the last x plane in
     which any of the tile's cells is liquid.  The hull-clamped passes (sweeps,
-    residual, CG direction and update) read only these cells; elsewhere a
-    sweep stores zeros."""
+    residual, CG direction and update) read and write only these cells."""
     import numpy as np
     liq = np.asarray(phi) < 0
     nx, ny, nz = liq.shape
@@ -451,11 +450,9 @@ def main():
     kl = sum(x["timed_kernel_launches"] for x in reports)
     avg_ms = kms / max(kl, 1)
     bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps)
-    # the timed passes touch the cells of the liquid hulls; a sweep stores a
-    # zero correction (s bytes) on every other cell
+    # the timed finest-level passes touch only the cells of the liquid hulls
     active = hull_cells(s.phi)
-    zero_b = 0.0 if args.timed_kernel in ("cg_direction", "cg_update") else (4 if args.precision == "fp32" else 2)
-    alg_bytes = bpc * active + zero_b * (cells - active)
+    alg_bytes = bpc * active
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # error against a tight FP64-refined solve of the same system (1e-12)
     err = None

@@ -165,7 +165,8 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const int chunk = lvl.sw.chunk, tiles_z = lvl.sw.tz, tiles_y = lvl.sw.ty;
   constexpr int TY = 256 / TZ;
   constexpr int EY = TY + 4, EZ = TZ + 4;    // old correction: -2 .. T+1
-  constexpr int WY = TY + 3, WZ = TZ + 3;    // coefficients: -2 .. T
+  // coefficients on the old correction's window: their liquid flags mask it
+  constexpr int WY = EY, WZ = EZ;
   constexpr int CY = TY + 2, CZ = TZ + 2;    // r, first-colour values: -1 .. T
   constexpr int WN = WY * WZ, EN = EY * EZ, CN = CY * CZ;
   constexpr int NW = (WN + 255) / 256, NE = ZERO ? 0 : (EN + 255) / 256, NR = (CN + 255) / 256;
@@ -192,11 +193,15 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const bool pairs = (nz & 1) == 0;
   if constexpr (TZ == kHullTZ) {
     // the level's 32 x 8 tiles carry liquid hulls (k_tile_hull): planes of
-    // the chunk outside the hull are all air in this tile and sweep to zero
+    // the chunk outside the hull are all air in this tile.  On coarse levels
+    // they sweep to zero (the prolongation reads coarse air as zero); the
+    // finest level's correction is read only inside the hulls or masked by
+    // the liquid flags (these sweeps' staged windows, the marches), so its
+    // planes outside the hull are left as they are.
     if (lvl.hull) {
       const int* hp = lvl.hull + 2 * (tyi * tiles_z + tzi);
       const int xs = max(x0, cload(hp)), xe = min(x1, cload(hp + 1) + 1);
-      if (xs > x0 || xe < x1) {
+      if (!top && (xs > x0 || xe < x1)) {
         const int zy = tid / (TZ / 2), zk = tid - (tid / (TZ / 2)) * (TZ / 2);
         const int gy = y0 + zy, gz = z0 + 2 * zk;
         if (tid < TY * (TZ / 2) && gy < ny && gz < nz)
@@ -313,10 +318,13 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
       if (k + 1 < NW || q < WN) s_w[s0][q] = bits_unless(pw[rs][k], pin && ow[k] >= 0);
     }
     if constexpr (!ZERO) {
+      // air cells stage a zero correction whatever the buffer holds there
+      static_assert(WN == EN && NW == NE, "coefficients and correction share the staged window");
 #pragma unroll
       for (int k = 0; k < NE; ++k) {
         const int q = tid + 256 * k;
-        if (k + 1 < NE || q < EN) s_e[e0 + q] = value_unless(pe[rs][k], pin && oe[k] >= 0);
+        const W4<T> wq = __builtin_bit_cast(W4<T>, pw[rs][k]);
+        if (k + 1 < NE || q < EN) s_e[e0 + q] = value_unless(pe[rs][k], pin && oe[k] >= 0 && wf(wq.id) > 0.0f);
       }
     }
 #pragma unroll

@@ -2075,9 +2075,14 @@ int Projector::probe_vcycle(hipStream_t s, UniformGrid* g, const fp_options* opt
   FLAUNCH(k_reset_state<Part>, p->one, s);
   setup_tables(s, *p, opt->phi_preserve_levels);
   HIPCHK(hipMemcpyAsync(&g->host_rec()->st->rscale, &rscale, sizeof(float), hipMemcpyHostToDevice, s));
+  // the finest level's sweeps leave its air planes outside the liquid hulls
+  // untouched: the probe returns zeros there
+  const LevelRec& l0 = g->level(0);
+  const size_t ebytes = (size_t)l0.n * (opt->precision == FP_PRECISION_FP32 ? 4 : 2);
+  HIPCHK(hipMemsetAsync(l0.e, 0, ebytes, s));
+  HIPCHK(hipMemsetAsync(l0.e2, 0, ebytes, s));
   HIPCHK(hipStreamSynchronize(s));
   enqueue_vcycle(s, *p);
-  const LevelRec& l0 = g->level(0);
   const size_t bytes = (size_t)l0.n * (opt->precision == FP_PRECISION_FP32 ? 4 : 2);
   HIPCHK(hipMemcpyAsync(z_out, l0.efinal ? l0.e2 : l0.e, bytes, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
