@@ -368,15 +368,34 @@ __device__ __forceinline__ void band_segment(const TallGeo& g, const int* __rest
 // Work items of the segment passes: per (x, z chunk of FP_BLOCK), item 0 is
 // the column's (bottom, top) pair and items 1.. its band segments (py0 =
 // 2 + kSegRows (k-1)); the item index is uniform across the workgroup and
-// each lane takes one z.
+// each lane takes one z.  Workgroup b runs on XCD b % 8 (a part starts at a
+// multiple of 8 and the stride is a multiple of 8), so when nx and the stride
+// divide by 8 the items are dealt as eight contiguous blocks of nx / 8
+// columns, one per XCD, walked x fastest within each (segment, z chunk): the
+// +-x neighbour columns an item reads were just read by the workgroups before
+// it on the same XCD, whose L2 still holds them.
+__device__ __forceinline__ void tall_item_of(int it, int nx, int ni, int zch, bool xcd, int& x, int& k, int& zc) {
+  if (xcd) {
+    const int x8 = nx >> 3, l = it >> 3;
+    const int sl = l / x8;
+    x = (it & 7) * x8 + (l - sl * x8);
+    k = sl / zch;
+    zc = sl - k * zch;
+  } else {
+    const int xk = it / zch;
+    zc = it - xk * zch;
+    k = xk % ni;
+    x = xk / ni;
+  }
+}
 #define TALL_ITEMS(g) for (int item_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK, \
-                           ni_ = 1 + ((g).B + kSegRows - 1) / kSegRows;                \
+                           ni_ = 1 + ((g).B + kSegRows - 1) / kSegRows,                \
+                           xcd_ = ((g).nx & 7) == 0 && (nblk & 7) == 0;                \
                            item_ < (g).nx * ni_ * zch_; item_ += nblk)
 #define TALL_ITEM(g, x, k, z)                                   \
-  const int xk_ = item_ / zch_;                                 \
-  const int z = (item_ - xk_ * zch_) * FP_BLOCK + (int)threadIdx.x; \
-  const int k = xk_ % ni_;                                      \
-  const int x = xk_ / ni_;                                      \
+  int x, k, zc_;                                                \
+  tall_item_of(item_, (g).nx, ni_, zch_, xcd_, x, k, zc_);      \
+  const int z = zc_ * FP_BLOCK + (int)threadIdx.x;              \
   if (z >= (g).nz) continue;
 
 // --------------------------------------------------------------------------
