@@ -68,17 +68,18 @@ constexpr int kHullTZ = 32, kHullTY = 256 / kHullTZ;
 // each chunk, slab b going to XCD b % 8: neighbouring tiles in a slab, which
 // read each other's halo rows, share an L2, and liquid that fills only part
 // of the grid's height still reaches all eight XCDs (the hull-clamped passes
-// do work only where liquid is).  Grids whose tile rows do not split into
-// 2-row slabs for 8 XCDs take one contiguous slab per XCD (tiles % 8 == 0),
-// else the identity.
+// do work only where liquid is).  A slab is 16 grid rows high: two rows of
+// 8-row tiles (slab_rows 2) or one row of 16-row tiles (slab_rows 1).  Grids
+// whose tile rows do not split into slabs for 8 XCDs take one contiguous slab
+// per XCD (tiles % 8 == 0), else the identity.
 __device__ __forceinline__ void tile_of_group(int g, int tiles_z, int tiles_y, int ngroups, int& tzi, int& tyi,
-                                              int& cxi) {
+                                              int& cxi, int slab_rows = 2) {
   (void)ngroups;
   const int tiles = tiles_z * tiles_y;
   int l = g;
-  if ((tiles_y & 15) == 0) {
+  if (tiles_y % (8 * slab_rows) == 0) {
     const int S = tiles >> 3;          // tiles per XCD per chunk
-    const int Bt = 2 * tiles_z;        // tiles per slab
+    const int Bt = slab_rows * tiles_z;  // tiles per slab
     const int i = g >> 3;
     const int chunk = i / S, r = i - chunk * S;
     const int bx = r / Bt;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   const float rm = dyn ? rmul * cload(&st->rscale) : rmul;
   const int nx = lvl.d[0], ny = lvl.d[1], nz = lvl.d[2];
   int tzi, tyi, cxi;
-  tile_of_group(blk, tiles_z, tiles_y, nblk, tzi, tyi, cxi);
+  tile_of_group(blk, tiles_z, tiles_y, nblk, tzi, tyi, cxi, TY >= 16 ? 1 : 2);
   const int z0 = tzi * TZ, y0 = tyi * TY;
   int x0 = cxi * chunk;
   int x1 = min(x0 + chunk, nx);
@@ -191,7 +192,8 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
   // even rows: both colours of a plane leave in one pass of paired stores
   // (colour 2 writes each (z even, z odd) pair); odd nz keeps single stores
   const bool pairs = (nz & 1) == 0;
-  if constexpr (TZ == kHullTZ) {
+  static_assert(kHullTZ % TZ == 0 && TY % kHullTY == 0, "a sweep tile lies in one column of hull tiles");
+  {
     // the level's 32 x 8 tiles carry liquid hulls (k_tile_hull): planes of
     // the chunk outside the hull are all air in this tile.  On coarse levels
     // they sweep to zero (the prolongation reads coarse air as zero); the
@@ -199,8 +201,15 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
     // the liquid flags (these sweeps' staged windows, the marches), so its
     // planes outside the hull are left as they are.
     if (lvl.hull) {
-      const int* hp = lvl.hull + 2 * (tyi * tiles_z + tzi);
-      const int xs = max(x0, cload(hp)), xe = min(x1, cload(hp + 1) + 1);
+      // the union of the hulls of the march tiles this tile lies in
+      const int hz = z0 / kHullTZ, hy0 = y0 / kHullTY, hy1 = min((y0 + TY - 1) / kHullTY, lvl.mg.ty - 1);
+      int hlo = nx, hhi = -1;
+      for (int hy = hy0; hy <= hy1; ++hy) {
+        const int* hp = lvl.hull + 2 * (hy * lvl.mg.tz + hz);
+        hlo = min(hlo, cload(hp));
+        hhi = max(hhi, cload(hp + 1));
+      }
+      const int xs = max(x0, hlo), xe = min(x1, hhi + 1);
       if (!top && (xs > x0 || xe < x1)) {
         const int zy = tid / (TZ / 2), zk = tid - (tid / (TZ / 2)) * (TZ / 2);
         const int gy = y0 + zy, gz = z0 + 2 * zk;
