@@ -263,17 +263,38 @@ struct VecDir {
   __device__ __forceinline__ float operator()(size_t i) const { return z[i] + beta * d[i]; }
 };
 
+// Item it of a pass over nx columns x ni rows (or segments) x zch z chunks:
+// (x, k, zc).  With xcd, workgroup b (XCD b % 8) takes the items of XCD
+// b % 8's block of nx / 8 columns, x fastest within each (k, zc); otherwise
+// zc fastest, then k, then x.
+__device__ __forceinline__ void tall_item_of(int it, int nx, int ni, int zch, bool xcd, int& x, int& k, int& zc) {
+  if (xcd) {
+    const int x8 = nx >> 3, l = it >> 3;
+    const int sl = l / x8;
+    x = (it & 7) * x8 + (l - sl * x8);
+    k = sl / zch;
+    zc = sl - k * zch;
+  } else {
+    const int xk = it / zch;
+    zc = it - xk * zch;
+    k = xk % ni;
+    x = xk / ni;
+  }
+}
+
 // Sample loops run over lines of FP_BLOCK samples along z at one (x, packed
 // row): the line index is uniform across the workgroup (scalar division),
 // each lane takes one z, and a wave shares its packed row (no divergence
-// between tall and band rows).
-#define TALL_LINES(g) for (int line_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK; \
+// between tall and band rows).  As the segment passes below, the lines are
+// dealt to the XCDs in blocks of nx / 8 columns walked x fastest (when nx and
+// the stride divide by 8), so that neighbour columns share an L2.
+#define TALL_LINES(g) for (int line_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK, \
+                           xcd_ = ((g).nx & 7) == 0 && (nblk & 7) == 0;                 \
                            line_ < (g).nx * (g).P * zch_; line_ += nblk)
 #define TALL_LINE(g, idx, x, py, z)                       \
-  const int xp_ = line_ / zch_;                           \
-  const int z = (line_ - xp_ * zch_) * FP_BLOCK + (int)threadIdx.x; \
-  const int py = xp_ % (g).P;                             \
-  const int x = xp_ / (g).P;                              \
+  int x, py, zc_;                                         \
+  tall_item_of(line_, (g).nx, (g).P, zch_, xcd_, x, py, zc_); \
+  const int z = zc_ * FP_BLOCK + (int)threadIdx.x;        \
   if (z >= (g).nz) continue;                              \
   const size_t idx = ((size_t)x * (g).P + py) * (g).nz + z;
 
@@ -374,20 +395,6 @@ __device__ __forceinline__ void band_segment(const TallGeo& g, const int* __rest
 // columns, one per XCD, walked x fastest within each (segment, z chunk): the
 // +-x neighbour columns an item reads were just read by the workgroups before
 // it on the same XCD, whose L2 still holds them.
-__device__ __forceinline__ void tall_item_of(int it, int nx, int ni, int zch, bool xcd, int& x, int& k, int& zc) {
-  if (xcd) {
-    const int x8 = nx >> 3, l = it >> 3;
-    const int sl = l / x8;
-    x = (it & 7) * x8 + (l - sl * x8);
-    k = sl / zch;
-    zc = sl - k * zch;
-  } else {
-    const int xk = it / zch;
-    zc = it - xk * zch;
-    k = xk % ni;
-    x = xk / ni;
-  }
-}
 #define TALL_ITEMS(g) for (int item_ = blk, zch_ = ((g).nz + FP_BLOCK - 1) / FP_BLOCK, \
                            ni_ = 1 + ((g).B + kSegRows - 1) / kSegRows,                \
                            xcd_ = ((g).nx & 7) == 0 && (nblk & 7) == 0;                \
