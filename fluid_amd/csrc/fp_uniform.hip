@@ -837,6 +837,109 @@ __global__ __launch_bounds__(kBottomThreads) void k_bottom(const Part* parts, in
   }
 }
 
+// k_bottom with the bottom levels held in LDS: the level sets of the levels
+// from lb on and their correction, right-hand side and residual live in the
+// workgroup's dynamic LDS (bottom_lds_bytes), the grid record is copied to
+// LDS with those arrays pointing there, and the same per-cell steps run on it
+// (they address memory through generic pointers), so every phase is an LDS
+// round trip instead of an L2 one.  In: the level sets and level lb's
+// restricted right-hand side; out: level lb's correction, the only array of
+// these levels read after the bottom.
+constexpr size_t kBottomLdsAlign = 16;
+__host__ __device__ inline size_t bottom_lds_round(size_t b) { return (b + kBottomLdsAlign - 1) & ~(kBottomLdsAlign - 1); }
+template <typename T>
+__host__ __device__ inline size_t bottom_lds_bytes_of(const long long* n, int count) {
+  size_t b = 0;
+  for (int q = 0; q < count; ++q) b += bottom_lds_round(4 * (size_t)n[q]) + 3 * bottom_lds_round(sizeof(T) * (size_t)n[q]);
+  return b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBottomThreads) void k_bottom_lds(const Part* parts, int pre, int post) {
+  extern __shared__ __align__(16) unsigned char s_bot[];
+  __shared__ __align__(16) unsigned char s_rec[sizeof(GridRec)];  // raw: the record's type has initialisers
+  GridRec& Gs = *reinterpret_cast<GridRec*>(s_rec);
+  const GridRec& Gg = *parts[blockIdx.x].g;
+  if (Gg.st->done) return;
+  {
+    const int* src = reinterpret_cast<const int*>(&Gg);
+    int* dst = reinterpret_cast<int*>(&Gs);
+    for (int q = threadIdx.x; q < (int)(sizeof(GridRec) / 4); q += blockDim.x) dst[q] = src[q];
+  }
+  __syncthreads();
+  const int lb = Gs.lb, lc = Gs.L - 1;
+  if (threadIdx.x == 0) {
+    unsigned char* p = s_bot;
+    for (int l = lb; l <= lc; ++l) {
+      LevelRec& lv = Gs.lv[l];
+      const size_t n = (size_t)lv.n;
+      lv.phi = reinterpret_cast<float*>(p);
+      p += bottom_lds_round(4 * n);
+      lv.e = p;
+      p += bottom_lds_round(sizeof(T) * n);
+      lv.r = p;
+      p += bottom_lds_round(sizeof(T) * n);
+      lv.t = p;
+      p += bottom_lds_round(sizeof(T) * n);
+      lv.e2 = lv.e;
+    }
+  }
+  __syncthreads();
+  for (int l = lb; l <= lc; ++l) {
+    const size_t n = (size_t)Gs.lv[l].n;
+    const float* __restrict__ src = Gg.lv[l].phi;
+    float* dst = Gs.lv[l].phi;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = ldg(src, i);
+  }
+  {
+    const size_t n = (size_t)Gs.lv[lb].n;
+    const T* __restrict__ src = (const T*)Gg.lv[lb].r;
+    T* dst = (T*)Gs.lv[lb].r;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = ldg(src, i);
+  }
+  __syncthreads();
+  const GridRec& G = Gs;
+  for (int l = lb; l < lc; ++l) {
+    const LevelRec& lv = G.lv[l];
+    const size_t n = (size_t)lv.n;
+    const float rs = lv.Sr / lv.Se, rt = lv.Sr / lv.St;
+    if (pre == 0) {
+      for (size_t i = threadIdx.x; i < n; i += blockDim.x) stT((T*)lv.e, i, 0.0f);
+      __syncthreads();
+    }
+    for (int sw = 0; sw < pre; ++sw)
+      for (int colour = 0; colour < 2; ++colour) {
+        const int zi = (sw == 0 && colour == 0) ? 1 : 0;
+        for (size_t i = threadIdx.x; i < n; i += blockDim.x) smooth_cell<T, T>(G, l, colour, zi, rs, i);
+        __syncthreads();
+      }
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) residual_cell<T, T>(G, l, rt, i);
+    __syncthreads();
+    for (size_t ci = threadIdx.x; ci < (size_t)G.lv[l + 1].n; ci += blockDim.x) restrict_cell<T>(G, l, ci);
+    __syncthreads();
+  }
+  coarse_solve_block<T, T>(G);
+  __syncthreads();
+  for (int l = lc - 1; l >= lb; --l) {
+    const LevelRec& lv = G.lv[l];
+    const size_t n = (size_t)lv.n;
+    const float rs = lv.Sr / lv.Se;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) prolong_cell<T>(G, l, 0, i);
+    __syncthreads();
+    for (int sw = 0; sw < post; ++sw)
+      for (int cc = 0; cc < 2; ++cc) {
+        for (size_t i = threadIdx.x; i < n; i += blockDim.x) smooth_cell<T, T>(G, l, 1 - cc, 0, rs, i);
+        __syncthreads();
+      }
+  }
+  {
+    const size_t n = (size_t)Gs.lv[lb].n;
+    const T* src = (const T*)Gs.lv[lb].e;
+    T* __restrict__ dst = (T*)Gg.lv[lb].e;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) stg(dst, i, src[i]);
+  }
+}
+
 // Packed coefficients of a fused level.
 template <typename T>
 __global__ void k_build_weights(const Part* parts, int np, int level) {
@@ -1417,6 +1520,7 @@ struct Projector::Plan {
   PartTable faces[3];   // face arrays
   PartTable march0;     // finest-level column-tile march
   PartTable coarse_top, coarse_deep;  // coarsest level = finest / deeper
+  long long bottom_n[kMaxLevels] = {};  // per bottom level (from lb): most cells over coarse_deep's grids
   struct Level {
     PartTable sweep32, sweep16;  // fused grids by sweep tile width
     PartTable res;               // fused grids: residual march
@@ -1504,6 +1608,13 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
     faces[2].push_back({g, Plan::blocks_for((size_t)R.nw, 1)});
     march0.push_back({g, R.lv[0].mg.ngroups});
     (R.L == 1 ? ctop : cdeep).push_back({g, 1});
+  }
+  // LDS of the one-workgroup bottom: sized by the largest bottom of the set
+  for (int q = 0; q < kMaxLevels; ++q) p->bottom_n[q] = 0;
+  for (UniformGrid* g : grids) {
+    const GridRec& R = *g->host_rec();
+    if (R.L == 1) continue;
+    for (int l = R.lb; l < R.L; ++l) p->bottom_n[l - R.lb] = std::max(p->bottom_n[l - R.lb], R.lv[l].n);
   }
   p->maxL = maxL;
   p->add(p->one, one);
@@ -1646,8 +1757,17 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
   FLAUNCH((k_coarse_solve<T, float>), p.coarse_top, s);
   // grids with a coarser level: the levels from lb on and the coarsest solve
   // in one workgroup per grid
-  if (p.coarse_deep.n > 0)
-    hipLaunchKernelGGL((k_bottom<T>), dim3(p.coarse_deep.n), dim3(kBottomThreads), 0, s, p.coarse_deep.dev, pre, post);
+  if (p.coarse_deep.n > 0) {
+    // LDS-resident bottom when its levels fit beside the static LDS (the
+    // grid record, the coarse solve's right-hand side)
+    const size_t lds = bottom_lds_bytes_of<T>(p.bottom_n, kMaxLevels);
+    if (lds <= kBottomLdsMax)
+      hipLaunchKernelGGL((k_bottom_lds<T>), dim3(p.coarse_deep.n), dim3(kBottomThreads), lds, s, p.coarse_deep.dev,
+                         pre, post);
+    else
+      hipLaunchKernelGGL((k_bottom<T>), dim3(p.coarse_deep.n), dim3(kBottomThreads), 0, s, p.coarse_deep.dev, pre,
+                         post);
+  }
   for (int l = p.maxL - 2; l >= 0; --l) {
     Plan::Level& T_ = p.lv[l];
     const bool top = l == 0;

@@ -24,6 +24,9 @@ namespace fp {
 
 constexpr int kMaxLevels = 16;
 constexpr unsigned kMaxBlocks = 16384;
+// Dynamic LDS of the LDS-resident V-cycle bottom (k_bottom_lds): with its
+// static LDS (grid record, coarse right-hand side) within 64 KB.
+constexpr size_t kBottomLdsMax = 48 * 1024;
 constexpr long long kBottomCells = 4096;  // levels below this size run in the one-workgroup bottom of the V-cycle
 constexpr int kBottomThreads = 1024;
 
