@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 530.7,
-    ("tall", 512, "fp32", "smooth_fine", 2): 454.8,
+    ("uniform", 384, "fp16", "smooth_fine", 2): 528.4,
+    ("tall", 512, "fp32", "smooth_fine", 2): 283.8,
 }
 
 
