@@ -188,6 +188,39 @@ def test_uneven_grids(fpl, oracle, dims):
     g.destroy()
 
 
+@pytest.mark.parametrize("precision", ["fp16", "fp32"])
+def test_bottom_in_lds_and_in_l2(fpl, oracle, precision):
+    """32 x 32 x 28: the one-workgroup bottom of the V-cycle starts at the
+    16 x 16 x 14 level (4172 cells down to the coarsest).  Its FP16 arrays fit
+    the LDS budget (41.7 KB), its FP32 ones (66.8 KB) do not, so the two
+    precisions run the LDS-resident and the L2-resident bottom kernels; level 0
+    (nz = 28) sweeps in 16-wide tiles clamped to the liquid hulls."""
+    dims = (32, 32, 28)
+    od, _, _ = oracle.plan_levels(*dims)
+    assert [int(np.prod(x)) for x in od] == [28672, 3584, 448, 112, 28]
+    rng = np.random.default_rng(7)
+    nx, ny, nz = dims
+    h = 1.0 / nx
+    c = lambda m: (np.arange(m) + 0.5) * h
+    phi = scenes.box_sdf(c(nx)[:, None, None], c(ny)[None, :, None], c(nz)[None, None, :],
+                         (0, 0, 0), (0.6, 0.55, 0.5)).astype(np.float32)
+    mu, mv, mw = scenes.fluid_face_masks(phi)
+    u = np.where(mu, rng.uniform(-0.3, 0.3, mu.shape), 0).astype(np.float32)
+    v = np.where(mv, rng.uniform(-0.3, 0.3, mv.shape) - 0.08, 0).astype(np.float32)
+    w = np.where(mw, rng.uniform(-0.3, 0.3, mw.shape), 0).astype(np.float32)
+    op, (ou, ov, ow), ob, orep = oracle.uniform_projection(phi, u, v, w, h, 1 / 120, 998.2, rel_tol=1e-12)
+    g = fpl.UniformPressureProjection(dims, h)
+    try:
+        res = g.project(phi, u, v, w, 1 / 120, 998.2, precision=precision, relative_tolerance=1e-7)
+        assert res.report["flags"] == 0, res.report
+        assert res.report["relative_residual"] <= 1e-7
+        assert rel_l2(res.pressure, op) < 1e-5
+        for a, b in ((res.u, ou), (res.v, ov), (res.w, ow)):
+            assert rel_l2(a, b) < 1e-5
+    finally:
+        g.destroy()
+
+
 def test_warm_start_is_same_solution_fewer_iterations(fpl, c0, c0_oracle):
     s = c0
     g = fpl.UniformPressureProjection(s.shape, s.h)
