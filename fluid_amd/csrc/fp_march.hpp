@@ -40,7 +40,9 @@ namespace fp {
 // Launch-uniform parameters of a march (the same for every grid of a batch).
 struct MarchLaunch {
   int level;    // multigrid level (V-cycle residual)
-  int which_e;  // correction buffer (0: e, 1: e2) the residual reads
+  int which_e;  // correction buffer (0: e, 1: e2) the residual reads; FP64
+                // true residual: 1 on the first refinement pass (x = 0 on
+                // grids that are not warm-started)
 };
 
 constexpr int kMarchTZ = kHullTZ;
@@ -461,7 +463,8 @@ struct OpTrueResidual {
   const double* b;
   const double* xv;
   float* r;
-  static __device__ __forceinline__ OpTrueResidual make(const GridRec* Gp, const MarchLaunch&) {
+  bool x_zero;  // the iterate is still zero: A x is not loaded
+  static __device__ __forceinline__ OpTrueResidual make(const GridRec* Gp, const MarchLaunch& la) {
     const LevelRec l0 = cload(&Gp->lv[0]);
     OpTrueResidual o;
     o.ny = l0.d[1];
@@ -470,6 +473,7 @@ struct OpTrueResidual {
     o.b = cload(&Gp->b);
     o.xv = cload(&Gp->x);
     o.r = cload(&Gp->r);
+    o.x_zero = la.which_e == 1 && !cload(&Gp->warm);
     return o;
   }
   __device__ __forceinline__ void prepare() {}
@@ -479,7 +483,7 @@ struct OpTrueResidual {
     if (inside) {
       const size_t i = ((size_t)x * ny + y) * nz + zz;
       c.p = ldg(phi, i);
-      c.x = ldg(xv, i);
+      if (!x_zero) c.x = ldg(xv, i);
     }
     return c;
   }

@@ -1908,7 +1908,8 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   int counts[2] = {0, 0};
   for (int pass = 0; pass <= max_outer; ++pass) {
     const int new_pass = pass < max_outer;
-    TIMED(FP_KERNEL_TRUE_RESIDUAL, PLAUNCH(k_march<OpTrueResidual>, p.march0, s, 2, MarchLaunch{0, 0}));
+    TIMED(FP_KERNEL_TRUE_RESIDUAL,
+          PLAUNCH(k_march<OpTrueResidual>, p.march0, s, 2, MarchLaunch{0, pass == 0 ? 1 : 0}));
     FLAUNCH(k_fin_true<Part>, p.march0, s, new_pass);
     int rc = poll(s, p, counts);
     if (rc) return rc;
