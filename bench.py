@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
     ("uniform", 384, "fp16", "smooth_fine", 2): 528.4,
-    ("tall", 512, "fp32", "smooth_fine", 2): 283.8,
+    ("tall", 512, "fp32", "smooth_fine", 2): 286.3,
 }
 
 
