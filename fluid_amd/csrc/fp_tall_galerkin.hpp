@@ -21,7 +21,8 @@
 //
 // Smoother: kTallCoarseSweeps scaled l1-Jacobi sweeps before and after the
 // coarse correction, e += theta (r - A e) / sum_j |a_ij| with theta = 1.8
-// (on the 512^2 wave grid: 29 PCG iterations at 1.4, 28 at 1.6, 26 at 1.8).
+// (on the 512^2 wave grid: 29 PCG iterations at 1.4, 28 at 1.6, 27 at 1.7,
+// 26 at 1.8, 27 at 1.9).
 // The coarse operators are not diagonally dominant (the affine profile inside
 // tall cells gives positive couplings; lambda_max(D^-1 A) reaches 3), so plain
 // damped Jacobi has no safe fixed weight.  With the l1 row sums,
