@@ -114,31 +114,55 @@ def host_cpu() -> dict:
     return {"model": model, "logical_cpus": os.cpu_count()}
 
 
-def cpu_baseline(seconds_budget: float = 20.0):
-    """The reference's CPU MAC/PCG algorithm (sparse-atlas Jacobi-PCG, restated
-    in C: oracle/fp_oracle.c) on a bounded sample of the same workload."""
+def _median_solve(oracle, s, solver, runs=3):
+    """Median wall time of `runs` FP64 projections of scene s (assembly ->
+    divergence -> solve to 1e-6 -> gradient subtract; generation excluded)."""
+    times, rep = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, _, _, rep = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-6,
+                                                 solver=solver)
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), rep
+
+
+def cpu_baseline(n_all: int = 96, n_one: int = 64, runs: int = 3):
+    """The reference's CPU MAC/PCG algorithm restated in C (oracle/fp_oracle.c)
+    on bounded samples of the same workload (the dam + 8 spheres generator at
+    reduced size): Jacobi-PCG (sparse-atlas-composite-projection.ts:1346-1457,
+    the reference's own CPU solver) and an FP64 MG-PCG variant, each on all
+    host cores and on one core, median of `runs`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle
     from fluid_amd import scenes
 
     oracle.build()
-    n = 128
-    s = scenes.dam_spheres_scene(n, seed=0)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    p, vel, b, rep = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-6,
-                                               solver="jacobi")
-    dt = time.perf_counter() - t0
+    all_threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    s_all = scenes.dam_spheres_scene(n_all, seed=0)
+    s_one = scenes.dam_spheres_scene(n_one, seed=0)
+    out = {}
+    try:
+        for label, threads, s in (("all_cores", all_threads, s_all), ("one_core", 1, s_one)):
+            oracle.set_threads(threads)
+            for solver in ("jacobi", "mgpcg"):
+                t, rep = _median_solve(oracle, s, solver, runs)
+                out[f"{solver}_{label}"] = {"mcells_per_s": round(s.cells / t / 1e6, 4), "seconds": round(t, 3),
+                                            "threads": threads, "grid": f"{s.shape[0]}^3",
+                                            "iterations": rep["iterations"],
+                                            "relative_residual": rep["relative_residual"]}
+    finally:
+        oracle.set_threads(all_threads)
+    head = out["jacobi_all_cores"]
     return {
-        "value": round(s.cells / dt / 1e6, 4),
+        "value": head["mcells_per_s"],
         "unit": "Mcells/s",
-        "cores": threads,
+        "cores": all_threads,
         "kind": "port",
-        "sample": f"dam+8 spheres scene generator at {n}^3 (seed 0), one projection, Jacobi-PCG "
+        "sample": f"dam+8 spheres generator at {n_all}^3 (seed 0; 1-core figures at {n_one}^3), one projection "
+                  f"to 1e-6 relative residual, median of {runs}: Jacobi-PCG "
                   f"(sparse-atlas-composite-projection.ts:1346-1457 restated in C, FP64, OpenMP stencils) "
-                  f"to 1e-6 relative residual: {rep['iterations']} iterations in {dt:.2f} s",
-        "relative_residual": rep["relative_residual"],
+                  f"{head['iterations']} iterations in {head['seconds']:.2f} s; variants in 'variants'",
+        "variants": out,
         "host": host_cpu(),
     }
 
@@ -169,7 +193,7 @@ def tall_cpu_baseline(n: int = 96):
     }
 
 
-def bench_tall(args):
+def bench_tall(args, summary=False):
     """Restricted tall-cell workload: 512^2 columns, 512 fine rows, 32-row band."""
     import torch
     from fluid_amd import scenes
@@ -179,52 +203,72 @@ def bench_tall(args):
     t0 = time.perf_counter()
     sc = scenes.tall_wave_scene(args.tall_n, args.tall_n, band=32, seed=0)
     L = sc.layout
-    print(f"# tall scene {args.tall_n}^2 columns generated in {time.perf_counter() - t0:.1f} s, bases "
-          f"{L.bases.min()}..{L.bases.max()}", file=sys.stderr, flush=True)
+    log(f"tall scene {args.tall_n}^2 columns generated in {time.perf_counter() - t0:.1f} s, bases "
+        f"{L.bases.min()}..{L.bases.max()}")
     phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (sc.phi, sc.u, sc.v, sc.w))
     out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
     g = TallCellPressureProjection.from_layout(L)
     common = dict(precision=args.precision, relative_tolerance=1e-6, pre_sweeps=args.sweeps,
                   post_sweeps=args.sweeps, graphs=0 if args.no_graphs else 1)
     opts = default_options(**common)
-    for i in range(args.warmup):
+    steps = args.sub_steps if summary else args.steps
+    for i in range(max(1, args.warmup)):
         r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=opts)
-        print(f"# warmup {i}: {r.report['device_ms']:.2f} ms iters={r.report['iterations']} "
-              f"rel={r.report['relative_residual']:.2e}", file=sys.stderr, flush=True)
-    timed = default_options(timed_kernel=args.timed_kernel, **common)
+        log(f"tall warmup {i}: {r.report['device_ms']:.2f} ms iters={r.report['iterations']} "
+            f"rel={r.report['relative_residual']:.2e}")
     torch.cuda.synchronize()
     lat, reports = [], []
     t_start = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         t1 = time.perf_counter()
-        r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=timed)
+        r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=opts)
         lat.append((time.perf_counter() - t1) * 1e3)
         reports.append(r.report)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     samples = sc.samples
-    value = samples * args.steps / elapsed / 1e6
+    value = samples * steps / elapsed / 1e6
     rep = reports[-1]
-    kms = sum(x["timed_kernel_ms"] for x in reports)
-    kl = sum(x["timed_kernel_launches"] for x in reports)
+    p_out = out[0].cpu().numpy()
+    # per-launch time of the timed kernel: untimed projections after the timed
+    # region, launched one by one (HIP events cannot split a replayed graph)
+    timed = default_options(timed_kernel=args.timed_kernel, **common)
+    kms = kl = 0
+    for _ in range(2):
+        r = g.project(phi, u, v, w, sc.dt, sc.rho, out=out, options=timed)
+        kms += r.report["timed_kernel_ms"]
+        kl += r.report["timed_kernel_launches"]
     avg_ms = kms / max(kl, 1)
     bps = {"smooth_fine": 20.0, "residual_fine": 16.0, "cg_direction": 24.0, "cg_update": 24.0}.get(
         args.timed_kernel, 0.0)
     achieved = bps * samples / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    g.destroy()
     err = None
-    if not args.no_error_solve:
-        ref = g.project(phi, u, v, w, sc.dt, sc.rho, relative_tolerance=1e-12)
-        p_ref = ref.pressure.double()
-        err = float(torch.linalg.norm(out[0].double() - p_ref) / torch.linalg.norm(p_ref))
+    if not args.no_error_solve and args.tall_n <= 128:
+        err = tall_oracle_error(sc, p_out)
     lat_sorted = sorted(lat)
+    solve = {"iterations": rep["iterations"], "outer": rep["outer_iterations"], "levels": rep["levels"],
+             "relative_residual": rep["relative_residual"], "max_div_after": rep["max_div_after"],
+             "max_div_before": rep["max_div_before"], "device_ms": round(rep["device_ms"], 3)}
+    roof = {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_bytes(("tall", args.tall_n, "fp32", args.timed_kernel, args.sweeps)),
+            "avg_launch_ms": round(avg_ms, 5), "bytes_per_sample": bps,
+            "algorithmic_bytes": round(bps * samples), "launches": kl,
+            "measured_in": "2 untimed projections after the timed region, kernels launched one by one"}
+    if summary:
+        return {"workload": f"tall-cell {args.tall_n}^2 columns x {args.tall_n} rows, band 32 (wave surface)",
+                "msamples_per_s": round(value, 2), "ms": round(elapsed / steps * 1e3, 3), "steps": steps,
+                "samples": samples, "equivalent_fine_mcells_per_s": round(L.equivalent_cells * steps / elapsed / 1e6, 1),
+                "solve": solve, "roofline": roof}
     line = {
         "metric": "MGPCG projection Mcells/s at 1e-6 rel. residual",
         "value": round(value, 2),
         "unit": "Mcells/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -233,44 +277,49 @@ def bench_tall(args):
         "config": {"workload": f"restricted tall-cell grid {args.tall_n}^2 columns x {args.tall_n} rows, band 32, "
                                "one projection to 1e-6 (cells = packed samples)",
                    "samples": samples, "equivalent_fine_cells": L.equivalent_cells,
-                   "equivalent_mcells_per_s": round(L.equivalent_cells * args.steps / elapsed / 1e6, 1),
+                   "equivalent_mcells_per_s": round(L.equivalent_cells * steps / elapsed / 1e6, 1),
                    "liquid_samples": int(rep["liquid_samples"]), "sweeps": args.sweeps},
         "latency_ms": {"p50": round(statistics.median(lat), 3),
                        "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
-        "solve": {"iterations": rep["iterations"], "outer": rep["outer_iterations"], "levels": rep["levels"],
-                  "relative_residual": rep["relative_residual"], "max_div_after": rep["max_div_after"],
-                  "max_div_before": rep["max_div_before"], "fp64_pressure_rel_l2_error": err,
-                  "device_ms": round(rep["device_ms"], 3)},
-        "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_bytes(("tall", args.tall_n, "fp32", args.timed_kernel, args.sweeps)),
-                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_sample": bps,
-                     "algorithmic_bytes": round(bps * samples), "launches": kl},
+        "solve": dict(solve, fp64_oracle_pressure_rel_l2_error=err),
+        "roofline": roof,
     }
-    g.destroy()
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = tall_cpu_baseline()
         except Exception as e:
             line["cpu_baseline"] = {"error": repr(e)}
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def tall_oracle_error(sc, p_gpu):
+    """Relative L2 error of a tall-cell pressure against the FP64 oracle solve."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    L = sc.layout
+    t = oracle.Tall(L.bases, sc.phi, L.fine_ny, L.band, L.h, L.max_delta)
+    op, _, _, _ = t.projection(sc.u, sc.v, sc.w, sc.dt, sc.rho, rel_tol=1e-11)
+    return float(np.linalg.norm(p_gpu.astype(np.float64) - op) / np.linalg.norm(op))
 
 
 def bench_batch(args):
     """96 uneven cubic grids (edges log-uniform in [32,192]) solved to 1e-6 each:
     one batched FP16-multigrid solve against the sequential per-grid FP32 loop
-    (and the per-grid FP16 loop, to separate batching from precision)."""
+    (and the per-grid FP16 loop, to separate batching from precision); the
+    pressure of every 8th grid is checked against the FP64 oracle."""
     import numpy as np
     import torch
     from fluid_amd import scenes
     from fluid_amd.projection import UniformPressureProjection, default_options, project_batch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
     ss = scenes.batch_scenes(args.batch_count, seed=0)
     cells = sum(s.cells for s in ss)
-    print(f"# {len(ss)} grids, {cells / 1e6:.1f} M cells, generated in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    log(f"{len(ss)} grids, {cells / 1e6:.1f} M cells, generated in {time.perf_counter() - t0:.1f} s")
     grids = [UniformPressureProjection(s.shape, s.h) for s in ss]
     reqs = []
     for s in ss:
@@ -300,74 +349,96 @@ def bench_batch(args):
 
     for _ in range(max(1, args.warmup)):
         batched()
-    t_b, lat_b, res_b = timed(batched, args.steps)
-    print(f"# batched fp16: {t_b * 1e3:.1f} ms per batch", file=sys.stderr, flush=True)
+    t_b, lat_b, res_b = timed(batched, args.batch_steps)
+    log(f"batched fp16: {t_b * 1e3:.1f} ms per batch")
+    p_b = [res_b[i].pressure.cpu().numpy() for i in range(0, len(grids), 8)]
     loop(plain)
     t_p, lat_p, res_p = timed(lambda: loop(plain), 1)
-    print(f"# per-grid fp32 loop: {t_p * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    log(f"per-grid fp32 loop: {t_p * 1e3:.1f} ms")
     loop(mixed)
     t_m, lat_m, res_m = timed(lambda: loop(mixed), 1)
-    print(f"# per-grid fp16 loop: {t_m * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    log(f"per-grid fp16 loop: {t_m * 1e3:.1f} ms")
     worst = max(r.report["relative_residual"] for r in res_b)
     flags = sorted({r.report["flags"] for r in res_b})
-    # FP64 error of the batched pressures against tight solves of every 8th grid
+    for g in grids:
+        g.destroy()
+    # FP64 oracle error of the batched pressures (every 8th grid)
     errs = []
-    for i in range(0, len(grids), 8):
-        g, r = grids[i], reqs[i]
-        ref = g.project(r["phi"], r["u"], r["v"], r["w"], r["dt"], r["rho"], precision="fp16",
-                        relative_tolerance=1e-12)
-        pb = res_b[i].pressure.double()
-        pr = ref.pressure.double()
-        errs.append(float(torch.linalg.norm(pb - pr) / max(float(torch.linalg.norm(pr)), 1e-300)))
-    line = {
-        "metric": "batched MGPCG projection Mcells/s at 1e-6 rel. residual (96 uneven grids)",
-        "value": round(cells / t_b / 1e6, 2),
-        "unit": "Mcells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(t_b * 1e3, 3),
-        "higher_is_better": True,
+    for k, i in enumerate(range(0, len(ss), 8)):
+        s = ss[i]
+        op, _, _, _ = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-11)
+        errs.append(float(np.linalg.norm(p_b[k].astype(np.float64) - op) / max(np.linalg.norm(op), 1e-300)))
+    return {
+        "workload": f"{len(ss)} cubic grids, edges log-uniform in [32,192] (seed 0), one batched solve",
+        "cells": cells,
+        "mcells_per_s": round(cells / t_b / 1e6, 2),
+        "ms": round(t_b * 1e3, 3),
+        "steps": args.batch_steps,
         "dtype": "fp16 multigrid / fp32 CG / fp64 refinement",
-        "data": "synthetic (seeded uneven dam boxes, uniform face noise)",
-        "config": {"workload": f"{len(ss)} cubic grids, edges log-uniform in [32,192] (seed 0), one batched solve",
-                   "cells": cells},
-        "plain_fp32_per_grid_loop": {"value": round(cells / t_p / 1e6, 2), "ms": round(t_p * 1e3, 3)},
-        "fp16_per_grid_loop": {"value": round(cells / t_m / 1e6, 2), "ms": round(t_m * 1e3, 3)},
+        "plain_fp32_per_grid_loop": {"mcells_per_s": round(cells / t_p / 1e6, 2), "ms": round(t_p * 1e3, 3)},
+        "fp16_per_grid_loop": {"mcells_per_s": round(cells / t_m / 1e6, 2), "ms": round(t_m * 1e3, 3)},
         "speedup_vs_plain_fp32_loop": round(t_p / t_b, 2),
         "worst_relative_residual": worst,
         "flags": flags,
-        "fp64_pressure_rel_l2_error_max": max(errs),
-        "latency_ms": {"p50": round(statistics.median(lat_b), 3), "p99": round(sorted(lat_b)[-1], 3)},
+        "fp64_oracle_pressure_rel_l2_error_max": max(errs),
+        "oracle_checked_grids": len(errs),
+        "latency_ms": {"p50": round(statistics.median(lat_b), 3), "max": round(max(lat_b), 3)},
     }
-    for g in grids:
-        g.destroy()
-    print(json.dumps(line), flush=True)
 
 
-def bench_serve(args):
+def bench_serve(args, duration=None):
     """Served stream: Poisson arrivals (rate, duration) of 20-step sessions of
     64^3 / 128^3 / 192^3 uniform grids and 256^2-column tall-cell grids (32-row
     band) with probabilities 0.4 / 0.3 / 0.2 / 0.1, admitted against a 200 GB HBM
     budget, warm-started steps, p50/p99 per-solve latency."""
     from fluid_amd.serve import ServeLoop
-    loop = ServeLoop(rate_hz=args.rate, duration_s=args.duration, steps=20, seed=0)
+    duration = args.duration if duration is None else duration
+    loop = ServeLoop(rate_hz=args.rate, duration_s=duration, steps=20, seed=0)
     stats = loop.run()
-    line = {
-        "metric": "served projections: p50/p99 solve latency (ms) and Mcells/s at 1e-6",
-        "value": stats["latency_ms"]["p99"],
-        "unit": "ms (p99)",
-        "higher_is_better": False,
-        "n_gpus": 1,
-        "dtype": "fp16 multigrid / fp32 CG / fp64 refinement",
-        "data": "synthetic (seeded dam boxes per session, uniform face noise, gravity steps)",
-        "config": {"workload": f"Poisson sessions at {args.rate}/s for {args.duration} s, 20 steps each, "
-                               "sizes 64^3:0.4 128^3:0.3 192^3:0.2 tall-cell 256^2 x 32-band:0.1 "
-                               "(Mcells/s counts packed samples for tall-cell grids)",
-                   "hbm_budget_gb": 200},
-        "serve": stats,
-    }
-    print(json.dumps(line), flush=True)
+    stats["workload"] = (f"Poisson sessions at {args.rate}/s for {duration} s, 20 warm-started steps each, sizes "
+                         "64^3:0.4 128^3:0.3 192^3:0.2 tall-cell 256^2 x 32-band:0.1, 200 GB HBM budget "
+                         "(Mcells/s counts packed samples for tall-cell grids)")
+    return stats
+
+
+def bench_c0(args):
+    """The CPU-runnable configuration: 64^3 corner dam break, one projection to
+    1e-6, FP16 multigrid (and plain FP32), inputs resident."""
+    import numpy as np
+    import torch
+    from fluid_amd import scenes
+    from fluid_amd.projection import UniformPressureProjection, default_options
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    dev = torch.device("cuda:0")
+    s = scenes.dam_break_scene(64, 0)
+    phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (s.phi, s.u, s.v, s.w))
+    out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
+    g = UniformPressureProjection(s.shape, s.h)
+    res = {}
+    for prec in ("fp16", "fp32"):
+        o = default_options(precision=prec, relative_tolerance=1e-6)
+        for _ in range(3):
+            g.project(phi, u, v, w, s.dt, s.rho, out=out, options=o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.sub_steps):
+            r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=o)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.sub_steps * 1e3
+        res[prec] = {"ms": round(ms, 4), "mcells_per_s": round(s.cells / ms / 1e3, 2),
+                     "iterations": r.report["iterations"], "relative_residual": r.report["relative_residual"]}
+    p = out[0].cpu().numpy()
+    g.destroy()
+    op, _, _, _ = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-12)
+    res["fp32"]["fp64_oracle_pressure_rel_l2_error"] = float(np.linalg.norm(p - op) / np.linalg.norm(op))
+    res["workload"] = "64^3 corner dam break (CPU-runnable configuration), one projection to 1e-6"
+    return res
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -381,15 +452,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweeps", type=int, default=2, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (profiling)")
-    ap.add_argument("--no-error-solve", action="store_true", help="skip the 1e-12 reference solve")
+    ap.add_argument("--no-error-solve", action="store_true", help="skip the FP64 oracle solve")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="uniform workload: skip the C0 / batch / served / tall-cell summaries")
     ap.add_argument("--copy-probe-mib", type=int, default=0, help="only run a device copy of this size")
     ap.add_argument("--tall-n", type=int, default=512, help="tall: columns per side (and fine rows)")
     ap.add_argument("--workload", default="uniform", choices=["uniform", "batch", "serve", "tall"],
-                    help="uniform: one 384^3 grid; batch: 96 uneven grids in one batched solve; "
-                         "serve: Poisson-arriving 20-step sessions through fp_submit/fp_poll")
+                    help="uniform: one 384^3 grid (plus summaries of the others); batch: 96 uneven grids in "
+                         "one batched solve; serve: Poisson-arriving 20-step sessions through fp_submit/fp_poll")
     ap.add_argument("--batch-count", type=int, default=96)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--sub-steps", type=int, default=5, help="timed steps of the summary workloads")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: session arrivals per second")
-    ap.add_argument("--duration", type=float, default=20.0, help="serve: arrival window in seconds")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: arrival window in seconds")
+    ap.add_argument("--serve-summary-duration", type=float, default=15.0,
+                    help="arrival window of the served stream in the default run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build measures one GPU only; "
@@ -400,15 +477,52 @@ def main():
         copy_probe(args.copy_probe_mib, 5)
         return
     if args.workload == "batch":
-        bench_batch(args)
+        b = bench_batch(args)
+        line = {"metric": "batched MGPCG projection Mcells/s at 1e-6 rel. residual (96 uneven grids)",
+                "value": b["mcells_per_s"], "unit": "Mcells/s", "n_gpus": 1, "steps": args.batch_steps,
+                "warmup": args.warmup, "ms_per_step": b["ms"], "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": b["dtype"], "data": "synthetic (seeded uneven dam boxes, uniform "
+                "face noise)", "config": {"workload": b["workload"], "cells": b["cells"]}, "batch": b}
+        print(json.dumps(line), flush=True)
         return
     if args.workload == "serve":
-        bench_serve(args)
+        st = bench_serve(args)
+        line = {"metric": "served projections: p50/p99 solve latency (ms) and Mcells/s at 1e-6",
+                "value": st["latency_ms"]["p99"], "unit": "ms (p99)", "higher_is_better": False, "n_gpus": 1,
+                "dtype": "fp16 multigrid / fp32 CG / fp64 refinement",
+                "data": "synthetic (seeded dam boxes per session, uniform face noise, gravity steps)",
+                "config": {"workload": st["workload"], "hbm_budget_gb": 200}, "serve": st}
+        print(json.dumps(line), flush=True)
         return
     if args.workload == "tall":
-        bench_tall(args)
+        print(json.dumps(bench_tall(args)), flush=True)
         return
+    line = bench_uniform(args)
+    if not args.headline_only:
+        for key, fn in (("c0_64", lambda: bench_c0(args)), ("batch", lambda: bench_batch(args)),
+                        ("served", lambda: bench_serve(args, args.serve_summary_duration)),
+                        ("tall", lambda: bench_tall(args, summary=True))):
+            t0 = time.perf_counter()
+            try:
+                line[key] = fn()
+            except Exception as e:  # a summary never hides the headline
+                line[key] = {"error": repr(e)}
+            log(f"{key} summary done in {time.perf_counter() - t0:.1f} s")
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline is reported, never required
+            line["cpu_baseline"] = {"error": repr(e)}
+        log(f"cpu baseline done in {time.perf_counter() - t0:.1f} s")
+    print(json.dumps(line), flush=True)
 
+
+def bench_uniform(args):
+    """The headline: one projection of the 384^3 dam + 8 spheres grid per
+    step, FP16 multigrid, each CG + V-cycle iteration pair replayed as a
+    hipGraph; then the plain FP32 MGPCG arm on the same grid, the smoother's
+    per-launch time and the FP64 oracle's error, all outside the timed region."""
     import numpy as np
     import torch
     from fluid_amd import scenes
@@ -417,9 +531,7 @@ def main():
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
     s = scenes.dam_spheres_scene(args.n, seed=0)
-    gen_s = time.perf_counter() - t0
-    print(f"# scene {args.n}^3 generated in {gen_s:.1f} s, liquid {(s.phi < 0).mean():.3f}", file=sys.stderr,
-          flush=True)
+    log(f"scene {args.n}^3 generated in {time.perf_counter() - t0:.1f} s, liquid {(s.phi < 0).mean():.3f}")
     phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (s.phi, s.u, s.v, s.w))
     out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
     g = UniformPressureProjection(s.shape, s.h)
@@ -428,16 +540,15 @@ def main():
     opts = default_options(**common)
     for i in range(args.warmup):
         r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=opts)
-        print(f"# warmup {i}: {r.report['device_ms']:.2f} ms iters={r.report['iterations']} "
-              f"rel={r.report['relative_residual']:.2e}", file=sys.stderr, flush=True)
-    timed = default_options(timed_kernel=args.timed_kernel, **common)
+        log(f"warmup {i}: {r.report['device_ms']:.2f} ms iters={r.report['iterations']} "
+            f"rel={r.report['relative_residual']:.2e}")
     torch.cuda.synchronize()
     lat = []
     reports = []
     t_start = time.perf_counter()
     for i in range(args.steps):
         t1 = time.perf_counter()
-        r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=timed)
+        r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=opts)
         lat.append((time.perf_counter() - t1) * 1e3)
         reports.append(r.report)
     torch.cuda.synchronize()
@@ -446,22 +557,56 @@ def main():
     cells = s.cells
     value = cells * args.steps / elapsed / 1e6
     rep = reports[-1]
-    kms = sum(x["timed_kernel_ms"] for x in reports)
-    kl = sum(x["timed_kernel_launches"] for x in reports)
+    host_out = [a.cpu().numpy() for a in out]
+    log(f"timed: {ms_per_step:.3f} ms per projection")
+    # the smoother's per-launch time: untimed projections after the timed
+    # region with kernels launched one by one (HIP events on the library's
+    # stream cannot split a replayed graph)
+    timed = default_options(timed_kernel=args.timed_kernel, **dict(common, graphs=0))
+    kms = kl = 0
+    for _ in range(2):
+        r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=timed)
+        kms += r.report["timed_kernel_ms"]
+        kl += r.report["timed_kernel_launches"]
     avg_ms = kms / max(kl, 1)
+    # the plain FP32 MGPCG arm on the same grid (graphs, same tolerance)
+    plain = {}
+    if args.precision != "fp32":
+        o32 = default_options(**dict(common, precision="fp32"))
+        g.project(phi, u, v, w, s.dt, s.rho, out=out, options=o32)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.sub_steps):
+            r32 = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=o32)
+        torch.cuda.synchronize()
+        ms32 = (time.perf_counter() - t1) / args.sub_steps * 1e3
+        plain = {"ms": round(ms32, 3), "mcells_per_s": round(cells / ms32 / 1e3, 2), "steps": args.sub_steps,
+                 "iterations": r32.report["iterations"], "relative_residual": r32.report["relative_residual"],
+                 "p_fp32": out[0].cpu().numpy()}
+    g.destroy()
     bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps)
     # the timed finest-level passes touch only the cells of the liquid hulls
     active = hull_cells(s.phi)
     alg_bytes = bpc * active
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    # error against a tight FP64-refined solve of the same system (1e-12)
+    # error against the FP64 oracle (MG-PCG to 1e-10 on the same inputs)
     err = None
     if not args.no_error_solve:
-        ref = g.project(phi, u, v, w, s.dt, s.rho, precision=args.precision, relative_tolerance=1e-12)
-        p_ref = ref.pressure.double()
-        err = float(torch.linalg.norm(out[0].double() - p_ref) / torch.linalg.norm(p_ref))
+        t1 = time.perf_counter()
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        op, (ou, ov, ow), _, orep = oracle.uniform_projection(s.phi, s.u, s.v, s.w, s.h, s.dt, s.rho, rel_tol=1e-10)
+        rl = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+        err = {"pressure_rel_l2": rl(host_out[0], op), "u_rel_l2": rl(host_out[1], ou),
+               "v_rel_l2": rl(host_out[2], ov), "w_rel_l2": rl(host_out[3], ow),
+               "oracle": f"oracle/fp_oracle.c FP64 MG-PCG on the same {args.n}^3 inputs to "
+                         f"{orep['relative_residual']:.1e} relative residual"}
+        if "p_fp32" in plain:
+            plain["fp64_oracle_pressure_rel_l2_error"] = rl(plain["p_fp32"], op)
+        log(f"oracle error computed in {time.perf_counter() - t1:.1f} s")
+    plain.pop("p_fp32", None)
     lat_sorted = sorted(lat)
-    line = {
+    return {
         "metric": "MGPCG projection Mcells/s at 1e-6 rel. residual",
         "value": round(value, 2),
         "unit": "Mcells/s",
@@ -476,26 +621,21 @@ def main():
         "data": "synthetic (seeded dam + 8 spheres level set, Gaussian face velocities)",
         "config": {"workload": f"uniform MAC grid {args.n}^3 dam+8 spheres, one projection to 1e-6",
                    "cells": cells, "liquid_cells": int(rep["liquid_samples"]), "precision": args.precision,
-                   "sweeps": args.sweeps},
+                   "sweeps": args.sweeps, "graphs": not args.no_graphs},
         "latency_ms": {"p50": round(statistics.median(lat), 3),
                        "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
         "solve": {"iterations": rep["iterations"], "outer": rep["outer_iterations"],
                   "relative_residual": rep["relative_residual"], "max_div_after": rep["max_div_after"],
-                  "max_div_before": rep["max_div_before"], "fp64_pressure_rel_l2_error": err,
-                  "device_ms": round(rep["device_ms"], 3)},
+                  "max_div_before": rep["max_div_before"], "device_ms": round(rep["device_ms"], 3)},
+        "fp64_oracle_error": err,
+        "plain_fp32_mgpcg": plain or None,
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps)),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc, "hull_cells": active,
-                     "algorithmic_bytes": round(alg_bytes), "launches": kl},
+                     "algorithmic_bytes": round(alg_bytes), "launches": kl,
+                     "measured_in": "2 untimed projections after the timed region, kernels launched one by one"},
     }
-    g.destroy()
-    if not args.no_cpu_baseline:
-        try:
-            line["cpu_baseline"] = cpu_baseline()
-        except Exception as e:  # the baseline is reported, never required
-            line["cpu_baseline"] = {"error": repr(e)}
-    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -28,6 +28,7 @@ struct fp_ctx {
   uint64_t budget = 0;
   uint64_t in_use = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t order = nullptr;    // orders the library stream after a caller's stream
   int grids = 0;
   fp::Projector* proj = nullptr;
   fp::TallProjector* tall = nullptr;
@@ -43,9 +44,9 @@ struct fp_grid {
   uint64_t bytes = 0;
 };
 
-// Tall-cell max neighbour base delta: fp_grid_desc.reserved, 0 meaning the
+// Tall-cell max neighbour base delta: fp_grid_desc.max_delta, 0 meaning the
 // maximumNeighborDelta of the high preset (lib/core/tall-cell-grid.ts:55).
-static int tall_delta(const fp_grid_desc* d) { return d->reserved > 0 ? d->reserved : 4; }
+static int tall_delta(const fp_grid_desc* d) { return d->max_delta > 0 ? d->max_delta : 4; }
 
 #define CHECK_ARG(cond, ...)              \
   do {                                    \
@@ -90,6 +91,7 @@ int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out) {
   }
   // default priority stream (no priority streams on this pool)
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->order, hipEventDisableTiming);
   if (e != hipSuccess) {
     set_error("hipStreamCreate: %s", hipGetErrorString(e));
     delete ctx->proj;
@@ -113,10 +115,24 @@ int fp_ctx_destroy(fp_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   delete ctx->proj;
   delete ctx->tall;
+  if (ctx->order) (void)hipEventDestroy(ctx->order);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return FP_OK;
 }
+
+// A grid with a submitted projection that has not been polled to completion.
+static bool grid_busy(const fp_grid* g) {
+  return g->ctx->server && g->ctx->server->jobs_on(g->uniform ? (const void*)g->uniform : (const void*)g->tall) > 0;
+}
+
+#define CHECK_IDLE(g, what)                                                                      \
+  do {                                                                                           \
+    if (grid_busy(g)) {                                                                          \
+      set_error("%s: the grid has a submitted projection in flight (fp_poll it first)", what);   \
+      return FP_ERR_STATE;                                                                       \
+    }                                                                                            \
+  } while (0)
 
 int fp_ctx_bytes_in_use(const fp_ctx* ctx, uint64_t* bytes) {
   CHECK_ARG(ctx && bytes, "fp_ctx_bytes_in_use: NULL argument");
@@ -150,7 +166,7 @@ static int validate_desc(const fp_grid_desc* d) {
   if (d->layout == FP_LAYOUT_TALL_CELL) {
     CHECK_ARG(d->column_bases != nullptr, "tall-cell grid: column_bases is NULL");
     CHECK_ARG(d->band >= 1 && d->band <= 1024, "tall-cell band rows must be in [1,1024] (got %d)", d->band);
-    CHECK_ARG(d->reserved >= 0 && tall_delta(d) <= d->band, "tall-cell max neighbour delta %d must be in [1, band]",
+    CHECK_ARG(d->max_delta >= 0 && tall_delta(d) <= d->band, "tall-cell max neighbour delta %d must be in [1, band]",
               tall_delta(d));
   }
   return FP_OK;
@@ -220,10 +236,7 @@ int fp_grid_create(fp_ctx* ctx, const fp_grid_desc* desc, fp_grid** out) {
 
 int fp_grid_destroy(fp_grid* g) {
   if (!g) return FP_OK;
-  if (g->ctx->server && g->ctx->server->jobs_on(g->uniform ? (const void*)g->uniform : (const void*)g->tall) > 0) {
-    set_error("fp_grid_destroy: the grid has submitted projections still running (fp_poll them first)");
-    return FP_ERR_STATE;
-  }
+  CHECK_IDLE(g, "fp_grid_destroy");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
@@ -312,6 +325,7 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
     CHECK_ARG(grids[i] != nullptr, "fp_project_batch: grid %d is NULL", i);
     CHECK_ARG(grids[i]->ctx == ctx, "fp_project_batch: grid %d belongs to another context", i);
     for (int j = 0; j < i; ++j) CHECK_ARG(grids[j] != grids[i], "fp_project_batch: grid %d repeated", i);
+    CHECK_IDLE(grids[i], "fp_project");
     int rc = validate_inputs(&in[i], &out[i], o);
     if (rc) return rc;
     if (grids[i]->tall) {
@@ -328,6 +342,16 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
   }
   std::lock_guard<std::mutex> lock(ctx->gpu);
   (void)hipSetDevice(ctx->device);
+  // device work after what the callers queued on their streams
+  for (int i = 0; i < count; ++i) {
+    if (!in[i].stream) continue;
+    hipError_t e = hipEventRecord(ctx->order, (hipStream_t)in[i].stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->order, 0);
+    if (e != hipSuccess) {
+      set_error("fp_project: ordering after the caller's stream: %s", hipGetErrorString(e));
+      return FP_ERR_DEVICE;
+    }
+  }
   for (size_t k = 0; k < list.size(); ++k) {
     int rc = list[k]->bind(ctx->stream, &in[ui[k]], &out[ui[k]], o);
     if (rc) return rc;
@@ -374,7 +398,7 @@ int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options
     ctx->server = new (std::nothrow) fp::Server(ctx->stream, ctx->device, ctx->proj, ctx->tall, &ctx->gpu);
     if (!ctx->server) return FP_ERR_OUT_OF_MEMORY;
   }
-  return ctx->server->submit(g->uniform, g->tall, in, out, o, ticket);
+  return ctx->server->submit(g->uniform, g->tall, in, out, o, ticket);  // FP_ERR_STATE on a busy grid
 }
 
 int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report, double* latency_ms) {
@@ -388,6 +412,7 @@ int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report,
 
 int fp_probe_apply(fp_grid* g, const float* phi, const double* p, double* out) {
   CHECK_ARG(g && phi && p && out, "fp_probe_apply: NULL argument");
+  CHECK_IDLE(g, "fp_probe_apply");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   if (g->tall) return g->ctx->tall->probe_apply(g->ctx->stream, g->tall, phi, p, out);
@@ -397,6 +422,7 @@ int fp_probe_apply(fp_grid* g, const float* phi, const double* p, double* out) {
 int fp_probe_rhs(fp_grid* g, const fp_inputs* in, double* b) {
   CHECK_ARG(g && in && b && in->phi && in->u && in->v && in->w, "fp_probe_rhs: NULL argument");
   CHECK_ARG(in->memory == FP_MEMORY_HOST, "fp_probe_rhs takes host arrays");
+  CHECK_IDLE(g, "fp_probe_rhs");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   if (g->tall) return g->ctx->tall->probe_rhs(g->ctx->stream, g->tall, in, b);
@@ -408,6 +434,7 @@ int fp_probe_rhs(fp_grid* g, const fp_inputs* in, double* b) {
 int fp_probe_vcycle(fp_grid* g, const float* phi, const fp_options* o, const float* r, float* z) {
   CHECK_ARG(g && phi && o && r && z, "fp_probe_vcycle: NULL argument");
   CHECK_ARG(g->uniform != nullptr, "fp_probe_vcycle: uniform grids only");
+  CHECK_IDLE(g, "fp_probe_vcycle");
   CHECK_ARG(o->precision >= 0 && o->precision <= 2, "unknown precision %d", o->precision);
   CHECK_ARG(o->pre_sweeps == o->post_sweeps && o->pre_sweeps >= 0 && o->pre_sweeps <= 16,
             "pre and post sweeps must match and lie in [0,16]");
@@ -445,6 +472,7 @@ int fp_probe_vcycle(fp_grid* g, const float* phi, const fp_options* o, const flo
 int fp_probe_level_phi(fp_grid* g, const float* phi, int32_t level, int32_t preserve, float* out) {
   CHECK_ARG(g && phi && out, "fp_probe_level_phi: NULL argument");
   CHECK_ARG(g->uniform != nullptr, "fp_probe_level_phi: uniform grids only");
+  CHECK_IDLE(g, "fp_probe_level_phi");
   CHECK_ARG(level >= 0 && level < g->uniform->levels(), "level %d out of range [0,%d)", level, g->uniform->levels());
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
@@ -457,9 +485,22 @@ int fp_probe_level_phi(fp_grid* g, const float* phi, int32_t level, int32_t pres
   return g->ctx->proj->probe_level_phi(g->ctx->stream, g->uniform, preserve, level, out);
 }
 
+int fp_probe_taps(fp_grid* g, int32_t level, int32_t axis, int32_t* prolong_coarse, float* prolong_weight,
+                  int32_t* restrict_fine, float* restrict_weight) {
+  CHECK_ARG(g && prolong_coarse && prolong_weight && restrict_fine && restrict_weight, "fp_probe_taps: NULL argument");
+  CHECK_ARG(g->uniform != nullptr, "fp_probe_taps: uniform grids only");
+  CHECK_ARG(level >= 1 && level < g->uniform->levels(), "level %d out of range [1,%d)", level, g->uniform->levels());
+  CHECK_ARG(axis >= 0 && axis < 3, "axis %d out of range [0,3)", axis);
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
+  (void)hipSetDevice(g->ctx->device);
+  return fp::probe_taps(g->ctx->stream, g->uniform, level, axis, prolong_coarse, prolong_weight, restrict_fine,
+                        restrict_weight);
+}
+
 int fp_probe_tall_layout(fp_grid* g, const float* phi, int32_t* rows, uint8_t* unknown) {
   CHECK_ARG(g && phi && rows && unknown, "fp_probe_tall_layout: NULL argument");
   CHECK_ARG(g->tall != nullptr, "fp_probe_tall_layout: tall-cell grids only");
+  CHECK_IDLE(g, "fp_probe_tall_layout");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   return g->ctx->tall->probe_layout(g->ctx->stream, g->tall, phi, rows, unknown);

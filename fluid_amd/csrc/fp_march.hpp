@@ -453,7 +453,7 @@ struct OpResidualW {
 struct OpTrueResidual {
   struct Cell { double x; float p; };
   static constexpr int DEPTH = 1;  // depth 2 is no faster
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;     // ||b - A x||^2 and the sum of x (null-space mean)
   static constexpr bool WITH_MAX = true;
   // r is read on liquid cells (the sweeps' own right-hand side) and over the
   // hulls (CG update, r.z)
@@ -463,6 +463,7 @@ struct OpTrueResidual {
   const double* b;
   const double* xv;
   float* r;
+  double bmean;  // mean of b removed on an all-liquid grid (else 0)
   bool x_zero;  // the iterate is still zero: A x is not loaded
   static __device__ __forceinline__ OpTrueResidual make(const GridRec* Gp, const MarchLaunch& la) {
     const LevelRec l0 = cload(&Gp->lv[0]);
@@ -473,6 +474,7 @@ struct OpTrueResidual {
     o.b = cload(&Gp->b);
     o.xv = cload(&Gp->x);
     o.r = cload(&Gp->r);
+    o.bmean = cload(&cload(&Gp->st)->bmean);
     o.x_zero = la.which_e == 1 && !cload(&Gp->warm);
     return o;
   }
@@ -502,8 +504,9 @@ struct OpTrueResidual {
 #pragma unroll
       for (int f = 0; f < 6; ++f)
         if (has[f]) ap += face_weight_d(c.p, nb[f].p) * (c.x - nb[f].x);
-      res = bv - ap;
+      res = (bv - bmean) - ap;
       acc[0] += res * res;
+      acc[1] += c.x;
       *mx = fmaxf(*mx, (float)fabs(res));
     }
     stg(r, i, (float)res);

@@ -39,8 +39,25 @@ int Server::submit(UniformGrid* g, TallGrid* t, const fp_inputs* in, const fp_ou
   j->out = *out;
   j->opt = *opt;
   j->t_submit = wall_seconds();
+  const void* key = g ? (const void*)g : (const void*)t;
   {
     std::lock_guard<std::mutex> lk(mu_);
+    // one projection in flight per grid: jobs share the grid's buffers
+    if (jobs_on_locked(key) > 0) {
+      set_error("fp_submit: the grid has a submitted projection in flight (fp_poll it first)");
+      return FP_ERR_STATE;
+    }
+    if (in->stream) {
+      // the worker's launches for this job wait for the caller's queued work
+      (void)hipSetDevice(device_);
+      hipError_t e = hipEventCreateWithFlags(&j->ready, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(j->ready, (hipStream_t)in->stream);
+      if (e != hipSuccess) {
+        if (j->ready) (void)hipEventDestroy(j->ready);
+        set_error("fp_submit: ordering after the caller's stream: %s", hipGetErrorString(e));
+        return FP_ERR_DEVICE;
+      }
+    }
     j->ticket = next_ticket_++;
     *ticket = j->ticket;
     incoming_.push_back(std::move(j));
@@ -51,6 +68,10 @@ int Server::submit(UniformGrid* g, TallGrid* t, const fp_inputs* in, const fp_ou
 
 int Server::jobs_on(const void* g) {
   std::lock_guard<std::mutex> lk(mu_);
+  return jobs_on_locked(g);
+}
+
+int Server::jobs_on_locked(const void* g) {
   int n = 0;
   for (const auto& j : incoming_) n += (const void*)j->grid == g || (const void*)j->tall == g;
   for (const auto& j : active_) n += (const void*)j->grid == g || (const void*)j->tall == g;
@@ -97,6 +118,8 @@ void Server::loop() {
       work_cv_.wait(lk, [&] { return stop_ || !incoming_.empty() || !active_.empty(); });
       if (stop_) return;
       while (!incoming_.empty()) {
+        Job* j = incoming_.front().get();
+        if (j->ready) (void)hipStreamWaitEvent(stream_, j->ready, 0);
         active_.push_back(std::move(incoming_.front()));
         incoming_.pop_front();
       }
@@ -122,6 +145,10 @@ void Server::loop() {
       }
       for (auto it = active_.begin(); it != active_.end();) {
         if ((*it)->stage == Job::DONE) {
+          if ((*it)->ready) {
+            (void)hipEventDestroy((*it)->ready);
+            (*it)->ready = nullptr;
+          }
           const uint64_t t = (*it)->ticket;
           done_[t] = std::move(*it);
           it = active_.erase(it);

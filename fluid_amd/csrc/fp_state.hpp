@@ -60,17 +60,34 @@ __global__ void k_fin_rhs(const P* parts) {
   const P p = parts[blockIdx.x];
   const auto& G = *p.g;
   State* st = G.st;
-  double out[2];
+  double out[3];
   float mx;
-  sum_partials_max<2>(G.partials, p.nblocks, out, mx);
+  sum_partials_max<3>(G.partials, p.nblocks, out, mx);
   if (threadIdx.x == 0) {
     st->divmax_before = __float_as_uint(mx);
-    st->bb = out[0];
+    const long long liquid = (long long)out[1];
+    st->liquid = liquid;
+    double bb = out[0];
+    // Every sample liquid (a tall-cell grid: and every band up to the lid):
+    // no free-surface (Dirichlet) face anchors the pressure, A is singular
+    // with the constants as its null space
+    // (sparse-atlas-composite-projection.ts:1086-1146,1194-1211).  Walls are
+    // the only other boundary, and a liquid component that is not the whole
+    // domain has a neighbour outside it, which is air, so this is the only
+    // unanchored component a grid can have.  The mean of b is removed (the
+    // system becomes consistent) and the pressure is returned with zero mean.
+    if (liquid > 0 && liquid == G.ns_liquid) {
+      const double mean = out[2] / (double)liquid;
+      st->nullspace = 1;
+      st->bmean = mean;
+      st->flags |= 0x10;  // FP_FLAG_UNANCHORED_COMPONENT
+      bb = fmax(0.0, bb - (double)liquid * mean * mean);
+    }
+    st->bb = bb;
     // The refinement stops at a quarter of the requested relative residual:
     // the max-norm divergence left in the FP32 output then stays below that
     // of an FP64 solve stopped at the requested tolerance.
-    st->tol2_bb = 0.0625 * G.tol2 * out[0];
-    st->liquid = (long long)out[1];
+    st->tol2_bb = 0.0625 * G.tol2 * bb;
   }
 }
 
@@ -82,12 +99,13 @@ __global__ void k_fin_true(const P* parts, int new_pass) {
   const P p = parts[blockIdx.x];
   State* st = p.g->st;
   if (st->outer_done) return;
-  double out[1];
+  double out[2];
   float m;
-  sum_partials_max<1>(p.g->partials, p.nblocks, out, m);
+  sum_partials_max<2>(p.g->partials, p.nblocks, out, m);
   if (threadIdx.x == 0) {
     const double rr = out[0];
     st->rr64 = rr;
+    st->xsum = out[1];
     const bool finite = isfinite(rr);
     st->outer_done = (!finite) || rr <= st->tol2_bb;
     if (!finite) st->flags |= 0x1;  // FP_FLAG_NONFINITE

@@ -615,7 +615,7 @@ __global__ void k_tall_rhs(const TallPart* parts, int np, int mode) {
   if (mode == 1 && (!s.u || !s.v || !s.w)) return;
   double* __restrict__ bv = G.b;
   const double sf = G.s_flux, inv_h = G.inv_h;
-  double acc[2] = {0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0};
   float dmax = 0.0f;
   TALL_LINES(g) {
     TALL_LINE(g, idx, x, py, z);
@@ -629,6 +629,7 @@ __global__ void k_tall_rhs(const TallPart* parts, int np, int mode) {
         dmax = fmaxf(dmax, (float)(fabs(f) * inv_h));
         acc[0] += val * val;
         acc[1] += 1.0;
+        acc[2] += val;
       }
       if (mode == 0) bv[idx] = val;
     } else {
@@ -641,13 +642,14 @@ __global__ void k_tall_rhs(const TallPart* parts, int np, int mode) {
       if (ut) dmax = fmaxf(dmax, (float)(fabs(ft) * inv_h / support));
       acc[0] += vb * vb + vt * vt;
       acc[1] += (ub ? 1.0 : 0.0) + (ut ? 1.0 : 0.0);
+      acc[2] += vb + vt;
       if (mode == 0) {
         bv[idx] = vb;
         bv[g.sidx(x, 1, z)] = vt;
       }
     }
   }
-  block_partials_max<2>(acc, dmax, G.partials, blk, nblk);
+  block_partials_max<3>(acc, dmax, G.partials, blk, nblk);
 }
 
 // max |div| of the projected field into State::divmax_after.
@@ -655,9 +657,9 @@ template <class P>
 __global__ void k_tall_fin_divafter(const P* parts) {
   const P p = parts[blockIdx.x];
   if (!p.g->out_u || !p.g->out_v || !p.g->out_w) return;
-  double out[2];
+  double out[3];
   float mx;
-  sum_partials_max<2>(p.g->partials, p.nblocks, out, mx);
+  sum_partials_max<3>(p.g->partials, p.nblocks, out, mx);
   if (threadIdx.x == 0) p.g->st->divmax_after = __float_as_uint(mx);
 }
 
@@ -674,7 +676,8 @@ __global__ void k_tall_true(const TallPart* parts, int np) {
   const double* __restrict__ bv = G.b;
   float* __restrict__ r = G.r;
   const VecD xv{G.x};
-  double acc[1] = {0.0};
+  const double bmean = G.st->bmean;  // removed on an unanchored grid (else 0)
+  double acc[2] = {0.0, 0.0};
   float rmax = 0.0f;
   TALL_LINES(g) {
     TALL_LINE(g, idx, x, py, z);
@@ -685,7 +688,8 @@ __global__ void k_tall_true(const TallPart* parts, int np) {
       if (is_unknown(g, phi, b, x, py, z)) {
         double ap, dg;
         band_row<double>(g, bases, phi, x, py, z, b, xv, ap, dg);
-        res = bv[idx] - ap;
+        res = (bv[idx] - bmean) - ap;
+        acc[1] += xv(idx);
       }
       r[idx] = (float)res;
       acc[0] += res * res;
@@ -693,15 +697,17 @@ __global__ void k_tall_true(const TallPart* parts, int np) {
     } else {
       const PairOut<double> o = tall_pair<double>(g, bases, phi, G.cols, x, z, b, xv);
       const size_t it = g.sidx(x, 1, z);
-      const double rb = is_unknown(g, phi, b, x, 0, z) ? bv[idx] - o.ab : 0.0;
-      const double rt = is_unknown(g, phi, b, x, 1, z) ? bv[it] - o.at : 0.0;
+      const bool ub = is_unknown(g, phi, b, x, 0, z), ut = is_unknown(g, phi, b, x, 1, z);
+      const double rb = ub ? (bv[idx] - bmean) - o.ab : 0.0;
+      const double rt = ut ? (bv[it] - bmean) - o.at : 0.0;
       r[idx] = (float)rb;
       r[it] = (float)rt;
       acc[0] += rb * rb + rt * rt;
+      acc[1] += (ub ? xv(idx) : 0.0) + (ut ? xv(it) : 0.0);
       rmax = fmaxf(rmax, fmaxf(fabsf((float)rb), fabsf((float)rt)));
     }
   }
-  block_partials_max<1>(acc, rmax, G.partials, blk, nblk);
+  block_partials_max<2>(acc, rmax, G.partials, blk, nblk);
 }
 
 // Block-Jacobi sweep: out = in + omega B^-1 (r - A in); first: in = 0.
@@ -947,9 +953,12 @@ __global__ void k_tall_pressure_out(const TallPart* parts, int np) {
   float* __restrict__ p = G.out_p;
   if (!p) return;
   const TallGeo g = geo_of(G);
+  // an unanchored grid's pressure is defined up to a constant: its mean is 0
+  const State* st = G.st;
+  const double xmean = st->nullspace ? st->xsum / (double)st->liquid : 0.0;
   TALL_LINES(g) {
     TALL_LINE(g, idx, x, py, z);
-    p[idx] = is_unknown(g, G.phi, G.bases[g.col(x, z)], x, py, z) ? (float)G.x[idx] : 0.0f;
+    p[idx] = is_unknown(g, G.phi, G.bases[g.col(x, z)], x, py, z) ? (float)(G.x[idx] - xmean) : 0.0f;
   }
 }
 
@@ -1207,6 +1216,11 @@ int TallGrid::create(int nx, int ny, int nz, int band, int max_delta, double h, 
   R.h = h;
   R.n = (long long)nx * R.P * nz;
   R.ncol = (long long)nx * nz;
+  // no air row above any band: all in-domain samples liquid leave the
+  // pressure unanchored (rows past the lid are outside the domain)
+  R.ns_liquid = 0;
+  for (int i = 0; i < nx * nz && R.ns_liquid >= 0; ++i)
+    R.ns_liquid = bases_host[i] + band >= ny ? R.ns_liquid + 2 + (ny - bases_host[i]) : -1;
   R.nu = (long long)(nx + 1) * R.S * nz;
   R.nv = R.n;
   R.nw = (long long)nx * R.S * (nz + 1);

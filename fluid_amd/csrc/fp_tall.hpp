@@ -56,6 +56,8 @@ struct TallRec {
   int lb;              // first lattice level of the V-cycle's one-workgroup bottom (L-1: the coarsest only)
   int zfinal;          // V-cycle buffer (0: z, 1: z2) holding M^-1 r
   long long n, nu, nv, nw, ncol;
+  long long ns_liquid;  // unknowns at which no air face anchors the pressure: every in-domain
+                        // sample, when every column's band reaches the lid (else -1: never)
   double h;
   const int* bases;    // (nx, nz)
   float* phi;          // samples (private copy)

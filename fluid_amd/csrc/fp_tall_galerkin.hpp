@@ -484,7 +484,10 @@ __global__ void k_lat_prolong(const TallPart* parts, int np, int level, int sel)
   FP_STRIDE(i, (size_t)G.lv[level].n) lat_prolong_node(G, level, sel, i);
 }
 
-// Dense FP64 inverse of the coarsest lattice (one workgroup per grid).
+// Dense FP64 inverse of the coarsest lattice (one workgroup per grid).  As on
+// uniform grids, the vanishing last pivot of a component without an air face
+// (a grid that is liquid everywhere, its band up to the lid) gets its row's
+// original diagonal added: the inverse of A + a_pp e_p e_p^T, SPD.
 __global__ void k_tall_coarse_setup(const TallPart* parts) {
   const TallRec& G = *parts[blockIdx.x].g;
   const TallLevel& L = G.lv[G.L - 1];
@@ -492,6 +495,7 @@ __global__ void k_tall_coarse_setup(const TallPart* parts) {
   int* __restrict__ cmap = G.cmap;
   double* __restrict__ a = G.minv;
   __shared__ int s_n, s_fail;
+  __shared__ double s_diag[kTallMaxCoarse];
   const size_t n = (size_t)L.n;
   if (threadIdx.x == 0) {
     int c = 0;
@@ -523,8 +527,11 @@ __global__ void k_tall_coarse_setup(const TallPart* parts) {
     lattice_row_entries(L, K, K, K, x, y, z, emit);
   }
   __syncthreads();
+  for (int r = threadIdx.x; r < m; r += blockDim.x) s_diag[r] = a[(size_t)r * m + r];
+  __syncthreads();
   for (int p = 0; p < m; ++p) {
-    const double piv = a[p * m + p];
+    double piv = a[p * m + p];
+    if (piv <= 1e-9 * s_diag[p] && piv > -1e-9 * s_diag[p]) piv += s_diag[p];  // singular component
     if (!(piv > 0.0) && threadIdx.x == 0) s_fail = 1;
     __syncthreads();
     if (s_fail) break;

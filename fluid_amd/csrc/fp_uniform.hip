@@ -125,7 +125,7 @@ __global__ void k_rhs(const Part* parts, int np) {
   const float* __restrict__ q = G.in_q;
   double* __restrict__ b = G.b;
   const double s_flux = G.s_flux, s_q = G.s_q, inv_h = G.inv_h;
-  double acc[2] = {0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0};
   float dmax = 0.0f;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   FP_STRIDE(idx, g.n()) {
@@ -145,10 +145,11 @@ __global__ void k_rhs(const Part* parts, int np) {
       val = -s_flux * flux + s_q * qq;
       dmax = fmaxf(dmax, (float)fabs(flux * inv_h - qq));
       acc[0] += val * val;
+      acc[2] += val;
     }
     b[idx] = val;
   }
-  block_partials_max<2>(acc, dmax, G.partials, blk, nblk);
+  block_partials_max<3>(acc, dmax, G.partials, blk, nblk);
 }
 
 // x += e; e = 0 (folds one inner solve into the FP64 iterate).  A broken CG
@@ -174,6 +175,19 @@ __global__ __launch_bounds__(256) void k_accumulate(const Part* parts, int np) {
 // --------------------------------------------------------------------------
 // multigrid setup: coarse level set (CM11a Eq. 15-16 with C preserved levels)
 // --------------------------------------------------------------------------
+// FP32 sum of eight corner terms in the pairing of mgD4Sum8
+// (lib/methods/uniform/webgpu-uniform-pressure-multigrid.wgsl.ts:50-54); the
+// additions are rounded one by one (no contraction), so the CPU restatement
+// reproduces every bit.
+__device__ __forceinline__ float d4_sum8(const float* t) {
+  const float y0 = __fadd_rn(__fadd_rn(t[0], t[5]), __fadd_rn(t[1], t[4]));
+  const float y1 = __fadd_rn(__fadd_rn(t[2], t[7]), __fadd_rn(t[3], t[6]));
+  return __fadd_rn(y0, y1);
+}
+
+// Coarse level set of mgDownsampleTopology (…multigrid.wgsl.ts:169-195): the
+// mean of the eight children, or the mean of the non-negative ones when their
+// signs are mixed and the destination is one of the C finest coarse levels.
 __global__ void k_downsample_phi(const Part* parts, int np, int level, int preserve) {
   FP_PART(parts, np);
   const LevelRec& fl = G.lv[level - 1];
@@ -186,36 +200,42 @@ __global__ void k_downsample_phi(const Part* parts, int np, int level, int prese
   FP_STRIDE(idx, c.n()) {
     int I, J, K;
     c.ijk(idx, I, J, K);
-    float sum = 0.0f, pos = 0.0f;
-    int cnt = 0, npos = 0, nneg = 0;
-    for (int oi = 0; oi < sx; ++oi) {
-      const int i = I * sx + oi;
-      if (i >= f.nx) continue;
-      for (int oj = 0; oj < sy; ++oj) {
-        const int j = J * sy + oj;
-        if (j >= f.ny) continue;
-        for (int ok = 0; ok < sz; ++ok) {
-          const int k = K * sz + ok;
-          if (k >= f.nz) continue;
-          const float v = pf[((size_t)i * f.ny + j) * f.nz + k];
-          sum += v;
-          ++cnt;
-          if (v >= 0.0f) { pos += v; ++npos; } else ++nneg;
-        }
-      }
+    // the eight corner children (bit 0: x, 1: y, 2: z); an axis that did not
+    // coarsen (stride 1) and the last cell of an odd axis repeat one child
+    float v[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int i = min(I * sx + (o & 1) * (sx - 1), f.nx - 1);
+      const int j = min(J * sy + ((o >> 1) & 1) * (sy - 1), f.ny - 1);
+      const int k = min(K * sz + ((o >> 2) & 1) * (sz - 1), f.nz - 1);
+      v[o] = pf[((size_t)i * f.ny + j) * f.nz + k];
     }
-    const bool mixed = npos > 0 && nneg > 0;
-    pc[idx] = (mixed && use_positive) ? pos / (float)npos : sum / (float)cnt;
+    float ps[8], pn[8], ng[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      ps[o] = v[o] >= 0.0f ? v[o] : 0.0f;
+      pn[o] = v[o] >= 0.0f ? 1.0f : 0.0f;
+      ng[o] = v[o] >= 0.0f ? 0.0f : 1.0f;
+    }
+    const float sum = d4_sum8(v), pos = d4_sum8(ps), npos = d4_sum8(pn), nneg = d4_sum8(ng);
+    const bool mixed = npos > 0.0f && nneg > 0.0f;
+    pc[idx] = (mixed && use_positive) ? __fdiv_rn(pos, fmaxf(npos, 1.0f)) : __fdiv_rn(sum, 8.0f);
   }
 }
 
 // Dense inverse of the coarsest level's liquid block (one workgroup, FP64
-// Gauss-Jordan in place; SPD so no pivoting).  A failed pivot falls back to
-// the diagonal (Jacobi) map so the preconditioner stays a fixed SPD map.
+// Gauss-Jordan in place; SPD so no pivoting).  A component of the block with
+// no air face (an all-liquid level) is singular, its last pivot vanishes to
+// rounding: that pivot gets its row's original diagonal added, which inverts
+// A + a_pp e_p e_p^T instead (SPD, exact on the range of A).  A negative or
+// non-finite pivot falls back to the diagonal (Jacobi) map so the
+// preconditioner stays a fixed SPD map.
+constexpr int kCoarseMax = 512;
 __device__ void coarse_setup(LevelArgs L, int* __restrict__ cmap, int* __restrict__ ncount,
                              double* __restrict__ a, int max_n, State* st) {
   __shared__ int s_n;
   __shared__ int s_fail;
+  __shared__ double s_diag[kCoarseMax];
   const Grid3 g = L.g;
   const size_t N = g.n();
   if (threadIdx.x == 0) {
@@ -264,10 +284,12 @@ __device__ void coarse_setup(LevelArgs L, int* __restrict__ cmap, int* __restric
       }
     }
     a[e] = val;
+    if (r == c) s_diag[r] = val;
   }
   __syncthreads();
   for (int p = 0; p < n; ++p) {
-    const double piv = a[p * n + p];
+    double piv = a[p * n + p];
+    if (piv <= 1e-9 * s_diag[p] && piv > -1e-9 * s_diag[p]) piv += s_diag[p];  // singular component
     if (!(piv > 0.0)) {
       if (threadIdx.x == 0) s_fail = 1;
     }
@@ -1088,7 +1110,10 @@ __global__ void k_pressure_out(const Part* parts, int np) {
   if (!p) return;
   const float* __restrict__ phi = G.lv[0].phi;
   const double* __restrict__ x = G.x;
-  FP_STRIDE(i, (size_t)G.n) p[i] = phi[i] < 0.0f ? (float)x[i] : 0.0f;
+  // an all-liquid grid's pressure is defined up to a constant: its mean is 0
+  const State* st = G.st;
+  const double xmean = st->nullspace ? st->xsum / (double)st->liquid : 0.0;
+  FP_STRIDE(i, (size_t)G.n) p[i] = phi[i] < 0.0f ? (float)(x[i] - xmean) : 0.0f;
 }
 
 // axis 0: faces (nx+1, ny, nz); 1: (nx, ny+1, nz); 2: (nx, ny, nz+1).
@@ -1344,6 +1369,7 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
   id_ = g_next_grid_id.fetch_add(1);
   R.nx = nx; R.ny = ny; R.nz = nz; R.L = L; R.h = h;
   R.n = (long long)nx * ny * nz;
+  R.ns_liquid = R.n;
   R.nu = (long long)(nx + 1) * ny * nz;
   R.nv = (long long)nx * (ny + 1) * nz;
   R.nw = (long long)nx * ny * (nz + 1);
@@ -1401,7 +1427,7 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
       break;
     }
   const LevelRec& c = R.lv[L - 1];
-  R.max_coarse = (int)std::min<long long>(c.n, 512);
+  R.max_coarse = (int)std::min<long long>(c.n, kCoarseMax);
   if ((rc = alloc((void**)&R.cmap, (size_t)R.max_coarse * 4 + 16))) return rc;
   if ((rc = alloc((void**)&R.ncoarse, 16))) return rc;
   if ((rc = alloc((void**)&R.minv, (size_t)R.max_coarse * R.max_coarse * 8 + 16))) return rc;
@@ -1844,16 +1870,12 @@ int Projector::ensure_graph(hipStream_t s, Plan& p) {
   return FP_OK;
 }
 
-int Projector::poll(hipStream_t s, Plan& p, int* counts) {
-  if (!dev_counts_) {
-    HIPCHK(hipMalloc((void**)&dev_counts_, 16));
-    HIPCHK(hipHostMalloc((void**)&host_counts_, 16, hipHostMallocDefault));
-  }
-  hipLaunchKernelGGL(k_status<Part>, dim3(1), dim3(FP_BLOCK), 0, s, p.one.dev, p.one.n, dev_counts_);
-  HIPCHK(hipMemcpyAsync(host_counts_, dev_counts_, 8, hipMemcpyDeviceToHost, s));
+int Projector::poll(hipStream_t s, Plan& p, int* dev_counts, int* host_counts, int* counts) {
+  hipLaunchKernelGGL(k_status<Part>, dim3(1), dim3(FP_BLOCK), 0, s, p.one.dev, p.one.n, dev_counts);
+  HIPCHK(hipMemcpyAsync(host_counts, dev_counts, 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  counts[0] = host_counts_[0];
-  counts[1] = host_counts_[1];
+  counts[0] = host_counts[0];
+  counts[1] = host_counts[1];
   return FP_OK;
 }
 
@@ -1878,22 +1900,11 @@ int Projector::setup_tables(hipStream_t s, Plan& p, int preserve) {
   return FP_OK;
 }
 
-int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps) {
-  if (grids.empty()) return FP_OK;
-  for (UniformGrid* g : grids) {
-    g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
-    int rc = g->upload(s);
-    if (rc) return rc;
-  }
-  Plan* pp = plan_for(grids, opt);
-  if (!pp) return FP_ERR_OUT_OF_MEMORY;
-  Plan& p = *pp;
-  timed_kernel_ = opt->timed_kernel;
-  timer_used_ = 0;
-  hipEvent_t ev0, ev1;
-  HIPCHK(hipEventCreate(&ev0));
-  HIPCHK(hipEventCreate(&ev1));
-  HIPCHK(hipEventRecord(ev0, s));
+// The solve of a prepared plan: set-up, rhs, FP64 refinement passes around
+// the CG + V-cycle iterations (replayed as graphs when `graphs`), gradient
+// subtract and outputs.  dev_counts / host_counts: 8 bytes each for the
+// host's convergence polls.
+int Projector::solve(hipStream_t s, Plan& p, const fp_options* opt, bool graphs, int* dev_counts, int* host_counts) {
   FLAUNCH(k_reset_state<Part>, p.one, s);
   setup_tables(s, p, opt->phi_preserve_levels);
   PLAUNCH(k_rhs, p.cells, s);
@@ -1904,14 +1915,13 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   const int max_outer = std::max(1, opt->max_outer);
   const int max_iter = std::max(1, opt->max_iterations);
   const int check = std::max(1, opt->check_every);
-  const bool graphs = opt->graphs && timed_kernel_ == 0;
   int counts[2] = {0, 0};
   for (int pass = 0; pass <= max_outer; ++pass) {
     const int new_pass = pass < max_outer;
     TIMED(FP_KERNEL_TRUE_RESIDUAL,
           PLAUNCH(k_march<OpTrueResidual>, p.march0, s, 2, MarchLaunch{0, pass == 0 ? 1 : 0}));
     FLAUNCH(k_fin_true<Part>, p.march0, s, new_pass);
-    int rc = poll(s, p, counts);
+    int rc = poll(s, p, dev_counts, host_counts, counts);
     if (rc) return rc;
     if (counts[0] == 0 || !new_pass) break;
     if (graphs) {
@@ -1921,7 +1931,7 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
       for (int it = 0; it < max_iter; it += 2) {
         HIPCHK(hipGraphLaunch(p.graph, s));
         if ((it + 2) % period == 0 || it + 2 >= max_iter) {
-          if ((rc = poll(s, p, counts))) return rc;
+          if ((rc = poll(s, p, dev_counts, host_counts, counts))) return rc;
           if (counts[1] == 0) break;
         }
       }
@@ -1929,7 +1939,7 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
       for (int it = 0; it < max_iter; ++it) {
         enqueue_iteration(s, p);
         if ((it + 1) % check == 0 || it + 1 == max_iter) {
-          if ((rc = poll(s, p, counts))) return rc;
+          if ((rc = poll(s, p, dev_counts, host_counts, counts))) return rc;
           if (counts[1] == 0) break;
         }
       }
@@ -1941,6 +1951,54 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   PLAUNCH(k_div_after, p.cells, s);
   FLAUNCH(k_fin_divafter<Part>, p.cells, s);
   PLAUNCH(k_pressure_out, p.cells, s);
+  return FP_OK;
+}
+
+static void fill_report(const State& S, const GridRec& R, float ms, double timed_ms, int timed_launches,
+                        fp_report* rep) {
+  std::memset(rep, 0, sizeof(*rep));
+  int flags = S.flags;
+  if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;  // FP_FLAG_NONCONVERGENCE
+  rep->flags = flags;
+  rep->iterations = S.total_iter;
+  rep->outer_iterations = S.outer;
+  rep->levels = R.L;
+  rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
+  rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
+  float fa, fb;
+  std::memcpy(&fa, &S.divmax_before, 4);
+  std::memcpy(&fb, &S.divmax_after, 4);
+  rep->max_div_before = fa;
+  rep->max_div_after = fb;
+  rep->device_ms = ms;
+  rep->samples = (int64_t)R.n;
+  rep->liquid_samples = (int64_t)S.liquid;
+  rep->timed_kernel_ms = timed_ms;
+  rep->timed_kernel_launches = timed_launches;
+}
+
+int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps) {
+  if (grids.empty()) return FP_OK;
+  for (UniformGrid* g : grids) {
+    g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
+    int rc = g->upload(s);
+    if (rc) return rc;
+  }
+  Plan* pp = plan_for(grids, opt);
+  if (!pp) return FP_ERR_OUT_OF_MEMORY;
+  Plan& p = *pp;
+  if (!dev_counts_) {
+    HIPCHK(hipMalloc((void**)&dev_counts_, 16));
+    HIPCHK(hipHostMalloc((void**)&host_counts_, 16, hipHostMallocDefault));
+  }
+  timed_kernel_ = opt->timed_kernel;
+  timer_used_ = 0;
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, s));
+  int rc = solve(s, p, opt, opt->graphs && timed_kernel_ == 0, dev_counts_, host_counts_);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(ev1, s));
   hipLaunchKernelGGL(k_gather_states<Part>, dim3((unsigned)((grids.size() + 255) / 256)), dim3(256), 0, s, p.one.dev,
                      p.one.n, p.gathered);
@@ -1959,35 +2017,11 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
   const int timed_launches = timer_used_;
   timed_kernel_ = 0;
   timer_used_ = 0;
-  if (reps) {
-    for (size_t i = 0; i < grids.size(); ++i) {
-      const State& S = p.gathered_host[i];
-      const GridRec& R = *grids[i]->host_rec();
-      fp_report* rep = &reps[i];
-      std::memset(rep, 0, sizeof(*rep));
-      int flags = S.flags;
-      if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;  // FP_FLAG_NONCONVERGENCE
-      rep->flags = flags;
-      rep->iterations = S.total_iter;
-      rep->outer_iterations = S.outer;
-      rep->levels = R.L;
-      rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
-      rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
-      float fa, fb;
-      std::memcpy(&fa, &S.divmax_before, 4);
-      std::memcpy(&fb, &S.divmax_after, 4);
-      rep->max_div_before = fa;
-      rep->max_div_after = fb;
-      rep->device_ms = ms;
-      rep->samples = (int64_t)R.n;
-      rep->liquid_samples = (int64_t)S.liquid;
-      rep->timed_kernel_ms = timed_ms;
-      rep->timed_kernel_launches = timed_launches;
-    }
-  }
+  if (reps)
+    for (size_t i = 0; i < grids.size(); ++i)
+      fill_report(p.gathered_host[i], *grids[i]->host_rec(), ms, timed_ms, timed_launches, &reps[i]);
   return FP_OK;
 }
-
 
 // ---- continuous batching ---------------------------------------------------
 static double now_seconds() {
@@ -2243,6 +2277,57 @@ int probe_apply(hipStream_t s, const UniformGrid* g, const float* phi_host, cons
   (void)hipFree(phi);
   (void)hipFree(p);
   (void)hipFree(o);
+  return FP_OK;
+}
+
+// The 1-D transfer taps along one axis: per fine index its prolongation taps
+// (ptaps: two coarse cells, -1 where unused) and per coarse index its
+// restriction taps (restrict_taps: four fine cells, -1 where unused).
+__global__ void k_probe_taps(int s, int nf, int nc, int* pc, float* pw, int* rf, float* rw) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < nf) {
+    int c[2] = {-1, -1};
+    float w[2] = {0.0f, 0.0f};
+    ptaps(t, s, nc, c, w);
+    for (int q = 0; q < 2; ++q) {
+      pc[2 * t + q] = c[q];
+      pw[2 * t + q] = w[q];
+    }
+  }
+  if (t < nc) {
+    int f[4] = {-1, -1, -1, -1};
+    float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    restrict_taps(t, s, nf, nc, f, w);
+    for (int q = 0; q < 4; ++q) {
+      rf[4 * t + q] = f[q];
+      rw[4 * t + q] = w[q];
+    }
+  }
+}
+
+int probe_taps(hipStream_t s, const UniformGrid* g, int level, int axis, int* pc_host, float* pw_host, int* rf_host,
+               float* rw_host) {
+  const LevelRec& fl = g->level(level - 1);
+  const LevelRec& cl = g->level(level);
+  const int nf = fl.d[axis], nc = cl.d[axis], st = cl.s[axis];
+  int* pc = nullptr;
+  int* rf = nullptr;
+  float *pw = nullptr, *rw = nullptr;
+  HIPCHK(hipMalloc(&pc, (size_t)nf * 2 * 4));
+  HIPCHK(hipMalloc(&pw, (size_t)nf * 2 * 4));
+  HIPCHK(hipMalloc(&rf, (size_t)nc * 4 * 4));
+  HIPCHK(hipMalloc(&rw, (size_t)nc * 4 * 4));
+  const unsigned blocks = (unsigned)((std::max(nf, nc) + FP_BLOCK - 1) / FP_BLOCK);
+  hipLaunchKernelGGL(k_probe_taps, dim3(blocks), dim3(FP_BLOCK), 0, s, st, nf, nc, pc, pw, rf, rw);
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipMemcpy(pc_host, pc, (size_t)nf * 2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pw_host, pw, (size_t)nf * 2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rf_host, rf, (size_t)nc * 4 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rw_host, rw, (size_t)nc * 4 * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(pc);
+  (void)hipFree(pw);
+  (void)hipFree(rf);
+  (void)hipFree(rw);
   return FP_OK;
 }
 

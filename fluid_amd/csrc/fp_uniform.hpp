@@ -51,7 +51,9 @@ struct State {
   int outer;             // refinement passes run
   float rscale;          // power-of-two normalisation of the V-cycle input
   unsigned divmax_before, divmax_after;
-  int pad;
+  int nullspace;         // every sample is liquid: A is singular (constants)
+  double bmean;          // mean of b removed on such a grid (0 otherwise)
+  double xsum;           // sum of the iterate over the liquid samples
 };
 
 // Launch geometry of the column-tile kernels on one level.
@@ -85,6 +87,7 @@ struct GridRec {
   int nx, ny, nz, L;
   int lb;             // first level of the V-cycle's one-workgroup bottom (levels >= 1 under kBottomCells)
   long long n, nu, nv, nw;
+  long long ns_liquid;  // liquid samples at which no air face anchors the pressure (all of them)
   double h;
   LevelRec lv[kMaxLevels];
   double* b;          // FP64 right-hand side (h^2-scaled)
@@ -231,6 +234,7 @@ struct Job {
   int rc = FP_OK;
   fp_report report{};
   double t_submit = 0, t_start = 0, t_done = 0;  // host seconds
+  hipEvent_t ready = nullptr;  // recorded on the caller's stream at submit (fp_inputs.stream)
 };
 
 // Device part table of one stage.
@@ -272,7 +276,8 @@ class Projector {
   int enqueue_vcycle(hipStream_t s, Plan& p);
   int enqueue_iteration(hipStream_t s, Plan& p);
   int ensure_graph(hipStream_t s, Plan& p);
-  int poll(hipStream_t s, Plan& p, int* counts);
+  int solve(hipStream_t s, Plan& p, const fp_options* opt, bool graphs, int* dev_counts, int* host_counts);
+  int poll(hipStream_t s, Plan& p, int* dev_counts, int* host_counts, int* counts);
   void timer_begin(hipStream_t s);
   void timer_end(hipStream_t s);
   std::map<std::vector<long long>, Plan*> plans_;
@@ -307,6 +312,7 @@ class Server {
 
  private:
   void loop();
+  int jobs_on_locked(const void* grid);
   hipStream_t stream_;
   int device_;
   Projector* proj_;
@@ -324,5 +330,8 @@ class Server {
 
 // FP64 operator probe out = A p on the grid's finest level (host arrays).
 int probe_apply(hipStream_t s, const UniformGrid* g, const float* phi_host, const double* p_host, double* out_host);
+// Transfer taps between level-1 (fine) and level (coarse) along one axis.
+int probe_taps(hipStream_t s, const UniformGrid* g, int level, int axis, int* pc_host, float* pw_host, int* rf_host,
+               float* rw_host);
 
 }  // namespace fp

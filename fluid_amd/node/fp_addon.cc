@@ -119,25 +119,36 @@ napi_value create_grid(napi_env env, napi_callback_info info) {
   d.ny = (int32_t)get_number(env, argv[1], "ny", 0);
   d.nz = (int32_t)get_number(env, argv[1], "nz", 0);
   d.band = (int32_t)get_number(env, argv[1], "band", 0);
-  d.reserved = (int32_t)get_number(env, argv[1], "maxDelta", 0);
+  d.max_delta = (int32_t)get_number(env, argv[1], "maxDelta", 0);
   d.h = get_number(env, argv[1], "h", 0.0);
   std::vector<int32_t> bases;
   napi_value cb;
   if (get_named(env, argv[1], "columnBases", &cb)) {
-    uint32_t n = 0;
-    napi_get_array_length(env, cb, &n);
+    // an Int32Array of nx * nz tall-cell heights (the reference stores them as
+    // float, so a Float32Array of integral values is accepted and converted)
     bool is_ta = false;
     napi_is_typedarray(env, cb, &is_ta);
-    if (is_ta) {
-      napi_typedarray_type type;
-      size_t len;
-      void* p;
-      napi_value ab;
-      size_t off;
-      napi_get_typedarray_info(env, cb, &type, &len, &p, &ab, &off);
-      bases.resize(len);
-      for (size_t i = 0; i < len; ++i)
-        bases[i] = type == napi_int32_array ? ((int32_t*)p)[i] : (int32_t)((float*)p)[i];
+    napi_typedarray_type type = napi_int8_array;
+    size_t len = 0, off = 0;
+    void* p = nullptr;
+    napi_value ab;
+    if (is_ta) napi_get_typedarray_info(env, cb, &type, &len, &p, &ab, &off);
+    if (!is_ta || (type != napi_int32_array && type != napi_float32_array)) {
+      napi_throw_type_error(env, nullptr, "columnBases must be an Int32Array (or Float32Array)");
+      return nullptr;
+    }
+    if (d.nx <= 0 || d.nz <= 0 || len != (size_t)d.nx * (size_t)d.nz) {
+      napi_throw_range_error(env, nullptr, "columnBases length must be nx * nz");
+      return nullptr;
+    }
+    bases.resize(len);
+    for (size_t i = 0; i < len; ++i) {
+      const double b = type == napi_int32_array ? (double)((int32_t*)p)[i] : (double)((float*)p)[i];
+      if (b != (double)(int32_t)b) {
+        napi_throw_range_error(env, nullptr, "columnBases must hold integers");
+        return nullptr;
+      }
+      bases[i] = (int32_t)b;
     }
     d.column_bases = bases.data();
   }

@@ -42,22 +42,23 @@ KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "
 EXPORTED_SYMBOLS = [
     "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
     "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
-    "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_submit", "fp_poll", "fp_probe_apply",
-    "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout",
+    "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_submit", "fp_poll",
+    "fp_probe_apply",
+    "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
 ]
 PENDING = 1
 
 
 class GridDesc(C.Structure):
     _fields_ = [("layout", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("band", C.c_int32), ("reserved", C.c_int32), ("h", C.c_double),
+                ("band", C.c_int32), ("max_delta", C.c_int32), ("h", C.c_double),
                 ("column_bases", C.c_void_p)]
 
 
 class Inputs(C.Structure):
     _fields_ = [("memory", C.c_int32), ("reserved", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p),
                 ("v", C.c_void_p), ("w", C.c_void_p), ("target_divergence", C.c_void_p),
-                ("dt", C.c_double), ("rho", C.c_double)]
+                ("dt", C.c_double), ("rho", C.c_double), ("stream", C.c_void_p)]
 
 
 class Outputs(C.Structure):
@@ -130,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_probe_vcycle.argtypes = [vp, vp, C.POINTER(Options), vp, vp]
     L.fp_probe_level_phi.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
     L.fp_probe_tall_layout.argtypes = [vp, vp, vp, vp]
+    L.fp_probe_taps.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
     for name in EXPORTED_SYMBOLS:
         if name not in ("fp_last_error", "fp_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -184,6 +186,15 @@ class Context:
 
 
 _default_ctx = None
+
+
+def grid_bytes(dims, precision: str = "fp16") -> int:
+    """Device bytes a uniform grid of these dims holds (fp_grid_bytes)."""
+    nx, ny, nz = (int(x) for x in dims)
+    b = C.c_uint64()
+    d = GridDesc(LAYOUT_UNIFORM, nx, ny, nz, 0, 0, 1.0 / nx, None)
+    _check(load_library().fp_grid_bytes(C.byref(d), PRECISIONS[precision], C.byref(b)))
+    return int(b.value)
 
 
 def default_context() -> Context:
@@ -268,7 +279,13 @@ class UniformPressureProjection:
         for arr, shp in zip((phi, u, v, w), shapes):
             if tuple(arr.shape) != shp:
                 raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
-        ins = Inputs(mem, 0, parts[0][0], parts[1][0], parts[2][0], parts[3][0], parts[4][0], float(dt), float(rho))
+        ins = Inputs(mem, 0, parts[0][0], parts[1][0], parts[2][0], parts[3][0], parts[4][0], float(dt), float(rho),
+                     None)
+        if mem == MEMORY_DEVICE:
+            # the library's work starts after what torch has queued on the
+            # current stream (producers of the inputs, earlier users of `out`)
+            import torch
+            ins.stream = torch.cuda.current_stream(phi.device).cuda_stream
         if mem == MEMORY_HOST:
             p = np.zeros(shapes[0], np.float32)
             uo = np.zeros(shapes[1], np.float32)
@@ -338,6 +355,20 @@ class UniformPressureProjection:
         phi = np.ascontiguousarray(phi, np.float32)
         _check(_lib.fp_probe_level_phi(self._g, phi.ctypes.data, level, preserve, out.ctypes.data))
         return out
+
+    def probe_taps(self, level, axis):
+        """Transfer taps between level-1 and level along axis, as the kernels
+        evaluate them: (prolong coarse index (nf, 2), weight (nf, 2)),
+        (restrict fine index (nc, 4), weight (nc, 4)); index -1 where unused."""
+        dims, _ = self.levels()
+        nf, nc = int(dims[level - 1][axis]), int(dims[level][axis])
+        pc = np.zeros((nf, 2), np.int32)
+        pw = np.zeros((nf, 2), np.float32)
+        rf = np.zeros((nc, 4), np.int32)
+        rw = np.zeros((nc, 4), np.float32)
+        _check(_lib.fp_probe_taps(self._g, level, axis, pc.ctypes.data, pw.ctypes.data, rf.ctypes.data,
+                                  rw.ctypes.data))
+        return (pc, pw), (rf, rw)
 
     def destroy(self):
         if self._g:

@@ -47,7 +47,8 @@ class ServeLoop:
     statistics."""
 
     def __init__(self, rate_hz: float, duration_s: float, mix=None, steps: int = 20, seed: int = 0,
-                 hbm_budget_bytes: int = 200 << 30, precision: str = "fp16", tolerance: float = 1e-6):
+                 hbm_budget_bytes: int = 200 << 30, precision: str = "fp16", tolerance: float = 1e-6,
+                 keep_first: bool = False):
         import torch
         self.torch = torch
         self.rate = rate_hz
@@ -59,6 +60,10 @@ class ServeLoop:
         self.opts_cold = fpl.default_options(precision=precision, relative_tolerance=tolerance)
         self.opts_warm = fpl.default_options(precision=precision, relative_tolerance=tolerance, warm_start=1)
         self.dev = torch.device("cuda:0")
+        # keep_first: the first served step of each size class (its scene and
+        # host copies of the outputs) is kept in self.first for checking
+        self.keep_first = keep_first
+        self.first = {}
 
     def _arrivals(self):
         t, out = 0.0, []
@@ -101,7 +106,6 @@ class ServeLoop:
 
     def _submit(self, s: Session, first: bool):
         phi, u, v, w, out = s.tensors
-        self.torch.cuda.current_stream().synchronize()  # host-side step work done before the worker reads
         s.ticket = s.grid.submit(phi, u, v, w, s.scene.dt, s.scene.rho, out=out,
                                  options=self.opts_cold if first else self.opts_warm)
 
@@ -148,6 +152,8 @@ class ServeLoop:
                 progressed = True
                 s.latencies.append(s.ticket.latency_ms)
                 s.iterations.append(res.report["iterations"])
+                if self.keep_first and s.steps_left == self.steps and s.n not in self.first:
+                    self.first[s.n] = (s.scene, tuple(a.cpu().numpy() for a in s.tensors[4]), dict(res.report))
                 if res.report["flags"] != 0 or res.report["relative_residual"] > self.opts_cold.relative_tolerance:
                     raise RuntimeError(f"session {s.sid} step failed: {res.report}")
                 cells_solved += s.cells

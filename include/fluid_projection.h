@@ -44,7 +44,10 @@
  * Errors: every entry point returns FP_OK (0) or a negative FP_ERR_* code and
  * then fp_last_error() describes it (thread-local).  Numerical outcomes are
  * never errors: they are FP_FLAG_* bits in fp_report.flags.
- * Threading: a context is used by one host thread at a time.
+ * Threading: a context is used by one host thread at a time.  A grid has at
+ * most one projection in flight: fp_project, fp_project_batch, fp_submit and
+ * the probes return FP_ERR_STATE for a grid whose submitted projection has
+ * not been polled to completion.
  */
 #ifndef FLUID_PROJECTION_H
 #define FLUID_PROJECTION_H
@@ -56,7 +59,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 1
+#define FP_ABI_VERSION 2
 
 /* return codes */
 #define FP_OK 0
@@ -96,7 +99,8 @@ typedef struct {
   int32_t ny;       /* uniform: cells along y; tall-cell: fine rows (fineNy) */
   int32_t nz;       /* cells along z */
   int32_t band;     /* tall-cell: cubic band rows B (regularLayers); else 0 */
-  int32_t reserved; /* tall-cell: max neighbour base delta D (0: 4, maximumNeighborDelta); else 0 */
+  int32_t max_delta; /* tall-cell: max neighbour base delta D (0: 4, maximumNeighborDelta,
+                        lib/core/tall-cell-grid.ts:55); else 0 */
   double h;         /* cubic cell size (m) */
   const int32_t* column_bases; /* tall-cell: (nx, nz) host array, tall height per column in [2, ny],
                                    neighbours within D (limitNeighboringTallCellBases) */
@@ -112,6 +116,10 @@ typedef struct {
   const float* target_divergence; /* optional desired divergence (1/s) per sample, or NULL */
   double dt;
   double rho;
+  void* stream;     /* optional hipStream_t of the caller (NULL: none).  The projection's
+                       device work starts after the work queued on it before the call
+                       (fp_project, fp_project_batch, fp_submit), so device arrays
+                       produced or reused there are read and written in order. */
 } fp_inputs;
 
 typedef struct {
@@ -221,6 +229,14 @@ int fp_probe_rhs(fp_grid* grid, const fp_inputs* in, double* b);
 int fp_probe_vcycle(fp_grid* grid, const float* phi, const fp_options* options, const float* r, float* z);
 /* The level-l level set of the multigrid hierarchy (FP32). */
 int fp_probe_level_phi(fp_grid* grid, const float* phi, int32_t level, int32_t phi_preserve_levels, float* out);
+/* Transfer taps between level-1 (fine) and level (coarse) along one axis, as
+ * the kernels evaluate them: per fine index f its two prolongation taps
+ * (prolong_coarse[2f+q], prolong_weight[2f+q]; coarse -1 where unused), per
+ * coarse index c its four restriction taps (restrict_fine[4c+q],
+ * restrict_weight[4c+q]; fine -1 where unused).  The reference's map is
+ * mgFineChild / mgTrilinearPressure (…multigrid.wgsl.ts:71-76,217-241). */
+int fp_probe_taps(fp_grid* grid, int32_t level, int32_t axis, int32_t* prolong_coarse, float* prolong_weight,
+                  int32_t* restrict_fine, float* restrict_weight);
 /* Tall-cell layout maps: the fine row of every packed sample and whether it
  * is an unknown (liquid and inside the domain).  No reference counterpart
  * (the reference computes these in buildInitialPhi, tall-cell-grid.ts:171-190). */

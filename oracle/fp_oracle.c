@@ -12,6 +12,7 @@
 #include "fp_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -19,6 +20,12 @@
 #define UIDX(i, j, k) (((size_t)(i) * ny + (j)) * nz + (k))
 #define VIDX(i, j, k) (((size_t)(i) * (ny + 1) + (j)) * nz + (k))
 #define WIDX(i, j, k) (((size_t)(i) * ny + (j)) * (nz + 1) + (k))
+
+/* OpenMP threads of the stencil loops (the CPU baseline times 1 and all). */
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 /* cm12GhostFluidTheta, lib/core/cm12-numerics.ts:37-43. */
 double orc_ghost_theta(double liquid_phi, double air_phi, double eps) {
@@ -495,33 +502,42 @@ typedef struct {
   orc_mg_options opt;
 } orc_mg;
 
-/* mgDownsampleTopology phi rule (multigrid.wgsl.ts:169-195): mean of the
- * children, or the mean of the non-negative children when their signs are
- * mixed and the destination is one of the C finest coarse levels. */
+/* FP32 sum of eight corner terms in the pairing of mgD4Sum8
+ * (webgpu-uniform-pressure-multigrid.wgsl.ts:50-54). */
+static float d4_sum8f(const float* t) {
+  float y0 = (t[0] + t[5]) + (t[1] + t[4]);
+  float y1 = (t[2] + t[7]) + (t[3] + t[6]);
+  return y0 + y1;
+}
+
+/* mgDownsampleTopology phi rule (multigrid.wgsl.ts:169-195), in FP32 as the
+ * reference's r32float level sets: the D4-ordered sum of the eight corner
+ * children (mgFineChild, :71-76: an axis of stride 1 repeats its child) over
+ * 8, or the sum of the non-negative children over their count when their
+ * signs are mixed and the destination is one of the C finest coarse levels.
+ * The last coarse cell of an odd axis (lattices the reference rejects, which
+ * this build ceil-halves) repeats its one child the same way. */
 static void downsample_phi(const orc_level* f, orc_level* c, int level, int preserve) {
   for (int I = 0; I < c->d[0]; ++I)
     for (int J = 0; J < c->d[1]; ++J)
       for (int K = 0; K < c->d[2]; ++K) {
-        double sum = 0.0, pos = 0.0;
-        int cnt = 0, npos = 0, nneg = 0;
-        for (int oi = 0; oi < c->s[0]; ++oi) {
-          int i = I * c->s[0] + oi;
-          if (i >= f->d[0]) continue;
-          for (int oj = 0; oj < c->s[1]; ++oj) {
-            int j = J * c->s[1] + oj;
-            if (j >= f->d[1]) continue;
-            for (int ok = 0; ok < c->s[2]; ++ok) {
-              int k = K * c->s[2] + ok;
-              if (k >= f->d[2]) continue;
-              double v = f->phi[((size_t)i * f->d[1] + j) * f->d[2] + k];
-              sum += v; ++cnt;
-              if (v >= 0.0) { pos += v; ++npos; } else ++nneg;
-            }
-          }
+        float v[8], ps[8], pn[8], ng[8];
+        for (int o = 0; o < 8; ++o) {
+          int i = I * c->s[0] + (o & 1) * (c->s[0] - 1);
+          int j = J * c->s[1] + ((o >> 1) & 1) * (c->s[1] - 1);
+          int k = K * c->s[2] + ((o >> 2) & 1) * (c->s[2] - 1);
+          if (i > f->d[0] - 1) i = f->d[0] - 1;
+          if (j > f->d[1] - 1) j = f->d[1] - 1;
+          if (k > f->d[2] - 1) k = f->d[2] - 1;
+          v[o] = (float)f->phi[((size_t)i * f->d[1] + j) * f->d[2] + k];
+          ps[o] = v[o] >= 0.0f ? v[o] : 0.0f;
+          pn[o] = v[o] >= 0.0f ? 1.0f : 0.0f;
+          ng[o] = v[o] >= 0.0f ? 0.0f : 1.0f;
         }
-        int mixed = npos > 0 && nneg > 0;
-        double out = (mixed && level <= preserve) ? pos / npos : sum / cnt;
-        c->phi[((size_t)I * c->d[1] + J) * c->d[2] + K] = out;
+        float sum = d4_sum8f(v), pos = d4_sum8f(ps), npos = d4_sum8f(pn), nneg = d4_sum8f(ng);
+        int mixed = npos > 0.0f && nneg > 0.0f;
+        float out = (mixed && level <= preserve) ? pos / (npos > 1.0f ? npos : 1.0f) : sum / 8.0f;
+        c->phi[((size_t)I * c->d[1] + J) * c->d[2] + K] = (double)out;
       }
 }
 
@@ -543,10 +559,15 @@ static void level_free(orc_level* l) {
   free(l->r); free(l->e); free(l->t);
 }
 
+/* A singular component of the coarsest block (no air face: an all-liquid
+ * level) leaves a pivot that vanishes to rounding; it gets its row's
+ * original diagonal added (the factor of A + a_jj e_j e_j^T, SPD). */
 static int cholesky(double* a, int n) {
   for (int j = 0; j < n; ++j) {
-    double s = a[j * n + j];
+    const double d0 = a[j * n + j];
+    double s = d0;
     for (int k = 0; k < j; ++k) s -= a[j * n + k] * a[j * n + k];
+    if (s <= 1e-9 * d0 && s > -1e-9 * d0) s += d0;
     if (!(s > 0.0)) return -1;
     double l = sqrt(s);
     a[j * n + j] = l;
@@ -651,6 +672,8 @@ static int taps1d(int f, int stride, int nc, int* idx, double* wt) {
   idx[0] = I; wt[0] = 0.75; idx[1] = J; wt[1] = 0.25;
   return 2;
 }
+
+int orc_taps1d(int f, int stride, int nc, int* idx, double* wt) { return taps1d(f, stride, nc, idx, wt); }
 
 /* e_f += P e_c on liquid fine cells. */
 static void prolong_add(orc_level* f, const orc_level* c) {

@@ -38,6 +38,8 @@ extern "C" {
 #define ORC_MAX_LEVELS 16
 
 /* ---- shared numerics ---------------------------------------------------- */
+/* Sets the OpenMP thread count (n > 0) and returns the count in effect. */
+int orc_set_threads(int n);
 double orc_ghost_theta(double liquid_phi, double air_phi, double eps);
 double orc_dot_kahan(const double* a, const double* b, size_t n);     /* sparse-atlas dot() */
 double orc_dot_neumaier(const double* a, const double* b, size_t n);  /* octree-pipelined-pcg dot() */
@@ -122,6 +124,11 @@ int orc_vcycle_apply(int nx, int ny, int nz, const float* phi, const orc_mg_opti
  * coarsening maps: writes the level-l phi into out (size of level l). */
 int orc_level_phi(int nx, int ny, int nz, const float* phi, int phi_preserve_levels,
                   int level, double* out);
+/* 1-D cell-centred prolongation taps of fine index f from a coarse axis of
+ * nc cells and stride 1 or 2 (mgTrilinearPressure, multigrid.wgsl.ts:217-241:
+ * weights 3/4 and 1/4, an out-of-grid tap dropped and the rest renormalised);
+ * returns the tap count (1 or 2).  Restriction is its transpose. */
+int orc_taps1d(int f, int stride, int nc, int* idx, double* wt);
 
 /* Generic direct-curvature PCG on a dense row-major SPD matrix with an
  * optional diagonal (Jacobi) preconditioner -- octree-pipelined-pcg.ts:305-594.

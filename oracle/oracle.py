@@ -65,6 +65,10 @@ def lib():
         L.orc_dot_neumaier.restype = d
         L.orc_dot_neumaier.argtypes = [_dp, _dp, C.c_size_t]
         L.orc_plan_levels.restype = i
+        L.orc_set_threads.restype = i
+        L.orc_set_threads.argtypes = [i]
+        L.orc_taps1d.restype = i
+        L.orc_taps1d.argtypes = [i, i, i, C.c_void_p, C.c_void_p]
         L.orc_plan_levels.argtypes = [i, i, i, _ip, _ip, C.POINTER(i)]
         L.orc_plan_levels_reference.restype = i
         L.orc_plan_levels_reference.argtypes = [i, i, i, _ip]
@@ -241,6 +245,28 @@ def vcycle(phi, r, pre=2, post=2, preserve=2):
     if lib().orc_vcycle_apply(nx, ny, nz, _f32(phi), C.byref(opt), _f64(r), z) != 0:
         raise ValueError("no multigrid plan")
     return z
+
+
+def set_threads(n: int = 0) -> int:
+    """OpenMP threads of the oracle's stencil loops (n > 0 sets it); returns
+    the count in effect."""
+    return int(lib().orc_set_threads(int(n)))
+
+
+def taps1d(nf, nc, stride):
+    """Prolongation taps of every fine index of an axis: (coarse index (nf, 2),
+    weight (nf, 2)), index -1 and weight 0 where a tap is unused."""
+    L = lib()
+    idx = np.full((nf, 2), -1, np.int32)
+    wt = np.zeros((nf, 2))
+    i2 = (C.c_int * 2)()
+    w2 = (C.c_double * 2)()
+    for f in range(nf):
+        n = L.orc_taps1d(f, stride, nc, i2, w2)
+        for q in range(n):
+            idx[f, q] = i2[q]
+            wt[f, q] = w2[q]
+    return idx, wt
 
 
 def level_phi(phi, level, preserve=2):

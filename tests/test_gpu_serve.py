@@ -113,3 +113,37 @@ def test_tall_and_uniform_served_together_equal_alone(fpl):
     finally:
         for g in grids:
             g.destroy()
+
+
+def test_served_stream_admission_and_oracle(fpl, oracle):
+    """The served-stream loop (Poisson sessions of warm-started steps, mixed
+    uniform and tall-cell sizes) under an HBM budget that holds a 64^3 grid
+    and a 32^3 one at most: arrivals wait for admission, every step reaches the
+    tolerance, and the first result of each size class matches the FP64
+    oracle (pressure and velocities within 1e-5 relative L2)."""
+    from fluid_amd.serve import ServeLoop
+
+    mix = {32: 0.4, 48: 0.3, 64: 0.2, "tall32": 0.1}
+    # a 64^3 session leaves room for one 32^3 grid beside it: others wait
+    budget = fpl.grid_bytes((64, 64, 64)) + fpl.grid_bytes((32, 32, 32))
+    loop = ServeLoop(rate_hz=40.0, duration_s=1.5, mix=mix, steps=4, seed=3, hbm_budget_bytes=budget,
+                     keep_first=True)
+    stats = loop.run()
+    print(stats)
+    assert stats["sessions"] >= 20
+    assert stats["solves"] == 4 * stats["sessions"]
+    assert stats["admission_wait_ms"]["max"] > 0.0
+    assert set(loop.first) == set(mix)
+    for n, (sc, outs, rep) in loop.first.items():
+        p, u, v, w = outs
+        if isinstance(n, str):
+            L = sc.layout
+            t = oracle.Tall(L.bases, sc.phi, L.fine_ny, L.band, L.h, L.max_delta)
+            op, (ou, ov, ow), _, _ = t.projection(sc.u, sc.v, sc.w, sc.dt, sc.rho, rel_tol=1e-11)
+        else:
+            op, (ou, ov, ow), _, _ = oracle.uniform_projection(sc.phi, sc.u, sc.v, sc.w, sc.h, sc.dt, sc.rho,
+                                                               rel_tol=1e-12)
+        for a, b in ((p, op), (u, ou), (v, ov), (w, ow)):
+            a = np.asarray(a, np.float64)
+            err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+            assert err < 1e-5, (n, err)

@@ -51,17 +51,52 @@ def test_level_plan_bit_exact(fpl, oracle, dims):
     g.destroy()
 
 
-def test_coarse_level_sets_and_labels(fpl, oracle, c0):
-    g = fpl.UniformPressureProjection(c0.shape, c0.h)
+@pytest.mark.parametrize("scene", ["dam64", "spheres128", "dam37", "dam48x32x40"])
+def test_coarse_level_sets_and_labels(fpl, oracle, scene):
+    """Coarse level sets (FP32, D4-ordered sums of the eight children) and the
+    labels derived from them are bit-exact on every level, with no mask."""
+    if scene == "dam48x32x40":
+        nx, ny, nz = 48, 32, 40
+        c = lambda m: (np.arange(m) + 0.5) / nx
+        phi = scenes.box_sdf(c(nx)[:, None, None], c(ny)[None, :, None], c(nz)[None, None, :],
+                             (0, 0, 0), (0.55, 0.5, 0.6)).astype(np.float32)
+    else:
+        phi = {"dam64": lambda: scenes.dam_break_scene(64, 0).phi,
+               "spheres128": lambda: scenes.dam_spheres_scene(128, seed=0).phi,
+               "dam37": lambda: scenes.dam_break_scene(37, 0).phi}[scene]()
+    g = fpl.UniformPressureProjection(phi.shape, 1.0 / phi.shape[0])
     dims, _ = g.levels()
     for level in range(1, len(dims)):
-        gp = g.probe_level_phi(c0.phi, level)
-        op = oracle.level_phi(c0.phi, level)
+        gp = g.probe_level_phi(phi, level)
+        op = oracle.level_phi(phi, level)
         assert gp.shape == op.shape
-        np.testing.assert_allclose(gp, op, rtol=1e-5, atol=1e-7 * (1 + np.abs(op).max()))
-        # labels bit-exact wherever the FP64 value is not within rounding of 0
-        sure = np.abs(op) > 1e-6 * np.abs(op).max()
-        assert np.array_equal((gp < 0)[sure], (op < 0)[sure])
+        assert np.array_equal(gp.astype(np.float64), op), (level, np.abs(gp - op).max())
+        assert np.array_equal(gp < 0, op < 0)
+    g.destroy()
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 64), (48, 32, 40), (37, 37, 37), (384, 384, 384), (128, 128, 8)])
+def test_transfer_taps_bit_exact(fpl, oracle, dims):
+    """Prolongation taps (and the restriction's, their transpose) as the
+    kernels evaluate them equal the oracle's taps1d on every level and axis."""
+    g = fpl.UniformPressureProjection(dims, 1.0 / dims[0])
+    ldims, strides = g.levels()
+    for level in range(1, len(ldims)):
+        for axis in range(3):
+            nf, nc, st = int(ldims[level - 1][axis]), int(ldims[level][axis]), int(strides[level][axis])
+            (pc, pw), (rf, rw) = g.probe_taps(level, axis)
+            oi, ow = oracle.taps1d(nf, nc, st)
+            assert np.array_equal(pc, oi), (level, axis)
+            assert np.array_equal(pw.astype(np.float64), ow), (level, axis)
+            # restriction rows: the transpose of the prolongation taps
+            want = [dict() for _ in range(nc)]
+            for f in range(nf):
+                for q in range(2):
+                    if oi[f, q] >= 0:
+                        want[oi[f, q]][f] = ow[f, q]
+            for c in range(nc):
+                got = {int(rf[c, q]): float(rw[c, q]) for q in range(4) if rf[c, q] >= 0}
+                assert got == want[c], (level, axis, c, got, want[c])
     g.destroy()
 
 
@@ -151,7 +186,9 @@ def test_hydrostatic_column_gpu(fpl):
     g.destroy()
 
 
-def test_empty_and_full_domains(fpl):
+def test_empty_domain(fpl):
+    """All air: nothing to solve, zero pressure and velocities (the all-liquid
+    domain is covered by test_gpu_nullspace_faults.py)."""
     n = 16
     g = fpl.UniformPressureProjection((n, n, n), 1 / n)
     rng = np.random.default_rng(0)

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(fpl):
 
 def test_abi_version_and_defaults(fpl):
     lib = fpl.load_library()
-    assert lib.fp_abi_version() == 1
+    assert lib.fp_abi_version() == 2
     o = fpl.default_options()
     assert o.relative_tolerance == 1e-6
     assert o.pre_sweeps == o.post_sweeps == 2

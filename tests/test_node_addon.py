@@ -24,7 +24,7 @@ def test_addon_exports_and_method_shape(fpl):
     out = json.loads(r.stdout)
     assert out["b"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
                                "project", "lastError", "abiVersion"])
-    assert out["id"] == "mi355x-projection" and out["abi"] == 1 and out["fn"] == "function"
+    assert out["id"] == "mi355x-projection" and out["abi"] == 2 and out["fn"] == "function"
 
 
 def test_addon_errors_are_js_errors_without_device(fpl):

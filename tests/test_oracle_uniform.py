@@ -228,3 +228,59 @@ def test_level_phi_preserves_bubbles(oracle):
     assert l1[2, 2, 2] == pytest.approx((0.05 + 6 * 0.01) / 7)
     l1n = oracle.level_phi(phi, 1, preserve=0)
     assert l1n[2, 2, 2] == pytest.approx((0.05 + 6 * 0.01 - 0.2) / 8)
+
+
+def test_level_phi_d4_pairing_and_children(oracle):
+    """The coarse level set sums the eight children in FP32 with the pairing
+    ((c0+c5)+(c1+c4)) + ((c2+c7)+(c3+c6)) of mgD4Sum8, corner bit 0 = x,
+    bit 1 = y, bit 2 = z: with c0 = 2^24, c5 = -2^24 and six ones that sum is
+    6 (a left-to-right sum rounds the ones away and gives 2), so the plain mean
+    is 0.75; children that are exact binary fractions average exactly."""
+    n = 16
+    phi = np.full((n, n, n), 0.5, np.float32)
+    phi[4:6, 6:8, 2:4] = 1.0
+    phi[4, 6, 2] = 2.0 ** 24   # corner 0 of coarse cell (2, 3, 1)
+    phi[5, 6, 3] = -2.0 ** 24  # corner 5: x + 1, z + 1
+    l1 = oracle.level_phi(phi, 1, preserve=0)
+    assert l1[2, 3, 1] == 0.75
+    # swapping corners 5 and 6 (x+1, z+1 -> y+1, z+1) pairs 2^24 with a one
+    phi[5, 6, 3] = 1.0
+    phi[4, 7, 3] = -2.0 ** 24
+    l1 = oracle.level_phi(phi, 1, preserve=0)
+    assert l1[2, 3, 1] == np.float32(np.float32(2.0 ** 24 + 1.0) + np.float32(2.0)
+                                     + np.float32(np.float32(1.0 - 2.0 ** 24) + np.float32(2.0))) / 8
+    # exact binary fractions: the mean of children 1/8 .. 8/8 is 9/16
+    phi = np.full((n, n, n), 0.5, np.float32)
+    for o in range(8):
+        phi[6 + (o & 1), 8 + ((o >> 1) & 1), 10 + ((o >> 2) & 1)] = (o + 1) / 8
+    assert oracle.level_phi(phi, 1, preserve=2)[3, 4, 5] == 9 / 16
+
+
+def test_level_phi_odd_axis_repeats_last_child(oracle):
+    """An odd lattice (37, ceil-halved to 19) averages the last coarse cell's
+    one child along that axis with itself."""
+    n = 37
+    phi = np.full((n, n, n), 0.25, np.float32)
+    phi[36, 0:2, 0:2] = np.array([[1.0, 2.0], [3.0, 4.0]], np.float32)
+    l1 = oracle.level_phi(phi, 1, preserve=0)
+    assert l1[18, 0, 0] == 2.5
+
+
+def test_taps1d_reproduce_linear_fields(oracle):
+    """Cell-centred linear prolongation: weights 3/4 + 1/4 sum to 1, a tap
+    outside the coarse grid is dropped and the other renormalised to 1, and a
+    linear field x is reproduced exactly at interior fine centres."""
+    nf, nc = 16, 8
+    idx, wt = oracle.taps1d(nf, nc, 2)
+    assert np.allclose(wt.sum(axis=1), 1.0)
+    coarse = 2.0 * np.arange(nc) + 1.0  # coarse centres in fine-cell units
+    for f in range(1, nf - 1):
+        val = sum(wt[f, q] * coarse[idx[f, q]] for q in range(2) if idx[f, q] >= 0)
+        assert val == f + 0.5
+    assert idx[0].tolist() == [0, -1] and wt[0].tolist() == [1.0, 0.0]
+    assert idx[nf - 1].tolist() == [nc - 1, -1]
+    i1, w1 = oracle.taps1d(5, 5, 1)
+    assert i1[:, 0].tolist() == list(range(5)) and np.all(w1[:, 0] == 1.0)
+    # odd fine axis (37 -> 19): fine 36 lies between coarse 18 and 17
+    i3, w3 = oracle.taps1d(37, 19, 2)
+    assert i3[36].tolist() == [18, 17] and w3[36].tolist() == [0.75, 0.25]
