@@ -344,7 +344,7 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
   (void)hipSetDevice(ctx->device);
   // device work after what the callers queued on their streams
   for (int i = 0; i < count; ++i) {
-    if (!in[i].stream) continue;
+    if (!in[i].stream_order) continue;
     hipError_t e = hipEventRecord(ctx->order, (hipStream_t)in[i].stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->order, 0);
     if (e != hipSuccess) {

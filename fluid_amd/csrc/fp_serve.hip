@@ -47,7 +47,7 @@ int Server::submit(UniformGrid* g, TallGrid* t, const fp_inputs* in, const fp_ou
       set_error("fp_submit: the grid has a submitted projection in flight (fp_poll it first)");
       return FP_ERR_STATE;
     }
-    if (in->stream) {
+    if (in->stream_order) {
       // the worker's launches for this job wait for the caller's queued work
       (void)hipSetDevice(device_);
       hipError_t e = hipEventCreateWithFlags(&j->ready, hipEventDisableTiming);

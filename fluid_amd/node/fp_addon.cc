@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,20 @@ void* unwrap_external(napi_env env, napi_value v) {
   void* p = nullptr;
   napi_get_value_external(env, v, &p);
   return p;
+}
+
+// Per grid (all on the JS thread): its context, and the projections queued
+// on it that have not completed.  destroyGrid refuses a grid with pending
+// work, whose buffers a worker thread may still be using.
+struct GridInfo {
+  fp_ctx* ctx = nullptr;
+  int pending = 0;
+};
+std::map<fp_grid*, GridInfo> g_grids;
+
+GridInfo* grid_info(fp_grid* g) {
+  auto it = g_grids.find(g);
+  return it == g_grids.end() ? nullptr : &it->second;
 }
 
 // ---- context ----------------------------------------------------------------
@@ -155,6 +170,7 @@ napi_value create_grid(napi_env env, napi_callback_info info) {
   fp_grid* g = nullptr;
   int rc = fp_grid_create(ctx, &d, &g);
   if (rc) return throw_fp(env, rc);
+  g_grids[g].ctx = ctx;
   napi_value ext;
   NAPI_OK(napi_create_external(env, g, nullptr, nullptr, &ext));
   return ext;
@@ -165,9 +181,28 @@ napi_value destroy_grid(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   if (argc < 1) return nullptr;
-  int rc = fp_grid_destroy((fp_grid*)unwrap_external(env, argv[0]));
+  fp_grid* g = (fp_grid*)unwrap_external(env, argv[0]);
+  GridInfo* gi = grid_info(g);
+  if (gi && gi->pending > 0) {
+    napi_throw_error(env, "FP_ERR_6", "destroyGrid: the grid has projections in flight (await them first)");
+    return nullptr;
+  }
+  int rc = fp_grid_destroy(g);
   if (rc) return throw_fp(env, rc);
+  g_grids.erase(g);
   return nullptr;
+}
+
+napi_value bytes_in_use(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  uint64_t b = 0;
+  int rc = fp_ctx_bytes_in_use(argc > 0 ? (const fp_ctx*)unwrap_external(env, argv[0]) : nullptr, &b);
+  if (rc) return throw_fp(env, rc);
+  napi_value v;
+  NAPI_OK(napi_create_double(env, (double)b, &v));
+  return v;
 }
 
 napi_value grid_levels(napi_env env, napi_callback_info info) {
@@ -193,24 +228,49 @@ napi_value grid_levels(napi_env env, napi_callback_info info) {
   return arr;
 }
 
-// ---- project (async) ---------------------------------------------------------------
-struct ProjectJob {
-  napi_async_work work = nullptr;
-  napi_deferred deferred = nullptr;
-  std::vector<napi_ref> keep;  // input/output typed arrays stay alive
+// ---- project / submit / batch (async) ----------------------------------------------
+// One grid's part of a job: its C records and the JS result object.
+struct GridIO {
   fp_grid* grid = nullptr;
   fp_inputs in{};
   fp_outputs out{};
-  fp_options opt{};
   fp_report rep{};
+  napi_ref result = nullptr;   // {pressure, u, v, w}
+};
+
+struct ProjectJob {
+  enum Kind { PROJECT, SUBMIT, BATCH } kind = PROJECT;
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  std::vector<napi_ref> keep;  // input/output typed arrays stay alive
+  std::vector<GridIO> io;
+  fp_options opt{};
+  fp_ctx* ctx = nullptr;       // SUBMIT: the context polled
+  uint64_t ticket = 0;
+  double latency_ms = 0.0;
   int rc = 0;
   std::string error;
-  napi_ref result = nullptr;   // {pressure, u, v, w}
 };
 
 void project_execute(napi_env, void* data) {
   ProjectJob* j = (ProjectJob*)data;
-  j->rc = fp_project(j->grid, &j->in, &j->out, &j->opt, &j->rep);
+  if (j->kind == ProjectJob::PROJECT) {
+    j->rc = fp_project(j->io[0].grid, &j->io[0].in, &j->io[0].out, &j->opt, &j->io[0].rep);
+  } else if (j->kind == ProjectJob::SUBMIT) {
+    j->rc = fp_poll(j->ctx, j->ticket, -1, &j->io[0].rep, &j->latency_ms);
+  } else {
+    std::vector<fp_grid*> grids;
+    std::vector<fp_inputs> in;
+    std::vector<fp_outputs> out;
+    std::vector<fp_report> reps(j->io.size());
+    for (const GridIO& g : j->io) {
+      grids.push_back(g.grid);
+      in.push_back(g.in);
+      out.push_back(g.out);
+    }
+    j->rc = fp_project_batch(grids.data(), (int32_t)grids.size(), in.data(), out.data(), &j->opt, reps.data());
+    for (size_t i = 0; i < reps.size(); ++i) j->io[i].rep = reps[i];
+  }
   if (j->rc) j->error = fp_last_error();
 }
 
@@ -237,6 +297,10 @@ napi_value report_object(napi_env env, const fp_report& r) {
 
 void project_complete(napi_env env, napi_status, void* data) {
   ProjectJob* j = (ProjectJob*)data;
+  for (const GridIO& g : j->io) {
+    GridInfo* gi = grid_info(g.grid);
+    if (gi) gi->pending -= 1;
+  }
   if (j->rc) {
     napi_value err, msg, code;
     napi_create_string_utf8(env, j->error.c_str(), NAPI_AUTO_LENGTH, &msg);
@@ -246,19 +310,128 @@ void project_complete(napi_env env, napi_status, void* data) {
     napi_create_error(env, code, msg, &err);
     napi_reject_deferred(env, j->deferred, err);
   } else {
-    napi_value res;
-    napi_get_reference_value(env, j->result, &res);
-    napi_set_named_property(env, res, "report", report_object(env, j->rep));
-    napi_resolve_deferred(env, j->deferred, res);
+    std::vector<napi_value> res(j->io.size());
+    for (size_t i = 0; i < j->io.size(); ++i) {
+      napi_get_reference_value(env, j->io[i].result, &res[i]);
+      napi_value rep = report_object(env, j->io[i].rep);
+      if (j->kind == ProjectJob::SUBMIT) {
+        napi_value lat;
+        napi_create_double(env, j->latency_ms, &lat);
+        napi_set_named_property(env, rep, "latencyMs", lat);
+      }
+      napi_set_named_property(env, res[i], "report", rep);
+    }
+    if (j->kind == ProjectJob::BATCH) {
+      napi_value arr;
+      napi_create_array_with_length(env, res.size(), &arr);
+      for (size_t i = 0; i < res.size(); ++i) napi_set_element(env, arr, (uint32_t)i, res[i]);
+      napi_resolve_deferred(env, j->deferred, arr);
+    } else {
+      napi_resolve_deferred(env, j->deferred, res[0]);
+    }
   }
   for (napi_ref r : j->keep) napi_delete_reference(env, r);
-  napi_delete_reference(env, j->result);
+  for (const GridIO& g : j->io)
+    if (g.result) napi_delete_reference(env, g.result);
   napi_delete_async_work(env, j->work);
   delete j;
 }
 
+// Parses one grid's {phi, u, v, w, targetDivergence?, dt, rho} and
+// {pressure, u, v, w} into g; false after throwing a JS error.
+bool parse_io(napi_env env, ProjectJob* j, GridIO& g, napi_value inputs, napi_value outputs) {
+  // array lengths the grid reads and writes: level-0 samples and the three face arrays
+  int32_t dims[3 * 16], levels = 0;
+  int64_t nf[3] = {0, 0, 0};
+  if (!g.grid || fp_grid_levels(g.grid, 16, dims, nullptr, &levels) != FP_OK ||
+      fp_grid_face_counts(g.grid, &nf[0], &nf[1], &nf[2]) != FP_OK) {
+    napi_throw_type_error(env, nullptr, "not a live projection grid");
+    return false;
+  }
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  const size_t want_in[5] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2], cells};
+  const size_t want_out[4] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2]};
+  g.in.memory = FP_MEMORY_HOST;
+  const char* in_names[5] = {"phi", "u", "v", "w", "targetDivergence"};
+  const float** in_ptrs[5] = {&g.in.phi, &g.in.u, &g.in.v, &g.in.w, &g.in.target_divergence};
+  for (int k = 0; k < 5; ++k) {
+    napi_value v;
+    if (!get_named(env, inputs, in_names[k], &v)) continue;
+    float* p;
+    size_t len;
+    if (!get_f32(env, v, &p, &len)) {
+      napi_throw_type_error(env, nullptr, "inputs must be Float32Array");
+      return false;
+    }
+    if (len != want_in[k]) {
+      napi_throw_range_error(env, nullptr, "input array length does not match the grid");
+      return false;
+    }
+    *in_ptrs[k] = p;
+    napi_ref r;
+    napi_create_reference(env, v, 1, &r);
+    j->keep.push_back(r);
+  }
+  g.in.dt = get_number(env, inputs, "dt", 0.0);
+  g.in.rho = get_number(env, inputs, "rho", 998.2);
+  const char* out_names[4] = {"pressure", "u", "v", "w"};
+  float** out_ptrs[4] = {&g.out.pressure, &g.out.u, &g.out.v, &g.out.w};
+  for (int k = 0; k < 4; ++k) {
+    napi_value v;
+    if (!get_named(env, outputs, out_names[k], &v)) continue;
+    float* p;
+    size_t len;
+    if (!get_f32(env, v, &p, &len)) {
+      napi_throw_type_error(env, nullptr, "outputs must be Float32Array");
+      return false;
+    }
+    if (len != want_out[k]) {
+      napi_throw_range_error(env, nullptr, "output array length does not match the grid");
+      return false;
+    }
+    *out_ptrs[k] = p;
+  }
+  napi_create_reference(env, outputs, 1, &g.result);
+  return true;
+}
+
+void parse_options(napi_env env, napi_value o, fp_options* opt) {
+  fp_default_options(opt);
+  napi_valuetype t;
+  napi_typeof(env, o, &t);
+  if (t != napi_object) return;
+  opt->relative_tolerance = get_number(env, o, "relativeTolerance", opt->relative_tolerance);
+  opt->max_iterations = (int32_t)get_number(env, o, "maxIterations", opt->max_iterations);
+  opt->max_outer = (int32_t)get_number(env, o, "maxOuter", opt->max_outer);
+  opt->precision = (int32_t)get_number(env, o, "precision", opt->precision);
+  opt->pre_sweeps = (int32_t)get_number(env, o, "preSweeps", opt->pre_sweeps);
+  opt->post_sweeps = (int32_t)get_number(env, o, "postSweeps", opt->post_sweeps);
+  opt->warm_start = (int32_t)get_number(env, o, "warmStart", opt->warm_start);
+}
+
+void discard(napi_env env, ProjectJob* j) {
+  for (napi_ref r : j->keep) napi_delete_reference(env, r);
+  for (const GridIO& g : j->io)
+    if (g.result) napi_delete_reference(env, g.result);
+  delete j;
+}
+
+napi_value queue(napi_env env, ProjectJob* j) {
+  for (const GridIO& g : j->io) {
+    GridInfo* gi = grid_info(g.grid);
+    if (gi) gi->pending += 1;
+  }
+  napi_value promise, name;
+  NAPI_OK(napi_create_promise(env, &j->deferred, &promise));
+  NAPI_OK(napi_create_string_utf8(env, "fluidProjection", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(napi_create_async_work(env, nullptr, name, project_execute, project_complete, j, &j->work));
+  NAPI_OK(napi_queue_async_work(env, j->work));
+  return promise;
+}
+
 // project(grid, {phi, u, v, w, targetDivergence?, dt, rho}, {pressure, u, v, w}, options?) -> Promise
-napi_value project(napi_env env, napi_callback_info info) {
+// submit(...same...) -> Promise (fp_submit now, fp_poll on a worker thread)
+napi_value project_or_submit(napi_env env, napi_callback_info info, bool submit) {
   size_t argc = 4;
   napi_value argv[4];
   NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
@@ -267,83 +440,59 @@ napi_value project(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   ProjectJob* j = new ProjectJob();
-  j->grid = (fp_grid*)unwrap_external(env, argv[0]);
+  j->kind = submit ? ProjectJob::SUBMIT : ProjectJob::PROJECT;
+  j->io.resize(1);
+  j->io[0].grid = (fp_grid*)unwrap_external(env, argv[0]);
+  if (!parse_io(env, j, j->io[0], argv[1], argv[2])) {
+    discard(env, j);
+    return nullptr;
+  }
+  if (argc > 3) parse_options(env, argv[3], &j->opt);
+  else fp_default_options(&j->opt);
+  if (submit) {
+    GridInfo* gi = grid_info(j->io[0].grid);
+    j->ctx = gi ? gi->ctx : nullptr;
+    int rc = fp_submit(j->io[0].grid, &j->io[0].in, &j->io[0].out, &j->opt, &j->ticket);
+    if (rc) {
+      discard(env, j);
+      return throw_fp(env, rc);
+    }
+  }
+  return queue(env, j);
+}
+
+napi_value project(napi_env env, napi_callback_info info) { return project_or_submit(env, info, false); }
+napi_value submit(napi_env env, napi_callback_info info) { return project_or_submit(env, info, true); }
+
+// projectBatch([grid...], [inputs...], [outputs...], options?) -> Promise<[result...]>
+napi_value project_batch(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  uint32_t n = 0, ni = 0, no = 0;
+  if (argc < 3 || napi_get_array_length(env, argv[0], &n) != napi_ok ||
+      napi_get_array_length(env, argv[1], &ni) != napi_ok || napi_get_array_length(env, argv[2], &no) != napi_ok ||
+      n == 0 || ni != n || no != n) {
+    napi_throw_type_error(env, nullptr, "projectBatch([grid...], [inputs...], [outputs...], options?) of equal lengths");
+    return nullptr;
+  }
+  ProjectJob* j = new ProjectJob();
+  j->kind = ProjectJob::BATCH;
+  j->io.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value g, in, out;
+    napi_get_element(env, argv[0], i, &g);
+    napi_get_element(env, argv[1], i, &in);
+    napi_get_element(env, argv[2], i, &out);
+    j->io[i].grid = (fp_grid*)unwrap_external(env, g);
+    if (!parse_io(env, j, j->io[i], in, out)) {
+      discard(env, j);
+      return nullptr;
+    }
+  }
   fp_default_options(&j->opt);
-  // array lengths the grid reads and writes: level-0 samples and the three face arrays
-  int32_t dims[3 * 16], levels = 0;
-  int64_t nf[3] = {0, 0, 0};
-  if (fp_grid_levels(j->grid, 16, dims, nullptr, &levels) != FP_OK ||
-      fp_grid_face_counts(j->grid, &nf[0], &nf[1], &nf[2]) != FP_OK) {
-    delete j;
-    return throw_fp(env, FP_ERR_INVALID_ARGUMENT);
-  }
-  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
-  const size_t want_in[5] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2], cells};
-  const size_t want_out[4] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2]};
-  j->in.memory = FP_MEMORY_HOST;
-  const char* in_names[5] = {"phi", "u", "v", "w", "targetDivergence"};
-  const float** in_ptrs[5] = {&j->in.phi, &j->in.u, &j->in.v, &j->in.w, &j->in.target_divergence};
-  for (int k = 0; k < 5; ++k) {
-    napi_value v;
-    if (!get_named(env, argv[1], in_names[k], &v)) continue;
-    float* p;
-    size_t len;
-    if (!get_f32(env, v, &p, &len)) {
-      delete j;
-      napi_throw_type_error(env, nullptr, "inputs must be Float32Array");
-      return nullptr;
-    }
-    if (len != want_in[k]) {
-      delete j;
-      napi_throw_range_error(env, nullptr, "input array length does not match the grid");
-      return nullptr;
-    }
-    *in_ptrs[k] = p;
-    napi_ref r;
-    napi_create_reference(env, v, 1, &r);
-    j->keep.push_back(r);
-  }
-  j->in.dt = get_number(env, argv[1], "dt", 0.0);
-  j->in.rho = get_number(env, argv[1], "rho", 998.2);
-  const char* out_names[4] = {"pressure", "u", "v", "w"};
-  float** out_ptrs[4] = {&j->out.pressure, &j->out.u, &j->out.v, &j->out.w};
-  for (int k = 0; k < 4; ++k) {
-    napi_value v;
-    if (!get_named(env, argv[2], out_names[k], &v)) continue;
-    float* p;
-    size_t len;
-    if (!get_f32(env, v, &p, &len)) {
-      delete j;
-      napi_throw_type_error(env, nullptr, "outputs must be Float32Array");
-      return nullptr;
-    }
-    if (len != want_out[k]) {
-      delete j;
-      napi_throw_range_error(env, nullptr, "output array length does not match the grid");
-      return nullptr;
-    }
-    *out_ptrs[k] = p;
-  }
-  napi_create_reference(env, argv[2], 1, &j->result);
-  if (argc > 3) {
-    napi_valuetype t;
-    napi_typeof(env, argv[3], &t);
-    if (t == napi_object) {
-      j->opt.relative_tolerance = get_number(env, argv[3], "relativeTolerance", j->opt.relative_tolerance);
-      j->opt.max_iterations = (int32_t)get_number(env, argv[3], "maxIterations", j->opt.max_iterations);
-      j->opt.max_outer = (int32_t)get_number(env, argv[3], "maxOuter", j->opt.max_outer);
-      j->opt.precision = (int32_t)get_number(env, argv[3], "precision", j->opt.precision);
-      j->opt.pre_sweeps = (int32_t)get_number(env, argv[3], "preSweeps", j->opt.pre_sweeps);
-      j->opt.post_sweeps = (int32_t)get_number(env, argv[3], "postSweeps", j->opt.post_sweeps);
-      j->opt.warm_start = (int32_t)get_number(env, argv[3], "warmStart", j->opt.warm_start);
-    }
-  }
-  napi_value promise, name;
-  NAPI_OK(napi_create_promise(env, &j->deferred, &promise));
-  NAPI_OK(napi_create_string_utf8(env, "fluidProjection", NAPI_AUTO_LENGTH, &name));
-  NAPI_OK(napi_create_async_work(env, nullptr, name, project_execute, project_complete, j, &j->work));
-  NAPI_OK(napi_queue_async_work(env, j->work));
-  return promise;
+  if (argc > 3) parse_options(env, argv[3], &j->opt);
+  return queue(env, j);
 }
 
 napi_value last_error(napi_env env, napi_callback_info) {
@@ -364,6 +513,7 @@ napi_value init(napi_env env, napi_value exports) {
     napi_callback cb;
   } fns[] = {{"createContext", create_context}, {"destroyContext", destroy_context}, {"createGrid", create_grid},
              {"destroyGrid", destroy_grid},     {"gridLevels", grid_levels},         {"project", project},
+             {"submit", submit},                {"projectBatch", project_batch},     {"bytesInUse", bytes_in_use},
              {"lastError", last_error},         {"abiVersion", abi_version}};
   for (auto& f : fns) {
     napi_value fn;

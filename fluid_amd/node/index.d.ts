@@ -47,6 +47,9 @@ export declare class UniformPressureProjection {
   readonly levelDimensions: number[][];
   diagnostics: ProjectionReport | undefined;
   project(inputs: ProjectionInputs, options?: ProjectionOptions): Promise<ProjectionResult>;
+  /** fp_submit / fp_poll: continuous batching with other queued projections. */
+  submit(inputs: ProjectionInputs, options?: ProjectionOptions): Promise<ProjectionResult>;
+  /** Frees the grid once no projection of it is in flight (idempotent). */
   destroy(): void;
 }
 
@@ -66,6 +69,52 @@ export declare class TallCellPressureProjection {
   diagnostics: ProjectionReport | undefined;
   // phi (nx, B+2, nz); u (nx+1, S, nz); v (nx, B+2, nz); w (nx, S, nz+1); S = B+2+maxDelta
   project(inputs: Omit<ProjectionInputs, 'targetDivergence'>, options?: ProjectionOptions): Promise<ProjectionResult>;
+  submit(inputs: Omit<ProjectionInputs, 'targetDivergence'>, options?: ProjectionOptions): Promise<ProjectionResult>;
+  destroy(): void;
+}
+
+export interface ProjectionReportWithLatency extends ProjectionReport {
+  latencyMs?: number;              // submit(): submit-to-completion latency
+}
+
+/** Independent projections of several grids in one batched solve. */
+export declare function projectBatch(projections: ReadonlyArray<UniformPressureProjection | TallCellPressureProjection>,
+                                     inputs: ReadonlyArray<ProjectionInputs>, options?: ProjectionOptions):
+  Promise<ProjectionResult[]>;
+
+// ---- the SimulationMethod plugin (lib/core/method-contract.ts:363-602) ----
+export type GPUQuality = 'balanced' | 'high' | 'ultra';
+export type MethodParamValues = Record<string, number | string | boolean>;
+
+export interface NumberParamSpec {
+  kind: 'number'; key: string; label: string; hint?: string; tier: 'coarse' | 'fine'; update?: 'runtime' | 'solver';
+  unit: string; min: number; max: number; step: number; digits?: number; default: number;
+}
+export interface SelectParamSpec {
+  kind: 'select'; key: string; label: string; hint?: string; tier: 'coarse' | 'fine'; update?: 'runtime' | 'solver';
+  options: ReadonlyArray<{ value: string; label: string }>; default: string;
+}
+
+/** The GPUEulerianInfo members this solver publishes (lib/core/webgpu-eulerian.ts:13-38). */
+export interface MI355XEulerianInfo {
+  nx: number; ny: number; nz: number; storedNy: number; cellCount: number; equivalentUniformCells: number;
+  compressionRatio: number; regularLayers: number; maximumNeighborDelta: number; gridKind: 'uniform';
+  pressureIterations: number; allocatedBytes: number; quality: GPUQuality; cellSize_m: number;
+  pressureSolver: string; pressureIterationsExecuted?: number; pressureRelativeResidual?: number;
+  pressureResidual?: number; maxDivergence_s?: number; maxDivergenceBefore_s?: number; maxDivergenceAfter_s?: number;
+  pressureSolveConverged?: boolean; pressureIterationCapReached?: boolean;
+  pressureConvergenceReason?: 'tolerance' | 'iteration-cap'; pressureCurvatureBreakdown?: boolean;
+  pressureFlags?: number; pressureDeviceMs?: number; time_s: number; steps: number; ordinaryGridFallback: boolean;
+}
+
+/** GPUSolverInstance (lib/core/method-contract.ts:209-361) without WebGPU:
+ * volumeTexture is undefined because the field lives in HIP device memory. */
+export interface MI355XSolverInstance {
+  readonly info: MI355XEulerianInfo;
+  readonly volumeTexture: undefined;
+  advanceTo(time_s: number, bodies: unknown[]): boolean;
+  applyRuntimeValues(values: MethodParamValues): void;
+  readStats(): Promise<MI355XEulerianInfo>;
   destroy(): void;
 }
 
@@ -74,13 +123,21 @@ export declare const mi355xProjectionMethod: {
   label: string;
   shortLabel: string;
   badge: string;
-  params: unknown[];
-  presetFor(quality: unknown): Record<string, unknown>;
-  createSolverAsync(device: unknown, scene: unknown, quality?: unknown, values?: ProjectionOptions): Promise<{
-    info: Record<string, unknown>;
-    volumeTexture: undefined;
-    advanceTo(time_s: number, bodies?: unknown[]): boolean;
-    readStats(): Promise<Record<string, unknown>>;
-    destroy(): void;
-  }>;
+  description: string;
+  detail: string;
+  backend: 'webgpu';
+  resource: { id: string; lane: 'fluid'; label: string; provides: readonly ['fluid-authority']; blocks: 'transport' };
+  qualityLabels: Record<GPUQuality, string>;
+  showQualityControl: boolean;
+  supportedFieldModes: readonly string[];
+  params: ReadonlyArray<NumberParamSpec | SelectParamSpec>;
+  pressureMapping: string;
+  presetFor(quality: GPUQuality): MethodParamValues;
+  normalizeValues(values: MethodParamValues): MethodParamValues;
+  runtimeParamKeys: readonly string[];
+  capabilities: { volumeRendering: boolean; adoptsRigidRosterShape: boolean };
+  /** scene: a SceneDescription (lib/core/model.ts:52-130); closed box containers. */
+  createSolverAsync(device: unknown, scene: unknown, quality?: GPUQuality, values?: MethodParamValues,
+                    onRigidLoads?: unknown, onProgress?: (p: { phase: string; label: string; completed: number;
+                      total: number }) => void, signal?: { aborted: boolean }): Promise<MI355XSolverInstance>;
 };

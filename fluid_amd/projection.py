@@ -56,7 +56,7 @@ class GridDesc(C.Structure):
 
 
 class Inputs(C.Structure):
-    _fields_ = [("memory", C.c_int32), ("reserved", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p),
+    _fields_ = [("memory", C.c_int32), ("stream_order", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p),
                 ("v", C.c_void_p), ("w", C.c_void_p), ("target_divergence", C.c_void_p),
                 ("dt", C.c_double), ("rho", C.c_double), ("stream", C.c_void_p)]
 
@@ -285,7 +285,8 @@ class UniformPressureProjection:
             # the library's work starts after what torch has queued on the
             # current stream (producers of the inputs, earlier users of `out`)
             import torch
-            ins.stream = torch.cuda.current_stream(phi.device).cuda_stream
+            ins.stream_order = 1
+            ins.stream = torch.cuda.current_stream(phi.device).cuda_stream  # 0: the null stream
         if mem == MEMORY_HOST:
             p = np.zeros(shapes[0], np.float32)
             uo = np.zeros(shapes[1], np.float32)

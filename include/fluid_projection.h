@@ -108,7 +108,7 @@ typedef struct {
 
 typedef struct {
   int32_t memory;   /* FP_MEMORY_HOST or FP_MEMORY_DEVICE for every array below */
-  int32_t reserved;
+  int32_t stream_order; /* 1: order the projection after `stream` below; 0: no ordering */
   const float* phi; /* level set at samples, metres, negative in liquid */
   const float* u;
   const float* v;
@@ -116,10 +116,11 @@ typedef struct {
   const float* target_divergence; /* optional desired divergence (1/s) per sample, or NULL */
   double dt;
   double rho;
-  void* stream;     /* optional hipStream_t of the caller (NULL: none).  The projection's
-                       device work starts after the work queued on it before the call
-                       (fp_project, fp_project_batch, fp_submit), so device arrays
-                       produced or reused there are read and written in order. */
+  void* stream;     /* hipStream_t of the caller, used when stream_order is 1 (NULL is the
+                       null stream).  The projection's device work then starts after the
+                       work queued on it before the call (fp_project, fp_project_batch,
+                       fp_submit), so device arrays produced or reused there are read and
+                       written in order. */
 } fp_inputs;
 
 typedef struct {

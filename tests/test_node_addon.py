@@ -16,15 +16,79 @@ def run_node(code):
     return subprocess.run([NODE, "-e", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
 
 
-def test_addon_exports_and_method_shape(fpl):
-    r = run_node("const m=require('./fluid_amd/node');"
-                 "console.log(JSON.stringify({b:Object.keys(m.binding).sort(), id:m.mi355xProjectionMethod.id,"
-                 "abi:m.binding.abiVersion(), fn:typeof m.mi355xProjectionMethod.createSolverAsync}))")
+def plugin_check(mode):
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "plugin_check.js"), mode], capture_output=True,
+                       text=True, cwd=ROOT, timeout=300)
     assert r.returncode == 0, r.stderr
-    out = json.loads(r.stdout)
-    assert out["b"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
-                               "project", "lastError", "abiVersion"])
-    assert out["id"] == "mi355x-projection" and out["abi"] == 2 and out["fn"] == "function"
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_method_satisfies_the_simulation_method_contract(fpl):
+    """Every required member of SimulationMethod (lib/core/method-contract.ts:
+    363-458): identity and copy, backend, a ResourcePluginDefinition
+    (lib/core/resource-plugin.ts:33-41), quality labels for each GPUQuality,
+    a parameter schema of NumberParamSpec / SelectParamSpec entries, the
+    pressure mapping, presets that name declared parameters; the addon binds
+    every C entry point the host needs."""
+    out = plugin_check("contract")
+    for k in ("id", "label", "shortLabel", "badge", "description", "detail", "backend", "pressureMapping"):
+        assert out[k] == "string", k
+    assert out["backend_value"] == "webgpu"
+    res = out["resource"]
+    assert res["lane"] in ("platform", "fluid", "svo", "optional") and res["blocks"] in ("viewport", "transport",
+                                                                                        "nothing")
+    assert set(res["provides"]) <= {"renderer", "live-scene", "fluid-authority", "water-presentation",
+                                    "sparse-voxel-presentation", "optional-tooling"} and res["id"] and res["label"]
+    assert out["qualityLabels"] == ["balanced", "high", "ultra"]
+    keys = {p["key"] for p in out["params"]}
+    assert all(p["ok"] and p["kind"] in ("number", "select") and p["tier"] in ("coarse", "fine")
+               for p in out["params"])
+    for q, values in out["presets"].items():
+        assert set(values) <= keys, q
+    assert out["normalized"] == {"precision": "fp16", "smoothingSweeps": 4, "layout": "uniform"}
+    assert out["createSolverAsync"] == "function"
+    assert out["binding"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
+                                     "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion"])
+    # the reference's default lattice sizing: 1.2 x 0.8 x 0.8 m at 2.5 cm
+    assert out["lattice"]["nx"] == 48 and out["lattice"]["ny"] == 32 and out["lattice"]["nz"] == 32
+    assert out["openTop"] == "RangeError"
+
+
+def test_addon_abi_version(fpl):
+    r = run_node("const m=require('./fluid_amd/node');console.log(m.binding.abiVersion())")
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "2"
+
+
+@pytest.mark.gpu
+def test_plugin_solver_runs_scene_steps_and_entry_points(fpl):
+    """createSolverAsync on a SceneDescription (uniform and tall-cell layouts):
+    advanceTo steps the fixed step, readStats resolves a GPUEulerianInfo with
+    every required member (lib/core/webgpu-eulerian.ts:13-38), destroy is
+    idempotent; submit and projectBatch through the addon equal project bit
+    for bit; destroy with a projection in flight waits for it."""
+    out = plugin_check("run")
+    for layout in ("uniform", "tall-cell"):
+        o = out[layout]
+        assert o["stepped"] == [True, True, False], layout
+        info = o["info"]
+        for k in ("nx", "ny", "nz", "storedNy", "cellCount", "equivalentUniformCells", "compressionRatio",
+                  "regularLayers", "maximumNeighborDelta", "gridKind", "pressureIterations", "allocatedBytes",
+                  "quality"):
+            assert k in info, (layout, k)
+        assert info["gridKind"] == "uniform" and info["quality"] == "high"
+        assert info["steps"] == 3 and info["pressureSolveConverged"] is True, (layout, info)
+        assert info["pressureFlags"] == 0 and info["pressureRelativeResidual"] <= 1e-6, (layout, info)
+        assert info["maxDivergenceAfter_s"] < 1e-3 * info["maxDivergenceBefore_s"], (layout, info)
+        assert info["allocatedBytes"] > 0
+        assert o["progress"][0] == "planning" and o["progress"][-1] == "attach"
+    t = out["tall-cell"]["info"]
+    assert t["storedNy"] == 34 and t["ny"] == 64 and t["regularLayers"] == 32
+    assert t["compressionRatio"] == pytest.approx(64 / 34)
+    e = out["entryPoints"]
+    assert e["submitEqualsProject"] and e["batchEqualsProject"] and e["flags"] == [0, 0, 0, 0]
+    assert e["latencyMs"] > 0
+    assert out["destroyWhilePending"] == {"flags": 0, "freed": True}
 
 
 def test_addon_errors_are_js_errors_without_device(fpl):
