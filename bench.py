@@ -126,7 +126,7 @@ def _median_solve(oracle, s, solver, runs=3):
     return statistics.median(times), rep
 
 
-def cpu_baseline(n_all: int = 96, n_one: int = 64, runs: int = 3):
+def cpu_baseline(n_all: int = 128, n_one: int = 64, runs: int = 3):
     """The reference's CPU MAC/PCG algorithm restated in C (oracle/fp_oracle.c)
     on bounded samples of the same workload (the dam + 8 spheres generator at
     reduced size): Jacobi-PCG (sparse-atlas-composite-projection.ts:1346-1457,

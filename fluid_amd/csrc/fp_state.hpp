@@ -9,6 +9,23 @@
 
 namespace fp {
 
+// Refinement targets: the FP64 refinement stops at kOuterFactor times the
+// requested relative residual; each inner FP32 solve at kInnerFactor of that
+// target, or after reducing its residual by kInnerCap (what FP32 CG can be
+// trusted to reduce in one pass).
+#ifndef FP_OUTER_FACTOR
+#define FP_OUTER_FACTOR 0.25
+#endif
+#ifndef FP_INNER_FACTOR
+#define FP_INNER_FACTOR 0.1
+#endif
+#ifndef FP_INNER_CAP
+#define FP_INNER_CAP 1e-6
+#endif
+constexpr double kOuterFactor = FP_OUTER_FACTOR;
+constexpr double kInnerFactor = FP_INNER_FACTOR;
+constexpr double kInnerCap = FP_INNER_CAP;
+
 // --------------------------------------------------------------------------
 // per-projection state and vector resets
 // --------------------------------------------------------------------------
@@ -87,7 +104,7 @@ __global__ void k_fin_rhs(const P* parts) {
     // The refinement stops at a quarter of the requested relative residual:
     // the max-norm divergence left in the FP32 output then stays below that
     // of an FP64 solve stopped at the requested tolerance.
-    st->tol2_bb = 0.0625 * G.tol2 * bb;
+    st->tol2_bb = kOuterFactor * kOuterFactor * G.tol2 * bb;
   }
 }
 
@@ -111,7 +128,7 @@ __global__ void k_fin_true(const P* parts, int new_pass) {
     if (!finite) st->flags |= 0x1;  // FP_FLAG_NONFINITE
     // inner solve: a tenth of the refinement target (margin for the max-norm
     // divergence), at most what FP32 CG can be trusted to reduce in one pass.
-    st->inner_target2 = fmax(0.01 * st->tol2_bb, 1e-12 * rr);
+    st->inner_target2 = fmax(kInnerFactor * kInnerFactor * st->tol2_bb, kInnerCap * kInnerCap * rr);
     st->iter = 0;
     st->done = st->outer_done || !new_pass;
     if (!st->outer_done && new_pass) st->outer += 1;

@@ -35,6 +35,9 @@ FLAG_NAMES = {
 }
 LAYOUT_UNIFORM, LAYOUT_TALL_CELL = 0, 1
 MEMORY_HOST, MEMORY_DEVICE = 0, 1
+# fp_* return codes
+(ERR_INVALID_ARGUMENT, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_LAYOUT, ERR_BUDGET, ERR_STATE,
+ ERR_UNSUPPORTED) = (-1, -2, -3, -4, -5, -6, -7)
 PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
 KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "cg_update": 4,
            "true_residual": 5, "prolong_fine": 6, "restrict_fine": 7}
@@ -194,6 +197,15 @@ def grid_bytes(dims, precision: str = "fp16") -> int:
     b = C.c_uint64()
     d = GridDesc(LAYOUT_UNIFORM, nx, ny, nz, 0, 0, 1.0 / nx, None)
     _check(load_library().fp_grid_bytes(C.byref(d), PRECISIONS[precision], C.byref(b)))
+    return int(b.value)
+
+
+def tall_grid_bytes(nx: int, fine_ny: int, nz: int, band: int, max_delta: int = 4) -> int:
+    """Device bytes a tall-cell grid of this description would hold."""
+    b = C.c_uint64()
+    bases = np.full((nx, nz), 2, np.int32)
+    d = GridDesc(LAYOUT_TALL_CELL, nx, fine_ny, nz, band, max_delta, 1.0 / fine_ny, bases.ctypes.data)
+    _check(load_library().fp_grid_bytes(C.byref(d), PRECISIONS["fp32"], C.byref(b)))
     return int(b.value)
 
 

@@ -336,6 +336,8 @@ def tall_dam_scene(n: int = 256, fine_ny: int | None = None, band: int = 32, see
     J = np.floor(fill[1] * ny - 0.5).astype(np.int64)  # last wet row: (J + 0.5) h <= fill_y
     J = min(max(J, -1), ny - 1)
     max_base = max(0, ny - band)
+    if max_base < 2:
+        raise ValueError(f"a {band}-row band leaves no tall cell in {ny} rows")
     wet_base = tl.choose_base(np.full((n, n), J), np.full((n, n), min(J + 1, ny - 1)), ny, band,
                               min(tl.LIQUID_HALO, band), min(tl.AIR_HALO, band))
     dry_base = min(2, max_base) if max_base <= 3 else max_base

@@ -80,20 +80,25 @@ class ServeLoop:
             sid += 1
 
     def _admit(self, s: Session) -> bool:
-        fill = tuple(float(x) for x in np.random.default_rng(1000 + s.sid).uniform(0.2, 0.8, 3))
         tall = isinstance(s.n, str)
-        try:
+        if s.scene is None:
+            # generated once: a session waiting for admission keeps its scene
+            fill = tuple(float(x) for x in np.random.default_rng(1000 + s.sid).uniform(0.2, 0.8, 3))
             if tall:
                 n = int(s.n[4:])
                 s.scene = scenes.tall_dam_scene(n, n, band=32, seed=2000 + s.sid, fill=fill)
+            else:
+                s.scene = scenes.dam_break_scene(s.n, seed=2000 + s.sid, fill=fill)
+        try:
+            if tall:
                 s.grid = fpl.TallCellPressureProjection.from_layout(s.scene.layout, context=self.ctx)
             else:
                 s.grid = fpl.UniformPressureProjection((s.n,) * 3, 1.0 / s.n, context=self.ctx)
-        except (fpl.FluidProjectionError, ValueError):
-            s.scene = None
-            return False  # over the HBM budget: wait
-        if not tall:
-            s.scene = scenes.dam_break_scene(s.n, seed=2000 + s.sid, fill=fill)
+        except fpl.FluidProjectionError as e:
+            if e.code == fpl.ERR_BUDGET:
+                return False  # over the HBM budget: wait until sessions finish
+            raise  # any other refusal would never clear: fail the run
+
         torch = self.torch
         phi, u, v, w = (torch.from_numpy(a).to(self.dev) for a in (s.scene.phi, s.scene.u, s.scene.v, s.scene.w))
         out = (torch.empty_like(phi), torch.empty_like(u), torch.empty_like(v), torch.empty_like(w))

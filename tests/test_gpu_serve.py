@@ -123,9 +123,10 @@ def test_served_stream_admission_and_oracle(fpl, oracle):
     oracle (pressure and velocities within 1e-5 relative L2)."""
     from fluid_amd.serve import ServeLoop
 
-    mix = {32: 0.4, 48: 0.3, 64: 0.2, "tall32": 0.1}
+    mix = {32: 0.4, 48: 0.3, 64: 0.2, "tall64": 0.1}
     # a 64^3 session leaves room for one 32^3 grid beside it: others wait
     budget = fpl.grid_bytes((64, 64, 64)) + fpl.grid_bytes((32, 32, 32))
+    assert fpl.tall_grid_bytes(64, 64, 64, 32) < budget
     loop = ServeLoop(rate_hz=40.0, duration_s=1.5, mix=mix, steps=4, seed=3, hbm_budget_bytes=budget,
                      keep_first=True)
     stats = loop.run()
