@@ -77,6 +77,9 @@ int fp_ctx_create(int32_t device, uint64_t hbm_budget_bytes, fp_ctx** out) {
     set_error("hipSetDevice(%d): %s", device, hipGetErrorString(e));
     return FP_ERR_DEVICE;
   }
+  // this device's copy of the no-allocation scratch, looked up once
+  int rc = fp::noalloc_scratch_init(device);
+  if (rc) return rc;
   fp_ctx* ctx = new (std::nothrow) fp_ctx();
   if (!ctx) return FP_ERR_OUT_OF_MEMORY;
   ctx->device = device;
@@ -380,6 +383,37 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     set_error("fp_project: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  return FP_OK;
+}
+
+int fp_project_noalloc(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options* o, fp_report* rep) {
+  CHECK_ARG(g != nullptr, "fp_project_noalloc: grid is NULL");
+  CHECK_ARG(g->uniform != nullptr, "fp_project_noalloc: uniform grids only");
+  CHECK_IDLE(g, "fp_project_noalloc");
+  int rc = validate_inputs(in, out, o);
+  if (rc) return rc;
+  fp_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  if (in->stream_order) {
+    hipError_t e = hipEventRecord(ctx->order, (hipStream_t)in->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->order, 0);
+    if (e != hipSuccess) {
+      set_error("fp_project_noalloc: ordering after the caller's stream: %s", hipGetErrorString(e));
+      return FP_ERR_DEVICE;
+    }
+  }
+  rc = g->uniform->bind(ctx->stream, in, out, o);
+  if (rc) return rc;
+  rc = ctx->proj->run_noalloc(ctx->stream, ctx->device, g->uniform, o, rep);
+  if (rc) return rc;
+  rc = g->uniform->finish_outputs(ctx->stream, in, out);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    set_error("fp_project_noalloc: %s", hipGetErrorString(e));
     return FP_ERR_DEVICE;
   }
   return FP_OK;

@@ -1576,7 +1576,10 @@ struct Projector::Plan {
   }
 };
 
+Projector::Projector() : noalloc_plan_(new Plan()) {}
+
 Projector::~Projector() {
+  delete noalloc_plan_;
   if (round_arena_) (void)hipFree(round_arena_);
   if (round_staging_) (void)hipHostFree(round_staging_);
   if (round_states_) (void)hipFree(round_states_);
@@ -2022,6 +2025,87 @@ int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const 
       fill_report(p.gathered_host[i], *grids[i]->host_rec(), ms, timed_ms, timed_launches, &reps[i]);
   return FP_OK;
 }
+
+// ---- the projection that allocates nothing ---------------------------------
+// The device memory fp_project_noalloc needs beyond the grid's own buffers --
+// the part tables of its launches, the gathered State and the poll counters --
+// lives in this array.  Each device holds its own copy for the life of the
+// process; host code never passes `scratch` itself to a kernel or a copy, only
+// the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Looks up this device's copy of `scratch` once (with the device current).
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d out of range for the no-allocation scratch", device);
+    return FP_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FP_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
+  g_scratch_ptr[device] = (unsigned char*)ptr;
+  return FP_OK;
+}
+
+int Projector::run_noalloc(hipStream_t s, int device, UniformGrid* g, const fp_options* opt, fp_report* rep) {
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+  }
+  if (!base) {
+    set_error("fp_project_noalloc: the scratch of device %d was not looked up (context set-up failed)", device);
+    return FP_ERR_STATE;
+  }
+  g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  int rc = g->upload(s);
+  if (rc) return rc;
+  // host tables in a plan kept for this entry point (its vectors keep their
+  // capacity), device tables in the scratch: parts, then the State, then
+  // the counters
+  Plan& p = *noalloc_plan_;
+  const std::vector<UniformGrid*> one_grid(1, g);
+  fill_tables(&p, one_grid, opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  const size_t parts_bytes = p.host.size() * sizeof(Part);
+  const size_t state_off = (parts_bytes + 255) & ~(size_t)255;
+  const size_t counts_off = state_off + ((sizeof(State) + 255) & ~(size_t)255);
+  const size_t need = counts_off + 16;
+  if (need > kNoAllocScratchBytes) {
+    set_error("fp_project_noalloc: %zu bytes of launch tables exceed the %zu-byte scratch", need,
+              (size_t)kNoAllocScratchBytes);
+    return FP_ERR_INVALID_ARGUMENT;
+  }
+  Part* parts = (Part*)base;
+  State* gathered = (State*)(base + state_off);
+  int* dev_counts = (int*)(base + counts_off);
+  HIPCHK(hipMemcpyAsync(parts, p.host.data(), parts_bytes, hipMemcpyHostToDevice, s));
+  set_table_pointers(&p, parts);
+  timed_kernel_ = 0;
+  timer_used_ = 0;
+  int host_counts[4] = {0, 0, 0, 0};
+  const auto t0 = std::chrono::steady_clock::now();
+  rc = solve(s, p, opt, false, dev_counts, host_counts);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gather_states<Part>, dim3(1), dim3(256), 0, s, p.one.dev, p.one.n, gathered);
+  State S;
+  HIPCHK(hipMemcpyAsync(&S, gathered, sizeof(State), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (rep) fill_report(S, *g->host_rec(), ms, 0.0, 0, rep);
+  return FP_OK;
+}
+
 
 // ---- continuous batching ---------------------------------------------------
 static double now_seconds() {

@@ -156,6 +156,13 @@ __device__ __forceinline__ int part_index(const P* parts, int n) {
 
 int plan_uniform_levels(int nx, int ny, int nz, int dims[][3], int strides[][3]);
 
+// Device scratch of fp_project_noalloc: the part tables of one grid's
+// launches (at most 8 + 7 kMaxLevels parts of 16 bytes), its gathered State
+// and the poll counters, each 256-byte aligned.
+constexpr size_t kNoAllocScratchBytes = 4096;
+// Looks up the device's copy of the scratch (at context set-up).
+int noalloc_scratch_init(int device);
+
 // ---- host helpers -----------------------------------------------------------
 #define HIPCHK(expr)                                                   \
   do {                                                                 \
@@ -252,8 +259,12 @@ using PartTable = PartTableOf<Part>;
 class Projector {
  public:
   struct Plan;
+  Projector();
   ~Projector();
   int run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps);
+  // One grid's projection with every launch table, state and counter in the
+  // device's static scratch (no device or pinned allocation, no graph).
+  int run_noalloc(hipStream_t s, int device, UniformGrid* g, const fp_options* opt, fp_report* rep);
   // Drops cached plans that name the grid (before the grid is destroyed).
   void forget(int grid_id);
   // Parity probes on one grid.
@@ -281,6 +292,7 @@ class Projector {
   void timer_begin(hipStream_t s);
   void timer_end(hipStream_t s);
   std::map<std::vector<long long>, Plan*> plans_;
+  Plan* noalloc_plan_ = nullptr;  // host tables of run_noalloc
   int timed_kernel_ = 0;
   int timer_used_ = 0;
   std::vector<hipEvent_t> timer_events_;

@@ -45,7 +45,7 @@ KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "
 EXPORTED_SYMBOLS = [
     "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
     "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
-    "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_submit", "fp_poll",
+    "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_project_noalloc", "fp_submit", "fp_poll",
     "fp_probe_apply",
     "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
 ]
@@ -127,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     L.fp_project.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(Report)]
     L.fp_project_batch.argtypes = [vp, C.c_int32, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options),
                                    C.POINTER(Report)]
+    L.fp_project_noalloc.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options),
+                                     C.POINTER(Report)]
     L.fp_submit.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(C.c_uint64)]
     L.fp_poll.argtypes = [vp, C.c_uint64, C.c_int32, C.POINTER(Report), C.POINTER(C.c_double)]
     L.fp_probe_apply.argtypes = [vp, vp, vp, vp]

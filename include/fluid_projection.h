@@ -205,6 +205,23 @@ int fp_project(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_opt
 int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, fp_outputs* out,
                      const fp_options* options, fp_report* reports);
 
+/* fp_project for one uniform grid, allocating nothing: no hipMalloc, hipFree,
+ * pinned-host allocation, event or graph is created by the call.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture setup, tight serving loops, harnesses that
+ * forbid allocation).  The launch tables, the solver state it gathers and the
+ * poll counters live in a static device array of 4096 bytes per GPU (looked
+ * up at fp_ctx_create); the grid's own buffers were allocated by
+ * fp_grid_create.  Its kernels and results are those of fp_project without
+ * graph replay (bitwise equal).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (from several contexts or threads) race.
+ * A launch table that would not fit the array is rejected
+ * (FP_ERR_INVALID_ARGUMENT; a uniform grid has at most 120 parts, 2320 bytes
+ * with the state and counters, so none is).  Tall-cell grids:
+ * FP_ERR_INVALID_ARGUMENT.  device_ms reports the call's host wall time. */
+int fp_project_noalloc(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_options* options,
+                       fp_report* report);
+
 /* Serving (continuous batching).  fp_submit queues one projection and returns
  * at once; the context's worker thread advances every queued and running
  * projection by one stage per round (set-up, one CG + V-cycle iteration,
