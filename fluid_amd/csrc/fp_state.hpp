@@ -17,7 +17,7 @@ namespace fp {
 #define FP_OUTER_FACTOR 0.25
 #endif
 #ifndef FP_INNER_FACTOR
-#define FP_INNER_FACTOR 0.1
+#define FP_INNER_FACTOR 0.5
 #endif
 #ifndef FP_INNER_CAP
 #define FP_INNER_CAP 1e-6
