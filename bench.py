@@ -326,7 +326,7 @@ def bench_batch(args):
         phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (s.phi, s.u, s.v, s.w))
         out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
         reqs.append(dict(phi=phi, u=u, v=v, w=w, dt=s.dt, rho=s.rho, out=out))
-    mixed = default_options(precision="fp16", relative_tolerance=1e-6)
+    mixed = default_options(precision="fp16", relative_tolerance=1e-6, persistent=1 if args.persistent else 0)
     plain = default_options(precision="fp32", relative_tolerance=1e-6)
 
     def timed(fn, steps):
@@ -371,6 +371,7 @@ def bench_batch(args):
     return {
         "workload": f"{len(ss)} cubic grids, edges log-uniform in [32,192] (seed 0), one batched solve",
         "cells": cells,
+        "persistent_sweeps": bool(args.persistent),
         "mcells_per_s": round(cells / t_b / 1e6, 2),
         "ms": round(t_b * 1e3, 3),
         "steps": args.batch_steps,
@@ -452,6 +453,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweeps", type=int, default=2, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (profiling)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="column-tile sweeps as persistent launches (a fixed grid walking each stage's table)")
     ap.add_argument("--no-error-solve", action="store_true", help="skip the FP64 oracle solve")
     ap.add_argument("--headline-only", action="store_true",
                     help="uniform workload: skip the C0 / batch / served / tall-cell summaries")
@@ -536,7 +539,8 @@ def bench_uniform(args):
     out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
     g = UniformPressureProjection(s.shape, s.h)
     common = dict(precision=args.precision, relative_tolerance=1e-6, pre_sweeps=args.sweeps,
-                  post_sweeps=args.sweeps, graphs=0 if args.no_graphs else 1)
+                  post_sweeps=args.sweeps, graphs=0 if args.no_graphs else 1,
+                  persistent=1 if args.persistent else 0)
     opts = default_options(**common)
     for i in range(args.warmup):
         r = g.project(phi, u, v, w, s.dt, s.rho, out=out, options=opts)
@@ -621,7 +625,7 @@ def bench_uniform(args):
         "data": "synthetic (seeded dam + 8 spheres level set, Gaussian face velocities)",
         "config": {"workload": f"uniform MAC grid {args.n}^3 dam+8 spheres, one projection to 1e-6",
                    "cells": cells, "liquid_cells": int(rep["liquid_samples"]), "precision": args.precision,
-                   "sweeps": args.sweeps, "graphs": not args.no_graphs},
+                   "sweeps": args.sweeps, "graphs": not args.no_graphs, "persistent_sweeps": bool(args.persistent)},
         "latency_ms": {"p50": round(statistics.median(lat), 3),
                        "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
         "solve": {"iterations": rep["iterations"], "outer": rep["outer_iterations"],

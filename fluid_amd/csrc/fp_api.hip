@@ -280,6 +280,26 @@ int fp_grid_levels(const fp_grid* g, int32_t max_levels, int32_t* dims, int32_t*
   return FP_OK;
 }
 
+int fp_grid_set_tuning(fp_grid* g, const fp_tuning* t) {
+  CHECK_ARG(g && t, "fp_grid_set_tuning: NULL argument");
+  if (!g->uniform) {
+    set_error("fp_grid_set_tuning: uniform grids only");
+    return FP_ERR_UNSUPPORTED;
+  }
+  CHECK_ARG(t->sweep_tile_z == 0 || t->sweep_tile_z == 16 || t->sweep_tile_z == 32,
+            "sweep_tile_z must be 0, 16 or 32 (got %d)", t->sweep_tile_z);
+  CHECK_ARG(t->sweep_tile_z != 32 || g->desc.nz >= 32, "32-wide sweep tiles need nz >= 32 (nz = %d)", g->desc.nz);
+  CHECK_ARG(t->sweep_groups == 0 || (t->sweep_groups >= 256 && t->sweep_groups <= 65536),
+            "sweep_groups must be 0 or in [256, 65536] (got %d)", t->sweep_groups);
+  CHECK_IDLE(g, "fp_grid_set_tuning");
+  std::lock_guard<std::mutex> lock(g->ctx->gpu);
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  g->ctx->proj->forget(g->uniform->id());  // cached plans hold the old launch shapes
+  g->uniform->set_tuning(t->sweep_tile_z, t->sweep_groups);
+  return FP_OK;
+}
+
 int fp_grid_face_counts(const fp_grid* g, int64_t* nu, int64_t* nv, int64_t* nw) {
   CHECK_ARG(g && nu && nv && nw, "fp_grid_face_counts: NULL argument");
   const fp_grid_desc& d = g->desc;
@@ -307,6 +327,7 @@ static int validate_inputs(const fp_inputs* in, const fp_outputs* out, const fp_
   CHECK_ARG(o->pre_sweeps >= 0 && o->post_sweeps >= 0 && o->pre_sweeps <= 16 && o->post_sweeps <= 16,
             "sweep counts must be in [0,16]");
   CHECK_ARG(o->pre_sweeps == o->post_sweeps, "pre and post sweeps must match (symmetric preconditioner)");
+  CHECK_ARG(o->precision_test_shift >= 0 && o->precision_test_shift <= 64, "precision_test_shift must be in [0,64]");
   return FP_OK;
 }
 

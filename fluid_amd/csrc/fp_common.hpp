@@ -265,4 +265,39 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned* p, float v) {
   if (v > 0.0f) atomicMax(p, __float_as_uint(v));
 }
 
+// y = M x for the dense row-major FP64 n x n inverse of a coarsest level, on
+// the matrix cores (v_mfma_f64_16x16x4f64).  Each wave of the calling
+// workgroup takes 16-row blocks of M; per step of 4 columns, lane l supplies
+// A = M[row0 + l % 16][k + l / 16] and, in column 0 of B (lanes l % 16 == 0),
+// x[k + l / 16] (0 in the other columns), so column 0 of the 16 x 16
+// accumulator is the block's slice of M x: lane l holds column l % 16, rows
+// l / 16 + 4 i in its registers i = 0..3 (measured on gfx950 with an
+// identity A and a B of distinct entries).  out(row, value) receives each row
+// once.  The calling workgroup's size is a multiple of 64.
+typedef double fp_f64x4 __attribute__((ext_vector_type(4)));
+template <class Out>
+__device__ __forceinline__ void gemv_f64_mfma(const double* __restrict__ M, int n, const double* __restrict__ x,
+                                              Out&& out) {
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6), nwaves = (int)(blockDim.x >> 6);
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int nb = (n + 15) >> 4;
+  for (int rb = wave; rb < nb; rb += nwaves) {
+    fp_f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int row = rb * 16 + r16;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      const int k = k0 + kq;
+      const double a = (row < n && k < n) ? M[(size_t)row * n + k] : 0.0;
+      const double b = (r16 == 0 && k < n) ? x[k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (r16 == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = rb * 16 + kq + 4 * i;
+        if (r < n) out(r, acc[i]);
+      }
+    }
+  }
+}
+
 }  // namespace fp

@@ -132,7 +132,7 @@ __global__ void k_fin_true(const P* parts, int new_pass) {
     st->iter = 0;
     st->done = st->outer_done || !new_pass;
     if (!st->outer_done && new_pass) st->outer += 1;
-    st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
+    st->rscale = p.g->rscale_bias * ((m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f);
   }
 }
 
@@ -192,7 +192,7 @@ __global__ void k_fin_update(const P* parts) {
     } else if (rr <= st->inner_target2 || st->iter >= p.g->max_iter) {
       st->done = 1;
     }
-    st->rscale = (m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f;
+    st->rscale = p.g->rscale_bias * ((m > 0.0f && isfinite(m)) ? exp2f(-ceilf(log2f(m))) : 1.0f);
   }
 }
 

@@ -150,9 +150,9 @@ __device__ __forceinline__ void gst_pair(T* p, size_t i, float a, float b) {
 // first-colour cells of the ring-extended tile (CY rows of CZ/2), colour-2
 // lanes the second-colour cells of the tile (TY rows of TZ/2).
 template <int TZ, int D, typename TR, typename T, bool ZERO>
-__global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int level, int which_in, int first) {
+__device__ __forceinline__ void sweep3_body(const Part* parts, int np, int vb, int level, int which_in, int first) {
   static_assert(D == 1 || D == 2 || D == 4, "the prefetch depth must divide the unroll of 4");
-  FP_PART(parts, np);
+  FP_PART_AT(parts, np, vb);
   const State* st = cload(&Gp->st);
   if (cload(&st->done)) return;
   const LevelRec lvl = cload(&Gp->lv[level]);
@@ -437,6 +437,24 @@ __global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int l
 #pragma unroll
     for (int d = 0; d < 4; ++d)
       if (L0 + d <= x1 + 1) step(L0 + d, d);
+  }
+}
+
+// One launch per sweep: workgroup b is the table's workgroup b.
+template <int TZ, int D, typename TR, typename T, bool ZERO>
+__global__ __launch_bounds__(256) void k_sweep3(const Part* parts, int np, int level, int which_in, int first) {
+  sweep3_body<TZ, D, TR, T, ZERO>(parts, np, (int)blockIdx.x, level, which_in, first);
+}
+
+// The same sweep as a persistent launch: kPersistGroups workgroups (or
+// fewer) walk the table's `total` workgroups with a stride of the grid size,
+// a multiple of 8, so every workgroup keeps the XCD its table slots assume.
+template <int TZ, int D, typename TR, typename T, bool ZERO>
+__global__ __launch_bounds__(256) void k_sweep3_persistent(const Part* parts, int np, int total, int level,
+                                                           int which_in, int first) {
+  for (int vb = (int)blockIdx.x; vb < total; vb += (int)gridDim.x) {
+    sweep3_body<TZ, D, TR, T, ZERO>(parts, np, vb, level, which_in, first);
+    __syncthreads();  // the next table slot reuses the LDS planes
   }
 }
 

@@ -1324,6 +1324,7 @@ int TallGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp
   R.tol2 = opt ? opt->relative_tolerance * opt->relative_tolerance : 0.0;
   R.max_iter = opt ? std::max(1, opt->max_iterations) : 1;
   R.warm = opt ? (opt->warm_start ? 1 : 0) : 0;
+  R.rscale_bias = 1.0f;
   const int pre = opt ? opt->pre_sweeps : 2, post = opt ? opt->post_sweeps : 2;
   // z / z2 ping-pong: sweep 1 writes z, every later sweep flips
   const int sweeps = std::max(pre, 1) + post;

@@ -94,6 +94,7 @@ struct TallRec {
   double s_flux, inv_h, tol2, sub_scale;
   float omega;         // block-Jacobi damping at the sample level
   int warm;
+  float rscale_bias;   // 1 (the tall-cell multigrid is FP32)
 };
 
 using TallPart = PartOf<TallRec>;

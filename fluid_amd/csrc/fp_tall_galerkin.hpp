@@ -560,7 +560,8 @@ __global__ void k_tall_coarse_setup(const TallPart* parts) {
   }
 }
 
-// e = Minv r on the coarsest level's active nodes (the calling workgroup).
+// e = Minv r on the coarsest level's active nodes (the calling workgroup),
+// the FP64 product on the matrix cores.
 __device__ __forceinline__ void coarse_solve_block(const TallRec& G) {
   __shared__ double rs[kTallMaxCoarse];
   const TallLevel& L = G.lv[G.L - 1];
@@ -568,11 +569,9 @@ __device__ __forceinline__ void coarse_solve_block(const TallRec& G) {
   for (size_t i = threadIdx.x; i < (size_t)L.n; i += blockDim.x) L.e[i] = 0.0f;
   for (int q = threadIdx.x; q < m; q += blockDim.x) rs[q] = (double)L.r[G.cmap[q]];
   __syncthreads();
-  for (int row = threadIdx.x; row < m; row += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < m; ++q) s += G.minv[(size_t)row * m + q] * rs[q];
-    L.e[G.cmap[row]] = (float)s;
-  }
+  float* __restrict__ e = L.e;
+  const int* __restrict__ cmap = G.cmap;
+  gemv_f64_mfma(G.minv, m, rs, [&](int row, double v) { e[cmap[row]] = (float)v; });
 }
 
 // The bottom of the V-cycle in one workgroup per grid: levels lb .. L-2 (at

@@ -799,11 +799,8 @@ __device__ __forceinline__ void coarse_solve_block(const GridRec& G) {
   for (size_t idx = threadIdx.x; idx < g.n(); idx += blockDim.x) stT(e, idx, 0.0f);
   for (int q = threadIdx.x; q < n; q += blockDim.x) rs[q] = (double)(ldT(r, (size_t)cmap[q]) * rm);
   __syncthreads();
-  for (int row = threadIdx.x; row < n; row += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < n; ++q) s += minv[(size_t)row * n + q] * rs[q];
-    stT(e, (size_t)cmap[row], (float)(s * fac));
-  }
+  // the FP64 product on the matrix cores
+  gemv_f64_mfma(minv, n, rs, [&](int row, double v) { stT(e, (size_t)cmap[row], (float)(v * fac)); });
 }
 
 template <typename T, typename TR>
@@ -1287,13 +1284,13 @@ static bool fused_dims(const int* d, long long n) { return d[2] >= 16 && d[1] >=
 #ifndef FP_SWEEP_GROUPS
 #define FP_SWEEP_GROUPS 8192
 #endif
-static TileGeo sweep_geometry(const int* d, int TZ, int level) {
+static TileGeo sweep_geometry(const int* d, int TZ, int level, int groups = 0) {
   const int TY = 256 / TZ;
   TileGeo g;
   g.tz = (d[2] + TZ - 1) / TZ;
   g.ty = (d[1] + TY - 1) / TY;
   const long tiles = (long)g.tz * g.ty;
-  const long cx_target = std::max<long>(1, FP_SWEEP_GROUPS / tiles);
+  const long cx_target = std::max<long>(1, (groups > 0 ? groups : FP_SWEEP_GROUPS) / tiles);
   const long min_chunk = min_march_chunk((long long)d[0] * d[1] * d[2], level);
   int chunk = (int)std::max<long>(min_chunk, (d[0] + cx_target - 1) / cx_target);
   chunk = std::min(chunk, d[0]);
@@ -1482,6 +1479,7 @@ int UniformGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const
   R.tol2 = opt ? opt->relative_tolerance * opt->relative_tolerance : 0.0;
   R.max_iter = opt ? std::max(1, opt->max_iterations) : 1;
   R.warm = opt ? (opt->warm_start ? 1 : 0) : 0;
+  R.rscale_bias = opt ? std::ldexp(1.0f, opt->precision_test_shift) : 1.0f;
   phi_staged_ = true;
   return FP_OK;
 }
@@ -1495,6 +1493,16 @@ int UniformGrid::finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* 
   if (out->pressure) HIPCHK(hipMemcpyAsync(out->pressure, R.r, (size_t)R.n * 4, hipMemcpyDeviceToHost, s));
   (void)in;
   return FP_OK;
+}
+
+void UniformGrid::set_tuning(int tile_z, int groups) {
+  GridRec& R = *rec_;
+  for (int l = 0; l < R.L; ++l) {
+    LevelRec& lv = R.lv[l];
+    const int tz = (tile_z == 32 && lv.d[2] >= 32) ? 32 : (tile_z == 16 ? 16 : (lv.d[2] >= 32 ? 32 : 16));
+    lv.sweep_tz = tz;
+    lv.sw = sweep_geometry(lv.d, tz, l, groups);
+  }
 }
 
 void UniformGrid::set_level_scales(int precision, int pre, int post) {
@@ -1534,6 +1542,7 @@ int UniformGrid::upload(hipStream_t s) {
 struct Projector::Plan {
   std::vector<UniformGrid*> grids;
   int precision = 1, pre = 2, post = 2, maxL = 1;
+  bool persistent = false;  // sweeps as persistent launches (fp_options.persistent)
   std::vector<Part> host;
   Part* arena = nullptr;
   struct Pending {
@@ -1692,10 +1701,12 @@ Projector::Plan* Projector::plan_for(const std::vector<UniformGrid*>& grids, con
   key.push_back(opt->precision);
   key.push_back(opt->pre_sweeps);
   key.push_back(opt->post_sweeps);
+  key.push_back(opt->persistent ? 1 : 0);
   auto it = plans_.find(key);
   if (it != plans_.end()) return it->second;
   Plan* p = new Plan();
   fill_tables(p, grids, opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  p->persistent = opt->persistent != 0;
   bool ok = hipMalloc((void**)&p->arena, std::max<size_t>(1, p->host.size()) * sizeof(Part)) == hipSuccess;
   ok = ok && hipMemcpy(p->arena, p->host.data(), p->host.size() * sizeof(Part), hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMalloc((void**)&p->gathered, grids.size() * sizeof(State)) == hipSuccess;
@@ -1737,9 +1748,31 @@ void Projector::timer_end(hipStream_t s) {
     if (_t) timer_end(s);                       \
   } while (0)
 
+template <typename TR, typename T, bool ZERO>
+static void sweep_launch_persistent(hipStream_t s, const PartTable& t, int tz, int level, int which_in, int first) {
+  if (t.n <= 0) return;
+  const int grid = std::min(t.blocks, kPersistGroups);
+  if (tz == 32)
+    hipLaunchKernelGGL((k_sweep3_persistent<32, kSweepDepth, TR, T, ZERO>), dim3(grid), dim3(FP_BLOCK), 0, s, t.dev,
+                       t.n, t.blocks, level, which_in, first);
+  else
+    hipLaunchKernelGGL((k_sweep3_persistent<16, kSweepDepth, TR, T, ZERO>), dim3(grid), dim3(FP_BLOCK), 0, s, t.dev,
+                       t.n, t.blocks, level, which_in, first);
+}
+
 template <typename TR, typename T>
 static void sweep_launch(hipStream_t s, const PartTable& t32, const PartTable& t16, int level, int which_in, int first,
-                         bool zero) {
+                         bool zero, bool persistent) {
+  if (persistent) {
+    if (zero) {
+      sweep_launch_persistent<TR, T, true>(s, t32, 32, level, which_in, first);
+      sweep_launch_persistent<TR, T, true>(s, t16, 16, level, which_in, first);
+    } else {
+      sweep_launch_persistent<TR, T, false>(s, t32, 32, level, which_in, first);
+      sweep_launch_persistent<TR, T, false>(s, t16, 16, level, which_in, first);
+    }
+    return;
+  }
   if (zero) {
     PLAUNCH((k_sweep3<32, kSweepDepth, TR, T, true>), t32, s, level, which_in, first);
     PLAUNCH((k_sweep3<16, kSweepDepth, TR, T, true>), t16, s, level, which_in, first);
@@ -1762,8 +1795,8 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
     }
     for (int sw = 0; sw < pre; ++sw) {
       TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0)));
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)));
       for (int colour = 0; colour < 2; ++colour) {
         const int zi = (sw == 0 && colour == 0) ? 1 : 0;
         if (top) PLAUNCH((k_smooth<float, T>), T_.smooth, s, l, colour, zi);
@@ -1808,8 +1841,8 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
     }
     for (int sw = 0; sw < post; ++sw) {
       TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false)));
+            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)
+                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)));
     }
     // ---- unfused levels ----
     TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_add<T>, T_.smooth, s, l, 0));
@@ -1966,7 +1999,8 @@ static void fill_report(const State& S, const GridRec& R, float ms, double timed
   rep->iterations = S.total_iter;
   rep->outer_iterations = S.outer;
   rep->levels = R.L;
-  rep->rhs_norm = std::sqrt(std::max(0.0, S.bb));
+  // a non-finite right-hand side (non-finite inputs) reads as NaN
+  rep->rhs_norm = std::isfinite(S.bb) ? std::sqrt(std::max(0.0, S.bb)) : std::nan("");
   rep->relative_residual = S.bb > 0 ? std::sqrt(std::max(0.0, S.rr64) / S.bb) : 0.0;
   float fa, fb;
   std::memcpy(&fa, &S.divmax_before, 4);
@@ -1980,7 +2014,41 @@ static void fill_report(const State& S, const GridRec& R, float ms, double timed
   rep->timed_kernel_launches = timed_launches;
 }
 
+// Projections of a grid set; grids whose FP16 multigrid overflowed (a
+// non-finite preconditioned residual with a finite right-hand side: the
+// FP16 range, not the inputs) are projected again with the BF16 multigrid
+// and reported with FP_FLAG_PRECISION_FALLBACK.
 int Projector::run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps) {
+  std::vector<fp_report> local;
+  if (!reps) {
+    local.resize(grids.size());
+    reps = local.data();
+  }
+  int rc = run_once(s, grids, opt, reps);
+  if (rc || opt->precision != FP_PRECISION_MIXED_FP16) return rc;
+  std::vector<UniformGrid*> redo;
+  std::vector<size_t> at;
+  for (size_t i = 0; i < grids.size(); ++i)
+    if ((reps[i].flags & 0x1) && std::isfinite(reps[i].rhs_norm)) {
+      redo.push_back(grids[i]);
+      at.push_back(i);
+    }
+  if (redo.empty()) return FP_OK;
+  fp_options bf = *opt;
+  bf.precision = FP_PRECISION_MIXED_BF16;
+  std::vector<fp_report> again(redo.size());
+  if ((rc = run_once(s, redo, &bf, again.data()))) return rc;
+  for (size_t k = 0; k < redo.size(); ++k) {
+    const double first_ms = reps[at[k]].device_ms;
+    reps[at[k]] = again[k];
+    reps[at[k]].flags |= 0x20;  // FP_FLAG_PRECISION_FALLBACK
+    reps[at[k]].device_ms += first_ms;
+  }
+  return FP_OK;
+}
+
+int Projector::run_once(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt,
+                        fp_report* reps) {
   if (grids.empty()) return FP_OK;
   for (UniformGrid* g : grids) {
     g->set_level_scales(opt->precision, opt->pre_sweeps, opt->post_sweeps);
@@ -2077,6 +2145,7 @@ int Projector::run_noalloc(hipStream_t s, int device, UniformGrid* g, const fp_o
   Plan& p = *noalloc_plan_;
   const std::vector<UniformGrid*> one_grid(1, g);
   fill_tables(&p, one_grid, opt->precision, opt->pre_sweeps, opt->post_sweeps);
+  p.persistent = opt->persistent != 0;
   const size_t parts_bytes = p.host.size() * sizeof(Part);
   const size_t state_off = (parts_bytes + 255) & ~(size_t)255;
   const size_t counts_off = state_off + ((sizeof(State) + 255) & ~(size_t)255);
@@ -2118,8 +2187,8 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
     std::vector<Job*> jobs;
   };
   auto key_of = [](const fp_options& o) {
-    return ((long long)o.precision << 32) | ((long long)o.pre_sweeps << 24) | ((long long)o.post_sweeps << 16) |
-           (long long)(o.phi_preserve_levels & 0xffff);
+    return ((long long)o.persistent << 40) | ((long long)o.precision << 32) | ((long long)o.pre_sweeps << 24) |
+           ((long long)o.post_sweeps << 16) | (long long)(o.phi_preserve_levels & 0xffff);
   };
   std::map<long long, Group> setup, accum, iter, resid[2];
   std::vector<Job*> finish, live;
@@ -2150,6 +2219,7 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
     for (Job* j : js) gs.push_back(j->grid);
     plans.emplace_back(new Plan());
     fill_tables(plans.back().get(), gs, o.precision, o.pre_sweeps, o.post_sweeps);
+    plans.back()->persistent = o.persistent != 0;
     return plans.back().get();
   };
   std::vector<std::pair<Plan*, const Group*>> setup_p, accum_p, iter_p, resid_p[2];
@@ -2254,7 +2324,15 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
         break;
       }
       case Job::ITERATE:
-        if (S.done) j->stage = Job::ACCUMULATE;
+        if (S.done && (S.flags & 0x1) && j->opt.precision == FP_PRECISION_MIXED_FP16 && std::isfinite(S.bb)) {
+          // the FP16 multigrid overflowed: start again with the BF16 multigrid
+          j->opt.precision = FP_PRECISION_MIXED_BF16;
+          j->fallback = 1;
+          j->passes = 0;
+          j->stage = Job::SETUP;
+        } else if (S.done) {
+          j->stage = Job::ACCUMULATE;
+        }
         break;
       case Job::FINISH: {
         const GridRec& R = *j->grid->host_rec();
@@ -2262,6 +2340,7 @@ int Projector::serve_round(hipStream_t s, std::vector<Job*>& jobs) {
         std::memset(rep, 0, sizeof(*rep));
         int flags = S.flags;
         if (!S.outer_done || (S.flags & 0x1)) flags |= 0x8;
+        if (j->fallback) flags |= 0x20;  // FP_FLAG_PRECISION_FALLBACK
         rep->flags = flags;
         rep->iterations = S.total_iter;
         rep->outer_iterations = S.outer;

@@ -114,7 +114,7 @@ struct GridRec {
   float* out_p;
   double s_flux, s_q, inv_h, tol2, sub_scale;
   int warm;           // keep x from the previous projection (warm start)
-  int pad;
+  float rscale_bias;  // 2^precision_test_shift (1 in production)
 };
 
 // One grid's slice of a part-table launch (R: the grid record type).
@@ -131,8 +131,8 @@ using Part = PartOf<GridRec>;
 // of 8, so a grid's workgroup b runs on XCD b % 8 as tile_of_group assumes;
 // workgroups in the padding return at once.
 template <class P>
-__device__ __forceinline__ int part_index(const P* parts, int n) {
-  const int b = (int)blockIdx.x;
+__device__ __forceinline__ int part_index(const P* parts, int n, int b = -1) {
+  if (b < 0) b = (int)blockIdx.x;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -149,6 +149,19 @@ __device__ __forceinline__ int part_index(const P* parts, int n) {
   const int blk = (int)blockIdx.x - fp_part_.block0; \
   const int nblk = fp_part_.nblocks;                 \
   if (blk >= nblk) return;
+
+// FP_PART for virtual workgroup vb of a persistent launch, whose workgroups
+// walk the table's workgroups with a stride of the (multiple of 8) grid size.
+#define FP_PART_AT(parts, np, vb)                                   \
+  const auto fp_part_ = cload((parts) + part_index(parts, np, vb)); \
+  const auto* Gp = fp_part_.g;                                      \
+  const auto& G = *Gp;                                              \
+  const int blk = (vb) - fp_part_.block0;                           \
+  const int nblk = fp_part_.nblocks;                                \
+  if (blk >= nblk) return;
+
+// Workgroups of a persistent sweep launch (a multiple of 8: each keeps its XCD).
+constexpr int kPersistGroups = 2048;
 
 // Grid-stride loop over one grid's workgroups.
 #define FP_STRIDE(idx, N) \
@@ -197,6 +210,9 @@ class UniformGrid {
   // take device pointers, the output targets and the scalars.
   int bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt);
   void set_level_scales(int precision, int pre, int post);
+  // Sweep tile width along z (0, 16 or 32) and workgroups per sweep (0 or
+  // 256..65536) of the fused levels; 0 restores the automatic choice.
+  void set_tuning(int tile_z, int groups);
   int upload(hipStream_t s);  // record -> device (async)
   int finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* out);
 
@@ -238,6 +254,7 @@ struct Job {
   fp_options opt{};
   int stage = SETUP;
   int passes = 0;     // FP64 true residuals evaluated
+  int fallback = 0;   // re-solved with the BF16 multigrid after an FP16 overflow
   int rc = FP_OK;
   fp_report report{};
   double t_submit = 0, t_start = 0, t_done = 0;  // host seconds
@@ -262,6 +279,7 @@ class Projector {
   Projector();
   ~Projector();
   int run(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps);
+  int run_once(hipStream_t s, const std::vector<UniformGrid*>& grids, const fp_options* opt, fp_report* reps);
   // One grid's projection with every launch table, state and counter in the
   // device's static scratch (no device or pinned allocation, no graph).
   int run_noalloc(hipStream_t s, int device, UniformGrid* g, const fp_options* opt, fp_report* rep);

@@ -45,6 +45,7 @@ KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "
 EXPORTED_SYMBOLS = [
     "fp_last_error", "fp_abi_version", "fp_ctx_create", "fp_ctx_destroy", "fp_ctx_bytes_in_use",
     "fp_default_options", "fp_grid_bytes", "fp_grid_create", "fp_grid_destroy", "fp_grid_levels",
+    "fp_grid_set_tuning",
     "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_project_noalloc", "fp_submit", "fp_poll",
     "fp_probe_apply",
     "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
@@ -64,6 +65,10 @@ class Inputs(C.Structure):
                 ("dt", C.c_double), ("rho", C.c_double), ("stream", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("sweep_tile_z", C.c_int32), ("sweep_groups", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
 class Outputs(C.Structure):
     _fields_ = [("pressure", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p)]
 
@@ -72,7 +77,8 @@ class Options(C.Structure):
     _fields_ = [("relative_tolerance", C.c_double), ("max_iterations", C.c_int32), ("max_outer", C.c_int32),
                 ("precision", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32),
                 ("warm_start", C.c_int32), ("phi_preserve_levels", C.c_int32), ("check_every", C.c_int32),
-                ("timed_kernel", C.c_int32), ("graphs", C.c_int32), ("reserved", C.c_int32)]
+                ("timed_kernel", C.c_int32), ("graphs", C.c_int32), ("precision_test_shift", C.c_int32),
+                ("persistent", C.c_int32)]
 
 
 class Report(C.Structure):
@@ -123,6 +129,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_grid_create.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(vp)]
     L.fp_grid_destroy.argtypes = [vp]
     L.fp_grid_levels.argtypes = [vp, C.c_int32, vp, vp, C.POINTER(C.c_int32)]
+    L.fp_grid_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.fp_grid_face_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fp_project.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options), C.POINTER(Report)]
     L.fp_project_batch.argtypes = [vp, C.c_int32, C.POINTER(Inputs), C.POINTER(Outputs), C.POINTER(Options),
@@ -370,6 +377,11 @@ class UniformPressureProjection:
         phi = np.ascontiguousarray(phi, np.float32)
         _check(_lib.fp_probe_level_phi(self._g, phi.ctypes.data, level, preserve, out.ctypes.data))
         return out
+
+    def set_tuning(self, tile_z: int = 0, groups: int = 0):
+        """Sweep tile width along z (16 or 32) and workgroups per sweep of the
+        fused levels (fp_grid_set_tuning); 0 restores the automatic choice."""
+        _check(_lib.fp_grid_set_tuning(self._g, C.byref(Tuning(int(tile_z), int(groups)))))
 
     def probe_taps(self, level, axis):
         """Transfer taps between level-1 and level along axis, as the kernels

@@ -20,6 +20,11 @@
  *   fp_submit/fp_poll   no reference counterpart (the reference solves one grid per
  *                       frame); they serve many independent projections.
  *
+ * Precision fallback: an FP16 multigrid whose values overflow (a non-finite
+ * preconditioned residual with a finite right-hand side) is solved again with
+ * the BF16 multigrid and reported with FP_FLAG_PRECISION_FALLBACK
+ * (fp_project, fp_project_batch, fp_submit).
+ *
  * Conventions
  *   Arrays are C order with z fastest.  Uniform grid (nx,ny,nz):
  *     cells  phi, pressure, target_divergence : (nx, ny, nz)
@@ -142,7 +147,10 @@ typedef struct {
   int32_t check_every;       /* host convergence poll period in iterations (default 1) */
   int32_t timed_kernel;      /* FP_KERNEL_* whose launches are timed with HIP events (0: none) */
   int32_t graphs;            /* 1: replay each pair of CG + V-cycle iterations as one hipGraph (default) */
-  int32_t reserved;
+  int32_t precision_test_shift; /* test hook (0 in production): scales the multigrid input normalisation
+                                   by 2^k, k in [0, 64], to force an FP16 overflow from the host */
+  int32_t persistent;        /* 1: the column-tile sweeps run as persistent launches (a fixed grid of
+                                workgroups walking each stage's table) instead of one workgroup per tile */
 } fp_options;
 
 /* kernel classes for fp_options.timed_kernel */
@@ -190,6 +198,17 @@ int fp_grid_destroy(fp_grid* grid);
 int fp_grid_levels(const fp_grid* grid, int32_t max_levels, int32_t* dims, int32_t* strides, int32_t* levels);
 /* Face counts of the grid's three velocity arrays. */
 int fp_grid_face_counts(const fp_grid* grid, int64_t* nu, int64_t* nv, int64_t* nw);
+
+/* Launch shapes of a uniform grid's column-tile sweeps, picked per grid size
+ * by the autotuner (fluid_amd/autotune.py).  No reference counterpart (the
+ * reference's dispatch shapes are fixed 4x4x4 workgroups). */
+typedef struct {
+  int32_t sweep_tile_z; /* 0: automatic (32 when nz >= 32, else 16); 16 or 32 (32 needs nz >= 32) */
+  int32_t sweep_groups; /* 0: automatic (8192); workgroups a level's sweep is split into, in [256, 65536],
+                           which sets the x chunk each workgroup marches */
+  int32_t reserved[2];
+} fp_tuning;
+int fp_grid_set_tuning(fp_grid* grid, const fp_tuning* tuning);
 
 /* ---- projection ------------------------------------------------------------ */
 /* One complete projection: labels + theta assembly -> divergence/rhs ->
