@@ -23,25 +23,37 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
 # HBM bytes per launch of the timed kernel, measured with rocprofv3 FETCH_SIZE
 # and WRITE_SIZE (each in its own run, scaled by a 1 GiB copy probe: reads 2x,
-# writes 1x) -- profiles/uniform_384_sweep_and_tall_512_jacobi_hbm_traffic.md.
+# writes 1x) -- profiles/uniform_384_sweep_and_tall_512_jacobi_hbm_traffic.md and
+# profiles/uniform_384_sweep_pairs.md.
 # The bench cannot read hardware counters itself; a configuration without a
 # measurement reports null.
 MEASURED_TRAFFIC_MB = {
-    ("uniform", 384, "fp16", "smooth_fine", 2): 528.4,
+    ("uniform", 384, "fp16", "smooth_fine", 2, False): 499.9,
+    ("uniform", 384, "fp16", "smooth_fine", 2, True): 569.3,
     ("tall", 512, "fp32", "smooth_fine", 2): 286.3,
 }
 
 
-def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float:
+def sweep_launches(precision: str, sweeps: int, pairs: bool) -> int:
+    """Finest-level sweep launches of one V-cycle (both sides): with pairs,
+    two sweeps per launch (one more for an odd count per side)."""
+    if pairs and precision != "fp32":
+        return 2 * (sweeps // 2 + sweeps % 2)
+    return 2 * sweeps
+
+
+def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2, pairs: bool = False) -> float:
     """Algorithmic (compulsory) HBM bytes per grid cell for one launch.
 
-    smooth_fine: the fused red-black sweeps over the finest level, averaged
-    over the launches one V-cycle times (per side `sweeps`): each reads the
-    packed face coefficients (4 values of the storage type, 4s B) and the FP32
-    right-hand side (4 B) and writes the new correction (s B); all but the
-    first pre-smoothing sweep (zero guess) read the old correction (s B).  The
-    coarse correction is added by its own pass before the post-smoothing
-    sweeps.
+    smooth_fine: the red-black sweeps over the finest level, averaged over
+    the launches one V-cycle times (per side `sweeps`).  Per sweep (the unit
+    of work): read the packed face coefficients (4 values of the storage
+    type, 4s B) and the FP32 right-hand side (4 B), write the new correction
+    (s B); all but the first pre-smoothing sweep (zero guess) read the old
+    correction (s B).  A launch of a pair of sweeps does two units of work
+    and counts both (its own HBM traffic is about one sweep's: the measured
+    traffic shows it).  The coarse correction is added by its own pass
+    before the post-smoothing sweeps.
     residual_fine: read coefficients (4s), r (4), e (s); write t (s).
     cg_direction: read z (s), d_old (4), phi (4); write d (4), q (4).
     cg_update: read d, q, e, r (16); write e, r (8).
@@ -50,7 +62,7 @@ def kernel_bytes_per_cell(kernel: str, precision: str, sweeps: int = 2) -> float
     """
     s = 4 if precision == "fp32" else 2
     plain = 4 * s + 4 + 2 * s
-    smooth = (2 * sweeps * plain - s) / (2 * sweeps) if sweeps > 0 else plain
+    smooth = (2 * sweeps * plain - s) / sweep_launches(precision, sweeps, pairs) if sweeps > 0 else plain
     return {
         "smooth_fine": smooth,
         "cg_direction": s + 4 + 4 + 4 + 4,
@@ -456,6 +468,8 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="column-tile sweeps as persistent launches (a fixed grid walking each stage's table)")
     ap.add_argument("--no-error-solve", action="store_true", help="skip the FP64 oracle solve")
+    ap.add_argument("--sweep-pairs", action="store_true",
+                    help="finest-level sweeps two to a launch (fp_tuning.sweep_pairs)")
     ap.add_argument("--headline-only", action="store_true",
                     help="uniform workload: skip the C0 / batch / served / tall-cell summaries")
     ap.add_argument("--copy-probe-mib", type=int, default=0, help="only run a device copy of this size")
@@ -538,6 +552,9 @@ def bench_uniform(args):
     phi, u, v, w = (torch.from_numpy(a).to(dev) for a in (s.phi, s.u, s.v, s.w))
     out = tuple(torch.empty_like(a) for a in (phi, u, v, w))
     g = UniformPressureProjection(s.shape, s.h)
+    pairs = args.sweep_pairs and args.precision != "fp32"
+    if pairs:
+        g.set_tuning(pairs=1)
     common = dict(precision=args.precision, relative_tolerance=1e-6, pre_sweeps=args.sweeps,
                   post_sweeps=args.sweeps, graphs=0 if args.no_graphs else 1,
                   persistent=1 if args.persistent else 0)
@@ -588,7 +605,8 @@ def bench_uniform(args):
                  "iterations": r32.report["iterations"], "relative_residual": r32.report["relative_residual"],
                  "p_fp32": out[0].cpu().numpy()}
     g.destroy()
-    bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps)
+    bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps,
+                                pairs and args.timed_kernel == "smooth_fine")
     # the timed finest-level passes touch only the cells of the liquid hulls
     active = hull_cells(s.phi)
     alg_bytes = bpc * active
@@ -625,7 +643,8 @@ def bench_uniform(args):
         "data": "synthetic (seeded dam + 8 spheres level set, Gaussian face velocities)",
         "config": {"workload": f"uniform MAC grid {args.n}^3 dam+8 spheres, one projection to 1e-6",
                    "cells": cells, "liquid_cells": int(rep["liquid_samples"]), "precision": args.precision,
-                   "sweeps": args.sweeps, "graphs": not args.no_graphs, "persistent_sweeps": bool(args.persistent)},
+                   "sweeps": args.sweeps, "graphs": not args.no_graphs, "persistent_sweeps": bool(args.persistent),
+                   "sweep_pairs": pairs},
         "latency_ms": {"p50": round(statistics.median(lat), 3),
                        "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
         "solve": {"iterations": rep["iterations"], "outer": rep["outer_iterations"],
@@ -635,7 +654,7 @@ def bench_uniform(args):
         "plain_fp32_mgpcg": plain or None,
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps)),
+                     "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps, pairs)),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_cell": bpc, "hull_cells": active,
                      "algorithmic_bytes": round(alg_bytes), "launches": kl,
                      "measured_in": "2 untimed projections after the timed region, kernels launched one by one"},

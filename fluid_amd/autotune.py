@@ -2,8 +2,9 @@
 
 For a grid size class it times every candidate configuration on a seeded
 scene of that size -- smoothing sweeps per side, the finest-level sweep
-tile width along z (16 or 32 lanes) and the number of workgroups a sweep is
-split into (which sets the x chunk each workgroup marches) -- keeps the
+tile width along z (16 or 32 lanes), the number of workgroups a sweep is
+split into (which sets the x chunk each workgroup marches) and whether the
+finest level's sweeps run two to a launch (FP16/BF16) -- keeps the
 fastest one that reaches the tolerance with no flags, and persists the
 choice per size class to a JSON file.  ``apply`` reads the file back and
 configures a grid (fp_grid_set_tuning) and the options of its projections.
@@ -29,6 +30,7 @@ CLASS_EDGES = (32, 48, 64, 96, 128, 192, 256, 384, 512)
 SWEEPS = (1, 2, 3)
 TILES = (32, 16)
 GROUPS = (4096, 8192, 16384)
+PAIRS = (0, 1)
 
 
 def size_class(dims) -> int:
@@ -42,14 +44,16 @@ def size_class(dims) -> int:
     return best
 
 
-def candidates(dims):
-    """Every (sweeps, tile_z, groups) the grid can run: 32-wide tiles need
-    nz >= 32."""
+def candidates(dims, precision="fp16"):
+    """Every (sweeps, tile_z, groups, pairs) the grid can run: 32-wide tiles
+    need nz >= 32; sweep pairs need an FP16/BF16 multigrid."""
     out = []
-    for sw, tz, gr in itertools.product(SWEEPS, TILES, GROUPS):
+    for sw, tz, gr, pa in itertools.product(SWEEPS, TILES, GROUPS, PAIRS):
         if tz == 32 and int(dims[2]) < 32:
             continue
-        out.append({"sweeps": sw, "tile_z": tz, "groups": gr})
+        if pa and precision == "fp32":
+            continue
+        out.append({"sweeps": sw, "tile_z": tz, "groups": gr, "pairs": pa})
     return out
 
 
@@ -57,7 +61,7 @@ def time_config(grid, tensors, scene, cfg, precision="fp16", tol=1e-6, reps=3):
     """Median wall time (ms) of `reps` projections with a configuration, and
     the last report."""
     import torch
-    grid.set_tuning(tile_z=cfg["tile_z"], groups=cfg["groups"])
+    grid.set_tuning(tile_z=cfg["tile_z"], groups=cfg["groups"], pairs=cfg.get("pairs", 0))
     opts = fpl.default_options(precision=precision, relative_tolerance=tol, pre_sweeps=cfg["sweeps"],
                                post_sweeps=cfg["sweeps"])
     phi, u, v, w, out = tensors
@@ -90,7 +94,7 @@ def tune(dims, scene=None, precision="fp16", tol=1e-6, reps=3, log=None):
     g = fpl.UniformPressureProjection(scene.shape, scene.h)
     trials = []
     try:
-        for cfg in candidates(scene.shape):
+        for cfg in candidates(scene.shape, precision):
             ms, rep = time_config(g, (phi, u, v, w, out), scene, cfg, precision, tol, reps)
             ok = rep["flags"] == 0 and rep["relative_residual"] <= tol
             trials.append(dict(cfg, ms=round(ms, 4), iterations=rep["iterations"], ok=ok))
@@ -103,9 +107,9 @@ def tune(dims, scene=None, precision="fp16", tol=1e-6, reps=3, log=None):
         raise RuntimeError(f"no configuration converged for {tuple(scene.shape)}")
     best = min(good, key=lambda t: t["ms"])
     default = next((t for t in trials if t["sweeps"] == 2 and t["tile_z"] == (32 if scene.shape[2] >= 32 else 16)
-                    and t["groups"] == 8192), None)
+                    and t["groups"] == 8192 and not t.get("pairs", 0)), None)
     return {"dims": [int(d) for d in scene.shape], "precision": precision, "tolerance": tol,
-            "best": {k: best[k] for k in ("sweeps", "tile_z", "groups")}, "best_ms": best["ms"],
+            "best": {k: best.get(k, 0) for k in ("sweeps", "tile_z", "groups", "pairs")}, "best_ms": best["ms"],
             "default_ms": default["ms"] if default else None, "trials": trials}
 
 
@@ -129,11 +133,13 @@ def load(path=DEFAULT_FILE):
 
 
 def choice_for(dims, path=DEFAULT_FILE):
-    """The tuned (sweeps, tile_z, groups) of a grid's size class, or None."""
+    """The tuned (sweeps, tile_z, groups, pairs) of a grid's size class, or
+    None."""
     rec = load(path).get(str(size_class(dims)))
     if not rec:
         return None
     best = dict(rec["best"])
+    best.setdefault("pairs", 0)
     if best["tile_z"] == 32 and int(dims[2]) < 32:
         best["tile_z"] = 16
     return best
@@ -145,7 +151,7 @@ def apply(grid, path=DEFAULT_FILE, **opts):
     best = choice_for(grid.dims, path)
     if best is None:
         return fpl.default_options(**opts), None
-    grid.set_tuning(tile_z=best["tile_z"], groups=best["groups"])
+    grid.set_tuning(tile_z=best["tile_z"], groups=best["groups"], pairs=best["pairs"])
     return fpl.default_options(pre_sweeps=best["sweeps"], post_sweeps=best["sweeps"], **opts), best
 
 

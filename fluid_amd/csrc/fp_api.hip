@@ -291,12 +291,13 @@ int fp_grid_set_tuning(fp_grid* g, const fp_tuning* t) {
   CHECK_ARG(t->sweep_tile_z != 32 || g->desc.nz >= 32, "32-wide sweep tiles need nz >= 32 (nz = %d)", g->desc.nz);
   CHECK_ARG(t->sweep_groups == 0 || (t->sweep_groups >= 256 && t->sweep_groups <= 65536),
             "sweep_groups must be 0 or in [256, 65536] (got %d)", t->sweep_groups);
+  CHECK_ARG(t->sweep_pairs == 0 || t->sweep_pairs == 1, "sweep_pairs must be 0 or 1 (got %d)", t->sweep_pairs);
   CHECK_IDLE(g, "fp_grid_set_tuning");
   std::lock_guard<std::mutex> lock(g->ctx->gpu);
   (void)hipSetDevice(g->ctx->device);
   (void)hipStreamSynchronize(g->ctx->stream);
   g->ctx->proj->forget(g->uniform->id());  // cached plans hold the old launch shapes
-  g->uniform->set_tuning(t->sweep_tile_z, t->sweep_groups);
+  g->uniform->set_tuning(t->sweep_tile_z, t->sweep_groups, t->sweep_pairs);
   return FP_OK;
 }
 

@@ -154,6 +154,16 @@ __device__ __forceinline__ double atomic_load_d(const double* p) {
 // keeps the compiler from moving LDS accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// The value of an FP32 expression rounded to FP32 in a register.  The
+// compiler may otherwise fold a product and its conversion to FP16/BF16 into
+// one mixed-precision FMA that rounds once (v_fma_mixlo_f16), at some sites
+// and not at others; the smoothers round every update to FP32 first, so two
+// kernels that compute the same sweep store the same bits.
+__device__ __forceinline__ float f32_rounded(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Two-kernel deterministic reduction.  Producers store one partial per
 // workgroup of their grid (no fence: the kernel boundary publishes them); a
 // one-workgroup-per-grid finisher sums them in a fixed order.  This keeps

@@ -339,7 +339,7 @@ __device__ __forceinline__ void sweep3_body(const Part* parts, int np, int vb, i
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
       const int q = tid + 256 * k;
-      const float v = Store<TR>::ld(&pr[rs][k], 0) * rm;
+      const float v = f32_rounded(Store<TR>::ld(&pr[rs][k], 0) * rm);
       if (k + 1 < NR || q < CN) s_r[s0][q] = to_store<T>((pin && orr[k] >= 0) ? v : 0.0f);
     }
     lds_barrier();  // A
@@ -385,7 +385,7 @@ __device__ __forceinline__ void sweep3_body(const Part* parts, int np, int vb, i
           acc += wf(wc.wz) * wf(ev[4]);
           acc += wf(wn[2]) * wf(ev[5]);
         }
-        const float val = idg > 0.0f ? acc * idg : 0.0f;
+        const float val = idg > 0.0f ? f32_rounded(acc * idg) : 0.0f;
         s_c[s1][ci] = to_store<T>(val);
         if (!pairs && R >= x0 && R < x1 && c1st[h])
           Store<T>::gst(eout + oR, (size_t)(c1gy * nz + z0 + lz - 1), val);
@@ -416,7 +416,7 @@ __device__ __forceinline__ void sweep3_body(const Part* parts, int np, int vb, i
       acc += wf(wy_) * wf(c3);
       acc += wf(wc.wz) * wf(c4);
       acc += wf(wz_) * wf(c5);
-      const float val = idg > 0.0f ? acc * idg : 0.0f;
+      const float val = idg > 0.0f ? f32_rounded(acc * idg) : 0.0f;
       if (c2st[h]) {
         if (pairs) {
           // the cell pair (z even, z odd) of this row in one store: the

@@ -22,6 +22,7 @@
 //   subtract ....... project (webgpu-uniform-reference.wgsl.ts:1052-1099)
 #include "fp_uniform.hpp"
 #include "fp_sweep.hpp"
+#include "fp_sweep2.hpp"
 #include "fp_march.hpp"
 #include "fp_state.hpp"
 
@@ -693,16 +694,19 @@ __device__ __forceinline__ void prolong_cell(const GridRec& G, int level, int wh
 // its march tiles (the other planes are air and keep e = 0): each lane walks its
 // cell column along x, and every load of four planes (level set, correction,
 // eight coarse taps each, coarse air holding e = 0) is issued before the
-// arithmetic.  which: the fine buffer, updated in place.
+// arithmetic.  which: the fine buffer read; which_out: the buffer written
+// (the same one: updated in place).
 template <typename T>
-__global__ __launch_bounds__(256) void k_prolong_march(const Part* parts, int np, int level, int which) {
+__global__ __launch_bounds__(256) void k_prolong_march(const Part* parts, int np, int level, int which,
+                                                       int which_out) {
   FP_PART(parts, np);
   if (cload(&cload(&Gp->st)->done)) return;
   const LevelRec fl = cload(&Gp->lv[level]);
   const LevelRec cl = cload(&Gp->lv[level + 1]);
   const TileSpan sp = tile_span(fl, blk, nblk);
   if (!sp.inside || sp.x0 >= sp.x1) return;
-  T* __restrict__ ef = (T*)(which ? fl.e2 : fl.e);
+  const T* __restrict__ ef = (const T*)(which ? fl.e2 : fl.e);
+  T* __restrict__ eo = (T*)(which_out ? fl.e2 : fl.e);
   const T* __restrict__ ec = (const T*)(cl.efinal ? cl.e2 : cl.e);
   const float* __restrict__ phi = fl.phi;
   const float fac = cl.Se / fl.Se;
@@ -758,7 +762,7 @@ __global__ __launch_bounds__(256) void k_prolong_march(const Part* parts, int np
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (x + u < sp.x1 && ph[u] < 0.0f) Store<T>::gst(ef, (size_t)(x + u) * fplane + sp.col, ev[u] + fac * acc[u]);
+      if (x + u < sp.x1 && ph[u] < 0.0f) Store<T>::gst(eo, (size_t)(x + u) * fplane + sp.col, ev[u] + fac * acc[u]);
   }
 }
 
@@ -1495,8 +1499,9 @@ int UniformGrid::finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* 
   return FP_OK;
 }
 
-void UniformGrid::set_tuning(int tile_z, int groups) {
+void UniformGrid::set_tuning(int tile_z, int groups, int pairs) {
   GridRec& R = *rec_;
+  sweep_pairs_ = pairs;
   for (int l = 0; l < R.L; ++l) {
     LevelRec& lv = R.lv[l];
     const int tz = (tile_z == 32 && lv.d[2] >= 32) ? 32 : (tile_z == 16 ? 16 : (lv.d[2] >= 32 ? 32 : 16));
@@ -1543,6 +1548,7 @@ struct Projector::Plan {
   std::vector<UniformGrid*> grids;
   int precision = 1, pre = 2, post = 2, maxL = 1;
   bool persistent = false;  // sweeps as persistent launches (fp_options.persistent)
+  bool pairs = false;       // finest-level sweeps in fused pairs (FP16/BF16 multigrid, every grid opted in)
   std::vector<Part> host;
   Part* arena = nullptr;
   struct Pending {
@@ -1609,7 +1615,8 @@ Projector::~Projector() {
 void Projector::forget(int grid_id) {
   for (auto it = plans_.begin(); it != plans_.end();) {
     bool uses = false;
-    for (size_t i = 0; i + 3 < it->first.size(); ++i)
+    // keys: the grid ids, then precision, pre, post, persistent
+    for (size_t i = 0; i + 4 < it->first.size(); ++i)
       if (it->first[i] == grid_id) uses = true;
     if (uses) {
       Plan* p = it->second;
@@ -1631,6 +1638,8 @@ void Projector::fill_tables(Plan* p, const std::vector<UniformGrid*>& grids, int
   p->precision = precision;
   p->pre = pre;
   p->post = post;
+  p->pairs = precision != FP_PRECISION_FP32;
+  for (UniformGrid* g : grids) p->pairs = p->pairs && g->sweep_pairs() == 1;
   p->host.clear();
   p->pending.clear();
   using Items = std::vector<std::pair<UniformGrid*, int>>;
@@ -1782,6 +1791,28 @@ static void sweep_launch(hipStream_t s, const PartTable& t32, const PartTable& t
   }
 }
 
+// Two finest-level sweeps in one launch (FP16/BF16 storage only: the FP32
+// staging would not leave room for three workgroups per CU).
+template <typename T, bool ZERO>
+static void sweep2_launch(hipStream_t s, const PartTable& t32, const PartTable& t16, int which_in, int first,
+                          bool persistent) {
+  if constexpr (sizeof(T) == 2) {
+    if (persistent) {
+      if (t32.n > 0)
+        hipLaunchKernelGGL((k_sweep2_persistent<32, float, T, ZERO>), dim3(std::min(t32.blocks, kPersistGroups)),
+                           dim3(FP_BLOCK), 0, s, t32.dev, t32.n, t32.blocks, which_in, first);
+      if (t16.n > 0)
+        hipLaunchKernelGGL((k_sweep2_persistent<16, float, T, ZERO>), dim3(std::min(t16.blocks, kPersistGroups)),
+                           dim3(FP_BLOCK), 0, s, t16.dev, t16.n, t16.blocks, which_in, first);
+      return;
+    }
+    PLAUNCH((k_sweep2<32, float, T, ZERO>), t32, s, which_in, first);
+    PLAUNCH((k_sweep2<16, float, T, ZERO>), t16, s, which_in, first);
+  } else {
+    (void)s, (void)t32, (void)t16, (void)which_in, (void)first, (void)persistent;
+  }
+}
+
 template <typename T>
 int Projector::vcycle_t(hipStream_t s, Plan& p) {
   const int pre = p.pre, post = p.post;
@@ -1793,10 +1824,29 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
       PLAUNCH(k_zero_level<T>, T_.fusedcells, s, l);
       PLAUNCH(k_zero_level<T>, T_.smooth, s, l);
     }
+    if (top && p.pairs && pre > 0) {
+      // fused levels: launches of two sweeps (and one of one for an odd
+      // count), each reading one buffer and writing the other; the first,
+      // from zero, writes the buffer that leaves the result in pre & 1
+      const int launches = pre / 2 + (pre & 1);
+      int buf = (pre & 1) ^ ((launches - 1) & 1);
+      TIMED(FP_KERNEL_SMOOTH_FINE, (pre >= 2 ? sweep2_launch<T, true>(s, T_.sweep32, T_.sweep16, buf ? 0 : 1, 0,
+                                                                       p.persistent)
+                                             : sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, buf ? 0 : 1, 0,
+                                                                      true, p.persistent)));
+      for (int left = pre - (pre >= 2 ? 2 : 1); left > 0; left -= 2) {
+        TIMED(FP_KERNEL_SMOOTH_FINE, (left >= 2 ? sweep2_launch<T, false>(s, T_.sweep32, T_.sweep16, buf, 0,
+                                                                           p.persistent)
+                                                : sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, buf, 0,
+                                                                         false, p.persistent)));
+        buf ^= 1;
+      }
+    }
     for (int sw = 0; sw < pre; ++sw) {
-      TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)));
+      if (!(top && p.pairs))
+        TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
+              (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)
+                   : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, sw & 1, 0, sw == 0, p.persistent)));
       for (int colour = 0; colour < 2; ++colour) {
         const int zi = (sw == 0 && colour == 0) ? 1 : 0;
         if (top) PLAUNCH((k_smooth<float, T>), T_.smooth, s, l, colour, zi);
@@ -1836,13 +1886,29 @@ int Projector::vcycle_t(hipStream_t s, Plan& p) {
     // ---- fused levels: prolongation over the liquid hulls, then the sweeps ----
     if (post == 0) {
       PLAUNCH(k_prolong_add<T>, T_.fusedcells, s, l, pre & 1);
+    } else if (top && p.pairs) {
+      // launches of two sweeps flip the buffer once per pair: the
+      // prolongation writes the other buffer when that leaves the result in
+      // (pre + post) & 1
+      const int launches = post / 2 + (post & 1);
+      int buf = pre & 1;
+      const int out = (launches & 1) == (post & 1) ? buf : buf ^ 1;
+      TIMED(FP_KERNEL_PROLONG_FINE, PLAUNCH(k_prolong_march<T>, T_.res, s, l, buf, out));
+      buf = out;
+      for (int left = post; left > 0; left -= 2) {
+        TIMED(FP_KERNEL_SMOOTH_FINE, (left >= 2 ? sweep2_launch<T, false>(s, T_.sweep32, T_.sweep16, buf, 1,
+                                                                           p.persistent)
+                                                : sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, buf, 1,
+                                                                         false, p.persistent)));
+        buf ^= 1;
+      }
     } else {
-      TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_march<T>, T_.res, s, l, pre & 1));
-    }
-    for (int sw = 0; sw < post; ++sw) {
-      TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
-            (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)
-                 : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)));
+      TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_march<T>, T_.res, s, l, pre & 1, pre & 1));
+      for (int sw = 0; sw < post; ++sw) {
+        TIMED(top ? FP_KERNEL_SMOOTH_FINE : -1,
+              (top ? sweep_launch<float, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)
+                   : sweep_launch<T, T>(s, T_.sweep32, T_.sweep16, l, (pre + sw) & 1, 1, false, p.persistent)));
+      }
     }
     // ---- unfused levels ----
     TIMED(top ? FP_KERNEL_PROLONG_FINE : -1, PLAUNCH(k_prolong_add<T>, T_.smooth, s, l, 0));

@@ -212,7 +212,10 @@ class UniformGrid {
   void set_level_scales(int precision, int pre, int post);
   // Sweep tile width along z (0, 16 or 32) and workgroups per sweep (0 or
   // 256..65536) of the fused levels; 0 restores the automatic choice.
-  void set_tuning(int tile_z, int groups);
+  // pairs 1: pairs of finest-level sweeps run as one launch when the
+  // multigrid stores FP16/BF16; 0: one launch per sweep.
+  void set_tuning(int tile_z, int groups, int pairs = 0);
+  int sweep_pairs() const { return sweep_pairs_; }
   int upload(hipStream_t s);  // record -> device (async)
   int finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* out);
 
@@ -234,6 +237,7 @@ class UniformGrid {
   float *in_u_ = nullptr, *in_v_ = nullptr, *in_w_ = nullptr, *in_q_ = nullptr;  // host-input staging
   bool host_io_ = false;
   bool phi_staged_ = false;
+  int sweep_pairs_ = 0;
   int id_ = 0;
   std::vector<void*> allocations_;
   size_t bytes_ = 0;

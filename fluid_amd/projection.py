@@ -66,7 +66,8 @@ class Inputs(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("sweep_tile_z", C.c_int32), ("sweep_groups", C.c_int32), ("reserved", C.c_int32 * 2)]
+    _fields_ = [("sweep_tile_z", C.c_int32), ("sweep_groups", C.c_int32), ("sweep_pairs", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Outputs(C.Structure):
@@ -378,10 +379,12 @@ class UniformPressureProjection:
         _check(_lib.fp_probe_level_phi(self._g, phi.ctypes.data, level, preserve, out.ctypes.data))
         return out
 
-    def set_tuning(self, tile_z: int = 0, groups: int = 0):
+    def set_tuning(self, tile_z: int = 0, groups: int = 0, pairs: int = 0):
         """Sweep tile width along z (16 or 32) and workgroups per sweep of the
-        fused levels (fp_grid_set_tuning); 0 restores the automatic choice."""
-        _check(_lib.fp_grid_set_tuning(self._g, C.byref(Tuning(int(tile_z), int(groups)))))
+        fused levels (fp_grid_set_tuning); 0 restores the automatic choice.
+        pairs 1 runs the finest level's sweeps two to a launch (FP16/BF16
+        multigrid; the same bits as one launch per sweep)."""
+        _check(_lib.fp_grid_set_tuning(self._g, C.byref(Tuning(int(tile_z), int(groups), int(pairs)))))
 
     def probe_taps(self, level, axis):
         """Transfer taps between level-1 and level along axis, as the kernels

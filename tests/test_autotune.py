@@ -15,9 +15,10 @@ def test_size_classes():
 
 
 def test_candidates_respect_the_tile_width():
-    assert len(autotune.candidates((64, 64, 64))) == 18
+    assert len(autotune.candidates((64, 64, 64))) == 36
+    assert len(autotune.candidates((64, 64, 64), "fp32")) == 18
     narrow = autotune.candidates((64, 64, 24))
-    assert len(narrow) == 9 and all(c["tile_z"] == 16 for c in narrow)
+    assert len(narrow) == 18 and all(c["tile_z"] == 16 for c in narrow)
 
 
 def test_persisted_table(tmp_path):
@@ -29,7 +30,7 @@ def test_persisted_table(tmp_path):
         assert rec["best"]["groups"] in autotune.GROUPS
         assert rec["best_ms"] <= rec["default_ms"]
         assert all(t["ok"] for t in rec["trials"])
-    assert autotune.choice_for((384, 384, 384)) == table["384"]["best"]
+    assert autotune.choice_for((384, 384, 384)) == dict(table["384"]["best"], pairs=table["384"]["best"].get("pairs", 0))
     # a 32-wide tile falls back to 16 on a grid thinner than 32 along z
     c = autotune.choice_for((128, 128, 24 * 8))
     assert c is not None
