@@ -1,4 +1,5 @@
 // fp_api.hip -- the C ABI (include/fluid_projection.h) over the HIP solvers.
+#include "fp_cm11a.hpp"
 #include "fp_tall.hpp"
 #include "fp_uniform.hpp"
 
@@ -41,6 +42,7 @@ struct fp_grid {
   fp_grid_desc desc{};
   fp::UniformGrid* uniform = nullptr;  // FP_LAYOUT_UNIFORM
   fp::TallGrid* tall = nullptr;        // FP_LAYOUT_TALL_CELL
+  fp::Cm11a* cm11a = nullptr;          // the CM11a solver's levels, from its first projection
   uint64_t bytes = 0;
 };
 
@@ -247,6 +249,8 @@ int fp_grid_destroy(fp_grid* g) {
   g->ctx->grids -= 1;
   if (g->uniform) g->ctx->proj->forget(g->uniform->id());
   if (g->tall) g->ctx->tall->forget(g->tall->id());
+  if (g->cm11a) g->ctx->in_use -= g->cm11a->bytes();
+  delete g->cm11a;
   delete g->uniform;
   delete g->tall;
   delete g;
@@ -329,6 +333,64 @@ static int validate_inputs(const fp_inputs* in, const fp_outputs* out, const fp_
             "sweep counts must be in [0,16]");
   CHECK_ARG(o->pre_sweeps == o->post_sweeps, "pre and post sweeps must match (symmetric preconditioner)");
   CHECK_ARG(o->precision_test_shift >= 0 && o->precision_test_shift <= 64, "precision_test_shift must be in [0,64]");
+  CHECK_ARG(o->solver == FP_SOLVER_MGPCG || o->solver == FP_SOLVER_CM11A, "unknown solver %d", o->solver);
+  CHECK_ARG(o->open_top == 0 || o->open_top == 1, "open_top must be 0 or 1 (got %d)", o->open_top);
+  CHECK_ARG(o->full_cycles >= 0 && o->full_cycles <= 64, "full_cycles must be in [0,64]");
+  CHECK_ARG(o->v_cycles >= 0 && o->v_cycles <= 256, "v_cycles must be in [0,256]");
+  CHECK_ARG(o->lcp_sweeps >= 0 && o->lcp_sweeps <= 64, "lcp_sweeps must be in [0,64]");
+  if (o->solver == FP_SOLVER_MGPCG && o->open_top) {
+    set_error("an open top needs FP_SOLVER_CM11A (the MGPCG system has closed walls)");
+    return FP_ERR_UNSUPPORTED;
+  }
+  return FP_OK;
+}
+
+// One CM11a projection of a uniform grid (fp_project with FP_SOLVER_CM11A).
+static int project_cm11a(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options* o, fp_report* rep) {
+  fp_ctx* ctx = g->ctx;
+  if (!g->uniform) {
+    set_error("FP_SOLVER_CM11A: uniform grids only");
+    return FP_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  if (!g->cm11a) {
+    const fp_grid_desc& d = g->desc;
+    const uint64_t need = fp::Cm11a::estimate_bytes(d.nx, d.ny, d.nz);
+    if (ctx->budget && ctx->in_use + need > ctx->budget) {
+      set_error("FP_SOLVER_CM11A: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)",
+                (unsigned long long)need, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
+      return FP_ERR_BUDGET;
+    }
+    fp::Cm11a* c = new (std::nothrow) fp::Cm11a();
+    if (!c) return FP_ERR_OUT_OF_MEMORY;
+    int rc = c->create(d.nx, d.ny, d.nz, d.h);
+    if (rc) {
+      delete c;
+      return rc;
+    }
+    g->cm11a = c;
+    ctx->in_use += c->bytes();
+  }
+  if (in->stream_order) {
+    hipError_t e = hipEventRecord(ctx->order, (hipStream_t)in->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, ctx->order, 0);
+    if (e != hipSuccess) {
+      set_error("fp_project: ordering after the caller's stream: %s", hipGetErrorString(e));
+      return FP_ERR_DEVICE;
+    }
+  }
+  int rc = g->uniform->bind(ctx->stream, in, out, o);
+  if (rc) return rc;
+  rc = g->cm11a->run(ctx->stream, g->uniform, o, in->dt, in->rho, rep);
+  if (rc) return rc;
+  rc = g->uniform->finish_outputs(ctx->stream, in, out);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    set_error("fp_project: %s", hipGetErrorString(e));
+    return FP_ERR_DEVICE;
+  }
   return FP_OK;
 }
 
@@ -343,6 +405,16 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
   CHECK_ARG(in != nullptr && out != nullptr, "fp_project_batch: NULL inputs/outputs");
   fp_ctx* ctx = grids[0] ? grids[0]->ctx : nullptr;
   CHECK_ARG(ctx != nullptr, "fp_project_batch: grid 0 is NULL");
+  if (o && o->solver == FP_SOLVER_CM11A) {
+    if (count != 1) {
+      set_error("FP_SOLVER_CM11A: one grid per call (fp_project)");
+      return FP_ERR_UNSUPPORTED;
+    }
+    CHECK_IDLE(grids[0], "fp_project");
+    int rc = validate_inputs(&in[0], &out[0], o);
+    if (rc) return rc;
+    return project_cm11a(grids[0], &in[0], &out[0], o, reps);
+  }
   std::vector<fp::UniformGrid*> list;
   std::vector<fp::TallGrid*> tlist;
   std::vector<int> ui, ti;  // batch positions of the uniform / tall-cell grids
@@ -416,6 +488,10 @@ int fp_project_noalloc(fp_grid* g, const fp_inputs* in, fp_outputs* out, const f
   CHECK_IDLE(g, "fp_project_noalloc");
   int rc = validate_inputs(in, out, o);
   if (rc) return rc;
+  if (o->solver != FP_SOLVER_MGPCG) {
+    set_error("fp_project_noalloc: FP_SOLVER_MGPCG only");
+    return FP_ERR_UNSUPPORTED;
+  }
   fp_ctx* ctx = g->ctx;
   std::lock_guard<std::mutex> lock(ctx->gpu);
   (void)hipSetDevice(ctx->device);
@@ -445,6 +521,10 @@ int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options
   CHECK_ARG(g != nullptr && ticket != nullptr, "fp_submit: NULL argument");
   int rc = validate_inputs(in, out, o);
   if (rc) return rc;
+  if (o->solver != FP_SOLVER_MGPCG) {
+    set_error("fp_submit: FP_SOLVER_MGPCG only (CM11a projections run through fp_project)");
+    return FP_ERR_UNSUPPORTED;
+  }
   fp_ctx* ctx = g->ctx;
   if (g->tall && in->target_divergence) {
     set_error("fp_submit: target_divergence is not supported on tall-cell grids");

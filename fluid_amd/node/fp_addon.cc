@@ -292,6 +292,9 @@ napi_value report_object(napi_env env, const fp_report& r) {
   num("deviceMs", r.device_ms);
   num("liquidSamples", (double)r.liquid_samples);
   num("samples", (double)r.samples);
+  num("lcpResidual", r.lcp_residual);
+  num("coarseSweeps", r.coarse_sweeps);
+  num("coarseFailures", r.coarse_failures);
   return o;
 }
 
@@ -407,6 +410,11 @@ void parse_options(napi_env env, napi_value o, fp_options* opt) {
   opt->pre_sweeps = (int32_t)get_number(env, o, "preSweeps", opt->pre_sweeps);
   opt->post_sweeps = (int32_t)get_number(env, o, "postSweeps", opt->post_sweeps);
   opt->warm_start = (int32_t)get_number(env, o, "warmStart", opt->warm_start);
+  opt->solver = (int32_t)get_number(env, o, "solver", opt->solver);
+  opt->open_top = (int32_t)get_number(env, o, "openTop", opt->open_top);
+  opt->full_cycles = (int32_t)get_number(env, o, "fullCycles", opt->full_cycles);
+  opt->v_cycles = (int32_t)get_number(env, o, "vCycles", opt->v_cycles);
+  opt->lcp_sweeps = (int32_t)get_number(env, o, "lcpSweeps", opt->lcp_sweeps);
 }
 
 void discard(napi_env env, ProjectJob* j) {

@@ -11,6 +11,9 @@ export interface ProjectionReport {
   deviceMs: number;
   liquidSamples: number;
   samples: number;
+  lcpResidual: number;             // CM11a: max projected residual (1/s)
+  coarseSweeps: number;            // CM11a: most sweeps of one coarsest LCP solve
+  coarseFailures: number;          // CM11a: coarsest solves that reached the 4096-sweep cap
 }
 
 export interface ProjectionInputs {
@@ -31,6 +34,11 @@ export interface ProjectionOptions {
   preSweeps?: number;
   postSweeps?: number;
   warmStart?: 0 | 1;
+  solver?: 'mgpcg' | 'cm11a' | 0 | 1;  // cm11a: the reference's separating-wall LCP multigrid
+  openTop?: 0 | 1;                 // cm11a: air beyond the +y side
+  fullCycles?: number;             // cm11a: full cycles (default 3)
+  vCycles?: number;                // cm11a: V-cycles after them (default 4)
+  lcpSweeps?: number;              // cm11a: projected sweeps per side (default 6)
 }
 
 export interface ProjectionResult {

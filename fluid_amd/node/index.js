@@ -23,6 +23,7 @@ const path = require('path');
 const binding = require(path.join(__dirname, '..', '_lib', 'fluid_projection.node'));
 
 const PRECISION = { fp32: 0, fp16: 1, bf16: 2 };
+const SOLVER = { mgpcg: 0, cm11a: 1 };
 const FLAGS = {
   NONFINITE: 0x1, NONPOSITIVE_CURVATURE: 0x2, NONPOSITIVE_PRECONDITIONER: 0x4, NONCONVERGENCE: 0x8,
   UNANCHORED_COMPONENT: 0x10, PRECISION_FALLBACK: 0x20, COARSE_SINGULAR: 0x40,
@@ -39,6 +40,10 @@ function nativeOptions(options) {
   if (typeof opts.precision === 'string') {
     if (!(opts.precision in PRECISION)) throw new RangeError(`unknown precision ${opts.precision}`);
     opts.precision = PRECISION[opts.precision];
+  }
+  if (typeof opts.solver === 'string') {
+    if (!(opts.solver in SOLVER)) throw new RangeError(`unknown solver ${opts.solver}`);
+    opts.solver = SOLVER[opts.solver];
   }
   return opts;
 }
@@ -247,9 +252,6 @@ function sceneCheck(scene) {
   if (!scene || !scene.container || !scene.voxelDomain || !scene.fluid || !scene.numerics) {
     throw new TypeError('createSolverAsync needs a SceneDescription (container, voxelDomain, fluid, numerics)');
   }
-  if (scene.container.top === 'open') {
-    throw new RangeError('the MI355X projection closes all six walls; open-top containers are not supported');
-  }
   if (scene.container.shape === 'sphere') {
     throw new RangeError('the MI355X projection supports box containers only');
   }
@@ -351,6 +353,11 @@ const PARAMS = [
       { value: 'fp32', label: 'FP32' }] },
   { kind: 'number', key: 'smoothingSweeps', label: 'Smoothing sweeps', tier: 'fine', update: 'runtime', unit: '',
     min: 1, max: 4, step: 1, default: 2, hint: 'Red-black Gauss-Seidel sweeps before and after each coarse correction.' },
+  { kind: 'select', key: 'pressureSolver', label: 'Pressure solver', tier: 'coarse', update: 'solver',
+    default: 'mgpcg', hint: 'MGPCG to an FP64 relative residual with closed walls, or the reference CM11a '
+      + 'multigrid (FP32, 3 full + 4 V-cycles) whose walls hold p >= 0 so liquid may separate from them; '
+      + 'open-top containers always use CM11a.',
+    options: [{ value: 'mgpcg', label: 'MGPCG (closed walls)' }, { value: 'cm11a', label: 'CM11a (separating walls)' }] },
 ];
 
 const PRESETS = {
@@ -392,6 +399,7 @@ const mi355xProjectionMethod = {
     v.smoothingSweeps = Math.max(1, Math.min(4, Math.round(Number(v.smoothingSweeps) || 2)));
     if (!(v.precision in PRECISION)) v.precision = 'fp16';
     if (v.layout !== 'tall-cell') v.layout = 'uniform';
+    if (!(v.pressureSolver in SOLVER) || v.layout === 'tall-cell') v.pressureSolver = 'mgpcg';
     return v;
   },
   runtimeParamKeys: ['precision', 'smoothingSweeps'],
@@ -406,6 +414,18 @@ const mi355xProjectionMethod = {
     const vals = this.normalizeValues(Object.assign({}, this.presetFor(quality), values));
     progress('planning', 'Planning the pressure lattice', 0, 3);
     const lat = sceneLattice(scene);
+    // an open top is air beyond the +y side: the CM11a solver's boundary
+    // (params.boundary.w of the reference); it needs the uniform layout and a
+    // lattice whose hierarchy halves exactly (planUniformCM11aHierarchy)
+    const openTop = scene.container.top === 'open';
+    if (openTop) {
+      if (vals.layout === 'tall-cell') throw new RangeError('open-top containers need the uniform layout (CM11a)');
+      vals.pressureSolver = 'cm11a';
+    }
+    if (vals.pressureSolver === 'cm11a' && !cm11aLattice(lat)) {
+      throw new RangeError(`the CM11a solver needs a lattice that halves exactly down to 256 haloed cells `
+        + `(got ${lat.nx} x ${lat.ny} x ${lat.nz})`);
+    }
     const rho = scene.fluid.density_kg_m3;
     const gy = scene.fluid.gravity_m_s2.y;
     const dt = scene.numerics.fixedDt_s;
@@ -452,7 +472,9 @@ const mi355xProjectionMethod = {
       equivalentUniformCells: equivalent, compressionRatio: equivalent / cellCount,
       regularLayers, maximumNeighborDelta: maxDelta, gridKind: 'uniform', quality,
       cellSize_m: lat.h, pressureIterations: 0,
-      pressureSolver: `MI355X HIP MGPCG (${vals.precision} multigrid, FP32 CG, FP64 refinement, `
+      pressureSolver: vals.pressureSolver === 'cm11a'
+        ? `MI355X HIP CM11a LCP multigrid (separating walls, FP32, 3 full + 4 V-cycles${openTop ? ', open top' : ''})`
+        : `MI355X HIP MGPCG (${vals.precision} multigrid, FP32 CG, FP64 refinement, `
         + `${vals.layout === 'tall-cell' ? 'restricted tall cells' : 'uniform grid'})`,
       allocatedBytes: binding.bytesInUse(proj.context),
       time_s: 0, steps: 0, ordinaryGridFallback,
@@ -461,7 +483,7 @@ const mi355xProjectionMethod = {
     let pending = Promise.resolve();
     let failure;
     let destroyed = false;
-    const opts = () => ({
+    const opts = () => (vals.pressureSolver === 'cm11a' ? { solver: 'cm11a', openTop: openTop ? 1 : 0 } : {
       precision: vals.precision, preSweeps: vals.smoothingSweeps, postSweeps: vals.smoothingSweeps,
       relativeTolerance: scene.numerics.pressureRelativeTolerance,
       maxIterations: Math.max(1, Math.min(1000, scene.numerics.pressureMaxIterations)),
@@ -495,6 +517,12 @@ const mi355xProjectionMethod = {
         info.pressureCurvatureBreakdown = (r.flags & FLAGS.NONPOSITIVE_CURVATURE) !== 0;
         info.pressureFlags = r.flags;
         info.pressureDeviceMs = r.deviceMs;
+        if (vals.pressureSolver === 'cm11a') {
+          // the reference's own CM11a readout: projected residual in 1/s, coarse-solve cap failures
+          info.pressureResidual = r.lcpResidual;
+          info.pressureIterationCapReached = r.coarseFailures > 0;
+          info.pressureConvergenceReason = r.coarseFailures > 0 ? 'iteration-cap' : 'tolerance';
+        }
         return res;
       });
     };
@@ -535,7 +563,28 @@ const mi355xProjectionMethod = {
   },
 };
 
+// True when the lattice's multigrid plan halves every axis exactly (or keeps
+// it) down to a coarsest lattice of at most 256 cells with its halo: the
+// hierarchies planUniformCM11aHierarchy accepts
+// (lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:149-169).
+function cm11aLattice(lat) {
+  const d = [lat.nx, lat.ny, lat.nz];
+  const haloed = (e) => (e[0] + 2) * (e[1] + 2) * (e[2] + 2);
+  const m = Math.min(...d);
+  if ((m & (m - 1)) === 0) {
+    const f = m >= 2 ? m / 2 : 1;
+    if (d.every((x) => x % f === 0) && haloed(d.map((x) => x / f)) <= 256) return true;
+  }
+  let e = d.slice();
+  for (;;) {
+    const n = e.map((x) => (x % 2 === 0 && x > 2 ? x / 2 : x));
+    if (n.every((x, a) => x === e[a])) break;
+    e = n;
+  }
+  return haloed(e) <= 256;
+}
+
 module.exports = {
   binding, UniformPressureProjection, TallCellPressureProjection, projectBatch, mi355xProjectionMethod,
-  sceneLattice, damBreakBox, tallLayout, PRECISION, FLAGS,
+  sceneLattice, damBreakBox, tallLayout, cm11aLattice, PRECISION, SOLVER, FLAGS,
 };

@@ -39,6 +39,7 @@ MEMORY_HOST, MEMORY_DEVICE = 0, 1
 (ERR_INVALID_ARGUMENT, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_LAYOUT, ERR_BUDGET, ERR_STATE,
  ERR_UNSUPPORTED) = (-1, -2, -3, -4, -5, -6, -7)
 PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+SOLVERS = {"mgpcg": 0, "cm11a": 1}
 KERNELS = {"none": 0, "smooth_fine": 1, "residual_fine": 2, "cg_direction": 3, "cg_update": 4,
            "true_residual": 5, "prolong_fine": 6, "restrict_fine": 7}
 
@@ -79,7 +80,8 @@ class Options(C.Structure):
                 ("precision", C.c_int32), ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32),
                 ("warm_start", C.c_int32), ("phi_preserve_levels", C.c_int32), ("check_every", C.c_int32),
                 ("timed_kernel", C.c_int32), ("graphs", C.c_int32), ("precision_test_shift", C.c_int32),
-                ("persistent", C.c_int32)]
+                ("persistent", C.c_int32), ("solver", C.c_int32), ("open_top", C.c_int32),
+                ("full_cycles", C.c_int32), ("v_cycles", C.c_int32), ("lcp_sweeps", C.c_int32)]
 
 
 class Report(C.Structure):
@@ -87,7 +89,8 @@ class Report(C.Structure):
                 ("levels", C.c_int32), ("relative_residual", C.c_double), ("rhs_norm", C.c_double),
                 ("max_div_before", C.c_double), ("max_div_after", C.c_double), ("device_ms", C.c_double),
                 ("liquid_samples", C.c_int64), ("samples", C.c_int64), ("timed_kernel_ms", C.c_double),
-                ("timed_kernel_launches", C.c_int32), ("reserved", C.c_int32)]
+                ("timed_kernel_launches", C.c_int32), ("coarse_sweeps", C.c_int32),
+                ("lcp_residual", C.c_double), ("coarse_failures", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -165,6 +168,8 @@ def default_options(**kw) -> Options:
             v = PRECISIONS[v]
         if k == "timed_kernel" and isinstance(v, str):
             v = KERNELS[v]
+        if k == "solver" and isinstance(v, str):
+            v = SOLVERS[v]
         setattr(o, k, v)
     return o
 

@@ -64,7 +64,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 2
+#define FP_ABI_VERSION 3
 
 /* return codes */
 #define FP_OK 0
@@ -94,6 +94,13 @@ extern "C" {
 #define FP_PRECISION_FP32 0        /* FP32 multigrid and CG, FP64 refinement */
 #define FP_PRECISION_MIXED_FP16 1  /* FP16 multigrid (per-level scaling), FP32 CG, FP64 refinement */
 #define FP_PRECISION_MIXED_BF16 2  /* BF16 multigrid, FP32 CG, FP64 refinement */
+
+/* fp_options.solver */
+#define FP_SOLVER_MGPCG 0  /* FP64-refined MGPCG of the Neumann-wall system (default) */
+#define FP_SOLVER_CM11A 1  /* the reference's CM11a separating-wall LCP multigrid
+                              (lib/methods/uniform/webgpu-uniform-pressure-multigrid.ts:488-576):
+                              box walls hold p >= 0, so liquid may pull away from them; FP32
+                              with a fixed cycle schedule; uniform grids, fp_project only */
 
 typedef struct fp_ctx fp_ctx;
 typedef struct fp_grid fp_grid;
@@ -151,6 +158,13 @@ typedef struct {
                                    by 2^k, k in [0, 64], to force an FP16 overflow from the host */
   int32_t persistent;        /* 1: the column-tile sweeps run as persistent launches (a fixed grid of
                                 workgroups walking each stage's table) instead of one workgroup per tile */
+  int32_t solver;            /* FP_SOLVER_* (ABI 3) */
+  int32_t open_top;          /* FP_SOLVER_CM11A: 1 opens the +y side of the box to air (p = 0 beyond it,
+                                params.boundary.w of the reference) */
+  int32_t full_cycles;       /* FP_SOLVER_CM11A: full multigrid cycles (0: 3, UNIFORM_CM11A_FULL_CYCLES) */
+  int32_t v_cycles;          /* FP_SOLVER_CM11A: V-cycles after them (0: 4) */
+  int32_t lcp_sweeps;        /* FP_SOLVER_CM11A: projected red-black sweeps before and after each
+                                coarse correction (0: 6) */
 } fp_options;
 
 /* kernel classes for fp_options.timed_kernel */
@@ -177,6 +191,12 @@ typedef struct {
   int64_t samples;
   double timed_kernel_ms;        /* summed device time of the timed kernel's launches */
   int32_t timed_kernel_launches; /* how many launches were timed */
+  int32_t coarse_sweeps;         /* FP_SOLVER_CM11A: most sweeps of one coarsest LCP solve */
+  double lcp_residual;           /* FP_SOLVER_CM11A: max projected residual in 1/s (mgMeasureFineResidual,
+                                    …multigrid.wgsl.ts:473-481); relative_residual is then the
+                                    2-norm of the projected residual over ||b|| */
+  int32_t coarse_failures;       /* FP_SOLVER_CM11A: coarsest solves that reached the 4096-sweep cap
+                                    (FP_FLAG_NONCONVERGENCE) */
   int32_t reserved;
 } fp_report;
 

@@ -173,6 +173,27 @@ int orc_tall_solve(const orc_tall* t, const double* b, double rel_tol, int max_i
 double orc_tall_max_divergence(const orc_tall* t, const double* u, const double* v, const double* w);
 int64_t orc_tall_coo(const orc_tall* t, int64_t* rows, int64_t* cols, double* vals, int64_t cap);
 
+/* ---- CM11a separating-wall LCP solve (fp_oracle_cm11a.c) ---- */
+typedef struct {
+  int full_cycles, v_cycles, pre, post, constraint_levels, preserve, coarse_cap;
+  float coarse_tol; /* coarsest projected residual, 1/s */
+  int open_top;
+} orc_cm11a_options;
+typedef struct {
+  int levels;
+  int coarse_sweeps;   /* most sweeps of one coarsest solve */
+  int coarse_failed;   /* coarsest solves that reached the cap */
+  int coarse_solves;
+  float lcp_residual;  /* finest projected residual max, 1/s */
+  float coarse_residual;
+} orc_cm11a_report;
+void orc_cm11a_default_options(orc_cm11a_options* o);
+int orc_cm11a_project(int nx, int ny, int nz, double h, const float* phi, const float* u, const float* v,
+                      const float* w, const float* q, double dt, double rho, const orc_cm11a_options* opt,
+                      float* p_padded, float* u_out, float* v_out, float* w_out, orc_cm11a_report* rep);
+int orc_cm11a_system(int nx, int ny, int nz, double h, const float* phi, const float* u, const float* v,
+                     const float* w, double dt, double rho, int open_top, float* coef, float* b, float* pmin);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,17 @@ class MGOptions(C.Structure):
     _fields_ = [("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32), ("phi_preserve_levels", C.c_int32)]
 
 
+class CM11aOptions(C.Structure):
+    _fields_ = [("full_cycles", C.c_int32), ("v_cycles", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
+                ("constraint_levels", C.c_int32), ("preserve", C.c_int32), ("coarse_cap", C.c_int32),
+                ("coarse_tol", C.c_float), ("open_top", C.c_int32)]
+
+
+class CM11aReport(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("coarse_sweeps", C.c_int32), ("coarse_failed", C.c_int32),
+                ("coarse_solves", C.c_int32), ("lcp_residual", C.c_float), ("coarse_residual", C.c_float)]
+
+
 class TallLayout(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("band", C.c_int32),
                 ("max_delta", C.c_int32), ("bases", C.c_void_p), ("phi", C.c_void_p), ("h", C.c_double)]
@@ -113,6 +124,12 @@ def lib():
         L.orc_tall_max_divergence.argtypes = [tp, _dp, _dp, _dp]
         L.orc_tall_coo.restype = C.c_int64
         L.orc_tall_coo.argtypes = [tp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_cm11a_default_options.argtypes = [C.POINTER(CM11aOptions)]
+        L.orc_cm11a_project.restype = i
+        L.orc_cm11a_project.argtypes = [i, i, i, d, _fp, _fp, _fp, _fp, C.c_void_p, d, d, C.POINTER(CM11aOptions),
+                                        _fp, _fp, _fp, _fp, C.POINTER(CM11aReport)]
+        L.orc_cm11a_system.restype = i
+        L.orc_cm11a_system.argtypes = [i, i, i, d, _fp, _fp, _fp, _fp, d, d, i, _fp, _fp, _fp]
         _lib = L
     return _lib
 
@@ -420,3 +437,46 @@ class Tall:
         rep["max_div_before"] = self.max_divergence(u, v, w)
         rep["max_div_after"] = self.max_divergence(uo, vo, wo)
         return p, (uo, vo, wo), b, rep
+
+
+def cm11a_options(**kw):
+    """The reference's CM11a schedule (webgpu-uniform-pressure-multigrid.ts:5-19),
+    with overrides."""
+    o = CM11aOptions()
+    lib().orc_cm11a_default_options(C.byref(o))
+    for k, val in kw.items():
+        setattr(o, k, val)
+    return o
+
+
+def cm11a_project(phi, u, v, w, h, dt, rho, q=None, **opts):
+    """CM11a separating-wall projection in FP32 (fp_oracle_cm11a.c): the
+    halo-padded finest pressure (nx+2, ny+2, nz+2), the projected
+    velocities and the report."""
+    phi = _f32(phi)
+    u, v, w = _f32(u), _f32(v), _f32(w)
+    nx, ny, nz = phi.shape
+    qq = _f32(q) if q is not None else None
+    p = np.zeros((nx + 2, ny + 2, nz + 2), np.float32)
+    uo, vo, wo = np.zeros_like(u), np.zeros_like(v), np.zeros_like(w)
+    rep = CM11aReport()
+    o = cm11a_options(**opts)
+    rc = lib().orc_cm11a_project(nx, ny, nz, h, phi, u, v, w, qq.ctypes.data if qq is not None else None, dt, rho,
+                                 C.byref(o), p, uo, vo, wo, C.byref(rep))
+    if rc <= 0:
+        raise RuntimeError(f"orc_cm11a_project failed ({rc})")
+    return p, (uo, vo, wo), {k: getattr(rep, k) for k, _ in CM11aReport._fields_}
+
+
+def cm11a_system(phi, u, v, w, h, dt, rho, open_top=0):
+    """The finest level's assembled CM11a LCP: coefficients (padded, 4 per
+    cell: +x, +y, +z, liquid flag), b and p_min."""
+    phi = _f32(phi)
+    u, v, w = _f32(u), _f32(v), _f32(w)
+    nx, ny, nz = phi.shape
+    D = (nx + 2, ny + 2, nz + 2)
+    coef = np.zeros(D + (4,), np.float32)
+    b = np.zeros(D, np.float32)
+    pmin = np.zeros(D, np.float32)
+    lib().orc_cm11a_system(nx, ny, nz, h, phi, u, v, w, dt, rho, open_top, coef, b, pmin)
+    return coef, b, pmin

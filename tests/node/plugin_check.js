@@ -59,6 +59,17 @@ async function run() {
     solver.destroy();
     out[layout] = { stepped, info, progress };
   }
+  // the CM11a solver: an open-top tank (32^3 lattice), and the closed tank by choice
+  const openTank = scene('dam-break', { container: { width_m: 0.8, height_m: 0.8, depth_m: 0.8, fillFraction: 0.22,
+    top: 'open', fluidWallMode: 'no-slip' } });
+  for (const [name, sc, values] of [['cm11aOpenTop', openTank, {}],
+    ['cm11aClosed', scene('dam-break'), { pressureSolver: 'cm11a' }]]) {
+    const solver = await meth.createSolverAsync(null, sc, 'high', values);
+    const stepped = [solver.advanceTo(0.016, []), solver.advanceTo(0.048, [])];
+    const info = await solver.readStats();
+    solver.destroy();
+    out[name] = { stepped, info };
+  }
   // addon entry points: project, submit and projectBatch agree
   const dims = [24, 20, 16];
   const h = 1 / 24;

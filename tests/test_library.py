@@ -33,11 +33,15 @@ def test_library_exports_every_declared_symbol(fpl):
 
 def test_abi_version_and_defaults(fpl):
     lib = fpl.load_library()
-    assert lib.fp_abi_version() == 2
+    assert lib.fp_abi_version() == 3
     o = fpl.default_options()
     assert o.relative_tolerance == 1e-6
     assert o.pre_sweeps == o.post_sweeps == 2
     assert o.precision == fpl.PRECISIONS["fp16"]
+    assert o.solver == fpl.SOLVERS["mgpcg"] and o.open_top == 0
+    assert o.full_cycles == o.v_cycles == o.lcp_sweeps == 0  # the CM11a schedule's defaults
+    # the structs match the header's layout
+    assert C.sizeof(fpl.Options) == 80 and C.sizeof(fpl.Report) == 104
 
 
 def test_grid_bytes_and_layout_rejection(fpl):

@@ -45,7 +45,8 @@ def test_method_satisfies_the_simulation_method_contract(fpl):
                for p in out["params"])
     for q, values in out["presets"].items():
         assert set(values) <= keys, q
-    assert out["normalized"] == {"precision": "fp16", "smoothingSweeps": 4, "layout": "uniform"}
+    assert out["normalized"] == {"precision": "fp16", "smoothingSweeps": 4, "layout": "uniform",
+                                 "pressureSolver": "mgpcg"}
     assert out["createSolverAsync"] == "function"
     assert out["binding"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
                                      "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion"])
@@ -57,7 +58,7 @@ def test_method_satisfies_the_simulation_method_contract(fpl):
 def test_addon_abi_version(fpl):
     r = run_node("const m=require('./fluid_amd/node');console.log(m.binding.abiVersion())")
     assert r.returncode == 0, r.stderr
-    assert r.stdout.strip() == "2"
+    assert r.stdout.strip() == "3"
 
 
 @pytest.mark.gpu
@@ -85,6 +86,13 @@ def test_plugin_solver_runs_scene_steps_and_entry_points(fpl):
     t = out["tall-cell"]["info"]
     assert t["storedNy"] == 34 and t["ny"] == 64 and t["regularLayers"] == 32
     assert t["compressionRatio"] == pytest.approx(64 / 34)
+    for name in ("cm11aOpenTop", "cm11aClosed"):
+        o = out[name]
+        info = o["info"]
+        assert o["stepped"] == [True, True] and info["steps"] == 3, (name, info)
+        assert "CM11a" in info["pressureSolver"] and ("open top" in info["pressureSolver"]) == (name == "cm11aOpenTop")
+        assert info["pressureFlags"] == 0 and info["pressureSolveConverged"] is True, (name, info)
+        assert info["maxDivergenceAfter_s"] < 0.1 * info["maxDivergenceBefore_s"], (name, info)
     e = out["entryPoints"]
     assert e["submitEqualsProject"] and e["batchEqualsProject"] and e["flags"] == [0, 0, 0, 0]
     assert e["latencyMs"] > 0
