@@ -839,49 +839,3 @@ int orc_cm11a_system(int nx, int ny, int nz, double h, const float* phi, const f
   return cells;
 }
 
-/* The coarsest level of the hierarchy (extrapolated level set, dual volumes,
- * padded dims) and the right-hand side of the first full cycle's coarsest
- * solve (b restricted down the levels), for the coarse-solve checks. */
-int orc_cm11a_coarsest(int nx, int ny, int nz, double h, const float* phi, const float* u, const float* v,
-                       const float* w, double dt, double rho, int open_top, int* dims, float* phi_out,
-                       float* topo_out, float* rhs_out, float* pmin_out, int cap) {
-  int pd[ORC_MAX_LEVELS * 3], strides[ORC_MAX_LEVELS * 3], ext = 0;
-  const int L = orc_plan_levels(nx, ny, nz, pd, strides, &ext);
-  if (L <= 0) return -1;
-  cml lv[ORC_MAX_LEVELS];
-  memset(lv, 0, sizeof(lv));
-  const int n[3] = {nx, ny, nz};
-  for (int l = 0; l < L; ++l) {
-    float hl[3];
-    for (int a = 0; a < 3; ++a) hl[a] = (float)(h * ((double)n[a] / (double)pd[3 * l + a]));
-    if (cml_alloc(&lv[l], pd + 3 * l, hl)) return -2;
-  }
-  const int c = L - 1;
-  if (lv[c].n > cap) {
-    for (int l = 0; l < L; ++l) cml_free(&lv[l]);
-    return -3;
-  }
-  float* b = calloc((size_t)lv[0].n, 4);
-  float* pmin0 = calloc((size_t)lv[0].n, 4);
-  finest_setup(&lv[0], n, phi, u, v, w, NULL, (float)dt, (float)rho, open_top, b, pmin0);
-  for (int l = 0; l + 1 < L; ++l) down_topology(&lv[l], &lv[l + 1], l + 1, L, 2, open_top);
-  for (int l = 0; l < L; ++l) extrapolate_phi(&lv[l]);
-  memcpy(lv[0].rhs1, b, 4 * (size_t)lv[0].n);
-  for (int l = 0; l + 1 < L; ++l) {
-    restrict_to(&lv[l], lv[l].rhs1, &lv[l + 1], lv[l + 1].rhs1);
-    if (l < 3)
-      down_minimum(&lv[l], l == 0 ? pmin0 : lv[l].pmin, 0, &lv[l + 1]);
-    else
-      fill(lv[l + 1].pmin, lv[l + 1].n, CM_NEG_BIG);
-  }
-  for (int a = 0; a < 3; ++a) dims[a] = lv[c].D[a];
-  memcpy(phi_out, lv[c].phi, 4 * (size_t)lv[c].n);
-  memcpy(topo_out, lv[c].topo, 16 * (size_t)lv[c].n);
-  memcpy(rhs_out, lv[c].rhs1, 4 * (size_t)lv[c].n);
-  memcpy(pmin_out, lv[c].pmin, 4 * (size_t)lv[c].n);
-  const int cells = lv[c].n;
-  free(b);
-  free(pmin0);
-  for (int l = 0; l < L; ++l) cml_free(&lv[l]);
-  return cells;
-}
