@@ -324,7 +324,7 @@ int fp_grid_face_counts(const fp_grid* g, int64_t* nu, int64_t* nv, int64_t* nw)
 static int validate_inputs(const fp_inputs* in, const fp_outputs* out, const fp_options* o) {
   CHECK_ARG(in && out && o, "fp_project: NULL argument");
   CHECK_ARG(in->memory == FP_MEMORY_HOST || in->memory == FP_MEMORY_DEVICE, "unknown memory kind %d", in->memory);
-  CHECK_ARG(in->phi && in->u && in->v && in->w, "fp_project: phi/u/v/w must be given");
+  CHECK_ARG((in->phi || in->density) && in->u && in->v && in->w, "fp_project: phi (or density)/u/v/w must be given");
   CHECK_ARG(in->dt > 0.0 && in->rho > 0.0, "fp_project: dt and rho must be positive");
   CHECK_ARG(o->relative_tolerance > 0.0 && o->relative_tolerance < 1.0, "relative_tolerance must be in (0,1)");
   CHECK_ARG(o->max_iterations >= 1 && o->max_outer >= 1, "iteration limits must be positive");
@@ -426,6 +426,10 @@ int fp_project_batch(fp_grid* const* grids, int32_t count, const fp_inputs* in, 
     int rc = validate_inputs(&in[i], &out[i], o);
     if (rc) return rc;
     if (grids[i]->tall) {
+      if (in[i].density) {
+        set_error("fp_project: density inputs are not supported on tall-cell grids (pass phi)");
+        return FP_ERR_UNSUPPORTED;
+      }
       if (in[i].target_divergence) {
         set_error("fp_project: target_divergence is not supported on tall-cell grids");
         return FP_ERR_UNSUPPORTED;
@@ -526,6 +530,10 @@ int fp_submit(fp_grid* g, const fp_inputs* in, fp_outputs* out, const fp_options
     return FP_ERR_UNSUPPORTED;
   }
   fp_ctx* ctx = g->ctx;
+  if (g->tall && in->density) {
+    set_error("fp_submit: density inputs are not supported on tall-cell grids (pass phi)");
+    return FP_ERR_UNSUPPORTED;
+  }
   if (g->tall && in->target_divergence) {
     set_error("fp_submit: target_divergence is not supported on tall-cell grids");
     return FP_ERR_UNSUPPORTED;

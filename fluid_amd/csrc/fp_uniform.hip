@@ -1443,13 +1443,22 @@ int UniformGrid::create(int nx, int ny, int nz, double h) {
   return FP_OK;
 }
 
+// pressurePhi (webgpu-uniform-reference.wgsl.ts:215-242) with open fraction
+// 1: phi = -(rho' - 1/2) dx from the surface density, in place
+__global__ void k_density_to_phi(float* __restrict__ a, size_t n, float h) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = -(a[i] - 0.5f) * h;
+}
+
 int UniformGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt) {
   GridRec& R = *rec_;
   const size_t n = (size_t)R.n;
   host_io_ = in->memory == FP_MEMORY_HOST;
-  // level 0 phi is a private copy, so callers may reuse their buffer
+  // level 0 phi is a private copy, so callers may reuse their buffer; a
+  // density input is copied there and converted in place
+  const float* phi_src = in->density ? in->density : in->phi;
   if (host_io_) {
-    HIPCHK(hipMemcpyAsync(R.lv[0].phi, in->phi, n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(R.lv[0].phi, phi_src, n * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(in_u_, in->u, (size_t)R.nu * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(in_v_, in->v, (size_t)R.nv * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(in_w_, in->w, (size_t)R.nw * 4, hipMemcpyHostToDevice, s));
@@ -1465,7 +1474,7 @@ int UniformGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const
     R.out_w = out && out->w ? in_w_ : nullptr;
     R.out_p = out && out->pressure ? R.r : nullptr;
   } else {
-    HIPCHK(hipMemcpyAsync(R.lv[0].phi, in->phi, n * 4, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(R.lv[0].phi, phi_src, n * 4, hipMemcpyDeviceToDevice, s));
     R.in_u = in->u;
     R.in_v = in->v;
     R.in_w = in->w;
@@ -1475,6 +1484,9 @@ int UniformGrid::bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const
     R.out_w = out ? out->w : nullptr;
     R.out_p = out ? out->pressure : nullptr;
   }
+  if (in->density)
+    hipLaunchKernelGGL(k_density_to_phi, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256), 0, s,
+                       R.lv[0].phi, n, (float)R.h);
   const double rho = in->rho, dt = in->dt, h = R.h;
   R.s_flux = rho * h / dt;
   R.s_q = rho * h * h / dt;

@@ -340,7 +340,7 @@ void project_complete(napi_env env, napi_status, void* data) {
   delete j;
 }
 
-// Parses one grid's {phi, u, v, w, targetDivergence?, dt, rho} and
+// Parses one grid's {phi | density, u, v, w, targetDivergence?, dt, rho} and
 // {pressure, u, v, w} into g; false after throwing a JS error.
 bool parse_io(napi_env env, ProjectJob* j, GridIO& g, napi_value inputs, napi_value outputs) {
   // array lengths the grid reads and writes: level-0 samples and the three face arrays
@@ -352,12 +352,12 @@ bool parse_io(napi_env env, ProjectJob* j, GridIO& g, napi_value inputs, napi_va
     return false;
   }
   const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
-  const size_t want_in[5] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2], cells};
+  const size_t want_in[6] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2], cells, cells};
   const size_t want_out[4] = {cells, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2]};
   g.in.memory = FP_MEMORY_HOST;
-  const char* in_names[5] = {"phi", "u", "v", "w", "targetDivergence"};
-  const float** in_ptrs[5] = {&g.in.phi, &g.in.u, &g.in.v, &g.in.w, &g.in.target_divergence};
-  for (int k = 0; k < 5; ++k) {
+  const char* in_names[6] = {"phi", "u", "v", "w", "targetDivergence", "density"};
+  const float** in_ptrs[6] = {&g.in.phi, &g.in.u, &g.in.v, &g.in.w, &g.in.target_divergence, &g.in.density};
+  for (int k = 0; k < 6; ++k) {
     napi_value v;
     if (!get_named(env, inputs, in_names[k], &v)) continue;
     float* p;

@@ -17,7 +17,8 @@ export interface ProjectionReport {
 }
 
 export interface ProjectionInputs {
-  phi: Float32Array;               // (nx, ny, nz), z fastest; liquid iff phi < 0
+  phi?: Float32Array;              // (nx, ny, nz), z fastest; liquid iff phi < 0
+  density?: Float32Array;          // instead of phi (uniform grids): surface density, phi = -(density - 1/2) h
   u: Float32Array;                 // (nx+1, ny, nz)
   v: Float32Array;                 // (nx, ny+1, nz)
   w: Float32Array;                 // (nx, ny, nz+1)

@@ -63,7 +63,7 @@ class GridDesc(C.Structure):
 class Inputs(C.Structure):
     _fields_ = [("memory", C.c_int32), ("stream_order", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p),
                 ("v", C.c_void_p), ("w", C.c_void_p), ("target_divergence", C.c_void_p),
-                ("dt", C.c_double), ("rho", C.c_double), ("stream", C.c_void_p)]
+                ("dt", C.c_double), ("rho", C.c_double), ("stream", C.c_void_p), ("density", C.c_void_p)]
 
 
 class Tuning(C.Structure):
@@ -293,11 +293,16 @@ class UniformPressureProjection:
         _check(_lib.fp_grid_levels(self._g, 16, dims.ctypes.data, strides.ctypes.data, C.byref(L)))
         return dims[: 3 * L.value].reshape(-1, 3), strides[: 3 * L.value].reshape(-1, 3)
 
-    def _prepare(self, phi, u, v, w, dt, rho, target_divergence=None, out=None):
+    def _prepare(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, density=None):
         """C input/output records for one projection, the result arrays and
-        the objects that must stay alive until the call returns."""
+        the objects that must stay alive until the call returns.  With
+        `density` (the surface density per cell) phi may be None."""
+        if phi is None and density is None:
+            raise ValueError("give phi or density")
+        dens = _ptr(density, np.float32)
         parts = [_ptr(phi, np.float32), _ptr(u, np.float32), _ptr(v, np.float32), _ptr(w, np.float32),
-                 _ptr(target_divergence, np.float32)]
+                 _ptr(target_divergence, np.float32), dens]
+        phi = phi if phi is not None else density
         kinds = {p[1] for p in parts if p[1] is not None}
         if len(kinds) != 1:
             raise TypeError("all arrays must live in the same memory (host or device)")
@@ -307,7 +312,7 @@ class UniformPressureProjection:
             if tuple(arr.shape) != shp:
                 raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
         ins = Inputs(mem, 0, parts[0][0], parts[1][0], parts[2][0], parts[3][0], parts[4][0], float(dt), float(rho),
-                     None)
+                     None, dens[0])
         if mem == MEMORY_DEVICE:
             # the library's work starts after what torch has queued on the
             # current stream (producers of the inputs, earlier users of `out`)
@@ -331,11 +336,13 @@ class UniformPressureProjection:
         return ins, outs, (p, uo, vo, wo), [q[2] for q in parts]
 
     def project(self, phi, u, v, w, dt, rho, target_divergence=None, out=None, options: Options | None = None,
-                **opts) -> ProjectionResult:
+                density=None, **opts) -> ProjectionResult:
         """One projection.  Host arrays in -> host arrays out; CUDA tensors in ->
-        results written into `out` tensors (or fresh tensors) on the device."""
+        results written into `out` tensors (or fresh tensors) on the device.
+        `density` (uniform grids) replaces phi by the reference's surface
+        density: phi = -(density - 1/2) h."""
         o = options or default_options(**opts)
-        ins, outs, res, keep = self._prepare(phi, u, v, w, dt, rho, target_divergence, out)
+        ins, outs, res, keep = self._prepare(phi, u, v, w, dt, rho, target_divergence, out, density)
         rep = Report()
         _check(_lib.fp_project(self._g, C.byref(ins), C.byref(outs), C.byref(o), C.byref(rep)))
         del keep

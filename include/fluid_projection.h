@@ -133,6 +133,10 @@ typedef struct {
                        work queued on it before the call (fp_project, fp_project_batch,
                        fp_submit), so device arrays produced or reused there are read and
                        written in order. */
+  const float* density; /* optional (ABI 3), uniform grids: the liquid volume fraction per cell, the
+                           reference's surface density.  When given, phi is not read and the level
+                           set is pressurePhi's phi = -(density - 1/2) h (webgpu-uniform-reference.
+                           wgsl.ts:215-242 with open fraction 1: box containers, no embedded solids) */
 } fp_inputs;
 
 typedef struct {
