@@ -254,6 +254,7 @@ def bench_tall(args, summary=False):
     bps = {"smooth_fine": 20.0, "residual_fine": 16.0, "cg_direction": 24.0, "cg_update": 24.0}.get(
         args.timed_kernel, 0.0)
     achieved = bps * samples / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    remesh = bench_tall_remesh(g, sc, phi, u, v, w)
     g.destroy()
     err = None
     if not args.no_error_solve and args.tall_n <= 128:
@@ -272,7 +273,7 @@ def bench_tall(args, summary=False):
         return {"workload": f"tall-cell {args.tall_n}^2 columns x {args.tall_n} rows, band 32 (wave surface)",
                 "msamples_per_s": round(value, 2), "ms": round(elapsed / steps * 1e3, 3), "steps": steps,
                 "samples": samples, "equivalent_fine_mcells_per_s": round(L.equivalent_cells * steps / elapsed / 1e6, 1),
-                "solve": solve, "roofline": roof}
+                "solve": solve, "roofline": roof, "remesh": remesh}
     line = {
         "metric": "MGPCG projection Mcells/s at 1e-6 rel. residual",
         "value": round(value, 2),
@@ -295,6 +296,7 @@ def bench_tall(args, summary=False):
                        "p99": round(lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
         "solve": dict(solve, fp64_oracle_pressure_rel_l2_error=err),
         "roofline": roof,
+        "remesh": remesh,
     }
     if not args.no_cpu_baseline:
         try:
@@ -302,6 +304,40 @@ def bench_tall(args, summary=False):
         except Exception as e:
             line["cpu_baseline"] = {"error": repr(e)}
     return line
+
+
+def bench_tall_remesh(g, sc, phi, u, v, w, calls: int = 6):
+    """fp_tall_remesh on device arrays, ping-ponging between two sets of
+    buffers: the surface alternately 4 fine rows lower and 4 rows higher than
+    the current one (samples clamped at -5 h stay liquid), so every call moves
+    most columns.  Host wall time of each call (the call returns when its
+    outputs are written); the algorithmic bytes are one read and one write
+    of phi, u, v and w plus the bases."""
+    import torch
+    h = sc.layout.h
+    cur = [a.clone() for a in (phi, u, v, w)]
+    nxt = [torch.empty_like(a) for a in cur]
+    ms, moved = [], []
+    for i in range(calls + 1):
+        shift = (4 if i % 2 == 0 else -4) * h
+        p = cur[0]
+        cur[0] = torch.where(p <= -5 * h * (1 - 1e-6), p, (p + shift).clamp(-5 * h, 5 * h)).contiguous()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = g.remesh(*cur, out=nxt)
+        if i > 0:  # the first call allocates the staging buffers
+            ms.append((time.perf_counter() - t1) * 1e3)
+            moved.append(res[4])
+        cur, nxt = nxt, cur
+    n = sum(int(a.numel()) for a in cur)
+    algo = 2 * 4 * n + 2 * 4 * sc.layout.bases.size
+    med = statistics.median(ms)
+    return {"calls": calls, "median_ms": round(med, 3), "min_ms": round(min(ms), 3),
+            "changed_columns": moved, "columns": int(sc.layout.bases.size),
+            "algorithmic_bytes": algo, "achieved_gbs": round(algo / (med * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(algo / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "note": "device arrays, out of place; host wall time per call, including the "
+                    "neighbour-limit convergence checks and the copy of the new bases to the host"}
 
 
 def tall_oracle_error(sc, p_gpu):

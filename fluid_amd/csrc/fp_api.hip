@@ -641,6 +641,35 @@ int fp_probe_taps(fp_grid* g, int32_t level, int32_t axis, int32_t* prolong_coar
                         restrict_weight);
 }
 
+int fp_tall_remesh(fp_grid* g, const fp_tall_remesh_args* a, int64_t* changed_columns) {
+  CHECK_ARG(g && a, "fp_tall_remesh: NULL argument");
+  CHECK_ARG(g->tall != nullptr, "fp_tall_remesh: tall-cell grids only");
+  CHECK_ARG(a->memory == FP_MEMORY_HOST || a->memory == FP_MEMORY_DEVICE, "fp_tall_remesh: memory must be 0 or 1");
+  CHECK_ARG(a->phi && a->u && a->v && a->w, "fp_tall_remesh: phi, u, v and w are required");
+  CHECK_ARG(a->liquid_halo >= 0 && a->air_halo >= 0, "fp_tall_remesh: halos must be >= 0");
+  CHECK_IDLE(g, "fp_tall_remesh");
+  fp_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  if (!g->tall->remesh_ready()) {
+    const fp_grid_desc& d = g->desc;
+    const uint64_t need = fp::TallGrid::remesh_bytes(d.nx, d.nz, d.band, tall_delta(&d));
+    if (ctx->budget && ctx->in_use + need > ctx->budget) {
+      set_error("fp_tall_remesh: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)",
+                (unsigned long long)need, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
+      return FP_ERR_BUDGET;
+    }
+  }
+  const size_t before = g->tall->bytes();
+  const int rc = g->tall->remesh(ctx->stream, a, changed_columns);
+  const size_t grown = g->tall->bytes() - before;
+  ctx->in_use += grown;
+  g->bytes += grown;
+  // plans hold per-grid state of the old layout
+  ctx->tall->forget(g->tall->id());
+  return rc;
+}
+
 int fp_probe_tall_layout(fp_grid* g, const float* phi, int32_t* rows, uint8_t* unknown) {
   CHECK_ARG(g && phi && rows && unknown, "fp_probe_tall_layout: NULL argument");
   CHECK_ARG(g->tall != nullptr, "fp_probe_tall_layout: tall-cell grids only");

@@ -111,6 +111,12 @@ class TallGrid {
   int bind(hipStream_t s, const fp_inputs* in, fp_outputs* out, const fp_options* opt);
   int upload(hipStream_t s);
   int finish_outputs(hipStream_t s, const fp_inputs* in, fp_outputs* out);
+  // Re-lay the grid out around the surface of a->phi and move the quantities
+  // to the new layout (fp_tall_remesh.hip); synchronous on s.
+  int remesh(hipStream_t s, const fp_tall_remesh_args* a, int64_t* changed_columns);
+  // Device bytes the first remesh adds (its staging buffers).
+  static size_t remesh_bytes(int nx, int nz, int band, int max_delta);
+  bool remesh_ready() const { return rm_in_ != nullptr; }
   int levels() const { return rec_->L; }
   const TallLevel& level(int l) const { return rec_->lv[l]; }
   size_t bytes() const { return bytes_; }
@@ -124,6 +130,8 @@ class TallGrid {
   TallRec* rec_ = nullptr;
   TallRec* dev_rec_ = nullptr;
   float *in_u_ = nullptr, *in_v_ = nullptr, *in_w_ = nullptr;
+  float *rm_in_ = nullptr, *rm_out_ = nullptr;  // remesh staging: phi, u, v, w in and out
+  int* rm_b_ = nullptr;                         // remesh bases (two buffers), flag, counter
   bool host_io_ = false;
   int id_ = 0;
   std::vector<void*> allocations_;

@@ -503,6 +503,83 @@ napi_value project_batch(napi_env env, napi_callback_info info) {
   return queue(env, j);
 }
 
+// remeshTallGrid(grid, {phi, u, v, w}, {phi, u, v, w}?, {liquidHalo, airHalo}?)
+//   -> {changedColumns, columnBases: Int32Array}
+// fp_tall_remesh, synchronous (about a millisecond on a 512^2-column grid);
+// without an output object the input arrays are updated in place.
+napi_value remesh_tall_grid(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) {
+    napi_throw_type_error(env, nullptr, "remeshTallGrid(grid, {phi, u, v, w}, outputs?, {liquidHalo, airHalo}?)");
+    return nullptr;
+  }
+  fp_grid* g = (fp_grid*)unwrap_external(env, argv[0]);
+  GridInfo* gi = g ? grid_info(g) : nullptr;
+  int32_t dims[3 * 16], levels = 0;
+  int64_t nf[3] = {0, 0, 0};
+  if (!gi || fp_grid_levels(g, 16, dims, nullptr, &levels) != FP_OK ||
+      fp_grid_face_counts(g, &nf[0], &nf[1], &nf[2]) != FP_OK) {
+    napi_throw_type_error(env, nullptr, "not a live projection grid");
+    return nullptr;
+  }
+  if (gi->pending > 0) {
+    napi_throw_error(env, "FP_ERR_6", "remeshTallGrid: the grid has a projection in flight (await it first)");
+    return nullptr;
+  }
+  const size_t samples = (size_t)dims[0] * dims[1] * dims[2];
+  const size_t want[4] = {samples, (size_t)nf[0], (size_t)nf[1], (size_t)nf[2]};
+  const char* names[4] = {"phi", "u", "v", "w"};
+  fp_tall_remesh_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.memory = FP_MEMORY_HOST;
+  const float** in_ptrs[4] = {&a.phi, &a.u, &a.v, &a.w};
+  float** out_ptrs[4] = {&a.phi_out, &a.u_out, &a.v_out, &a.w_out};
+  bool has_out = false;
+  if (argc > 2) {
+    napi_valuetype t;
+    napi_typeof(env, argv[2], &t);
+    has_out = t == napi_object;
+  }
+  for (int k = 0; k < 4; ++k) {
+    napi_value v;
+    float* p;
+    size_t len;
+    if (!get_named(env, argv[1], names[k], &v) || !get_f32(env, v, &p, &len) || len != want[k]) {
+      napi_throw_range_error(env, nullptr, "remeshTallGrid: phi, u, v, w must be Float32Arrays of the grid's sizes");
+      return nullptr;
+    }
+    *in_ptrs[k] = p;
+    *out_ptrs[k] = p;
+    if (has_out && get_named(env, argv[2], names[k], &v)) {
+      if (!get_f32(env, v, &p, &len) || len != want[k]) {
+        napi_throw_range_error(env, nullptr, "remeshTallGrid: outputs must be Float32Arrays of the grid's sizes");
+        return nullptr;
+      }
+      *out_ptrs[k] = p;
+    }
+  }
+  if (argc > 3) {
+    a.liquid_halo = (int32_t)get_number(env, argv[3], "liquidHalo", 0);
+    a.air_halo = (int32_t)get_number(env, argv[3], "airHalo", 0);
+  }
+  const size_t columns = (size_t)dims[0] * dims[2];
+  napi_value ab, bases, res, changed;
+  void* bp = nullptr;
+  NAPI_OK(napi_create_arraybuffer(env, columns * 4, &bp, &ab));
+  NAPI_OK(napi_create_typedarray(env, napi_int32_array, columns, ab, 0, &bases));
+  a.column_bases_out = (int32_t*)bp;
+  int64_t moved = 0;
+  int rc = fp_tall_remesh(g, &a, &moved);
+  if (rc) return throw_fp(env, rc);
+  NAPI_OK(napi_create_object(env, &res));
+  NAPI_OK(napi_create_double(env, (double)moved, &changed));
+  NAPI_OK(napi_set_named_property(env, res, "changedColumns", changed));
+  NAPI_OK(napi_set_named_property(env, res, "columnBases", bases));
+  return res;
+}
+
 napi_value last_error(napi_env env, napi_callback_info) {
   napi_value s;
   NAPI_OK(napi_create_string_utf8(env, fp_last_error(), NAPI_AUTO_LENGTH, &s));
@@ -522,7 +599,7 @@ napi_value init(napi_env env, napi_value exports) {
   } fns[] = {{"createContext", create_context}, {"destroyContext", destroy_context}, {"createGrid", create_grid},
              {"destroyGrid", destroy_grid},     {"gridLevels", grid_levels},         {"project", project},
              {"submit", submit},                {"projectBatch", project_batch},     {"bytesInUse", bytes_in_use},
-             {"lastError", last_error},         {"abiVersion", abi_version}};
+             {"lastError", last_error},         {"abiVersion", abi_version},         {"remeshTallGrid", remesh_tall_grid}};
   for (auto& f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

@@ -79,6 +79,12 @@ export declare class TallCellPressureProjection {
   // phi (nx, B+2, nz); u (nx+1, S, nz); v (nx, B+2, nz); w (nx, S, nz+1); S = B+2+maxDelta
   project(inputs: Omit<ProjectionInputs, 'targetDivergence'>, options?: ProjectionOptions): Promise<ProjectionResult>;
   submit(inputs: Omit<ProjectionInputs, 'targetDivergence'>, options?: ProjectionOptions): Promise<ProjectionResult>;
+  /** Re-lays the grid out around the surface of `fields.phi` and moves phi, u, v, w to the new
+   * layout in place (fp_tall_remesh); the grid projects with the new bases from then on.
+   * Throws while a projection of the grid is in flight. */
+  remesh(fields: { phi: Float32Array; u: Float32Array; v: Float32Array; w: Float32Array },
+         halos?: { liquidHalo?: number; airHalo?: number }): { changedColumns: number; columnBases: Int32Array };
+  columnBases: Int32Array;
   destroy(): void;
 }
 

@@ -146,8 +146,9 @@ class TallCellPressureProjection {
     this.context = context || defaultContext();
     this.nx = nx; this.fineNy = fineNy; this.nz = nz; this.band = band; this.maxDelta = maxDelta;
     this.h = h;
+    this.columnBases = Int32Array.from(columnBases);
     this.handle = new GridHandle(this.context, {
-      layout: 1, nx, ny: fineNy, nz, band, h, maxDelta, columnBases: Int32Array.from(columnBases),
+      layout: 1, nx, ny: fineNy, nz, band, h, maxDelta, columnBases: this.columnBases,
     });
     this.diagnostics = undefined;
   }
@@ -174,6 +175,17 @@ class TallCellPressureProjection {
   submit({ phi, u, v, w, dt, rho }, options = {}) {
     const run = binding.submit(this.handle.use(), { phi, u, v, w, dt, rho }, this.outputs(), nativeOptions(options));
     return this.handle.track(run).then((res) => { this.diagnostics = res.report; return res; });
+  }
+
+  /** Re-lay the grid out around the surface of phi and move phi, u, v, w to
+   * the new layout in place (fp_tall_remesh: chooseTallCellBase and the
+   * neighbour limit of tall-cell-grid.ts:61-106 on the device, the quantity
+   * transfer of the tallCells paper's remeshing). */
+  remesh({ phi, u, v, w }, { liquidHalo = 0, airHalo = 0 } = {}) {
+    if (this.handle.pending > 0) throw new Error('remesh: a projection of this grid is in flight');
+    const res = binding.remeshTallGrid(this.handle.use(), { phi, u, v, w }, undefined, { liquidHalo, airHalo });
+    this.columnBases = res.columnBases;
+    return res;
   }
 
   destroy() { this.handle.destroy(); }

@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = [
     "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_project_noalloc", "fp_submit", "fp_poll",
     "fp_probe_apply",
     "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
+    "fp_tall_remesh",
 ]
 PENDING = 1
 
@@ -69,6 +70,13 @@ class Inputs(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("sweep_tile_z", C.c_int32), ("sweep_groups", C.c_int32), ("sweep_pairs", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+class TallRemeshArgs(C.Structure):
+    _fields_ = [("memory", C.c_int32), ("liquid_halo", C.c_int32), ("air_halo", C.c_int32),
+                ("reserved", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("w", C.c_void_p), ("phi_out", C.c_void_p), ("u_out", C.c_void_p), ("v_out", C.c_void_p),
+                ("w_out", C.c_void_p), ("column_bases_out", C.c_void_p)]
 
 
 class Outputs(C.Structure):
@@ -148,6 +156,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_probe_level_phi.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
     L.fp_probe_tall_layout.argtypes = [vp, vp, vp, vp]
     L.fp_probe_taps.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
+    L.fp_tall_remesh.argtypes = [vp, C.POINTER(TallRemeshArgs), C.POINTER(C.c_int64)]
     for name in EXPORTED_SYMBOLS:
         if name not in ("fp_last_error", "fp_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -464,6 +473,42 @@ class TallCellPressureProjection(UniformPressureProjection):
 
     def probe_level_phi(self, *a, **k):
         raise FluidProjectionError(-1, "the level-set probe covers uniform grids")
+
+    def remesh(self, phi, u, v, w, liquid_halo: int = 0, air_halo: int = 0, out=None):
+        """Re-lay the grid out around the surface of phi and move phi and the
+        slot velocities to the new layout on the device (fp_tall_remesh).
+        Returns (phi, u, v, w, changed_columns) in the new layout -- host
+        arrays for host inputs, CUDA tensors (written into `out` when given;
+        it may be the inputs themselves) for CUDA tensors -- and updates
+        self.bases.  Halos 0 take tallCellSettings.high (16 liquid, 8 air)."""
+        parts = [_ptr(a, np.float32) for a in (phi, u, v, w)]
+        kinds = {q[1] for q in parts}
+        if len(kinds) != 1:
+            raise TypeError("all arrays must live in the same memory (host or device)")
+        mem = kinds.pop()
+        shapes = self.shapes()
+        for arr, shp in zip((phi, u, v, w), shapes):
+            if tuple(arr.shape) != shp:
+                raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
+        if mem == MEMORY_HOST:
+            res = [np.zeros(shp, np.float32) for shp in shapes]
+            optr = [r.ctypes.data for r in res]
+        else:
+            import torch
+            res = list(out) if out is not None else [torch.empty(shp, device=phi.device) for shp in shapes]
+            for r, shp in zip(res, shapes):
+                if tuple(r.shape) != shp or r.dtype != torch.float32 or not r.is_contiguous():
+                    raise ValueError("out must hold contiguous float32 tensors of the input shapes")
+            optr = [r.data_ptr() for r in res]
+            # the library's stream is ordered after the caller's work by a device sync
+            torch.cuda.current_stream(phi.device).synchronize()
+        bases = np.zeros(self.bases.shape, np.int32)
+        args = TallRemeshArgs(mem, int(liquid_halo), int(air_halo), 0, parts[0][0], parts[1][0], parts[2][0],
+                              parts[3][0], *optr, bases.ctypes.data)
+        changed = C.c_int64(0)
+        _check(_lib.fp_tall_remesh(self._g, C.byref(args), C.byref(changed)))
+        self.bases = bases
+        return (*res, int(changed.value))
 
     def probe_layout(self, phi):
         """(fine row, unknown flag) of every sample, computed on the device."""

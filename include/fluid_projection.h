@@ -284,6 +284,51 @@ int fp_project_noalloc(fp_grid* grid, const fp_inputs* in, fp_outputs* out, cons
 int fp_submit(fp_grid* grid, const fp_inputs* in, fp_outputs* out, const fp_options* options, uint64_t* ticket);
 int fp_poll(fp_ctx* ctx, uint64_t ticket, int32_t timeout_ms, fp_report* report, double* latency_ms);
 
+/* ---- tall-cell remeshing ---- */
+/* Re-lays a tall-cell grid out around the surface of phi and moves phi and the
+ * velocity face slots to the new layout, on the device.  Per column: the
+ * lowest and highest fine rows where liquid meets air vertically (phi read
+ * linearly inside the tall cell, from the band samples above it, 5 h beyond
+ * the band; isInitialSurfaceCell, lib/core/tall-cell-grid.ts:263-279), the base
+ * chooseTallCellBase picks for them with halos min(halo, band)
+ * (tall-cell-grid.ts:61-82,327; a dry column takes the highest base,
+ * :378-386), then base <= min(4-neighbour base) + max_delta to convergence
+ * (limitNeighboringTallCellBases, :84-106, run for nx + nz passes at :395).
+ * Quantities move as docs/papers/tallCells.txt:240-252 states: a band sample
+ * copies the old value at its fine row (linear inside the old tall cell), a
+ * tall cell's bottom and top samples are the least-squares line through the
+ * old values of its rows; v slot 0 is the mean of the old faces inside the new
+ * tall cell; u/w slots 0 and 1 are the least-squares affine fit over the rows
+ * both columns' new tall cells share, slot 2+j the old value at row m+j up to
+ * the higher band top (0 above).  Columns (faces) whose base(s) did not change
+ * are copied unchanged.  FP64 arithmetic without contraction.
+ * The grid keeps the new bases (fp_project then uses them); its resident
+ * warm-start pressure is cleared.  Outputs may alias the inputs; a NULL
+ * output is not written.  Device arrays must be complete when the call is
+ * made (it orders itself after no other stream); it returns when the outputs
+ * are written.  The first call allocates staging for one copy of
+ * the inputs and one of the outputs (FP_ERR_BUDGET if the context's budget
+ * cannot hold it).  Grids with fine_ny - band < 2 have no restricted layout
+ * (the reference then keeps an ordinary grid): FP_ERR_LAYOUT.
+ * No reference entry point: the reference builds a layout once per scene
+ * (createTallCellLayout, tall-cell-grid.ts:309-436). */
+typedef struct {
+  int32_t memory;       /* FP_MEMORY_HOST or FP_MEMORY_DEVICE for the arrays below */
+  int32_t liquid_halo;  /* G_L, cells of liquid kept below the surface (0: 16, tallCellSettings.high) */
+  int32_t air_halo;     /* G_A, cells of air kept above it (0: 8) */
+  int32_t reserved;
+  const float* phi;     /* (nx, B+2, nz) in the grid's current layout */
+  const float* u;       /* (nx+1, S, nz) */
+  const float* v;       /* (nx, B+2, nz) */
+  const float* w;       /* (nx, S, nz+1) */
+  float* phi_out;       /* the same shapes, in the new layout */
+  float* u_out;
+  float* v_out;
+  float* w_out;
+  int32_t* column_bases_out; /* (nx, nz) host array of the new bases, or NULL */
+} fp_tall_remesh_args;
+int fp_tall_remesh(fp_grid* grid, const fp_tall_remesh_args* args, int64_t* changed_columns);
+
 /* ---- parity probes (device work on host arrays; used by the test suite) ---- */
 /* out = A p (FP64) on the finest level, h^2-scaled. */
 int fp_probe_apply(fp_grid* grid, const float* phi, const double* p, double* out);

@@ -3,6 +3,7 @@ import json
 import os
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -49,7 +50,8 @@ def test_method_satisfies_the_simulation_method_contract(fpl):
                                  "pressureSolver": "mgpcg"}
     assert out["createSolverAsync"] == "function"
     assert out["binding"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
-                                     "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion"])
+                                     "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion",
+                                     "remeshTallGrid"])
     # the reference's default lattice sizing: 1.2 x 0.8 x 0.8 m at 2.5 cm
     assert out["lattice"]["nx"] == 48 and out["lattice"]["ny"] == 32 and out["lattice"]["nz"] == 32
     assert out["openTop"] == "RangeError"
@@ -128,3 +130,29 @@ def test_addon_projection_matches_c_abi(fpl, tmp_path):
     v = np.fromfile(tmp_path / "v.out.f32", np.float32).reshape(s.v.shape)
     assert np.abs(p - ref.pressure).max() <= 1e-6 * np.abs(ref.pressure).max()
     assert np.abs(v - ref.v).max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_addon_tall_remesh_matches_oracle(fpl, tmp_path):
+    """TallCellPressureProjection.remesh through the addon: the re-laid
+    fields and bases equal the CPU restatement bit for bit; refused while a
+    projection is in flight; a second remesh changes nothing; the grid then
+    projects."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import tall_remesh as orm
+    from remesh_cases import moved_scene
+    lay, phi, u, v, w = moved_scene(seed=11, drop=20)
+    lay.bases.astype(np.int32).tofile(tmp_path / "bases.i32")
+    for name, a in (("phi", phi), ("u", u), ("v", v), ("w", w)):
+        a.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "remesh_files.js"), str(tmp_path), str(lay.nx),
+                        str(lay.fine_ny), str(lay.nz), str(lay.band), repr(lay.h), str(lay.max_delta)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    want = orm.remesh(lay.bases, phi, u, v, w, lay.fine_ny, lay.band, lay.max_delta, lay.h)
+    assert out == {"refusedInFlight": True, "changed": want[5], "again": 0, "flags": 0, "basesKept": True}
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "bases.out.i32", np.int32).reshape(lay.bases.shape), want[0])
+    for name, a in zip(("phi", "u", "v", "w"), want[1:5]):
+        got = np.fromfile(tmp_path / f"{name}.out.f32", np.float32).reshape(a.shape)
+        assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name
