@@ -1,5 +1,6 @@
 // fp_api.hip -- the C ABI (include/fluid_projection.h) over the HIP solvers.
 #include "fp_cm11a.hpp"
+#include "fp_extrap.hpp"
 #include "fp_tall.hpp"
 #include "fp_uniform.hpp"
 
@@ -43,6 +44,7 @@ struct fp_grid {
   fp::UniformGrid* uniform = nullptr;  // FP_LAYOUT_UNIFORM
   fp::TallGrid* tall = nullptr;        // FP_LAYOUT_TALL_CELL
   fp::Cm11a* cm11a = nullptr;          // the CM11a solver's levels, from its first projection
+  fp::Extrapolator* extrap = nullptr;  // velocity extrapolation state, from its first call
   uint64_t bytes = 0;
 };
 
@@ -250,7 +252,9 @@ int fp_grid_destroy(fp_grid* g) {
   if (g->uniform) g->ctx->proj->forget(g->uniform->id());
   if (g->tall) g->ctx->tall->forget(g->tall->id());
   if (g->cm11a) g->ctx->in_use -= g->cm11a->bytes();
+  if (g->extrap) g->ctx->in_use -= g->extrap->bytes();
   delete g->cm11a;
+  delete g->extrap;
   delete g->uniform;
   delete g->tall;
   delete g;
@@ -639,6 +643,40 @@ int fp_probe_taps(fp_grid* g, int32_t level, int32_t axis, int32_t* prolong_coar
   (void)hipSetDevice(g->ctx->device);
   return fp::probe_taps(g->ctx->stream, g->uniform, level, axis, prolong_coarse, prolong_weight, restrict_fine,
                         restrict_weight);
+}
+
+int fp_extrapolate_velocity(fp_grid* g, const fp_extrapolate_args* a, fp_extrapolate_report* rep) {
+  CHECK_ARG(g && a, "fp_extrapolate_velocity: NULL argument");
+  CHECK_ARG(g->uniform != nullptr, "fp_extrapolate_velocity: uniform grids only");
+  CHECK_ARG(a->memory == FP_MEMORY_HOST || a->memory == FP_MEMORY_DEVICE,
+            "fp_extrapolate_velocity: memory must be 0 or 1");
+  CHECK_ARG(a->phi || a->density, "fp_extrapolate_velocity: phi or density is required");
+  CHECK_ARG(a->u && a->v && a->w && a->u_out && a->v_out && a->w_out,
+            "fp_extrapolate_velocity: u, v, w and their outputs are required");
+  CHECK_ARG(a->max_passes >= 0 && a->max_passes <= 16, "fp_extrapolate_velocity: max_passes must be in [0,16]");
+  CHECK_IDLE(g, "fp_extrapolate_velocity");
+  fp_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  if (!g->extrap) {
+    const fp_grid_desc& d = g->desc;
+    const uint64_t need = fp::Extrapolator::estimate_bytes(d.nx, d.ny, d.nz);
+    if (ctx->budget && ctx->in_use + need > ctx->budget) {
+      set_error("fp_extrapolate_velocity: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)",
+                (unsigned long long)need, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
+      return FP_ERR_BUDGET;
+    }
+    fp::Extrapolator* e = new (std::nothrow) fp::Extrapolator();
+    if (!e) return FP_ERR_OUT_OF_MEMORY;
+    int rc = e->create(d.nx, d.ny, d.nz, d.h);
+    if (rc) {
+      delete e;
+      return rc;
+    }
+    g->extrap = e;
+    ctx->in_use += e->bytes();
+  }
+  return g->extrap->run(ctx->stream, a, rep);
 }
 
 int fp_tall_remesh(fp_grid* g, const fp_tall_remesh_args* a, int64_t* changed_columns) {

@@ -50,7 +50,7 @@ EXPORTED_SYMBOLS = [
     "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_project_noalloc", "fp_submit", "fp_poll",
     "fp_probe_apply",
     "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
-    "fp_tall_remesh",
+    "fp_tall_remesh", "fp_extrapolate_velocity",
 ]
 PENDING = 1
 
@@ -77,6 +77,20 @@ class TallRemeshArgs(C.Structure):
                 ("reserved", C.c_int32), ("phi", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
                 ("w", C.c_void_p), ("phi_out", C.c_void_p), ("u_out", C.c_void_p), ("v_out", C.c_void_p),
                 ("w_out", C.c_void_p), ("column_bases_out", C.c_void_p)]
+
+
+class ExtrapolateArgs(C.Structure):
+    _fields_ = [("memory", C.c_int32), ("open_top", C.c_int32), ("max_passes", C.c_int32), ("reserved", C.c_int32),
+                ("phi", C.c_void_p), ("density", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("w", C.c_void_p), ("u_out", C.c_void_p), ("v_out", C.c_void_p), ("w_out", C.c_void_p)]
+
+
+class ExtrapolateReport(C.Structure):
+    _fields_ = [("passes", C.c_int32), ("active_left", C.c_int32), ("levels", C.c_int32), ("reserved", C.c_int32),
+                ("band_faces", C.c_int64), ("unknown_faces", C.c_int64), ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
 class Outputs(C.Structure):
@@ -157,6 +171,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_probe_tall_layout.argtypes = [vp, vp, vp, vp]
     L.fp_probe_taps.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
     L.fp_tall_remesh.argtypes = [vp, C.POINTER(TallRemeshArgs), C.POINTER(C.c_int64)]
+    L.fp_extrapolate_velocity.argtypes = [vp, C.POINTER(ExtrapolateArgs), C.POINTER(ExtrapolateReport)]
     for name in EXPORTED_SYMBOLS:
         if name not in ("fp_last_error", "fp_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -399,6 +414,42 @@ class UniformPressureProjection:
         phi = np.ascontiguousarray(phi, np.float32)
         _check(_lib.fp_probe_level_phi(self._g, phi.ctypes.data, level, preserve, out.ctypes.data))
         return out
+
+    def extrapolate(self, u, v, w, phi=None, density=None, open_top: bool = False, max_passes: int = 0, out=None):
+        """Velocity extrapolation into the air (fp_extrapolate_velocity): the
+        narrow band of Jeong-Ross-Whitaker passes, then the hierarchy fill.
+        Returns (u, v, w, report dict): host arrays for host inputs, CUDA
+        tensors (written into `out` when given; it may be the inputs) for CUDA
+        tensors."""
+        if (phi is None) == (density is None):
+            raise ValueError("give exactly one of phi and density")
+        field = phi if phi is not None else density
+        parts = [_ptr(a, np.float32) for a in (field, u, v, w)]
+        kinds = {q[1] for q in parts}
+        if len(kinds) != 1:
+            raise TypeError("all arrays must live in the same memory (host or device)")
+        mem = kinds.pop()
+        shapes = self.shapes()
+        for arr, shp in zip((field, u, v, w), shapes):
+            if tuple(arr.shape) != shp:
+                raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
+        if mem == MEMORY_HOST:
+            res = [np.zeros(shp, np.float32) for shp in shapes[1:]]
+            optr = [r.ctypes.data for r in res]
+        else:
+            import torch
+            res = list(out) if out is not None else [torch.empty(shp, device=field.device) for shp in shapes[1:]]
+            for r, shp in zip(res, shapes[1:]):
+                if tuple(r.shape) != shp or r.dtype != torch.float32 or not r.is_contiguous():
+                    raise ValueError("out must hold contiguous float32 tensors of the face shapes")
+            optr = [r.data_ptr() for r in res]
+            torch.cuda.current_stream(field.device).synchronize()
+        args = ExtrapolateArgs(mem, int(bool(open_top)), int(max_passes), 0,
+                               parts[0][0] if phi is not None else None, parts[0][0] if phi is None else None,
+                               parts[1][0], parts[2][0], parts[3][0], *optr)
+        rep = ExtrapolateReport()
+        _check(_lib.fp_extrapolate_velocity(self._g, C.byref(args), C.byref(rep)))
+        return (*res, rep.as_dict())
 
     def set_tuning(self, tile_z: int = 0, groups: int = 0, pairs: int = 0):
         """Sweep tile width along z (16 or 32) and workgroups per sweep of the

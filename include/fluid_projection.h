@@ -329,6 +329,53 @@ typedef struct {
 } fp_tall_remesh_args;
 int fp_tall_remesh(fp_grid* grid, const fp_tall_remesh_args* args, int64_t* changed_columns);
 
+/* ---- velocity extrapolation (uniform grids) ---- */
+/* Extends every velocity component from the faces next to liquid into the
+ * air, the stage after the projection in the reference's uniform solver
+ * (WebGPUUniformVelocityExtrapolator.encode,
+ * lib/methods/uniform/webgpu-uniform-velocity-extrapolation.ts:312-392, kernels
+ * in …extrapolation.wgsl.ts).  Each component lives on its lattice of
+ * positive faces (face u(i+1,j,k) of cell (i,j,k)); a face is open when it is
+ * interior or, with open_top, the top face (faceOpenFraction,
+ * webgpu-uniform-reference.wgsl.ts:511-519); a source face is open with liquid
+ * on either side (liquid iff phi < 0, or density > 1/2) and keeps its input
+ * value.  Jeong-Ross-Whitaker passes of the Godunov distance clipped to two
+ * cells, each updated face taking the first-order upwind extension of CM11b
+ * Eq. 7 (at most min(max(nx,ny,nz),16) passes, or max_passes), then the
+ * hierarchy fill of CM11b Sec. 3.3.1 (levels ceil-halved until the smallest
+ * extent is 1; renormalised trilinear restriction of known values, the
+ * vertical component's corresponding-cell fallback, prolongation of the
+ * unknown).  Outputs: known values on open faces, 0 on walls and on open
+ * faces the hierarchy leaves unknown (packTransportShell, :491-512).  FP32.
+ * Outputs may alias the inputs.  The first call on a grid allocates the
+ * extrapolation state (about 57 bytes per cell; FP_ERR_BUDGET beyond the
+ * context's budget).  Device arrays must be complete when the call is made;
+ * it returns when the outputs are written. */
+typedef struct {
+  int32_t memory;       /* FP_MEMORY_HOST or FP_MEMORY_DEVICE for the arrays below */
+  int32_t open_top;     /* 1: the top (+y) faces are open to the atmosphere */
+  int32_t max_passes;   /* band passes (0: min(max(nx,ny,nz), 16), the reference's ceiling) */
+  int32_t reserved;
+  const float* phi;     /* (nx,ny,nz) level set, or NULL when density is given */
+  const float* density; /* (nx,ny,nz) surface density (liquid iff > 1/2), used when phi is NULL */
+  const float* u;       /* (nx+1,ny,nz) */
+  const float* v;       /* (nx,ny+1,nz) */
+  const float* w;       /* (nx,ny,nz+1) */
+  float* u_out;
+  float* v_out;
+  float* w_out;
+} fp_extrapolate_args;
+typedef struct {
+  int32_t passes;        /* band passes that ran */
+  int32_t active_left;   /* band faces still active after the last pass (0: the front converged) */
+  int32_t levels;        /* hierarchy levels */
+  int32_t reserved;
+  int64_t band_faces;    /* faces known after the band (sources included) */
+  int64_t unknown_faces; /* open faces left unknown (written 0) */
+  double device_ms;      /* host wall time of the call */
+} fp_extrapolate_report;
+int fp_extrapolate_velocity(fp_grid* grid, const fp_extrapolate_args* args, fp_extrapolate_report* report);
+
 /* ---- parity probes (device work on host arrays; used by the test suite) ---- */
 /* out = A p (FP64) on the finest level, h^2-scaled. */
 int fp_probe_apply(fp_grid* grid, const float* phi, const double* p, double* out);

@@ -194,6 +194,23 @@ int orc_cm11a_project(int nx, int ny, int nz, double h, const float* phi, const 
 int orc_cm11a_system(int nx, int ny, int nz, double h, const float* phi, const float* u, const float* v,
                      const float* w, double dt, double rho, int open_top, float* coef, float* b, float* pmin);
 
+/* ---- uniform velocity extrapolation (fp_oracle_extrap.c) ---- */
+typedef struct {
+  int passes;          /* FIM passes executed */
+  int active_left;     /* active faces after the last pass (0: the front converged) */
+  int levels;          /* hierarchy levels */
+  int reserved;
+  int64_t band_known;  /* faces known after the narrow band, sources included */
+  int64_t unknown;     /* open faces left unknown */
+} orc_extrap_report;
+/* phi (or density when phi is NULL) per cell; u, v, w face arrays; outputs of
+ * the same shapes.  band_dist / band_val (3, nx, ny, nz) and band_known (nx, ny,
+ * nz; bit c) receive the narrow-band state when not NULL.  max_passes 0: the
+ * reference's min(max(dims), 16). */
+int orc_extrapolate(int nx, int ny, int nz, double h, int open_top, int max_passes, const float* phi,
+                    const float* density, const float* u, const float* v, const float* w, float* uo, float* vo,
+                    float* wo, float* band_dist, float* band_val, uint8_t* band_known, orc_extrap_report* rep);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,11 @@ class TallLayout(C.Structure):
                 ("max_delta", C.c_int32), ("bases", C.c_void_p), ("phi", C.c_void_p), ("h", C.c_double)]
 
 
+class ExtrapReport(C.Structure):
+    _fields_ = [("passes", C.c_int32), ("active_left", C.c_int32), ("levels", C.c_int32), ("reserved", C.c_int32),
+                ("band_known", C.c_int64), ("unknown", C.c_int64)]
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (gcc) into oracle/_build/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -130,6 +135,10 @@ def lib():
                                         _fp, _fp, _fp, _fp, C.POINTER(CM11aReport)]
         L.orc_cm11a_system.restype = i
         L.orc_cm11a_system.argtypes = [i, i, i, d, _fp, _fp, _fp, _fp, d, d, i, _fp, _fp, _fp]
+        vp = C.c_void_p
+        L.orc_extrapolate.restype = i
+        L.orc_extrapolate.argtypes = [i, i, i, d, i, i, vp, vp, _fp, _fp, _fp, _fp, _fp, _fp, vp, vp, vp,
+                                      C.POINTER(ExtrapReport)]
         _lib = L
     return _lib
 
@@ -480,3 +489,31 @@ def cm11a_system(phi, u, v, w, h, dt, rho, open_top=0):
     pmin = np.zeros(D, np.float32)
     lib().orc_cm11a_system(nx, ny, nz, h, phi, u, v, w, dt, rho, open_top, coef, b, pmin)
     return coef, b, pmin
+
+
+def extrapolate(u, v, w, h, phi=None, density=None, open_top=0, max_passes=0, band=False):
+    """Velocity extrapolation of the uniform solver (fp_oracle_extrap.c, FP32):
+    returns (u, v, w, report dict) and, with band=True, the narrow-band state
+    (dist (3, nx, ny, nz), value (3, nx, ny, nz), known bits (nx, ny, nz))."""
+    field = phi if phi is not None else density
+    nx, ny, nz = field.shape
+    f = _f32(field)
+    u, v, w = _f32(u), _f32(v), _f32(w)
+    uo, vo, wo = np.zeros_like(u), np.zeros_like(v), np.zeros_like(w)
+    bd = bv = bk = None
+    if band:
+        bd = np.zeros((3, nx, ny, nz), np.float32)
+        bv = np.zeros((3, nx, ny, nz), np.float32)
+        bk = np.zeros((nx, ny, nz), np.uint8)
+    rep = ExtrapReport()
+    rc = lib().orc_extrapolate(nx, ny, nz, h, int(open_top), int(max_passes),
+                               f.ctypes.data if phi is not None else None,
+                               f.ctypes.data if phi is None else None, u, v, w, uo, vo, wo,
+                               bd.ctypes.data if band else None, bv.ctypes.data if band else None,
+                               bk.ctypes.data if band else None, C.byref(rep))
+    if rc != 0:
+        raise RuntimeError(f"orc_extrapolate failed ({rc})")
+    r = {k: getattr(rep, k) for k, _ in ExtrapReport._fields_ if k != "reserved"}
+    if band:
+        return uo, vo, wo, r, (bd, bv, bk)
+    return uo, vo, wo, r
