@@ -571,6 +571,46 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def bench_extrapolation(g, s, phi, out, args, calls: int = 5):
+    """fp_extrapolate_velocity of the projected velocities of the headline grid
+    (device tensors, out of place), after the timed projections: host wall
+    time per call (the call returns when its outputs are written).  The
+    algorithmic bytes are one read of phi, u, v, w and one write of u, v, w
+    (28 B per cell).  CPU baseline: the oracle's scalar C restatement on the
+    same generator at 128^3."""
+    import torch
+    src = [a.clone() for a in out[1:]]
+    res = [torch.empty_like(a) for a in src]
+    rep = g.extrapolate(*src, phi=phi, out=res)[3]  # allocates the extrapolation state
+    ms = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rep = g.extrapolate(*src, phi=phi, out=res)[3]
+        ms.append((time.perf_counter() - t1) * 1e3)
+    med = statistics.median(ms)
+    cells = s.cells
+    algo = 28 * cells
+    line = {"calls": calls, "median_ms": round(med, 3), "min_ms": round(min(ms), 3),
+            "mcells_per_s": round(cells / med / 1e3, 1), "passes": rep["passes"],
+            "active_left": rep["active_left"], "levels": rep["levels"], "band_faces": rep["band_faces"],
+            "unknown_faces": rep["unknown_faces"], "algorithmic_bytes": algo,
+            "achieved_gbs": round(algo / (med * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(algo / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if not args.no_cpu_baseline:
+        from fluid_amd import scenes
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        c = scenes.dam_spheres_scene(128, seed=0)
+        t1 = time.perf_counter()
+        oracle.extrapolate(c.u, c.v, c.w, c.h, phi=c.phi)
+        dt = time.perf_counter() - t1
+        line["cpu_baseline"] = {"value": round(c.cells / dt / 1e6, 2), "unit": "Mcells/s", "cores": 1,
+                                "kind": "port", "sample": f"oracle/fp_oracle_extrap.c (scalar C) on the dam + 8 "
+                                f"spheres generator at 128^3, one extrapolation in {dt:.2f} s"}
+    return line
+
+
 def bench_uniform(args):
     """The headline: one projection of the 384^3 dam + 8 spheres grid per
     step, FP16 multigrid, each CG + V-cycle iteration pair replayed as a
@@ -640,6 +680,7 @@ def bench_uniform(args):
         plain = {"ms": round(ms32, 3), "mcells_per_s": round(cells / ms32 / 1e3, 2), "steps": args.sub_steps,
                  "iterations": r32.report["iterations"], "relative_residual": r32.report["relative_residual"],
                  "p_fp32": out[0].cpu().numpy()}
+    extrapolation = bench_extrapolation(g, s, phi, out, args)
     g.destroy()
     bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps,
                                 pairs and args.timed_kernel == "smooth_fine")
@@ -688,6 +729,7 @@ def bench_uniform(args):
                   "max_div_before": rep["max_div_before"], "device_ms": round(rep["device_ms"], 3)},
         "fp64_oracle_error": err,
         "plain_fp32_mgpcg": plain or None,
+        "extrapolation": extrapolation,
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps, pairs)),

@@ -30,35 +30,59 @@ namespace {
 constexpr float kInf = 65504.0f;  // DISTANCE_INFINITY (…extrapolation.wgsl.ts:61)
 constexpr int kThreads = 256;
 
+// Division by a launch-invariant divisor: floor(n / d) = (n * m) >> s for
+// every n < 2^31 with s = 31 + ceil(log2 d), m = ceil(2^s / d) < 2^32
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, Thm 4.2).  Cell indices stay below 2^31.
+struct FastDiv {
+  unsigned m;
+  int s;
+};
+FastDiv make_div(unsigned d) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  const unsigned long long m = ((1ull << (31 + l)) + d - 1) / d;
+  return FastDiv{(unsigned)m, 31 + l};
+}
+__device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) {
+  return (unsigned)(((unsigned long long)n * f.m) >> f.s);
+}
+
 struct ExGeo {
   int d[3];
   float h;
   int open_top;
   float eps_min, eps_max;  // fimConvergenceEpsilon's clamp (…wgsl.ts:114-119)
+  FastDiv dz, dy;          // division by nz and by ny
 };
 
-__device__ __forceinline__ long long eidx(const ExGeo& g, int x, int y, int z) {
-  return ((long long)x * g.d[1] + y) * g.d[2] + z;
+__device__ __forceinline__ unsigned eidx(const ExGeo& g, int x, int y, int z) {
+  return ((unsigned)x * g.d[1] + y) * g.d[2] + z;
 }
 __device__ __forceinline__ bool ein(const int* d, int x, int y, int z) {
   return x >= 0 && y >= 0 && z >= 0 && x < d[0] && y < d[1] && z < d[2];
 }
-__device__ __forceinline__ void edecode(const ExGeo& g, long long i, int& x, int& y, int& z) {
-  z = (int)(i % g.d[2]);
-  const long long t = i / g.d[2];
-  y = (int)(t % g.d[1]);
-  x = (int)(t / g.d[1]);
+// cell index -> (x, y, z) of a lattice with divisors dz = nz, dy = ny
+__device__ __forceinline__ void decode3(unsigned i, int ny, int nz, FastDiv dz, FastDiv dy, int& x, int& y, int& z) {
+  const unsigned t = fdiv(i, dz);
+  z = (int)(i - t * (unsigned)nz);
+  const unsigned xx = fdiv(t, dy);
+  y = (int)(t - xx * (unsigned)ny);
+  x = (int)xx;
+}
+__device__ __forceinline__ void edecode(const ExGeo& g, unsigned i, int& x, int& y, int& z) {
+  decode3(i, g.d[1], g.d[2], g.dz, g.dy, x, y, z);
 }
 // positive face of cell (x,y,z) along c in the (n_c + 1)-long face array
-__device__ __forceinline__ long long face_of(const ExGeo& g, int c, int x, int y, int z) {
-  return c == 0 ? ((long long)(x + 1) * g.d[1] + y) * g.d[2] + z
-       : c == 1 ? ((long long)x * (g.d[1] + 1) + y + 1) * g.d[2] + z
-                : ((long long)x * g.d[1] + y) * (g.d[2] + 1) + z + 1;
+__device__ __forceinline__ unsigned face_of(const ExGeo& g, int c, int x, int y, int z) {
+  return c == 0 ? ((unsigned)(x + 1) * g.d[1] + y) * g.d[2] + z
+       : c == 1 ? ((unsigned)x * (g.d[1] + 1) + y + 1) * g.d[2] + z
+                : ((unsigned)x * g.d[1] + y) * (g.d[2] + 1) + z + 1;
 }
 // XCD-contiguous chunk of a streaming launch: workgroup b runs on XCD b mod 8
-__device__ __forceinline__ long long chunk_of(long long chunks) {
-  const long long per = (chunks + 7) / 8;
-  return (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+__device__ __forceinline__ unsigned chunk_of(unsigned chunks) {
+  const unsigned per = (chunks + 7) / 8;
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 __device__ __forceinline__ float eps_of(const ExGeo& g, float reference) {
   float a = fabsf(reference);
@@ -80,59 +104,130 @@ __device__ __forceinline__ int block_sum(int v) {
 // ---- classes: open (bits 0-2) and source (bits 3-5) per component ----
 // faceOpenFraction (webgpu-uniform-reference.wgsl.ts:511-519) without solids,
 // sourceFace (…extrapolation.wgsl.ts:100-104)
-__device__ __forceinline__ bool liquid_at(const ExGeo& g, const float* phi, const float* dens, int x, int y, int z) {
-  if (!ein(g.d, x, y, z)) return false;
-  const long long i = eidx(g, x, y, z);
-  return phi ? phi[i] < 0.0f : dens[i] > 0.5f;
+// Streaming kernels take kCells cells per thread (kCells x kThreads per
+// workgroup, the cells of one pass kThreads apart) and issue every load of
+// their cells before the first store: neighbour loads are clamped to the cell
+// itself at the domain edge and masked afterwards, so no branch separates
+// them.  One cell per thread, or loads behind bounds branches, left these
+// kernels waiting on one memory round trip at a time.
+constexpr int kCells = 4;
+
+__device__ __forceinline__ unsigned cell_of(unsigned n, int j) {
+  const unsigned per_group = kThreads * kCells;
+  return chunk_of((n + per_group - 1) / per_group) * per_group + j * kThreads + threadIdx.x;
 }
 
-__global__ void k_ex_classify(ExGeo g, const float* __restrict__ phi, const float* __restrict__ dens,
-                              uint8_t* __restrict__ cls, uint8_t* __restrict__ cand, long long n) {
-  const long long i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
-  if (i >= n) return;
-  int x, y, z;
-  edecode(g, i, x, y, z);
-  const bool here = liquid_at(g, phi, dens, x, y, z);
-  const bool up[3] = {liquid_at(g, phi, dens, x + 1, y, z), liquid_at(g, phi, dens, x, y + 1, z),
-                      liquid_at(g, phi, dens, x, y, z + 1)};
-  const int p[3] = {x, y, z};
-  unsigned b = 0;
-  for (int c = 0; c < 3; ++c) {
-    const bool open = p[c] + 1 < g.d[c] || (c == 1 && g.open_top);
-    if (!open) continue;
-    b |= 1u << c;
-    if (here || up[c]) b |= 8u << c;
+// exclusive prefix of v over the workgroup, and the workgroup total
+__device__ __forceinline__ int block_scan(int v, int* total) {
+  __shared__ int wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  cls[i] = (uint8_t)b;
-  cand[i] = (uint8_t)(b >> 3);
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) {
+    before += k < wid ? wsum[k] : 0;
+    all += wsum[k];
+  }
+  *total = all;
+  return before + x - v;
 }
 
+__global__ __launch_bounds__(kThreads) void k_ex_classify(ExGeo g, const float* __restrict__ field, int is_phi,
+                                                         uint8_t* __restrict__ cls, uint8_t* __restrict__ cand,
+                                                         unsigned n) {
+  const unsigned sx = (unsigned)g.d[1] * g.d[2], sy = (unsigned)g.d[2];
+  float f[kCells][4];
+  int pos[kCells][3];
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = min(cell_of(n, j), n - 1);
+    int x, y, z;
+    edecode(g, i, x, y, z);
+    pos[j][0] = x;
+    pos[j][1] = y;
+    pos[j][2] = z;
+    f[j][0] = field[i];
+    f[j][1] = field[i + (x + 1 < g.d[0] ? sx : 0u)];
+    f[j][2] = field[i + (y + 1 < g.d[1] ? sy : 0u)];
+    f[j][3] = field[i + (z + 1 < g.d[2] ? 1u : 0u)];
+  }
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = cell_of(n, j);
+    if (i >= n) break;
+    bool wet[4];
+    for (int k = 0; k < 4; ++k) wet[k] = is_phi ? f[j][k] < 0.0f : f[j][k] > 0.5f;
+    unsigned b = 0;
+    for (int c = 0; c < 3; ++c) {
+      const bool inside = pos[j][c] + 1 < g.d[c];
+      const bool open = inside || (c == 1 && g.open_top);
+      if (!open) continue;
+      b |= 1u << c;
+      if (wet[0] || (inside && wet[1 + c])) b |= 8u << c;
+    }
+    cls[i] = (uint8_t)b;
+    cand[i] = (uint8_t)(b >> 3);
+  }
+}
+
+// bits 0-2 open(c), 3-5 source(c)
 __device__ __forceinline__ unsigned cls_at(const ExGeo& g, const uint8_t* cls, int x, int y, int z) {
   return ein(g.d, x, y, z) ? cls[eidx(g, x, y, z)] : 0u;
 }
 
-// seedActiveFront (…extrapolation.wgsl.ts:148-170); both sides get the seed
-__global__ void k_ex_seed(ExGeo g, const uint8_t* __restrict__ cls, const float* __restrict__ u,
-                          const float* __restrict__ v, const float* __restrict__ w, ExState a, ExState b,
-                          ExCounters* cnt, long long n) {
-  const long long i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
-  int active = 0;
-  if (i < n) {
+// the byte of each 6-neighbour (0 outside the domain), loaded without branches
+__device__ __forceinline__ void load6(const ExGeo& g, const uint8_t* __restrict__ a, unsigned i, int x, int y, int z,
+                                      unsigned* out) {
+  const unsigned sx = (unsigned)g.d[1] * g.d[2], sy = (unsigned)g.d[2];
+  const bool ok[6] = {x > 0, x + 1 < g.d[0], y > 0, y + 1 < g.d[1], z > 0, z + 1 < g.d[2]};
+  const unsigned idx[6] = {ok[0] ? i - sx : i, ok[1] ? i + sx : i, ok[2] ? i - sy : i,
+                           ok[3] ? i + sy : i, ok[4] ? i - 1 : i,  ok[5] ? i + 1 : i};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out[k] = a[idx[k]];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out[k] = ok[k] ? out[k] : 0u;
+}
+
+// seedActiveFront (…extrapolation.wgsl.ts:148-170), written to both sides
+__global__ __launch_bounds__(kThreads) void k_ex_seed(ExGeo g, const uint8_t* __restrict__ cls,
+                                                     const float* __restrict__ u, const float* __restrict__ v,
+                                                     const float* __restrict__ w, ExState a, ExState b,
+                                                     ExCounters* cnt, unsigned n) {
+  unsigned me[kCells], nb[kCells][6];
+  float in[kCells][3];
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = min(cell_of(n, j), n - 1);
     int x, y, z;
     edecode(g, i, x, y, z);
-    const unsigned me = cls[i];
+    me[j] = cls[i];
+    load6(g, cls, i, x, y, z, nb[j]);
+    // the positive face of every cell exists in the face arrays
+    in[j][0] = u[face_of(g, 0, x, y, z)];
+    in[j][1] = v[face_of(g, 1, x, y, z)];
+    in[j][2] = w[face_of(g, 2, x, y, z)];
+  }
+  int active = 0;
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = cell_of(n, j);
+    if (i >= n) break;
     unsigned srcnb = 0;  // per component: a 6-neighbour is a source
-    srcnb |= cls_at(g, cls, x - 1, y, z) | cls_at(g, cls, x + 1, y, z);
-    srcnb |= cls_at(g, cls, x, y - 1, z) | cls_at(g, cls, x, y + 1, z);
-    srcnb |= cls_at(g, cls, x, y, z - 1) | cls_at(g, cls, x, y, z + 1);
+    for (int k = 0; k < 6; ++k) srcnb |= nb[j][k];
     srcnb >>= 3;
-    const float* in[3] = {u, v, w};
     unsigned flags = 0;
     for (int c = 0; c < 3; ++c) {
       float val = 0.0f, dist = kInf;
-      if (me >> c & 1) {
-        if (me >> (3 + c) & 1) {
-          val = in[c][face_of(g, c, x, y, z)];
+      if (me[j] >> c & 1) {
+        if (me[j] >> (3 + c) & 1) {
+          val = in[j][c];
           dist = 0.0f;
           flags |= 1u << c;
         } else if (srcnb >> c & 1) {
@@ -153,53 +248,88 @@ __global__ void k_ex_seed(ExGeo g, const uint8_t* __restrict__ cls, const float*
 }
 
 // one 6-neighbour OR of the candidate bits; the last one appends the cells
-// with an open non-source candidate face to the band list
-__global__ void k_ex_dilate(ExGeo g, const uint8_t* __restrict__ cls, const uint8_t* __restrict__ in,
-                            uint8_t* __restrict__ out, int last, int* __restrict__ list, ExCounters* cnt, long long n) {
-  const long long i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
-  bool take = false;
-  if (i < n) {
+// with an open non-source candidate face to the band list (one atomic per
+// workgroup)
+__global__ __launch_bounds__(kThreads) void k_ex_dilate(ExGeo g, const uint8_t* __restrict__ cls,
+                                                       const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
+                                                       int last, int* __restrict__ list, ExCounters* cnt,
+                                                       unsigned n) {
+  unsigned m[kCells], me[kCells];
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = min(cell_of(n, j), n - 1);
     int x, y, z;
     edecode(g, i, x, y, z);
-    unsigned m = in[i];
-    m |= cls_at(g, in, x - 1, y, z) | cls_at(g, in, x + 1, y, z);
-    m |= cls_at(g, in, x, y - 1, z) | cls_at(g, in, x, y + 1, z);
-    m |= cls_at(g, in, x, y, z - 1) | cls_at(g, in, x, y, z + 1);
-    if (!last) {
-      out[i] = (uint8_t)m;
-    } else {
-      const unsigned me = cls[i];
-      take = (m & (me & 7u) & ~(me >> 3)) != 0;
+    unsigned nb[6];
+    load6(g, in, i, x, y, z, nb);
+    m[j] = in[i] | nb[0] | nb[1] | nb[2] | nb[3] | nb[4] | nb[5];
+    me[j] = last ? cls[i] : 0u;
+  }
+  if (!last) {
+#pragma unroll
+    for (int j = 0; j < kCells; ++j) {
+      const unsigned i = cell_of(n, j);
+      if (i < n) out[i] = (uint8_t)m[j];
     }
+    return;
   }
-  if (!last) return;
-  // wave-aggregated append
-  const unsigned long long ballot = __ballot(take);
-  if (!ballot) return;
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (lane == __ffsll((long long)ballot) - 1) base = atomicAdd(&cnt->list_count, __popcll(ballot));
-  base = __shfl(base, __ffsll((long long)ballot) - 1, 64);
-  if (take) {
-    const int below = __popcll(ballot & ((1ull << lane) - 1ull));
-    list[base + below] = (int)i;
-  }
+  unsigned takes = 0;
+#pragma unroll
+  for (int j = 0; j < kCells; ++j)
+    if (cell_of(n, j) < n && (m[j] & (me[j] & 7u) & ~(me[j] >> 3)) != 0) takes |= 1u << j;
+  int total;
+  int at = block_scan(__popc(takes), &total);
+  __shared__ int base;
+  if (threadIdx.x == 0) base = total ? atomicAdd(&cnt->list_count, total) : 0;
+  __syncthreads();
+  at += base;
+#pragma unroll
+  for (int j = 0; j < kCells; ++j)
+    if (takes >> j & 1) list[at++] = (int)cell_of(n, j);
 }
 
 // ---- the band pass (updateActiveFront and helpers, …extrapolation.wgsl.ts:172-331) ----
+// Neighbour state during a pass (the pass's source side).  Every accessor
+// loads at an index clamped into the domain and selects afterwards, so the
+// loads of a stencil are issued together instead of one branch at a time.
 struct Nb {
   const ExGeo& g;
   const uint8_t* cls;
   const ExState& s;
-  __device__ bool open(int x, int y, int z, int c) const { return cls_at(g, cls, x, y, z) >> c & 1; }
-  __device__ bool source(int x, int y, int z, int c) const { return cls_at(g, cls, x, y, z) >> (3 + c) & 1; }
+  __device__ unsigned at(int x, int y, int z, bool* inside) const {
+    *inside = ein(g.d, x, y, z);
+    return eidx(g, min(max(x, 0), g.d[0] - 1), min(max(y, 0), g.d[1] - 1), min(max(z, 0), g.d[2] - 1));
+  }
+  __device__ bool open(int x, int y, int z, int c) const {
+    bool in;
+    const unsigned i = at(x, y, z, &in);
+    return in && (cls[i] >> c & 1);
+  }
+  __device__ bool source(int x, int y, int z, int c) const {
+    bool in;
+    const unsigned i = at(x, y, z, &in);
+    return in && (cls[i] >> (3 + c) & 1);
+  }
   __device__ float dist(int x, int y, int z, int c) const {
-    return open(x, y, z, c) ? s.dist[c][eidx(g, x, y, z)] : kInf;
+    bool in;
+    const unsigned i = at(x, y, z, &in);
+    const unsigned cl = cls[i];
+    const float d = s.dist[c][i];
+    return in && (cl >> c & 1) ? d : kInf;
   }
   __device__ float value(int x, int y, int z, int c) const {
-    if (!open(x, y, z, c)) return 0.0f;
-    const long long i = eidx(g, x, y, z);
-    return (s.flags[i] >> c & 1) ? s.val[c][i] : 0.0f;
+    bool in;
+    const unsigned i = at(x, y, z, &in);
+    const unsigned cl = cls[i];
+    const unsigned fl = s.flags[i];
+    const float v = s.val[c][i];
+    return in && (cl >> c & 1) && (fl >> c & 1) ? v : 0.0f;
+  }
+  __device__ unsigned flags(int x, int y, int z) const {
+    bool in;
+    const unsigned i = at(x, y, z, &in);
+    const unsigned fl = s.flags[i];
+    return in ? fl : 0u;
   }
 };
 
@@ -279,9 +409,8 @@ __device__ float upwind(const Nb& nb, int x, int y, int z, int c, float solved) 
 // activeNodeConverges (…extrapolation.wgsl.ts:253-265)
 __device__ bool converges(const Nb& nb, int x, int y, int z, int c, float* updated) {
   if (!nb.open(x, y, z, c) || nb.source(x, y, z, c)) return false;
-  const long long i = eidx(nb.g, x, y, z);
-  if (!(nb.s.flags[i] >> (3 + c) & 1)) return false;
-  const float old = nb.s.dist[c][i];
+  if (!(nb.flags(x, y, z) >> (3 + c) & 1)) return false;
+  const float old = nb.dist(x, y, z, c);
   if (old >= 0.5f * kInf) return false;
   const float gd = godunov(nb, x, y, z, c);
   const float upd = old < gd ? old : gd;
@@ -290,7 +419,8 @@ __device__ bool converges(const Nb& nb, int x, int y, int z, int c, float* updat
   return fabsf(upd - old) <= eps_of(nb.g, old);
 }
 
-// activatedByConvergedUpwindNeighbor (…extrapolation.wgsl.ts:266-283)
+// activatedByConvergedUpwindNeighbor (…extrapolation.wgsl.ts:266-283); only
+// called when a neighbour is active, the first condition of activeNodeConverges
 __device__ bool activated(const Nb& nb, int x, int y, int z, int c, float own) {
   const float cand = godunov(nb, x, y, z, c);
   if (cand > 2.0f * nb.g.h) return false;
@@ -313,12 +443,22 @@ __global__ __launch_bounds__(kThreads) void k_ex_pass(ExGeo g, const uint8_t* __
   const int k = blockIdx.x * kThreads + threadIdx.x;
   int active = 0;
   if (k < count) {
-    const long long i = list[k];
+    const unsigned i = (unsigned)list[k];
     int x, y, z;
     edecode(g, i, x, y, z);
     const Nb nb{g, cls, src};
     const unsigned me = cls[i];
     unsigned flags = src.flags[i];
+    // active bits of the 6 neighbours per component: an inactive face with no
+    // active neighbour cannot be activated
+    unsigned nbact = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int side = -1; side <= 1; side += 2) {
+        int qx = x, qy = y, qz = z;
+        step(a, side, qx, qy, qz);
+        nbact |= nb.flags(qx, qy, qz);
+      }
+    nbact >>= 3;
     for (int c = 0; c < 3; ++c) {
       float val = src.val[c][i], dist = src.dist[c][i];
       if ((me >> c & 1) && !(me >> (3 + c) & 1)) {
@@ -335,7 +475,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_pass(ExGeo g, const uint8_t* __
           const bool conv = old < 0.5f * kInf && fabsf(upd - old) <= eps;
           if (!conv && upd <= 2.0f * g.h) flags |= 8u << c;
           else flags &= ~(8u << c);
-        } else if (activated(nb, x, y, z, c, dist)) {
+        } else if ((nbact >> c & 1) && activated(nb, x, y, z, c, dist)) {
           flags |= 8u << c;  // JRW07: a newly added point is updated from the next pass
         }
       }
@@ -380,13 +520,14 @@ __device__ float hier_sample(const ExSrc& S, const int* T, const int* p, int c, 
       wgt[a] = off[a] ? fr[a] : 1.0f - fr[a];
     }
     const float weight = wgt[0] * wgt[1] * wgt[2];
-    const long long i = ((long long)q[0] * S.d[1] + q[1]) * S.d[2] + q[2];
-    cx[corner] = 0.0f;
-    cw[corner] = 0.0f;
-    if (weight > 0.0f && (S.known[i] >> c & 1)) {
-      cx[corner] = weight * S.val[c][i];
-      cw[corner] = weight;
-    }
+    const unsigned i = ((unsigned)q[0] * S.d[1] + q[1]) * S.d[2] + q[2];
+    // load every corner (the clamped index is always valid) and select:
+    // branch-free, so the eight corners' loads are in flight together
+    const unsigned kn = S.known[i];
+    const float val = S.val[c][i];
+    const bool use = weight > 0.0f && (kn >> c & 1);
+    cx[corner] = use ? weight * val : 0.0f;
+    cw[corner] = use ? weight : 0.0f;
   }
   const float xs = ((cx[0] + cx[5]) + (cx[1] + cx[4])) + ((cx[2] + cx[7]) + (cx[3] + cx[6]));
   const float wsum = ((cw[0] + cw[5]) + (cw[1] + cw[4])) + ((cw[2] + cw[7]) + (cw[3] + cw[6]));
@@ -398,7 +539,7 @@ __device__ __forceinline__ void known_at(const ExSrc& S, int x, int y, int z, in
   *v = 0.0f;
   *w = 0.0f;
   if (!ein(S.d, x, y, z)) return;
-  const long long i = ((long long)x * S.d[1] + y) * S.d[2] + z;
+  const unsigned i = ((unsigned)x * S.d[1] + y) * S.d[2] + z;
   if (S.known[i] >> c & 1) {
     *v = S.val[c][i];
     *w = 1.0f;
@@ -422,22 +563,28 @@ __device__ float corresponding_sample(const ExSrc& S, const int* p, int c, float
   return sw;
 }
 
-// the resolved band side as a hierarchy source
-__device__ __forceinline__ ExSrc band_src(const ExGeo& g, const ExState* s, const ExCounters* cnt) {
-  const ExState& r = s[cnt->latest];
-  return ExSrc{{g.d[0], g.d[1], g.d[2]}, {r.val[0], r.val[1], r.val[2]}, r.flags};
+// resolveConvergedFront (…wgsl.ts:359-369): the band cells' final state,
+// on the side the last pass wrote, back into side A, which then holds the
+// band result of every cell
+__global__ __launch_bounds__(kThreads) void k_ex_resolve(const int* __restrict__ list, int count, ExState a,
+                                                        ExState b, const ExCounters* cnt) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count || cnt->latest == 0) return;
+  const unsigned i = (unsigned)list[k];
+  for (int c = 0; c < 3; ++c) {
+    a.val[c][i] = b.val[c][i];
+    a.dist[c][i] = b.dist[c][i];
+  }
+  a.flags[i] = b.flags[i];
 }
 
-// restrictKnownVelocity (…wgsl.ts:450-471); source level 0 is the band result
-__global__ void k_ex_restrict(ExGeo g, ExSrc src, int from_band, ExState s0, ExState s1, const ExCounters* cnt,
-                              ExLevel tgt) {
-  const long long i = chunk_of((tgt.n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
-  if (i >= tgt.n) return;
-  if (from_band) {
-    const ExState s[2] = {s0, s1};
-    src = band_src(g, s, cnt);
-  }
-  const int p[3] = {(int)(i / ((long long)tgt.d[1] * tgt.d[2])), (int)(i / tgt.d[2] % tgt.d[1]), (int)(i % tgt.d[2])};
+// restrictKnownVelocity (…wgsl.ts:450-471); level 1 samples the band result
+__global__ __launch_bounds__(kThreads) void k_ex_restrict(ExSrc src, ExLevel tgt, FastDiv tz, FastDiv ty) {
+  const unsigned n = (unsigned)tgt.n;
+  const unsigned i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int p[3];
+  decode3(i, tgt.d[1], tgt.d[2], tz, ty, p[0], p[1], p[2]);
   unsigned known = 0;
   for (int c = 0; c < 3; ++c) {
     float v;
@@ -450,10 +597,12 @@ __global__ void k_ex_restrict(ExGeo g, ExSrc src, int from_band, ExState s0, ExS
 }
 
 // prolongUnknownVelocity (…wgsl.ts:473-488) into a level's up arrays
-__global__ void k_ex_prolong(ExSrc src, ExLevel tgt) {
-  const long long i = chunk_of((tgt.n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
-  if (i >= tgt.n) return;
-  const int p[3] = {(int)(i / ((long long)tgt.d[1] * tgt.d[2])), (int)(i / tgt.d[2] % tgt.d[1]), (int)(i % tgt.d[2])};
+__global__ __launch_bounds__(kThreads) void k_ex_prolong(ExSrc src, ExLevel tgt, FastDiv tz, FastDiv ty) {
+  const unsigned n = (unsigned)tgt.n;
+  const unsigned i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int p[3];
+  decode3(i, tgt.d[1], tgt.d[2], tz, ty, p[0], p[1], p[2]);
   unsigned known = tgt.known[i];
   for (int c = 0; c < 3; ++c) {
     float out = tgt.val[c][i];
@@ -471,22 +620,36 @@ __global__ void k_ex_prolong(ExSrc src, ExLevel tgt) {
 
 // the last prolongation onto the band result, packed into the face arrays
 // (packTransportShell, …wgsl.ts:491-512): known values on open faces, 0 else
-__global__ void k_ex_finish(ExGeo g, ExSrc src, int have_src, ExState s0, ExState s1, const uint8_t* __restrict__ cls,
-                            ExCounters* cnt, float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
-                            long long n) {
-  const long long i = chunk_of((n + kThreads - 1) / kThreads) * kThreads + threadIdx.x;
+__global__ __launch_bounds__(kThreads) void k_ex_finish(ExGeo g, ExSrc src, int have_src, ExState r,
+                                                       const uint8_t* __restrict__ cls, ExCounters* cnt,
+                                                       float* __restrict__ uo, float* __restrict__ vo,
+                                                       float* __restrict__ wo, unsigned n) {
   int unknown = 0, band = 0;
-  if (i < n) {
+  unsigned mes[kCells], knowns[kCells];
+  float owns[kCells][3];
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = min(cell_of(n, j), n - 1);
+    mes[j] = cls[i];
+    knowns[j] = r.flags[i] & 7u;
+    owns[j][0] = r.val[0][i];
+    owns[j][1] = r.val[1][i];
+    owns[j][2] = r.val[2][i];
+  }
+#pragma unroll
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = cell_of(n, j);
+    if (i >= n) break;
     int x, y, z;
     edecode(g, i, x, y, z);
-    const ExState& r = cnt->latest ? s1 : s0;
-    const unsigned me = cls[i];
-    unsigned known = r.flags[i] & 7u;
-    band = __popc(known);
+    const unsigned me = mes[j];
+    const unsigned known = knowns[j];
+    const float* own = owns[j];
+    band += __popc(known);
     const int p[3] = {x, y, z};
     float* out[3] = {uo, vo, wo};
     for (int c = 0; c < 3; ++c) {
-      float val = r.val[c][i];
+      float val = own[c];
       bool k = known >> c & 1;
       if (!k && have_src) {
         float v;
@@ -500,9 +663,9 @@ __global__ void k_ex_finish(ExGeo g, ExSrc src, int have_src, ExState s0, ExStat
       out[c][face_of(g, c, x, y, z)] = open && k ? val : 0.0f;
     }
     // the negative domain faces are walls
-    if (x == 0) uo[((long long)0 * g.d[1] + y) * g.d[2] + z] = 0.0f;
-    if (y == 0) vo[((long long)x * (g.d[1] + 1) + 0) * g.d[2] + z] = 0.0f;
-    if (z == 0) wo[((long long)x * g.d[1] + y) * (g.d[2] + 1) + 0] = 0.0f;
+    if (x == 0) uo[(unsigned)y * g.d[2] + z] = 0.0f;
+    if (y == 0) vo[(unsigned)x * (g.d[1] + 1) * g.d[2] + z] = 0.0f;
+    if (z == 0) wo[((unsigned)x * g.d[1] + y) * (g.d[2] + 1)] = 0.0f;
   }
   const int su = block_sum(unknown);
   __syncthreads();
@@ -513,8 +676,9 @@ __global__ void k_ex_finish(ExGeo g, ExSrc src, int have_src, ExState s0, ExStat
   }
 }
 
-int chunked_grid(long long n) {
-  const long long chunks = (n + kThreads - 1) / kThreads;
+// workgroups of a streaming launch over n < 2^31 items, cells per thread k
+int chunked_grid(long long n, int k = 1) {
+  const long long chunks = (n + (long long)kThreads * k - 1) / ((long long)kThreads * k);
   return (int)(8 * ((chunks + 7) / 8));
 }
 
@@ -604,6 +768,11 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   const int nx = d_[0], ny = d_[1], nz = d_[2];
   const long long n = n_;
   const size_t nu = (size_t)(nx + 1) * ny * nz, nv = (size_t)nx * (ny + 1) * nz, nw = (size_t)nx * ny * (nz + 1);
+  if ((long long)(nx + 1) * ny * nz >= (1ll << 31) || (long long)nx * (ny + 1) * nz >= (1ll << 31) ||
+      (long long)nx * ny * (nz + 1) >= (1ll << 31)) {
+    set_error("fp_extrapolate_velocity: grids of 2^31 or more faces per component are not supported");
+    return FP_ERR_UNSUPPORTED;
+  }
   const float* field = a->phi ? a->phi : a->density;
   const float *u = a->u, *v = a->v, *w = a->w;
   // kernels write device memory only: host outputs go through the input
@@ -632,6 +801,8 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   g.d[2] = nz;
   g.h = h_;
   g.open_top = a->open_top ? 1 : 0;
+  g.dz = make_div((unsigned)nz);
+  g.dy = make_div((unsigned)ny);
   g.eps_min = 2.0f * h_;
   {
     const float dx = (float)nx * h_, dy = (float)ny * h_, dz = (float)nz * h_;
@@ -639,17 +810,17 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   }
   int ceiling = std::min(std::max(nx, std::max(ny, nz)), 16);
   if (a->max_passes > 0) ceiling = std::min(a->max_passes, 19);
-  const dim3 grid(chunked_grid(n)), block(kThreads);
+  const dim3 grid(chunked_grid(n, kCells)), block(kThreads);
   HIPCHK(hipMemsetAsync(counters_, 0, sizeof(ExCounters), s));
-  hipLaunchKernelGGL(k_ex_classify, grid, block, 0, s, g, a->phi ? field : nullptr, a->phi ? nullptr : field, cls_,
-                     cand_[0], n);
-  hipLaunchKernelGGL(k_ex_seed, grid, block, 0, s, g, (const uint8_t*)cls_, u, v, w, st_[0], st_[1], counters_, n);
+  const unsigned un = (unsigned)n;
+  hipLaunchKernelGGL(k_ex_classify, grid, block, 0, s, g, field, a->phi ? 1 : 0, cls_, cand_[0], un);
+  hipLaunchKernelGGL(k_ex_seed, grid, block, 0, s, g, (const uint8_t*)cls_, u, v, w, st_[0], st_[1], counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 0,
-                     list_, counters_, n);
+                     list_, counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], cand_[0], 0,
-                     list_, counters_, n);
+                     list_, counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 1,
-                     list_, counters_, n);
+                     list_, counters_, un);
   ExCounters host;
   HIPCHK(hipMemcpyAsync(&host, counters_, sizeof(ExCounters), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -658,19 +829,21 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
     for (int pass = 1; pass <= ceiling; ++pass)
       hipLaunchKernelGGL(k_ex_pass, dim3((count + kThreads - 1) / kThreads), block, 0, s, g, (const uint8_t*)cls_,
                          (const int*)list_, count, st_[(pass - 1) & 1], st_[pass & 1], counters_, pass);
+    hipLaunchKernelGGL(k_ex_resolve, dim3((count + kThreads - 1) / kThreads), block, 0, s, (const int*)list_, count,
+                       st_[0], st_[1], (const ExCounters*)counters_);
   }
   // hierarchy: restrict down, prolong up, the last prolongation fused with the packing
   const int L = (int)levels_.size();
   const ExSrc none{{0, 0, 0}, {nullptr, nullptr, nullptr}, nullptr};
   for (int l = 0; l < L; ++l) {
     const ExLevel& T = levels_[l];
-    ExSrc src = none;
+    ExSrc src{{nx, ny, nz}, {st_[0].val[0], st_[0].val[1], st_[0].val[2]}, st_[0].flags};
     if (l > 0) {
       const ExLevel& S = levels_[l - 1];
       src = ExSrc{{S.d[0], S.d[1], S.d[2]}, {S.val[0], S.val[1], S.val[2]}, S.known};
     }
-    hipLaunchKernelGGL(k_ex_restrict, dim3(chunked_grid(T.n)), block, 0, s, g, src, l == 0 ? 1 : 0, st_[0], st_[1],
-                       (const ExCounters*)counters_, T);
+    hipLaunchKernelGGL(k_ex_restrict, dim3(chunked_grid(T.n)), block, 0, s, src, T, make_div((unsigned)T.d[2]),
+                       make_div((unsigned)T.d[1]));
   }
   ExSrc coarser = none;
   if (L > 0) {
@@ -679,11 +852,12 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   }
   for (int l = L - 2; l >= 0; --l) {
     const ExLevel& T = levels_[l];
-    hipLaunchKernelGGL(k_ex_prolong, dim3(chunked_grid(T.n)), block, 0, s, coarser, T);
+    hipLaunchKernelGGL(k_ex_prolong, dim3(chunked_grid(T.n)), block, 0, s, coarser, T, make_div((unsigned)T.d[2]),
+                       make_div((unsigned)T.d[1]));
     coarser = ExSrc{{T.d[0], T.d[1], T.d[2]}, {T.up_val[0], T.up_val[1], T.up_val[2]}, T.up_known};
   }
-  hipLaunchKernelGGL(k_ex_finish, grid, block, 0, s, g, coarser, L > 0 ? 1 : 0, st_[0], st_[1], (const uint8_t*)cls_,
-                     counters_, uo, vo, wo, n);
+  hipLaunchKernelGGL(k_ex_finish, grid, block, 0, s, g, coarser, L > 0 ? 1 : 0, st_[0], (const uint8_t*)cls_,
+                     counters_, uo, vo, wo, un);
   if (a->memory == FP_MEMORY_HOST) {
     HIPCHK(hipMemcpyAsync(a->u_out, uo, nu * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(a->v_out, vo, nv * 4, hipMemcpyDeviceToHost, s));

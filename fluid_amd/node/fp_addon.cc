@@ -580,6 +580,95 @@ napi_value remesh_tall_grid(napi_env env, napi_callback_info info) {
   return res;
 }
 
+// extrapolateVelocity(grid, {phi | density, u, v, w}, {u, v, w}?, {openTop, maxPasses}?)
+//   -> {passes, activeLeft, levels, bandFaces, unknownFaces, deviceMs}
+// fp_extrapolate_velocity, synchronous; without an output object u, v, w are
+// updated in place.
+napi_value extrapolate_velocity(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) {
+    napi_throw_type_error(env, nullptr, "extrapolateVelocity(grid, {phi|density, u, v, w}, outputs?, options?)");
+    return nullptr;
+  }
+  fp_grid* g = (fp_grid*)unwrap_external(env, argv[0]);
+  GridInfo* gi = g ? grid_info(g) : nullptr;
+  int32_t dims[3 * 16], levels = 0;
+  int64_t nf[3] = {0, 0, 0};
+  if (!gi || fp_grid_levels(g, 16, dims, nullptr, &levels) != FP_OK ||
+      fp_grid_face_counts(g, &nf[0], &nf[1], &nf[2]) != FP_OK) {
+    napi_throw_type_error(env, nullptr, "not a live projection grid");
+    return nullptr;
+  }
+  if (gi->pending > 0) {
+    napi_throw_error(env, "FP_ERR_6", "extrapolateVelocity: the grid has a projection in flight (await it first)");
+    return nullptr;
+  }
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  fp_extrapolate_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.memory = FP_MEMORY_HOST;
+  napi_value val;
+  float* p;
+  size_t len;
+  const bool has_phi = get_named(env, argv[1], "phi", &val);
+  if (!has_phi && !get_named(env, argv[1], "density", &val)) {
+    napi_throw_type_error(env, nullptr, "extrapolateVelocity: phi or density is required");
+    return nullptr;
+  }
+  if (!get_f32(env, val, &p, &len) || len != cells) {
+    napi_throw_range_error(env, nullptr, "extrapolateVelocity: phi/density must be a Float32Array of the grid's cells");
+    return nullptr;
+  }
+  if (has_phi) a.phi = p;
+  else a.density = p;
+  const char* names[3] = {"u", "v", "w"};
+  const float** in_ptrs[3] = {&a.u, &a.v, &a.w};
+  float** out_ptrs[3] = {&a.u_out, &a.v_out, &a.w_out};
+  bool has_out = false;
+  if (argc > 2) {
+    napi_valuetype t;
+    napi_typeof(env, argv[2], &t);
+    has_out = t == napi_object;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (!get_named(env, argv[1], names[k], &val) || !get_f32(env, val, &p, &len) || len != (size_t)nf[k]) {
+      napi_throw_range_error(env, nullptr, "extrapolateVelocity: u, v, w must be Float32Arrays of the grid's faces");
+      return nullptr;
+    }
+    *in_ptrs[k] = p;
+    *out_ptrs[k] = p;
+    if (has_out && get_named(env, argv[2], names[k], &val)) {
+      if (!get_f32(env, val, &p, &len) || len != (size_t)nf[k]) {
+        napi_throw_range_error(env, nullptr, "extrapolateVelocity: outputs must be Float32Arrays of the grid's faces");
+        return nullptr;
+      }
+      *out_ptrs[k] = p;
+    }
+  }
+  if (argc > 3) {
+    a.open_top = (int32_t)get_number(env, argv[3], "openTop", 0);
+    a.max_passes = (int32_t)get_number(env, argv[3], "maxPasses", 0);
+  }
+  fp_extrapolate_report r;
+  int rc = fp_extrapolate_velocity(g, &a, &r);
+  if (rc) return throw_fp(env, rc);
+  napi_value res, num;
+  NAPI_OK(napi_create_object(env, &res));
+  const struct {
+    const char* name;
+    double value;
+  } fields[] = {{"passes", (double)r.passes},        {"activeLeft", (double)r.active_left},
+                {"levels", (double)r.levels},        {"bandFaces", (double)r.band_faces},
+                {"unknownFaces", (double)r.unknown_faces}, {"deviceMs", r.device_ms}};
+  for (const auto& f : fields) {
+    NAPI_OK(napi_create_double(env, f.value, &num));
+    NAPI_OK(napi_set_named_property(env, res, f.name, num));
+  }
+  return res;
+}
+
 napi_value last_error(napi_env env, napi_callback_info) {
   napi_value s;
   NAPI_OK(napi_create_string_utf8(env, fp_last_error(), NAPI_AUTO_LENGTH, &s));
@@ -599,7 +688,8 @@ napi_value init(napi_env env, napi_value exports) {
   } fns[] = {{"createContext", create_context}, {"destroyContext", destroy_context}, {"createGrid", create_grid},
              {"destroyGrid", destroy_grid},     {"gridLevels", grid_levels},         {"project", project},
              {"submit", submit},                {"projectBatch", project_batch},     {"bytesInUse", bytes_in_use},
-             {"lastError", last_error},         {"abiVersion", abi_version},         {"remeshTallGrid", remesh_tall_grid}};
+             {"lastError", last_error},         {"abiVersion", abi_version},         {"remeshTallGrid", remesh_tall_grid},
+             {"extrapolateVelocity", extrapolate_velocity}};
   for (auto& f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

@@ -58,8 +58,22 @@ export declare class UniformPressureProjection {
   project(inputs: ProjectionInputs, options?: ProjectionOptions): Promise<ProjectionResult>;
   /** fp_submit / fp_poll: continuous batching with other queued projections. */
   submit(inputs: ProjectionInputs, options?: ProjectionOptions): Promise<ProjectionResult>;
+  /** Velocity extrapolation into the air, in place (fp_extrapolate_velocity); throws while a
+   * projection of the grid is in flight. */
+  extrapolate(fields: { phi?: Float32Array; density?: Float32Array; u: Float32Array; v: Float32Array;
+                        w: Float32Array },
+              options?: { openTop?: boolean; maxPasses?: number }): ExtrapolationReport;
   /** Frees the grid once no projection of it is in flight (idempotent). */
   destroy(): void;
+}
+
+export interface ExtrapolationReport {
+  passes: number;        // band passes that ran
+  activeLeft: number;    // band faces still active after the last pass (0: converged)
+  levels: number;        // hierarchy levels
+  bandFaces: number;     // faces known after the band, sources included
+  unknownFaces: number;  // open faces left unknown (written 0)
+  deviceMs: number;
 }
 
 export interface TallCellLayoutDesc {

@@ -130,6 +130,18 @@ class UniformPressureProjection {
     return this.handle.track(run).then((res) => { this.diagnostics = res.report; return res; });
   }
 
+  /** Velocity extrapolation into the air (fp_extrapolate_velocity; the
+   * reference's WebGPUUniformVelocityExtrapolator stage,
+   * lib/methods/uniform/webgpu-uniform-velocity-extrapolation.ts): u, v, w
+   * are updated in place from the faces next to liquid (phi < 0, or
+   * density > 1/2).  Synchronous; returns the report
+   * {passes, activeLeft, levels, bandFaces, unknownFaces, deviceMs}. */
+  extrapolate({ phi, density, u, v, w }, { openTop = false, maxPasses = 0 } = {}) {
+    if (this.handle.pending > 0) throw new Error('extrapolate: a projection of this grid is in flight');
+    const fields = phi ? { phi, u, v, w } : { density, u, v, w };
+    return binding.extrapolateVelocity(this.handle.use(), fields, undefined, { openTop: openTop ? 1 : 0, maxPasses });
+  }
+
   destroy() { this.handle.destroy(); }
 }
 

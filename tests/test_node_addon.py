@@ -51,7 +51,7 @@ def test_method_satisfies_the_simulation_method_contract(fpl):
     assert out["createSolverAsync"] == "function"
     assert out["binding"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
                                      "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion",
-                                     "remeshTallGrid"])
+                                     "remeshTallGrid", "extrapolateVelocity"])
     # the reference's default lattice sizing: 1.2 x 0.8 x 0.8 m at 2.5 cm
     assert out["lattice"]["nx"] == 48 and out["lattice"]["ny"] == 32 and out["lattice"]["nz"] == 32
     assert out["openTop"] == "RangeError"
@@ -154,5 +154,26 @@ def test_addon_tall_remesh_matches_oracle(fpl, tmp_path):
     assert out == {"refusedInFlight": True, "changed": want[5], "again": 0, "flags": 0, "basesKept": True}
     np.testing.assert_array_equal(np.fromfile(tmp_path / "bases.out.i32", np.int32).reshape(lay.bases.shape), want[0])
     for name, a in zip(("phi", "u", "v", "w"), want[1:5]):
+        got = np.fromfile(tmp_path / f"{name}.out.f32", np.float32).reshape(a.shape)
+        assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name
+
+
+@pytest.mark.gpu
+def test_addon_extrapolation_matches_oracle(fpl, oracle, tmp_path):
+    """UniformPressureProjection.extrapolate through the addon: u, v, w and
+    the report equal the CPU restatement bit for bit."""
+    from fluid_amd import scenes
+    s = scenes.dam_spheres_scene(40, seed=2)
+    for name, a in (("phi", s.phi), ("u", s.u), ("v", s.v), ("w", s.w)):
+        a.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    nx, ny, nz = s.shape
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "extrapolate_files.js"), str(tmp_path), str(nx),
+                        str(ny), str(nz), repr(s.h)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout.strip().splitlines()[-1])
+    uo, vo, wo, want = oracle.extrapolate(s.u, s.v, s.w, s.h, phi=s.phi)
+    assert (rep["passes"], rep["activeLeft"], rep["levels"], rep["bandFaces"], rep["unknownFaces"]) == \
+        (want["passes"], want["active_left"], want["levels"], want["band_known"], want["unknown"])
+    for name, a in (("u", uo), ("v", vo), ("w", wo)):
         got = np.fromfile(tmp_path / f"{name}.out.f32", np.float32).reshape(a.shape)
         assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name
