@@ -591,12 +591,26 @@ def bench_extrapolation(g, s, phi, out, args, calls: int = 5):
     med = statistics.median(ms)
     cells = s.cells
     algo = 28 * cells
+    advection = {}
+    for scheme in ("semi-lagrangian", "maccormack"):
+        adv = [torch.empty_like(a) for a in src]
+        g.advect(*src, s.dt, phi=phi, transport=res, scheme=scheme, out=adv)  # allocates on first use
+        t_adv = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            g.advect(*src, s.dt, phi=phi, transport=res, scheme=scheme, out=adv)
+            t_adv.append((time.perf_counter() - t1) * 1e3)
+        advection[scheme] = {"median_ms": round(statistics.median(t_adv), 3),
+                             "mcells_per_s": round(cells / statistics.median(t_adv) / 1e3, 1)}
     line = {"calls": calls, "median_ms": round(med, 3), "min_ms": round(min(ms), 3),
             "mcells_per_s": round(cells / med / 1e3, 1), "passes": rep["passes"],
             "active_left": rep["active_left"], "levels": rep["levels"], "band_faces": rep["band_faces"],
             "unknown_faces": rep["unknown_faces"], "algorithmic_bytes": algo,
             "achieved_gbs": round(algo / (med * 1e-3) / 1e9, 1),
-            "frac_of_hbm_peak": round(algo / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "frac_of_hbm_peak": round(algo / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "advection": dict(advection, note="fp_advect_velocity of the same projected velocities through "
+                              "their extrapolation, dt of the scene, gravity, device tensors out of place")}
     if not args.no_cpu_baseline:
         from fluid_amd import scenes
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libfluid_projection.so")
-SOURCES = ["fp_uniform.hip", "fp_serve.hip", "fp_tall.hip", "fp_tall_remesh.hip", "fp_cm11a.hip", "fp_extrap.hip", "fp_api.hip"]
+SOURCES = ["fp_uniform.hip", "fp_serve.hip", "fp_tall.hip", "fp_tall_remesh.hip", "fp_cm11a.hip", "fp_extrap.hip", "fp_advect.hip", "fp_api.hip"]
 ARCH = os.environ.get("FP_OFFLOAD_ARCH", "gfx950")
 
 

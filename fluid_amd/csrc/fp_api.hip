@@ -1,4 +1,5 @@
 // fp_api.hip -- the C ABI (include/fluid_projection.h) over the HIP solvers.
+#include "fp_advect.hpp"
 #include "fp_cm11a.hpp"
 #include "fp_extrap.hpp"
 #include "fp_tall.hpp"
@@ -45,6 +46,7 @@ struct fp_grid {
   fp::TallGrid* tall = nullptr;        // FP_LAYOUT_TALL_CELL
   fp::Cm11a* cm11a = nullptr;          // the CM11a solver's levels, from its first projection
   fp::Extrapolator* extrap = nullptr;  // velocity extrapolation state, from its first call
+  fp::Advector* advect = nullptr;      // velocity advection state, from its first call
   uint64_t bytes = 0;
 };
 
@@ -253,8 +255,10 @@ int fp_grid_destroy(fp_grid* g) {
   if (g->tall) g->ctx->tall->forget(g->tall->id());
   if (g->cm11a) g->ctx->in_use -= g->cm11a->bytes();
   if (g->extrap) g->ctx->in_use -= g->extrap->bytes();
+  if (g->advect) g->ctx->in_use -= g->advect->bytes();
   delete g->cm11a;
   delete g->extrap;
+  delete g->advect;
   delete g->uniform;
   delete g->tall;
   delete g;
@@ -645,6 +649,29 @@ int fp_probe_taps(fp_grid* g, int32_t level, int32_t axis, int32_t* prolong_coar
                         restrict_weight);
 }
 
+// The grid's extrapolation state, created on first use within the context's budget.
+static int ensure_extrapolator(fp_grid* g, const char* what) {
+  if (g->extrap) return FP_OK;
+  fp_ctx* ctx = g->ctx;
+  const fp_grid_desc& d = g->desc;
+  const uint64_t need = fp::Extrapolator::estimate_bytes(d.nx, d.ny, d.nz);
+  if (ctx->budget && ctx->in_use + need > ctx->budget) {
+    set_error("%s: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)", what,
+              (unsigned long long)need, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
+    return FP_ERR_BUDGET;
+  }
+  fp::Extrapolator* e = new (std::nothrow) fp::Extrapolator();
+  if (!e) return FP_ERR_OUT_OF_MEMORY;
+  int rc = e->create(d.nx, d.ny, d.nz, d.h);
+  if (rc) {
+    delete e;
+    return rc;
+  }
+  g->extrap = e;
+  ctx->in_use += e->bytes();
+  return FP_OK;
+}
+
 int fp_extrapolate_velocity(fp_grid* g, const fp_extrapolate_args* a, fp_extrapolate_report* rep) {
   CHECK_ARG(g && a, "fp_extrapolate_velocity: NULL argument");
   CHECK_ARG(g->uniform != nullptr, "fp_extrapolate_velocity: uniform grids only");
@@ -658,25 +685,53 @@ int fp_extrapolate_velocity(fp_grid* g, const fp_extrapolate_args* a, fp_extrapo
   fp_ctx* ctx = g->ctx;
   std::lock_guard<std::mutex> lock(ctx->gpu);
   (void)hipSetDevice(ctx->device);
-  if (!g->extrap) {
+  int rc = ensure_extrapolator(g, "fp_extrapolate_velocity");
+  if (rc) return rc;
+  return g->extrap->run(ctx->stream, a, rep);
+}
+
+int fp_advect_velocity(fp_grid* g, const fp_advect_args* a, fp_advect_report* rep) {
+  CHECK_ARG(g && a, "fp_advect_velocity: NULL argument");
+  CHECK_ARG(g->uniform != nullptr, "fp_advect_velocity: uniform grids only");
+  CHECK_ARG(a->memory == FP_MEMORY_HOST || a->memory == FP_MEMORY_DEVICE, "fp_advect_velocity: memory must be 0 or 1");
+  CHECK_ARG(a->scheme == FP_ADVECT_SEMI_LAGRANGIAN || a->scheme == FP_ADVECT_MACCORMACK,
+            "fp_advect_velocity: unknown scheme %d", a->scheme);
+  CHECK_ARG(a->phi || a->density, "fp_advect_velocity: phi or density is required");
+  CHECK_ARG(a->u && a->v && a->w && a->u_out && a->v_out && a->w_out,
+            "fp_advect_velocity: u, v, w and their outputs are required");
+  CHECK_ARG((a->tu && a->tv && a->tw) || (!a->tu && !a->tv && !a->tw),
+            "fp_advect_velocity: give all of tu, tv, tw or none");
+  CHECK_ARG(std::isfinite(a->dt) && a->dt > 0.0 && std::isfinite(a->gravity) && std::isfinite(a->viscosity) &&
+                a->viscosity >= 0.0,
+            "fp_advect_velocity: dt must be positive, gravity and viscosity finite, viscosity >= 0");
+  CHECK_IDLE(g, "fp_advect_velocity");
+  fp_ctx* ctx = g->ctx;
+  std::lock_guard<std::mutex> lock(ctx->gpu);
+  (void)hipSetDevice(ctx->device);
+  int rc = ensure_extrapolator(g, "fp_advect_velocity");
+  if (rc) return rc;
+  if (!g->advect) {
     const fp_grid_desc& d = g->desc;
-    const uint64_t need = fp::Extrapolator::estimate_bytes(d.nx, d.ny, d.nz);
+    const uint64_t need = fp::Advector::estimate_bytes(d.nx, d.ny, d.nz);
     if (ctx->budget && ctx->in_use + need > ctx->budget) {
-      set_error("fp_extrapolate_velocity: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)",
+      set_error("fp_advect_velocity: %llu more bytes exceed the context's HBM budget (%llu of %llu in use)",
                 (unsigned long long)need, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
       return FP_ERR_BUDGET;
     }
-    fp::Extrapolator* e = new (std::nothrow) fp::Extrapolator();
-    if (!e) return FP_ERR_OUT_OF_MEMORY;
-    int rc = e->create(d.nx, d.ny, d.nz, d.h);
+    fp::Advector* v = new (std::nothrow) fp::Advector();
+    if (!v) return FP_ERR_OUT_OF_MEMORY;
+    rc = v->create(d.nx, d.ny, d.nz, d.h);
     if (rc) {
-      delete e;
+      delete v;
       return rc;
     }
-    g->extrap = e;
-    ctx->in_use += e->bytes();
+    g->advect = v;
+    ctx->in_use += v->bytes();
   }
-  return g->extrap->run(ctx->stream, a, rep);
+  const size_t before = g->advect->bytes();
+  rc = g->advect->run(ctx->stream, g->extrap, a, rep);
+  ctx->in_use += g->advect->bytes() - before;  // host staging, on the first host-memory call
+  return rc;
 }
 
 int fp_tall_remesh(fp_grid* g, const fp_tall_remesh_args* a, int64_t* changed_columns) {

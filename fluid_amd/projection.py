@@ -50,8 +50,9 @@ EXPORTED_SYMBOLS = [
     "fp_grid_face_counts", "fp_project", "fp_project_batch", "fp_project_noalloc", "fp_submit", "fp_poll",
     "fp_probe_apply",
     "fp_probe_rhs", "fp_probe_vcycle", "fp_probe_level_phi", "fp_probe_tall_layout", "fp_probe_taps",
-    "fp_tall_remesh", "fp_extrapolate_velocity",
+    "fp_tall_remesh", "fp_extrapolate_velocity", "fp_advect_velocity",
 ]
+ADVECT_SCHEMES = {"semi-lagrangian": 0, "maccormack": 1}
 PENDING = 1
 
 
@@ -91,6 +92,18 @@ class ExtrapolateReport(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class AdvectArgs(C.Structure):
+    _fields_ = [("memory", C.c_int32), ("scheme", C.c_int32), ("open_top", C.c_int32), ("no_slip", C.c_int32),
+                ("dt", C.c_double), ("gravity", C.c_double), ("viscosity", C.c_double),
+                ("phi", C.c_void_p), ("density", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("w", C.c_void_p), ("tu", C.c_void_p), ("tv", C.c_void_p), ("tw", C.c_void_p),
+                ("u_out", C.c_void_p), ("v_out", C.c_void_p), ("w_out", C.c_void_p)]
+
+
+class AdvectReport(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("reserved", C.c_int32 * 4)]
 
 
 class Outputs(C.Structure):
@@ -172,6 +185,7 @@ def load_library(path: str = LIB_PATH):
     L.fp_probe_taps.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
     L.fp_tall_remesh.argtypes = [vp, C.POINTER(TallRemeshArgs), C.POINTER(C.c_int64)]
     L.fp_extrapolate_velocity.argtypes = [vp, C.POINTER(ExtrapolateArgs), C.POINTER(ExtrapolateReport)]
+    L.fp_advect_velocity.argtypes = [vp, C.POINTER(AdvectArgs), C.POINTER(AdvectReport)]
     for name in EXPORTED_SYMBOLS:
         if name not in ("fp_last_error", "fp_abi_version"):
             getattr(L, name).restype = C.c_int
@@ -450,6 +464,45 @@ class UniformPressureProjection:
         rep = ExtrapolateReport()
         _check(_lib.fp_extrapolate_velocity(self._g, C.byref(args), C.byref(rep)))
         return (*res, rep.as_dict())
+
+    def advect(self, u, v, w, dt, phi=None, density=None, transport=None, gravity: float = -9.80665,
+               viscosity: float = 0.0, scheme: str = "semi-lagrangian", open_top: bool = False,
+               no_slip: bool = False, out=None):
+        """Velocity advection and body forces (fp_advect_velocity): u, v, w
+        traced back through `transport` (their extrapolation; computed by the
+        library when None), semi-Lagrangian or bounded MacCormack, then
+        gravity along y and viscosity.  Returns (u, v, w, report dict)."""
+        if (phi is None) == (density is None):
+            raise ValueError("give exactly one of phi and density")
+        field = phi if phi is not None else density
+        arrays = [field, u, v, w] + (list(transport) if transport is not None else [])
+        parts = [_ptr(a, np.float32) for a in arrays]
+        kinds = {q[1] for q in parts}
+        if len(kinds) != 1:
+            raise TypeError("all arrays must live in the same memory (host or device)")
+        mem = kinds.pop()
+        shapes = self.shapes()
+        for arr, shp in zip(arrays, shapes + shapes[1:]):
+            if tuple(arr.shape) != shp:
+                raise ValueError(f"array shape {tuple(arr.shape)} != {shp}")
+        if mem == MEMORY_HOST:
+            res = [np.zeros(shp, np.float32) for shp in shapes[1:]]
+            optr = [r.ctypes.data for r in res]
+        else:
+            import torch
+            res = list(out) if out is not None else [torch.empty(shp, device=field.device) for shp in shapes[1:]]
+            for r, shp in zip(res, shapes[1:]):
+                if tuple(r.shape) != shp or r.dtype != torch.float32 or not r.is_contiguous():
+                    raise ValueError("out must hold contiguous float32 tensors of the face shapes")
+            optr = [r.data_ptr() for r in res]
+            torch.cuda.current_stream(field.device).synchronize()
+        tr = [p[0] for p in parts[4:]] if transport is not None else [None, None, None]
+        args = AdvectArgs(mem, ADVECT_SCHEMES[scheme], int(bool(open_top)), int(bool(no_slip)), float(dt),
+                          float(gravity), float(viscosity), parts[0][0] if phi is not None else None,
+                          parts[0][0] if phi is None else None, parts[1][0], parts[2][0], parts[3][0], *tr, *optr)
+        rep = AdvectReport()
+        _check(_lib.fp_advect_velocity(self._g, C.byref(args), C.byref(rep)))
+        return (*res, {"device_ms": rep.device_ms})
 
     def set_tuning(self, tile_z: int = 0, groups: int = 0, pairs: int = 0):
         """Sweep tile width along z (16 or 32) and workgroups per sweep of the

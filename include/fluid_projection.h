@@ -376,6 +376,60 @@ typedef struct {
 } fp_extrapolate_report;
 int fp_extrapolate_velocity(fp_grid* grid, const fp_extrapolate_args* args, fp_extrapolate_report* report);
 
+/* ---- velocity advection and body forces (uniform grids) ---- */
+/* The step before the projection in the reference's uniform solver
+ * (Algorithm 1 step 3; lib/methods/uniform/webgpu-uniform-reference.wgsl.ts):
+ * every positive face is traced back along an RK2 characteristic through the
+ * transported field (the extrapolated velocity, a sample outside the domain
+ * reading 0; sampleVelocityComponent :261-270), in pieces of at most 1.5
+ * cells, at most 32, each stage clamped into the box (departurePoint :324-334),
+ * then:
+ *   FP_ADVECT_SEMI_LAGRANGIAN  the sample there (semiLagrangianAdvection
+ *                              :922-936; a positive wall face keeps
+ *                              min(sample, its current velocity));
+ *   FP_ADVECT_MACCORMACK       bounded MacCormack (advect :938-952, the
+ *                              prediction extrapolated, reverseAdvection
+ *                              :954-961, correctAdvection :993-1004 with
+ *                              boundedMacCormack :963-991);
+ * and body forces (applyVelocityForces :872-915): dt * viscosity * Laplacian
+ * of the current velocity where the cell holds liquid (the wall reflecting the
+ * velocity with no_slip), gravity * dt on the +y face where either cell holds
+ * liquid above 1e-5 occupancy (clamp(1/2 - phi/4h), or clamp(density)).
+ * Scope: box containers without embedded solids, inflow or surface tension.
+ * The negative domain faces are copied.  transport: the extrapolation of
+ * (u, v, w) (fp_extrapolate_velocity), or NULL for the library to compute it
+ * from phi / density, which also extrapolate the MacCormack prediction.
+ * FP32, in the reference's operation order.  Outputs may alias the inputs.
+ * The first call on a grid allocates its state (FP_ERR_BUDGET beyond the
+ * context's budget); device arrays must be complete when the call is made. */
+#define FP_ADVECT_SEMI_LAGRANGIAN 0
+#define FP_ADVECT_MACCORMACK 1
+typedef struct {
+  int32_t memory;       /* FP_MEMORY_HOST or FP_MEMORY_DEVICE for the arrays below */
+  int32_t scheme;       /* FP_ADVECT_* */
+  int32_t open_top;     /* 1: the top (+y) side is open (traces may leave it, no wall clamp) */
+  int32_t no_slip;      /* 1: viscosity sees walls reflecting the velocity (fluidWallMode no-slip) */
+  double dt;            /* s */
+  double gravity;       /* m/s^2 along +y (e.g. -9.80665) */
+  double viscosity;     /* kinematic, m^2/s (dynamicViscosity / density); 0: none */
+  const float* phi;     /* (nx,ny,nz) level set of the advected surface, or NULL when density is given */
+  const float* density; /* (nx,ny,nz) surface density, used when phi is NULL */
+  const float* u;       /* current (projected) velocity: (nx+1,ny,nz) */
+  const float* v;       /* (nx,ny+1,nz) */
+  const float* w;       /* (nx,ny,nz+1) */
+  const float* tu;      /* its extrapolation, the same shapes, or all three NULL */
+  const float* tv;
+  const float* tw;
+  float* u_out;
+  float* v_out;
+  float* w_out;
+} fp_advect_args;
+typedef struct {
+  double device_ms;     /* host wall time of the call */
+  int32_t reserved[4];
+} fp_advect_report;
+int fp_advect_velocity(fp_grid* grid, const fp_advect_args* args, fp_advect_report* report);
+
 /* ---- parity probes (device work on host arrays; used by the test suite) ---- */
 /* out = A p (FP64) on the finest level, h^2-scaled. */
 int fp_probe_apply(fp_grid* grid, const float* phi, const double* p, double* out);

@@ -211,6 +211,16 @@ int orc_extrapolate(int nx, int ny, int nz, double h, int open_top, int max_pass
                     const float* density, const float* u, const float* v, const float* w, float* uo, float* vo,
                     float* wo, float* band_dist, float* band_val, uint8_t* band_known, orc_extrap_report* rep);
 
+/* ---- uniform velocity advection and body forces (fp_oracle_advect.c) ---- */
+/* scheme 0: semi-Lagrangian, 1: bounded MacCormack (the predicted field is
+ * extrapolated with orc_extrapolate on the same phi / density).  u, v, w: the
+ * current (projected) face velocities; tu, tv, tw: their extrapolation (the
+ * transported field); outputs of the face shapes. */
+int orc_advect_velocity(int nx, int ny, int nz, double h, double dt, double gravity, double nu, int scheme,
+                        int open_top, int no_slip, const float* phi, const float* density, const float* u,
+                        const float* v, const float* w, const float* tu, const float* tv, const float* tw,
+                        float* uo, float* vo, float* wo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,9 @@ def lib():
         L.orc_cm11a_system.restype = i
         L.orc_cm11a_system.argtypes = [i, i, i, d, _fp, _fp, _fp, _fp, d, d, i, _fp, _fp, _fp]
         vp = C.c_void_p
+        L.orc_advect_velocity.restype = i
+        L.orc_advect_velocity.argtypes = [i, i, i, d, d, d, d, i, i, i, vp, vp, _fp, _fp, _fp, _fp, _fp, _fp,
+                                          _fp, _fp, _fp]
         L.orc_extrapolate.restype = i
         L.orc_extrapolate.argtypes = [i, i, i, d, i, i, vp, vp, _fp, _fp, _fp, _fp, _fp, _fp, vp, vp, vp,
                                       C.POINTER(ExtrapReport)]
@@ -517,3 +520,25 @@ def extrapolate(u, v, w, h, phi=None, density=None, open_top=0, max_passes=0, ba
     if band:
         return uo, vo, wo, r, (bd, bv, bk)
     return uo, vo, wo, r
+
+
+def advect_velocity(u, v, w, h, dt, phi=None, density=None, transport=None, gravity=-9.81, nu=0.0,
+                    scheme="semi-lagrangian", open_top=0, no_slip=0):
+    """Velocity advection and body forces of the uniform solver
+    (fp_oracle_advect.c, FP32).  `transport` is the extrapolated (u, v, w); by
+    default the oracle's own extrapolation of (u, v, w)."""
+    field = phi if phi is not None else density
+    nx, ny, nz = field.shape
+    f = _f32(field)
+    u, v, w = _f32(u), _f32(v), _f32(w)
+    if transport is None:
+        transport = extrapolate(u, v, w, h, phi=phi, density=density, open_top=open_top)[:3]
+    tu, tv, tw = (_f32(a) for a in transport)
+    uo, vo, wo = np.zeros_like(u), np.zeros_like(v), np.zeros_like(w)
+    sch = {"semi-lagrangian": 0, "maccormack": 1}[scheme]
+    rc = lib().orc_advect_velocity(nx, ny, nz, h, dt, gravity, nu, sch, int(open_top), int(no_slip),
+                                   f.ctypes.data if phi is not None else None,
+                                   f.ctypes.data if phi is None else None, u, v, w, tu, tv, tw, uo, vo, wo)
+    if rc != 0:
+        raise RuntimeError(f"orc_advect_velocity failed ({rc})")
+    return uo, vo, wo
