@@ -44,14 +44,20 @@ __device__ __forceinline__ unsigned fidx(const AdGeo& g, int c, int x, int y, in
 
 // the transport shell at padded index T: face T along c (cell T-1's positive
 // face), cell T-1 along the other axes; 0 outside
+// (the load is made at an index clamped into the array and selected
+// afterwards, so the corners of a sample are loaded together)
 __device__ __forceinline__ float shell(const AdGeo& g, const Faces& t, int c, int tx, int ty, int tz) {
   const int T[3] = {tx, ty, tz};
   bool inside = true;
+  int q[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) inside = inside && T[a] >= (a == c ? 0 : 1) && T[a] <= g.d[a];
-  if (!inside) return 0.0f;
-  const int x = c == 0 ? tx : tx - 1, y = c == 1 ? ty : ty - 1, z = c == 2 ? tz : tz - 1;
-  return t.f[c][fidx(g, c, x, y, z)];
+  for (int a = 0; a < 3; ++a) {
+    const int lo = a == c ? 0 : 1;
+    inside = inside && T[a] >= lo && T[a] <= g.d[a];
+    q[a] = min(max(T[a], lo), g.d[a]) - (a == c ? 0 : 1);
+  }
+  const float v = t.f[c][fidx(g, c, q[0], q[1], q[2])];
+  return inside ? v : 0.0f;
 }
 
 // sampleVelocityComponent (…wgsl.ts:261-270) with d4Sum8 (:103-106)
@@ -71,6 +77,11 @@ __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int
     b[a] = (int)f;
     fr[a] = q[a] - f;
   }
+  // all eight corners' values first (independent loads), then the weights
+  float vals[8];
+#pragma unroll
+  for (int corner = 0; corner < 8; ++corner)
+    vals[corner] = shell(g, t, c, b[0] + (corner & 1) + 1, b[1] + (corner >> 1 & 1) + 1, b[2] + (corner >> 2 & 1) + 1);
   float terms[8];
 #pragma unroll
   for (int corner = 0; corner < 8; ++corner) {
@@ -78,7 +89,7 @@ __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int
     float w[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) w[a] = o[a] ? fr[a] : 1.0f - fr[a];
-    terms[corner] = w[0] * w[1] * w[2] * shell(g, t, c, b[0] + o[0] + 1, b[1] + o[1] + 1, b[2] + o[2] + 1);
+    terms[corner] = w[0] * w[1] * w[2] * vals[corner];
   }
   const float y0 = (terms[0] + terms[5]) + (terms[1] + terms[4]);
   const float y1 = (terms[2] + terms[7]) + (terms[3] + terms[6]);
@@ -93,36 +104,66 @@ __device__ __forceinline__ void clamp_trace(const AdGeo& g, float* q) {
   if (!g.open_top) q[1] = q[1] > (float)g.d[1] ? (float)g.d[1] : q[1];
 }
 
-// departurePoint (…wgsl.ts:324-334): RK2 in pieces of at most 1.5 cells
-__device__ void departure(const AdGeo& g, const Faces& t, const float* pos, float dt, float* out) {
-  float point[3] = {pos[0], pos[1], pos[2]};
-  float remaining = fabsf(dt);
+// departurePoint (…wgsl.ts:324-334), RK2 in pieces of at most 1.5 cells, for
+// the cell's three faces at once: the same operations per face, the three
+// traces advanced together (a finished trace keeps its point), so each round
+// of samples serves three faces
+__device__ void departure3(const AdGeo& g, const Faces& t, const float (*pos)[3], float dt, float (*out)[3]) {
+  float point[3][3], remaining[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    point[f][0] = pos[f][0];
+    point[f][1] = pos[f][1];
+    point[f][2] = pos[f][2];
+    remaining[f] = fabsf(dt);
+  }
   const float direction = dt >= 0.0f ? 1.0f : -1.0f;
   const float h = g.h;
   for (int step = 0; step < 32; ++step) {
-    if (remaining <= 1e-7f) break;
-    float first[3];
-    for (int c = 0; c < 3; ++c) first[c] = sample_comp(g, t, point, c);
-    const float ax = fabsf(first[0]) / h, ay = fabsf(first[1]) / h, az = fabsf(first[2]) / h;
-    const float myz = ay > az ? ay : az;
-    const float rate = ax > myz ? ax : myz;
-    const float lim = 1.5f / (rate > 1e-6f ? rate : 1e-6f);
-    const float step_s = remaining < lim ? remaining : lim;
-    const float signed_step = direction * step_s;
-    float mid[3];
+    bool live[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) mid[a] = point[a] - 0.5f * first[a] * signed_step / h;
-    clamp_trace(g, mid);
-    float second[3];
-    for (int c = 0; c < 3; ++c) second[c] = sample_comp(g, t, mid, c);
+    for (int f = 0; f < 3; ++f) live[f] = !(remaining[f] <= 1e-7f);
+    if (!live[0] && !live[1] && !live[2]) break;
+    float first[3][3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) point[a] = point[a] - second[a] * signed_step / h;
-    clamp_trace(g, point);
-    remaining -= step_s;
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) first[f][c] = sample_comp(g, t, point[f], c);
+    float mid[3][3], step_s[3], signed_step[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const float ax = fabsf(first[f][0]) / h, ay = fabsf(first[f][1]) / h, az = fabsf(first[f][2]) / h;
+      const float myz = ay > az ? ay : az;
+      const float rate = ax > myz ? ax : myz;
+      const float lim = 1.5f / (rate > 1e-6f ? rate : 1e-6f);
+      step_s[f] = remaining[f] < lim ? remaining[f] : lim;
+      signed_step[f] = direction * step_s[f];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) mid[f][a] = point[f][a] - 0.5f * first[f][a] * signed_step[f] / h;
+      clamp_trace(g, mid[f]);
+    }
+    float second[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) second[f][c] = sample_comp(g, t, mid[f], c);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      float next[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) next[a] = point[f][a] - second[f][a] * signed_step[f] / h;
+      clamp_trace(g, next);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) point[f][a] = live[f] ? next[a] : point[f][a];
+      remaining[f] = live[f] ? remaining[f] - step_s[f] : remaining[f];
+    }
   }
-  out[0] = point[0];
-  out[1] = point[1];
-  out[2] = point[2];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    out[f][0] = point[f][0];
+    out[f][1] = point[f][1];
+    out[f][2] = point[f][2];
+  }
 }
 
 __device__ __forceinline__ void face_position(int x, int y, int z, int c, float* pos) {
@@ -132,12 +173,15 @@ __device__ __forceinline__ void face_position(int x, int y, int z, int c, float*
   pos[c] = (float)(c == 0 ? x : c == 1 ? y : z) + 1.0f;
 }
 
-// advectVelocityComponent (…wgsl.ts:415-423) without solids or the liquid-only toggle
-__device__ float advect_comp(const AdGeo& g, const Faces& t, int x, int y, int z, int c, float dt) {
-  float pos[3], dep[3];
-  face_position(x, y, z, c, pos);
-  departure(g, t, pos, dt, dep);
-  return sample_comp(g, t, dep, c);
+// advectVelocityComponent (…wgsl.ts:415-423) without solids or the
+// liquid-only toggle, for the three faces of cell (x,y,z) together
+__device__ void advect3(const AdGeo& g, const Faces& t, int x, int y, int z, float dt, float* val) {
+  float pos[3][3], dep[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) face_position(x, y, z, c, pos[c]);
+  departure3(g, t, pos, dt, dep);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) val[c] = sample_comp(g, t, dep[c], c);
 }
 
 // sampledFaceVelocity (…wgsl.ts:247-250)
@@ -150,11 +194,9 @@ __device__ __forceinline__ float donor(const AdGeo& g, const Faces& t, const int
 }
 
 // boundedMacCormack (…wgsl.ts:963-991)
-__device__ float bounded(const AdGeo& g, const Faces& t, int x, int y, int z, int c, float predicted, float original,
+// (dep: the face's departure point, the forward trace of the prediction)
+__device__ float bounded(const AdGeo& g, const Faces& t, const float* dep, int c, float predicted, float original,
                          float reversed) {
-  float pos[3], dep[3];
-  face_position(x, y, z, c, pos);
-  departure(g, t, pos, g.dt, dep);
   float q[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -253,15 +295,15 @@ __device__ __forceinline__ void carry_walls(const AdGeo& g, const Faces& cur, fl
 }
 
 // semiLagrangianAdvection (…wgsl.ts:922-936)
-__global__ __launch_bounds__(kThreads) void k_adv_sl(AdGeo g, Faces t, Faces cur, const float* __restrict__ phi,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_adv_sl(AdGeo g, Faces t, Faces cur, const float* __restrict__ phi,
                                                     const float* __restrict__ dens, float* uo, float* vo, float* wo) {
   int x, y, z;
   if (!cell_of(g, x, y, z)) return;
   float* out[3] = {uo, vo, wo};
   const int p[3] = {x, y, z};
   float val[3];
+  advect3(g, t, x, y, z, g.dt, val);
   for (int c = 0; c < 3; ++c) {
-    val[c] = advect_comp(g, t, x, y, z, c, g.dt);
     if (p[c] == g.d[c] - 1 && !(c == 1 && g.open_top)) {
       const float old = cur.f[c][fidx(g, c, x + (c == 0), y + (c == 1), z + (c == 2))];
       val[c] = val[c] < old ? val[c] : old;
@@ -274,15 +316,17 @@ __global__ __launch_bounds__(kThreads) void k_adv_sl(AdGeo g, Faces t, Faces cur
 
 // advect (prediction, …wgsl.ts:938-952) and reverseAdvection (:954-961):
 // the wall faces keep the field the trace starts from
-__global__ __launch_bounds__(kThreads) void k_adv_trace(AdGeo g, Faces t, Faces walls, float dt, float* uo, float* vo,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_adv_trace(AdGeo g, Faces t, Faces walls, float dt, float* uo, float* vo,
                                                        float* wo) {
   int x, y, z;
   if (!cell_of(g, x, y, z)) return;
   float* out[3] = {uo, vo, wo};
   const int p[3] = {x, y, z};
+  float adv[3];
+  advect3(g, t, x, y, z, dt, adv);
   for (int c = 0; c < 3; ++c) {
     const unsigned fi = fidx(g, c, x + (c == 0), y + (c == 1), z + (c == 2));
-    float val = advect_comp(g, t, x, y, z, c, dt);
+    float val = adv[c];
     if (p[c] == g.d[c] - 1 && !(c == 1 && g.open_top)) val = walls.f[c][fi];
     out[c][fi] = val;
   }
@@ -290,16 +334,18 @@ __global__ __launch_bounds__(kThreads) void k_adv_trace(AdGeo g, Faces t, Faces 
 }
 
 // correctAdvection (…wgsl.ts:993-1004)
-__global__ __launch_bounds__(kThreads) void k_adv_correct(AdGeo g, Faces t, Faces cur, Faces pred, Faces rev,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_adv_correct(AdGeo g, Faces t, Faces cur, Faces pred, Faces rev,
                                                          const float* __restrict__ phi, const float* __restrict__ dens,
                                                          float* uo, float* vo, float* wo) {
   int x, y, z;
   if (!cell_of(g, x, y, z)) return;
   float* out[3] = {uo, vo, wo};
-  float val[3];
+  float val[3], pos[3][3], dep[3][3];
+  for (int c = 0; c < 3; ++c) face_position(x, y, z, c, pos[c]);
+  departure3(g, t, pos, g.dt, dep);
   for (int c = 0; c < 3; ++c) {
     const unsigned fi = fidx(g, c, x + (c == 0), y + (c == 1), z + (c == 2));
-    val[c] = bounded(g, t, x, y, z, c, pred.f[c][fi], cur.f[c][fi], rev.f[c][fi]);
+    val[c] = bounded(g, t, dep[c], c, pred.f[c][fi], cur.f[c][fi], rev.f[c][fi]);
   }
   forces(g, cur, phi, dens, x, y, z, val);
   for (int c = 0; c < 3; ++c) out[c][fidx(g, c, x + (c == 0), y + (c == 1), z + (c == 2))] = val[c];
