@@ -571,13 +571,16 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_extrapolation(g, s, phi, out, args, calls: int = 5):
+def bench_extrapolation(g, s, phi, out, args, step_options, calls: int = 5):
     """fp_extrapolate_velocity of the projected velocities of the headline grid
     (device tensors, out of place), after the timed projections: host wall
     time per call (the call returns when its outputs are written).  The
     algorithmic bytes are one read of phi, u, v, w and one write of u, v, w
-    (28 B per cell).  CPU baseline: the oracle's scalar C restatement on the
-    same generator at 128^3."""
+    (28 B per cell).  Then fp_advect_velocity of the same velocities (both
+    schemes), and whole velocity steps of the grid: semi-Lagrangian advection
+    through the field's own extrapolation with gravity, then the projection
+    with `step_options`.  CPU baseline: the oracle's scalar C restatement on
+    the same generator at 128^3."""
     import torch
     src = [a.clone() for a in out[1:]]
     res = [torch.empty_like(a) for a in src]
@@ -603,6 +606,24 @@ def bench_extrapolation(g, s, phi, out, args, calls: int = 5):
             t_adv.append((time.perf_counter() - t1) * 1e3)
         advection[scheme] = {"median_ms": round(statistics.median(t_adv), 3),
                              "mcells_per_s": round(cells / statistics.median(t_adv) / 1e3, 1)}
+    # whole velocity steps: advect (extrapolating the current field), project
+    state = [a.clone() for a in src]
+    adv = [torch.empty_like(a) for a in src]
+    p_out = torch.empty_like(phi)
+    t_step, iters = [], []
+    for i in range(4):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g.advect(*state, s.dt, phi=phi, out=adv)
+        r = g.project(phi, *adv, s.dt, s.rho, out=(p_out, *state), options=step_options)
+        if i:  # the first step is a warmup
+            t_step.append((time.perf_counter() - t1) * 1e3)
+            iters.append(r.report["iterations"])
+    advection["velocity_step"] = {
+        "median_ms": round(statistics.median(t_step), 3),
+        "mcells_per_s": round(cells / statistics.median(t_step) / 1e3, 1), "steps": len(t_step),
+        "iterations": iters, "note": "semi-Lagrangian advection through the field's own extrapolation with gravity, "
+        "then the projection (headline options), consecutive steps of one state"}
     line = {"calls": calls, "median_ms": round(med, 3), "min_ms": round(min(ms), 3),
             "mcells_per_s": round(cells / med / 1e3, 1), "passes": rep["passes"],
             "active_left": rep["active_left"], "levels": rep["levels"], "band_faces": rep["band_faces"],
@@ -694,7 +715,7 @@ def bench_uniform(args):
         plain = {"ms": round(ms32, 3), "mcells_per_s": round(cells / ms32 / 1e3, 2), "steps": args.sub_steps,
                  "iterations": r32.report["iterations"], "relative_residual": r32.report["relative_residual"],
                  "p_fp32": out[0].cpu().numpy()}
-    extrapolation = bench_extrapolation(g, s, phi, out, args)
+    extrapolation = bench_extrapolation(g, s, phi, out, args, opts)
     g.destroy()
     bpc = kernel_bytes_per_cell(args.timed_kernel, args.precision, args.sweeps,
                                 pairs and args.timed_kernel == "smooth_fine")

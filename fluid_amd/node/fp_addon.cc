@@ -669,6 +669,96 @@ napi_value extrapolate_velocity(napi_env env, napi_callback_info info) {
   return res;
 }
 
+// advectVelocity(grid, {phi | density, u, v, w, transport?: {u, v, w}}, {u, v, w}?,
+//                {dt, scheme, gravity, viscosity, openTop, noSlip})
+//   -> {deviceMs}
+// fp_advect_velocity, synchronous; without an output object u, v, w are
+// updated in place.  scheme: 0 semi-Lagrangian, 1 bounded MacCormack.
+napi_value advect_velocity(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 4) {
+    napi_throw_type_error(env, nullptr, "advectVelocity(grid, {phi|density, u, v, w, transport?}, outputs|null, options)");
+    return nullptr;
+  }
+  fp_grid* g = (fp_grid*)unwrap_external(env, argv[0]);
+  GridInfo* gi = g ? grid_info(g) : nullptr;
+  int32_t dims[3 * 16], levels = 0;
+  int64_t nf[3] = {0, 0, 0};
+  if (!gi || fp_grid_levels(g, 16, dims, nullptr, &levels) != FP_OK ||
+      fp_grid_face_counts(g, &nf[0], &nf[1], &nf[2]) != FP_OK) {
+    napi_throw_type_error(env, nullptr, "not a live projection grid");
+    return nullptr;
+  }
+  if (gi->pending > 0) {
+    napi_throw_error(env, "FP_ERR_6", "advectVelocity: the grid has a projection in flight (await it first)");
+    return nullptr;
+  }
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  fp_advect_args a;
+  std::memset(&a, 0, sizeof(a));
+  a.memory = FP_MEMORY_HOST;
+  napi_value val, tr;
+  float* p;
+  size_t len;
+  const bool has_phi = get_named(env, argv[1], "phi", &val);
+  if (!has_phi && !get_named(env, argv[1], "density", &val)) {
+    napi_throw_type_error(env, nullptr, "advectVelocity: phi or density is required");
+    return nullptr;
+  }
+  if (!get_f32(env, val, &p, &len) || len != cells) {
+    napi_throw_range_error(env, nullptr, "advectVelocity: phi/density must be a Float32Array of the grid's cells");
+    return nullptr;
+  }
+  if (has_phi) a.phi = p;
+  else a.density = p;
+  const char* names[3] = {"u", "v", "w"};
+  const float** in_ptrs[3] = {&a.u, &a.v, &a.w};
+  const float** tr_ptrs[3] = {&a.tu, &a.tv, &a.tw};
+  float** out_ptrs[3] = {&a.u_out, &a.v_out, &a.w_out};
+  napi_valuetype t;
+  napi_typeof(env, argv[2], &t);
+  const bool has_out = t == napi_object;
+  const bool has_tr = get_named(env, argv[1], "transport", &tr);
+  for (int k = 0; k < 3; ++k) {
+    if (!get_named(env, argv[1], names[k], &val) || !get_f32(env, val, &p, &len) || len != (size_t)nf[k]) {
+      napi_throw_range_error(env, nullptr, "advectVelocity: u, v, w must be Float32Arrays of the grid's faces");
+      return nullptr;
+    }
+    *in_ptrs[k] = p;
+    *out_ptrs[k] = p;
+    if (has_tr) {
+      if (!get_named(env, tr, names[k], &val) || !get_f32(env, val, &p, &len) || len != (size_t)nf[k]) {
+        napi_throw_range_error(env, nullptr, "advectVelocity: transport u, v, w must be Float32Arrays of the grid's faces");
+        return nullptr;
+      }
+      *tr_ptrs[k] = p;
+    }
+    if (has_out && get_named(env, argv[2], names[k], &val)) {
+      if (!get_f32(env, val, &p, &len) || len != (size_t)nf[k]) {
+        napi_throw_range_error(env, nullptr, "advectVelocity: outputs must be Float32Arrays of the grid's faces");
+        return nullptr;
+      }
+      *out_ptrs[k] = p;
+    }
+  }
+  a.dt = (float)get_number(env, argv[3], "dt", 0.0);
+  a.scheme = (int32_t)get_number(env, argv[3], "scheme", FP_ADVECT_SEMI_LAGRANGIAN);
+  a.gravity = (float)get_number(env, argv[3], "gravity", -9.80665);
+  a.viscosity = (float)get_number(env, argv[3], "viscosity", 0.0);
+  a.open_top = (int32_t)get_number(env, argv[3], "openTop", 0);
+  a.no_slip = (int32_t)get_number(env, argv[3], "noSlip", 0);
+  fp_advect_report r;
+  int rc = fp_advect_velocity(g, &a, &r);
+  if (rc) return throw_fp(env, rc);
+  napi_value res, num;
+  NAPI_OK(napi_create_object(env, &res));
+  NAPI_OK(napi_create_double(env, r.device_ms, &num));
+  NAPI_OK(napi_set_named_property(env, res, "deviceMs", num));
+  return res;
+}
+
 napi_value last_error(napi_env env, napi_callback_info) {
   napi_value s;
   NAPI_OK(napi_create_string_utf8(env, fp_last_error(), NAPI_AUTO_LENGTH, &s));
@@ -689,7 +779,7 @@ napi_value init(napi_env env, napi_value exports) {
              {"destroyGrid", destroy_grid},     {"gridLevels", grid_levels},         {"project", project},
              {"submit", submit},                {"projectBatch", project_batch},     {"bytesInUse", bytes_in_use},
              {"lastError", last_error},         {"abiVersion", abi_version},         {"remeshTallGrid", remesh_tall_grid},
-             {"extrapolateVelocity", extrapolate_velocity}};
+             {"extrapolateVelocity", extrapolate_velocity}, {"advectVelocity", advect_velocity}};
   for (auto& f : fns) {
     napi_value fn;
     NAPI_OK(napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn));

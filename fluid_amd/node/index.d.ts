@@ -63,6 +63,13 @@ export declare class UniformPressureProjection {
   extrapolate(fields: { phi?: Float32Array; density?: Float32Array; u: Float32Array; v: Float32Array;
                         w: Float32Array },
               options?: { openTop?: boolean; maxPasses?: number }): ExtrapolationReport;
+  /** Velocity advection and body forces, in place (fp_advect_velocity): traced through `transport`
+   * when given, else through the field's own extrapolation; throws while a projection of the grid
+   * is in flight. */
+  advect(fields: { phi?: Float32Array; density?: Float32Array; u: Float32Array; v: Float32Array;
+                   w: Float32Array; transport?: { u: Float32Array; v: Float32Array; w: Float32Array } },
+         options: { dt: number; scheme?: 'semi-lagrangian' | 'maccormack'; gravity?: number;
+                    viscosity?: number; openTop?: boolean; noSlip?: boolean }): { deviceMs: number };
   /** Frees the grid once no projection of it is in flight (idempotent). */
   destroy(): void;
 }

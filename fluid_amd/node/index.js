@@ -24,6 +24,7 @@ const binding = require(path.join(__dirname, '..', '_lib', 'fluid_projection.nod
 
 const PRECISION = { fp32: 0, fp16: 1, bf16: 2 };
 const SOLVER = { mgpcg: 0, cm11a: 1 };
+const ADVECT_SCHEME = { 'semi-lagrangian': 0, maccormack: 1 };
 const FLAGS = {
   NONFINITE: 0x1, NONPOSITIVE_CURVATURE: 0x2, NONPOSITIVE_PRECONDITIONER: 0x4, NONCONVERGENCE: 0x8,
   UNANCHORED_COMPONENT: 0x10, PRECISION_FALLBACK: 0x20, COARSE_SINGULAR: 0x40,
@@ -140,6 +141,21 @@ class UniformPressureProjection {
     if (this.handle.pending > 0) throw new Error('extrapolate: a projection of this grid is in flight');
     const fields = phi ? { phi, u, v, w } : { density, u, v, w };
     return binding.extrapolateVelocity(this.handle.use(), fields, undefined, { openTop: openTop ? 1 : 0, maxPasses });
+  }
+
+  /** Velocity advection and body forces (fp_advect_velocity; the reference's
+   * semiLagrangianAdvection / bounded MacCormack passes,
+   * lib/methods/uniform/webgpu-uniform-reference.wgsl.ts:872-1004): u, v, w
+   * are updated in place, traced through `transport` when given, else through
+   * their own extrapolation.  Synchronous; returns {deviceMs}. */
+  advect({ phi, density, u, v, w, transport }, { dt, scheme = 'semi-lagrangian', gravity = -9.80665,
+    viscosity = 0, openTop = false, noSlip = false } = {}) {
+    if (this.handle.pending > 0) throw new Error('advect: a projection of this grid is in flight');
+    if (!(scheme in ADVECT_SCHEME)) throw new RangeError(`advect: unknown scheme ${scheme}`);
+    const fields = phi ? { phi, u, v, w } : { density, u, v, w };
+    if (transport) fields.transport = transport;
+    return binding.advectVelocity(this.handle.use(), fields, null, {
+      dt, scheme: ADVECT_SCHEME[scheme], gravity, viscosity, openTop: openTop ? 1 : 0, noSlip: noSlip ? 1 : 0 });
   }
 
   destroy() { this.handle.destroy(); }
@@ -382,6 +398,12 @@ const PARAMS = [
       + 'multigrid (FP32, 3 full + 4 V-cycles) whose walls hold p >= 0 so liquid may separate from them; '
       + 'open-top containers always use CM11a.',
     options: [{ value: 'mgpcg', label: 'MGPCG (closed walls)' }, { value: 'cm11a', label: 'CM11a (separating walls)' }] },
+  { kind: 'select', key: 'velocityTransport', label: 'Velocity advection', tier: 'coarse', update: 'runtime',
+    default: 'semi-lagrangian', hint: 'Uniform layout: the velocity is advected through its extrapolation with '
+      + 'gravity and viscosity on the GPU before each projection. Semi-Lagrangian traces once; bounded MacCormack '
+      + 'adds a forward prediction, its extrapolation, a reverse trace and a limited correction.',
+    options: [{ value: 'semi-lagrangian', label: 'Semi-Lagrangian · one pass' },
+      { value: 'maccormack', label: 'Bounded MacCormack · three passes' }] },
 ];
 
 const PRESETS = {
@@ -424,9 +446,10 @@ const mi355xProjectionMethod = {
     if (!(v.precision in PRECISION)) v.precision = 'fp16';
     if (v.layout !== 'tall-cell') v.layout = 'uniform';
     if (!(v.pressureSolver in SOLVER) || v.layout === 'tall-cell') v.pressureSolver = 'mgpcg';
+    if (!(v.velocityTransport in ADVECT_SCHEME)) v.velocityTransport = 'semi-lagrangian';
     return v;
   },
-  runtimeParamKeys: ['precision', 'smoothingSweeps'],
+  runtimeParamKeys: ['precision', 'smoothingSweeps', 'velocityTransport'],
   capabilities: { volumeRendering: false, adoptsRigidRosterShape: true },
 
   async createSolverAsync(_device, scene, quality = 'high', values = {}, _onRigidLoads, onProgress, signal) {
@@ -513,16 +536,18 @@ const mi355xProjectionMethod = {
       maxIterations: Math.max(1, Math.min(1000, scene.numerics.pressureMaxIterations)),
       warmStart: info.steps > 0 ? 1 : 0,
     });
+    const nu = scene.fluid.dynamicViscosity_Pa_s / rho;
+    const noSlip = scene.container.fluidWallMode === 'no-slip';
     const stepOnce = () => {
-      // gravity impulse on the y faces of the field, then one projection
       if (vmask) {
+        // tall cells: gravity impulse on the v slots inside the domain
         for (let i = 0; i < v.length; i += 1) if (vmask[i]) v[i] += gy * dt;
       } else {
-        const { nx, ny, nz } = lat;
-        for (let i = 0; i < nx; i += 1) for (let j = 1; j < ny; j += 1) for (let k = 0; k < nz; k += 1) {
-          const a = phi[(i * ny + j - 1) * nz + k] < 0, b = phi[(i * ny + j) * nz + k] < 0;
-          if (a || b) v[(i * (ny + 1) + j) * nz + k] += gy * dt;
-        }
+        // uniform: advection through the extrapolated field, gravity and
+        // viscosity on the GPU (the reference's step before its projection)
+        const r = proj.advect({ phi, u, v, w }, { dt, scheme: vals.velocityTransport, gravity: gy, viscosity: nu,
+          openTop, noSlip });
+        info.advectionDeviceMs = r.deviceMs;
       }
       return proj.project({ phi, u, v, w, dt, rho }, opts()).then((res) => {
         ({ u, v, w } = res);
@@ -571,6 +596,7 @@ const mi355xProjectionMethod = {
         const n = mi355xProjectionMethod.normalizeValues(Object.assign({}, vals, values));
         vals.precision = n.precision;
         vals.smoothingSweeps = n.smoothingSweeps;
+        vals.velocityTransport = n.velocityTransport;
       },
       async readStats() {
         await pending;
@@ -610,5 +636,5 @@ function cm11aLattice(lat) {
 
 module.exports = {
   binding, UniformPressureProjection, TallCellPressureProjection, projectBatch, mi355xProjectionMethod,
-  sceneLattice, damBreakBox, tallLayout, cm11aLattice, PRECISION, SOLVER, FLAGS,
+  sceneLattice, damBreakBox, tallLayout, cm11aLattice, PRECISION, SOLVER, ADVECT_SCHEME, FLAGS,
 };

@@ -47,11 +47,11 @@ def test_method_satisfies_the_simulation_method_contract(fpl):
     for q, values in out["presets"].items():
         assert set(values) <= keys, q
     assert out["normalized"] == {"precision": "fp16", "smoothingSweeps": 4, "layout": "uniform",
-                                 "pressureSolver": "mgpcg"}
+                                 "pressureSolver": "mgpcg", "velocityTransport": "semi-lagrangian"}
     assert out["createSolverAsync"] == "function"
     assert out["binding"] == sorted(["createContext", "destroyContext", "createGrid", "destroyGrid", "gridLevels",
                                      "project", "submit", "projectBatch", "bytesInUse", "lastError", "abiVersion",
-                                     "remeshTallGrid", "extrapolateVelocity"])
+                                     "remeshTallGrid", "extrapolateVelocity", "advectVelocity"])
     # the reference's default lattice sizing: 1.2 x 0.8 x 0.8 m at 2.5 cm
     assert out["lattice"]["nx"] == 48 and out["lattice"]["ny"] == 32 and out["lattice"]["nz"] == 32
     assert out["openTop"] == "RangeError"
@@ -175,5 +175,29 @@ def test_addon_extrapolation_matches_oracle(fpl, oracle, tmp_path):
     assert (rep["passes"], rep["activeLeft"], rep["levels"], rep["bandFaces"], rep["unknownFaces"]) == \
         (want["passes"], want["active_left"], want["levels"], want["band_known"], want["unknown"])
     for name, a in (("u", uo), ("v", vo), ("w", wo)):
+        got = np.fromfile(tmp_path / f"{name}.out.f32", np.float32).reshape(a.shape)
+        assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,nu,no_slip", [("semi-lagrangian", 0.0, 0), ("maccormack", 1e-3, 1)])
+def test_addon_advection_matches_oracle(fpl, oracle, tmp_path, scheme, nu, no_slip):
+    """UniformPressureProjection.advect through the addon (through the field's
+    own extrapolation, gravity, viscosity, both schemes): u, v, w equal the CPU
+    restatement bit for bit; an unknown scheme is a RangeError."""
+    from fluid_amd import scenes
+    s = scenes.dam_spheres_scene(40, seed=4)
+    for name, a in (("phi", s.phi), ("u", s.u), ("v", s.v), ("w", s.w)):
+        a.astype(np.float32).tofile(tmp_path / f"{name}.f32")
+    nx, ny, nz = s.shape
+    dt = 1.0 / 30.0
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "advect_files.js"), str(tmp_path), str(nx),
+                        str(ny), str(nz), repr(s.h), repr(dt), scheme, repr(nu), str(no_slip)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1]) == {"refusedScheme": True, "timed": True}
+    want = oracle.advect_velocity(s.u, s.v, s.w, s.h, np.float32(dt), phi=s.phi, gravity=np.float32(-9.80665),
+                                  nu=np.float32(nu), scheme=scheme, no_slip=no_slip)
+    for name, a in zip("uvw", want):
         got = np.fromfile(tmp_path / f"{name}.out.f32", np.float32).reshape(a.shape)
         assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name
