@@ -60,6 +60,33 @@ __device__ __forceinline__ float shell(const AdGeo& g, const Faces& t, int c, in
   return inside ? v : 0.0f;
 }
 
+// shell() at padded indices (tx, ty, tz) and (tx, ty, tz + 1): the two
+// z-adjacent faces come from one 8-byte load at an array index clamped so
+// that both words lie inside the array (every face array is at least 2 deep
+// in z: grids are at least 2 cells on each axis), each word selected
+// afterwards (or 0 outside).  Values equal two shell() calls.
+__device__ __forceinline__ void shell_pair(const AdGeo& g, const Faces& t, int c, int tx, int ty, int tz, float& v0,
+                                           float& v1) {
+  const int nzc = g.d[2] + (c == 2);
+  const int T[2] = {tx, ty};
+  bool inside = true;
+  int q[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int lo = a == c ? 0 : 1;
+    inside = inside && T[a] >= lo && T[a] <= g.d[a];
+    q[a] = min(max(T[a], lo), g.d[a]) - (a == c ? 0 : 1);
+  }
+  const int z0 = tz - (c == 2 ? 0 : 1);
+  const int zb = min(max(z0, 0), nzc - 2);
+  float2 pair;
+  __builtin_memcpy(&pair, t.f[c] + fidx(g, c, q[0], q[1], zb), sizeof(pair));
+  const bool in0 = inside && z0 >= 0 && z0 <= nzc - 1;
+  const bool in1 = inside && z0 + 1 >= 0 && z0 + 1 <= nzc - 1;
+  v0 = in0 ? (z0 == zb ? pair.x : pair.y) : 0.0f;
+  v1 = in1 ? (z0 + 1 == zb ? pair.x : pair.y) : 0.0f;
+}
+
 // sampleVelocityComponent (…wgsl.ts:261-270) with d4Sum8 (:103-106)
 __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int c) {
   float q[3];
@@ -78,10 +105,11 @@ __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int
     fr[a] = q[a] - f;
   }
   // all eight corners' values first (independent loads), then the weights
+  // (corners k and k + 4 differ only in z: one pair load each)
   float vals[8];
 #pragma unroll
-  for (int corner = 0; corner < 8; ++corner)
-    vals[corner] = shell(g, t, c, b[0] + (corner & 1) + 1, b[1] + (corner >> 1 & 1) + 1, b[2] + (corner >> 2 & 1) + 1);
+  for (int k = 0; k < 4; ++k)
+    shell_pair(g, t, c, b[0] + (k & 1) + 1, b[1] + (k >> 1 & 1) + 1, b[2] + 1, vals[k], vals[k + 4]);
   float terms[8];
 #pragma unroll
   for (int corner = 0; corner < 8; ++corner) {
@@ -375,6 +403,7 @@ size_t Advector::estimate_bytes(int nx, int ny, int nz) {
 }
 
 int Advector::create(int nx, int ny, int nz, double h) {
+  if (nx < 2 || ny < 2 || nz < 2) return FP_ERR_UNSUPPORTED;  // the paired z loads need 2 faces per row
   d_[0] = nx;
   d_[1] = ny;
   d_[2] = nz;
