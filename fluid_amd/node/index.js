@@ -636,5 +636,5 @@ function cm11aLattice(lat) {
 
 module.exports = {
   binding, UniformPressureProjection, TallCellPressureProjection, projectBatch, mi355xProjectionMethod,
-  sceneLattice, damBreakBox, tallLayout, cm11aLattice, PRECISION, SOLVER, ADVECT_SCHEME, FLAGS,
+  sceneLattice, damBreakBox, uniformLevelSet, tallLayout, cm11aLattice, PRECISION, SOLVER, ADVECT_SCHEME, FLAGS,
 };

@@ -52,6 +52,99 @@ class UniformScene:
         return int(self.phi.size)
 
 
+def scene_lattice(scene: dict):
+    """(nx, ny, nz, h) of a SceneDescription: voxelDomain.finestCellSize_m over
+    the container, at least 8 cells per axis (the sizing of
+    createTallCellLayout, lib/core/tall-cell-grid.ts:311-314).  The projection
+    needs cubic cells: other containers raise ValueError."""
+    c = scene["container"]
+    size = scene["voxelDomain"]["finestCellSize_m"]
+
+    def cells(extent):  # JavaScript Math.round: halves round up
+        return max(8, int(math.floor(extent / size + 0.5)))
+
+    nx, ny, nz = cells(c["width_m"]), cells(c["height_m"]), cells(c["depth_m"])
+    hx, hy, hz = c["width_m"] / nx, c["height_m"] / ny, c["depth_m"] / nz
+    if abs(hy - hx) > 1e-6 * hx or abs(hz - hx) > 1e-6 * hx:
+        raise ValueError(f"the projection needs cubic cells (container {c['width_m']} x {c['height_m']} x "
+                         f"{c['depth_m']} m at {size} m gives {hx}, {hy}, {hz} m)")
+    return nx, ny, nz, hx
+
+
+def dam_break_box(scene: dict):
+    """Normalised reservoir box (min, max) of a dam-break scene
+    (sceneDamBreakBox, lib/core/initial-fluid.ts:16-63): the fill-derived
+    square footprint of height max(0.92, fill) unless authored dimensions or
+    an origin are given."""
+    c, f = scene["container"], scene["fluid"]
+    dims = f.get("initialDamBreakDimensions_m")
+    if dims:
+        size = (dims["x"] / c["width_m"], dims["y"] / c["height_m"], dims["z"] / c["depth_m"])
+    else:
+        fill = max(0.0, min(1.0, c["fillFraction"]))
+        height = 0.0 if fill == 0 else max(0.92, fill)
+        foot = 0.0 if fill == 0 else math.sqrt(fill / height)
+        size = (foot, height, foot)
+    o = f.get("initialDamBreakOrigin_m")
+    lo = (o["x"] / c["width_m"], o["y"] / c["height_m"], o["z"] / c["depth_m"]) if o else (0.0, 0.0, 0.0)
+    return lo, tuple(a + b for a, b in zip(lo, size))
+
+
+def scene_from_description(scene) -> UniformScene:
+    """The inputs of a uniform grid for a SceneDescription
+    (lib/core/model.ts:52-130; a dict, JSON text or a path to a JSON file such
+    as lib/core/default-scene.json), built by the same rules as the Node
+    plugin's createSolverAsync: the lattice of scene_lattice, the level set of
+    the dam-break reservoir box (signed distance in metres) or of the
+    tank-fill level at cell centres (i + 1/2) h, the fluid at rest,
+    dt = numerics.fixedDt_s, rho = fluid.density_kg_m3.  ``meta`` carries
+    what the step needs: kinematic viscosity (dynamicViscosity / density),
+    gravity, open top, no-slip walls, the solver tolerance and iteration cap.
+    Box containers with the dam-break or tank-fill initial condition only
+    (ValueError otherwise)."""
+    import json
+    import os
+    if isinstance(scene, (str, os.PathLike)):
+        text = str(scene)
+        if not text.lstrip().startswith("{"):
+            with open(scene) as fh:
+                text = fh.read()
+        scene = json.loads(text)
+    for key in ("container", "voxelDomain", "fluid", "numerics"):
+        if key not in scene:
+            raise ValueError(f"a SceneDescription needs {key}")
+    c, f, num = scene["container"], scene["fluid"], scene["numerics"]
+    if c.get("shape", "box") != "box":
+        raise ValueError("box containers only")
+    ic = f["initialCondition"]
+    if ic not in ("dam-break", "tank-fill"):
+        raise ValueError(f"unsupported initial condition {ic}")
+    nx, ny, nz, h = scene_lattice(scene)
+    x = (np.arange(nx) + 0.5) * h
+    y = (np.arange(ny) + 0.5) * h
+    z = (np.arange(nz) + 0.5) * h
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    if ic == "dam-break":
+        blo, bhi = dam_break_box(scene)
+        ext = (c["width_m"], c["height_m"], c["depth_m"])
+        lo = [b * e for b, e in zip(blo, ext)]
+        hi = [b * e for b, e in zip(bhi, ext)]
+        q = [np.abs(P - (lo[a] + hi[a]) / 2) - (hi[a] - lo[a]) / 2 for a, P in enumerate((X, Y, Z))]
+        outside = np.sqrt(np.maximum(q[0], 0) ** 2 + np.maximum(q[1], 0) ** 2 + np.maximum(q[2], 0) ** 2)
+        phi = outside + np.minimum(np.maximum(np.maximum(q[0], q[1]), q[2]), 0)
+    else:
+        phi = Y - c["fillFraction"] * c["height_m"]
+    rho = float(f["density_kg_m3"])
+    meta = {"nu": float(f["dynamicViscosity_Pa_s"]) / rho, "gravity": float(f["gravity_m_s2"]["y"]),
+            "open_top": c.get("top") == "open", "no_slip": c.get("fluidWallMode") == "no-slip",
+            "relative_tolerance": float(num["pressureRelativeTolerance"]),
+            "max_iterations": max(1, min(1000, int(num["pressureMaxIterations"]))),
+            "scene_id": scene.get("sceneId", "")}
+    return UniformScene(phi=phi.astype(np.float32), u=np.zeros((nx + 1, ny, nz), np.float32),
+                        v=np.zeros((nx, ny + 1, nz), np.float32), w=np.zeros((nx, ny, nz + 1), np.float32),
+                        h=h, dt=float(num["fixedDt_s"]), rho=rho, name=scene.get("sceneId", "scene"), meta=meta)
+
+
 def _rng(seed: int) -> np.random.Generator:
     return np.random.Generator(np.random.PCG64(seed))
 
