@@ -42,6 +42,15 @@ __device__ __forceinline__ unsigned fidx(const AdGeo& g, int c, int x, int y, in
                 : ((unsigned)x * g.d[1] + y) * (g.d[2] + 1) + z;
 }
 
+// v clamped into [lo, hi] (lo < hi) in one instruction.  For finite v this
+// equals v < lo ? lo : (v > hi ? hi : v) up to the sign of a zero result;
+// positions are only ever used as p - off with off >= 1/2, or compared, so
+// that sign never reaches a result.
+__device__ __forceinline__ float med3(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
+
+// a * b for a, b < 2^24 and a product below 2^32: one full-rate 24-bit multiply
+__device__ __forceinline__ unsigned mul24(unsigned a, unsigned b) { return (a & 0xffffffu) * (b & 0xffffffu); }
+
 // the transport shell at padded index T: face T along c (cell T-1's positive
 // face), cell T-1 along the other axes; 0 outside
 // (the load is made at an index clamped into the array and selected
@@ -60,33 +69,6 @@ __device__ __forceinline__ float shell(const AdGeo& g, const Faces& t, int c, in
   return inside ? v : 0.0f;
 }
 
-// shell() at padded indices (tx, ty, tz) and (tx, ty, tz + 1): the two
-// z-adjacent faces come from one 8-byte load at an array index clamped so
-// that both words lie inside the array (every face array is at least 2 deep
-// in z: grids are at least 2 cells on each axis), each word selected
-// afterwards (or 0 outside).  Values equal two shell() calls.
-__device__ __forceinline__ void shell_pair(const AdGeo& g, const Faces& t, int c, int tx, int ty, int tz, float& v0,
-                                           float& v1) {
-  const int nzc = g.d[2] + (c == 2);
-  const int T[2] = {tx, ty};
-  bool inside = true;
-  int q[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int lo = a == c ? 0 : 1;
-    inside = inside && T[a] >= lo && T[a] <= g.d[a];
-    q[a] = min(max(T[a], lo), g.d[a]) - (a == c ? 0 : 1);
-  }
-  const int z0 = tz - (c == 2 ? 0 : 1);
-  const int zb = min(max(z0, 0), nzc - 2);
-  float2 pair;
-  __builtin_memcpy(&pair, t.f[c] + fidx(g, c, q[0], q[1], zb), sizeof(pair));
-  const bool in0 = inside && z0 >= 0 && z0 <= nzc - 1;
-  const bool in1 = inside && z0 + 1 >= 0 && z0 + 1 <= nzc - 1;
-  v0 = in0 ? (z0 == zb ? pair.x : pair.y) : 0.0f;
-  v1 = in1 ? (z0 + 1 == zb ? pair.x : pair.y) : 0.0f;
-}
-
 // sampleVelocityComponent (…wgsl.ts:261-270) with d4Sum8 (:103-106)
 __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int c) {
   float q[3];
@@ -94,22 +76,43 @@ __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int
   for (int a = 0; a < 3; ++a) {
     const float off = a == c ? 1.0f : 0.5f, lo = a == c ? -1.0f : 0.0f, hi = (float)(g.d[a] - 1);
     const float v = p[a] - off;
-    q[a] = v < lo ? lo : (v > hi ? hi : v);
+    q[a] = med3(v, lo, hi);
   }
   int b[3];
   float fr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float f = floorf(q[a]);
-    b[a] = (int)f;
+    // q lies in [lo, hi], so this clamp changes nothing; it keeps the
+    // indices below inside the arrays whatever the input
+    b[a] = min(max((int)f, a == c ? -1 : 0), g.d[a] - 1);
     fr[a] = q[a] - f;
   }
-  // all eight corners' values first (independent loads), then the weights
-  // (corners k and k + 4 differ only in z: one pair load each)
+  // All eight corners' values first, then the weights.  Corner o is face
+  // array element e = b + o (+1 along c): e >= 0 always, and only an o = 1
+  // corner at the top of an axis lies past the array (it reads 0, as the
+  // reference's padded shell does; its weight is 0).  Corners k and k + 4
+  // differ only in z and come from one 8-byte load at a row index clamped so
+  // that both words lie in the row (every row holds at least 2 faces: grids
+  // are at least 2 cells on each axis).  Byte offsets fit 32 bits (fewer than
+  // 2^30 faces per component), so the loads take a scalar base.
+  const int ex = g.d[0] + (c == 0), ey = g.d[1] + (c == 1), ez = g.d[2] + (c == 2);
+  const int ix0 = b[0] + (c == 0), iy0 = b[1] + (c == 1), iz0 = b[2] + (c == 2);
+  const bool inx1 = ix0 + 1 < ex, iny1 = iy0 + 1 < ey, topz = iz0 > ez - 2;
+  const int ix1 = min(ix0 + 1, ex - 1), iy1 = min(iy0 + 1, ey - 1), zb = min(iz0, ez - 2);
+  // (24-bit multiplies: fewer than 2^24 rows per component)
+  const unsigned r0 = mul24((unsigned)ix0, (unsigned)ey), r1 = mul24((unsigned)ix1, (unsigned)ey);
+  const unsigned rows[4] = {r0 + iy0, r1 + iy0, r0 + iy1, r1 + iy1};
+  const char* base = (const char*)t.f[c];
   float vals[8];
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    shell_pair(g, t, c, b[0] + (k & 1) + 1, b[1] + (k >> 1 & 1) + 1, b[2] + 1, vals[k], vals[k + 4]);
+  for (int k = 0; k < 4; ++k) {
+    float2 pair;
+    __builtin_memcpy(&pair, base + ((mul24(rows[k], (unsigned)ez) + (unsigned)zb) << 2), sizeof(pair));
+    const bool in = ((k & 1) == 0 || inx1) && ((k & 2) == 0 || iny1);
+    vals[k] = in ? (topz ? pair.y : pair.x) : 0.0f;
+    vals[k + 4] = in && !topz ? pair.y : 0.0f;
+  }
   float terms[8];
 #pragma unroll
   for (int corner = 0; corner < 8; ++corner) {
@@ -126,10 +129,9 @@ __device__ float sample_comp(const AdGeo& g, const Faces& t, const float* p, int
 
 // clampVelocityTraceToDomain (…wgsl.ts:317-322)
 __device__ __forceinline__ void clamp_trace(const AdGeo& g, float* q) {
-  q[0] = q[0] < 0.0f ? 0.0f : (q[0] > (float)g.d[0] ? (float)g.d[0] : q[0]);
-  q[2] = q[2] < 0.0f ? 0.0f : (q[2] > (float)g.d[2] ? (float)g.d[2] : q[2]);
-  q[1] = q[1] < 0.0f ? 0.0f : q[1];
-  if (!g.open_top) q[1] = q[1] > (float)g.d[1] ? (float)g.d[1] : q[1];
+  q[0] = med3(q[0], 0.0f, (float)g.d[0]);
+  q[2] = med3(q[2], 0.0f, (float)g.d[2]);
+  q[1] = g.open_top ? fmaxf(q[1], 0.0f) : med3(q[1], 0.0f, (float)g.d[1]);
 }
 
 // departurePoint (…wgsl.ts:324-334), RK2 in pieces of at most 1.5 cells, for
@@ -230,7 +232,7 @@ __device__ float bounded(const AdGeo& g, const Faces& t, const float* dep, int c
   for (int a = 0; a < 3; ++a) {
     const float off = a == c ? 1.0f : 0.5f, lo = a == c ? -1.0f : 0.0f, hi = (float)(g.d[a] - 1);
     const float v = dep[a] - off;
-    q[a] = v < lo ? lo : (v > hi ? hi : v);
+    q[a] = med3(v, lo, hi);
   }
   int b[3];
   float fr[3];
@@ -403,7 +405,14 @@ size_t Advector::estimate_bytes(int nx, int ny, int nz) {
 }
 
 int Advector::create(int nx, int ny, int nz, double h) {
-  if (nx < 2 || ny < 2 || nz < 2) return FP_ERR_UNSUPPORTED;  // the paired z loads need 2 faces per row
+  // the paired z loads need 2 faces per row; byte offsets of 32 bits
+  if (nx < 2 || ny < 2 || nz < 2) return FP_ERR_UNSUPPORTED;
+  const size_t most = (size_t)(std::max(nx, std::max(ny, nz)) + 1) * nx * ny * nz / std::min(nx, std::min(ny, nz));
+  if (most >= ((size_t)1 << 30) || (size_t)(nx + 1) * (ny + 1) >= ((size_t)1 << 24)) {
+    set_error("velocity advection: %d x %d x %d has 2^30 or more faces or 2^24 or more rows per component", nx, ny,
+              nz);
+    return FP_ERR_UNSUPPORTED;
+  }
   d_[0] = nx;
   d_[1] = ny;
   d_[2] = nz;
