@@ -36,37 +36,26 @@ struct Faces {
   const float* f[3];
 };
 
-__device__ __forceinline__ unsigned fidx(const AdGeo& g, int c, int x, int y, int z) {
-  return c == 0 ? ((unsigned)x * g.d[1] + y) * g.d[2] + z
-       : c == 1 ? ((unsigned)x * (g.d[1] + 1) + y) * g.d[2] + z
-                : ((unsigned)x * g.d[1] + y) * (g.d[2] + 1) + z;
-}
-
 // v clamped into [lo, hi] (lo < hi) in one instruction.  For finite v this
 // equals v < lo ? lo : (v > hi ? hi : v) up to the sign of a zero result;
 // positions are only ever used as p - off with off >= 1/2, or compared, so
 // that sign never reaches a result.
 __device__ __forceinline__ float med3(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 
-// a * b for a, b < 2^24 and a product below 2^32: one full-rate 24-bit multiply
-__device__ __forceinline__ unsigned mul24(unsigned a, unsigned b) { return (a & 0xffffffu) * (b & 0xffffffu); }
+// a * b for a, b < 2^24 and a product below 2^32: one full-rate 24-bit
+// multiply (written out: the compiler keeps a quarter-rate 32-bit multiply
+// when the operands' ranges are not visible inside the trace loop)
+__device__ __forceinline__ unsigned mul24(unsigned a, unsigned b) {
+  unsigned r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 
-// the transport shell at padded index T: face T along c (cell T-1's positive
-// face), cell T-1 along the other axes; 0 outside
-// (the load is made at an index clamped into the array and selected
-// afterwards, so the corners of a sample are loaded together)
-__device__ __forceinline__ float shell(const AdGeo& g, const Faces& t, int c, int tx, int ty, int tz) {
-  const int T[3] = {tx, ty, tz};
-  bool inside = true;
-  int q[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int lo = a == c ? 0 : 1;
-    inside = inside && T[a] >= lo && T[a] <= g.d[a];
-    q[a] = min(max(T[a], lo), g.d[a]) - (a == c ? 0 : 1);
-  }
-  const float v = t.f[c][fidx(g, c, q[0], q[1], q[2])];
-  return inside ? v : 0.0f;
+// face (x, y, z) of component c (fewer than 2^24 rows and 2^30 faces per
+// component, checked at set-up)
+__device__ __forceinline__ unsigned fidx(const AdGeo& g, int c, int x, int y, int z) {
+  const unsigned ey = g.d[1] + (c == 1), ez = g.d[2] + (c == 2);
+  return mul24(mul24((unsigned)x, ey) + (unsigned)y, ez) + (unsigned)z;
 }
 
 // sampleVelocityComponent (…wgsl.ts:261-270) with d4Sum8 (:103-106)
@@ -162,9 +151,11 @@ __device__ void departure3(const AdGeo& g, const Faces& t, const float (*pos)[3]
     float mid[3][3], step_s[3], signed_step[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const float ax = fabsf(first[f][0]) / h, ay = fabsf(first[f][1]) / h, az = fabsf(first[f][2]) / h;
+      // max(|u|/h, |v|/h, |w|/h) as max(|u|, |v|, |w|)/h: a correctly rounded
+      // division by h > 0 is monotonic, so both pick the same value
+      const float ax = fabsf(first[f][0]), ay = fabsf(first[f][1]), az = fabsf(first[f][2]);
       const float myz = ay > az ? ay : az;
-      const float rate = ax > myz ? ax : myz;
+      const float rate = (ax > myz ? ax : myz) / h;
       const float lim = 1.5f / (rate > 1e-6f ? rate : 1e-6f);
       step_s[f] = remaining[f] < lim ? remaining[f] : lim;
       signed_step[f] = direction * step_s[f];
@@ -214,14 +205,6 @@ __device__ void advect3(const AdGeo& g, const Faces& t, int x, int y, int z, flo
   for (int c = 0; c < 3; ++c) val[c] = sample_comp(g, t, dep[c], c);
 }
 
-// sampledFaceVelocity (…wgsl.ts:247-250)
-__device__ __forceinline__ float donor(const AdGeo& g, const Faces& t, const int* p, int c) {
-  if (p[c] < 0 || p[c] >= g.d[c]) return 0.0f;
-  int q[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) q[a] = p[a] < 0 ? 0 : (p[a] > g.d[a] - 1 ? g.d[a] - 1 : p[a]);
-  return shell(g, t, c, q[0] + 1, q[1] + 1, q[2] + 1);
-}
 
 // boundedMacCormack (…wgsl.ts:963-991)
 // (dep: the face's departure point, the forward trace of the prediction)
@@ -239,19 +222,47 @@ __device__ float bounded(const AdGeo& g, const Faces& t, const float* dep, int c
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float f = floorf(q[a]);
-    b[a] = (int)f;
+    b[a] = min(max((int)f, a == c ? -1 : 0), g.d[a] - 1);  // changes nothing; keeps indices in the arrays
     fr[a] = q[a] - f;
+  }
+  // The donors (sampledFaceVelocity, …wgsl.ts:247-250): corner o reads face
+  // array element clamp(b + o, 0, d - 1) (+1 along c), or 0 when b + o
+  // leaves [0, d - 1] along c.  The two z corners of each (x, y) come from
+  // one 8-byte load, as in sample_comp.
+  const int ey = g.d[1] + (c == 1), ez = g.d[2] + (c == 2);
+  int idx[3][2];
+  bool valid[2] = {true, true};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      const int p = b[a] + o;
+      idx[a][o] = min(max(p, 0), g.d[a] - 1) + (a == c);
+      if (a == c) valid[o] = p >= 0 && p < g.d[a];
+    }
+  const int zb = min(idx[2][0], ez - 2);
+  const bool z0hi = idx[2][0] > zb, z1hi = idx[2][1] > zb;
+  const char* base = (const char*)t.f[c];
+  float values[8];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const unsigned row = mul24((unsigned)idx[0][k & 1], (unsigned)ey) + (unsigned)idx[1][k >> 1];
+    float2 pair;
+    __builtin_memcpy(&pair, base + ((mul24(row, (unsigned)ez) + (unsigned)zb) << 2), sizeof(pair));
+    const int o[3] = {k & 1, k >> 1, 0};
+    values[k] = valid[o[c]] ? (z0hi ? pair.y : pair.x) : 0.0f;
+    const int o4[3] = {k & 1, k >> 1, 1};
+    values[k + 4] = valid[o4[c]] ? (z1hi ? pair.y : pair.x) : 0.0f;
   }
   float lower = 1e30f, upper = -1e30f;
 #pragma unroll
   for (int corner = 0; corner < 8; ++corner) {
     const int o[3] = {corner & 1, corner >> 1 & 1, corner >> 2 & 1};
-    const int p[3] = {b[0] + o[0], b[1] + o[1], b[2] + o[2]};
     float w[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) w[a] = o[a] ? fr[a] : 1.0f - fr[a];
     const float weight = w[0] * w[1] * w[2];
-    const float value = donor(g, t, p, c);
+    const float value = values[corner];
     if (weight > 0.0f) {
       lower = value < lower ? value : lower;
       upper = value > upper ? value : upper;
@@ -264,7 +275,7 @@ __device__ float bounded(const AdGeo& g, const Faces& t, const float* dep, int c
 
 __device__ __forceinline__ float occupancy(const AdGeo& g, const float* phi, const float* dens, int x, int y, int z) {
   if (x < 0 || y < 0 || z < 0 || x >= g.d[0] || y >= g.d[1] || z >= g.d[2]) return 0.0f;
-  const unsigned i = ((unsigned)x * g.d[1] + y) * g.d[2] + z;
+  const unsigned i = mul24(mul24((unsigned)x, (unsigned)g.d[1]) + (unsigned)y, (unsigned)g.d[2]) + (unsigned)z;
   const float v = phi ? 0.5f - phi[i] / (4.0f * g.h) : dens[i];
   return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
 }
