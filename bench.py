@@ -643,6 +643,14 @@ def bench_extrapolation(g, s, phi, out, args, step_options, calls: int = 5):
         line["cpu_baseline"] = {"value": round(c.cells / dt / 1e6, 2), "unit": "Mcells/s", "cores": 1,
                                 "kind": "port", "sample": f"oracle/fp_oracle_extrap.c (scalar C) on the dam + 8 "
                                 f"spheres generator at 128^3, one extrapolation in {dt:.2f} s"}
+        tr = oracle.extrapolate(c.u, c.v, c.w, c.h, phi=c.phi)[:3]
+        t1 = time.perf_counter()
+        oracle.advect_velocity(c.u, c.v, c.w, c.h, c.dt, phi=c.phi, transport=tr, gravity=-9.80665)
+        dt = time.perf_counter() - t1
+        line["advection"]["cpu_baseline"] = {
+            "value": round(c.cells / dt / 1e6, 2), "unit": "Mcells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/fp_oracle_advect.c (scalar C), semi-Lagrangian through the extrapolated field, dam + 8 "
+                      f"spheres generator at 128^3, one call in {dt:.2f} s"}
     return line
 
 
