@@ -140,3 +140,40 @@ def test_refused_forms(fpl):
         assert lib.fp_advect_velocity(g.handle, C.byref(partial), None) == fpl.ERR_INVALID_ARGUMENT
     finally:
         g.destroy()
+
+
+@pytest.mark.parametrize("cells,scheme", [(0.3, "semi-lagrangian"), (4.0, "semi-lagrangian"), (2.5, "maccormack")])
+def test_full_size_translation_is_exact(fpl, cells, scheme):
+    """At the 384^3 size of the large workload, where the oracle is too slow:
+    a constant transport velocity U along x carries a field linear in x
+    exactly (exact RK2 departure points of a constant field, in pieces of at
+    most 1.5 cells; exact trilinear interpolation of a linear field; the
+    MacCormack correction vanishes), so interior faces hold a + b (x - U dt)
+    and u stays U.  All liquid, no gravity."""
+    import torch
+    n = 384
+    h = 1.0 / n
+    U = 0.8
+    dt = cells * h / U
+    dev = torch.device("cuda:0")
+    phi = torch.full((n, n, n), -1.0, device=dev)
+    u = torch.full((n + 1, n, n), U, device=dev)
+    u[0] = 0.0
+    u[-1] = 0.0
+    xs = (torch.arange(n, device=dev, dtype=torch.float64) + 0.5) * h
+    v = (0.02 + 0.05 * xs).float()[:, None, None].expand(n, n + 1, n).contiguous()
+    w = torch.zeros((n, n, n + 1), device=dev)
+    print(f"384^3 translation {cells} cells {scheme}", flush=True)
+    g = fpl.UniformPressureProjection((n, n, n), h)
+    try:
+        uo, vo, wo, _ = g.advect(u, v, w, dt, phi=phi, transport=(u, v, w), gravity=0.0, scheme=scheme)
+        torch.cuda.synchronize()
+    finally:
+        g.destroy()
+    lo = int(np.ceil(cells)) + 3
+    hi = n - 3 - (int(np.ceil(cells)) if scheme == "maccormack" else 0)
+    want = (0.02 + 0.05 * (xs[lo:hi] - U * dt)).float()
+    got = vo[lo:hi, 3:-3, 3:-3]
+    assert float((got - want[:, None, None]).abs().max()) <= 2e-6
+    assert float((uo[lo:hi, 3:-3, 3:-3] - U).abs().max()) <= 2e-6
+    assert float(wo[:, :, 1:-1].abs().max()) == 0.0
