@@ -401,7 +401,9 @@ int fp_extrapolate_velocity(fp_grid* grid, const fp_extrapolate_args* args, fp_e
  * from phi / density, which also extrapolate the MacCormack prediction.
  * FP32, in the reference's operation order.  Outputs may alias the inputs.
  * The first call on a grid allocates its state (FP_ERR_BUDGET beyond the
- * context's budget); device arrays must be complete when the call is made. */
+ * context's budget); device arrays must be complete when the call is made.
+ * Grids of 2^30 or more faces per component, or of (nx+1)(ny+1) >= 2^24, are
+ * refused with FP_ERR_UNSUPPORTED (32-bit offsets, 24-bit multiplies). */
 #define FP_ADVECT_SEMI_LAGRANGIAN 0
 #define FP_ADVECT_MACCORMACK 1
 typedef struct {
