@@ -14,6 +14,9 @@ What fixes each check, independently of the restatement's formulas:
     discrete Laplacian of the current field (a quadratic in x: 2 / h^2);
   * boundedness: every result before forces lies within the range of the
     transported field (convex combinations, or the MacCormack limiter);
+  * the MacCormack limiter on a step, worked by hand: the correction kept
+    where it stays within the donors (0.72), the prediction kept where it
+    would undershoot them (0, not -0.12);
   * mirror symmetry: mirrored inputs give mirrored outputs up to rounding
     (the interpolation weights 1 - f and f of mirrored positions round
     differently, so not bit for bit; the viscosity's positive-face stencil
@@ -136,3 +139,38 @@ def test_mirror_symmetry(oracle, scheme):
     allowed = 0 if scheme == "semi-lagrangian" else 4
     for a, mirrored in ((uo, -uo[::-1]), (vo, vo[::-1]), (wo, wo[::-1])):
         assert int((np.abs(a - mirrored) > 1e-5).sum()) <= allowed
+
+
+def test_maccormack_limiter_on_a_step(oracle):
+    """A step in v (0 below x = 12 cells, 1 above) carried along x by a
+    constant u of 0.4 cells per step, worked by hand.  The prediction
+    (semi-Lagrangian) is 0.6 at the face x = 12.5 (departure 12.1, between the
+    0 at 11.5 and the 1 at 12.5) and 0 at 11.5.  Its reverse trace (+0.4
+    cells) gives 0.24 at 11.5 and 0.76 at 12.5, so the unlimited correction
+    pred + (orig - rev) / 2 is -0.12 at 11.5 (below every donor: the limiter
+    keeps the prediction, 0) and 0.6 + 0.12 = 0.72 at 12.5 (inside its donors
+    0 and 1: kept).  A missing limiter fails the [0, 1] bound; one that always
+    reverts fails the 0.72."""
+    n = 24
+    h = 1.0 / n
+    phi = -np.ones((n, n, n), np.float32)
+    U = 0.8
+    dt = 0.4 * h / U
+    u = np.full((n + 1, n, n), U, np.float32)
+    u[0] = 0.0
+    u[-1] = 0.0
+    v = np.zeros((n, n + 1, n), np.float32)
+    v[12:] = 1.0
+    w = np.zeros((n, n, n + 1), np.float32)
+    t = (u, v, w)
+    sl = oracle.advect_velocity(u, v, w, h, dt, phi=phi, transport=t, gravity=0.0)[1]
+    mc = oracle.advect_velocity(u, v, w, h, dt, phi=phi, transport=t, gravity=0.0, scheme="maccormack")[1]
+    inner = (slice(3, n - 3), slice(3, n - 2), slice(3, n - 3))
+    col_sl, col_mc = sl[inner][:, 5, 5], mc[inner][:, 5, 5]
+    np.testing.assert_allclose(col_sl[12 - 3], 0.6, atol=1e-6)
+    np.testing.assert_array_equal(col_sl[11 - 3], 0.0)
+    assert float(mc[inner].min()) >= 0.0 and float(mc[inner].max()) <= 1.0
+    np.testing.assert_array_equal(col_mc[11 - 3], 0.0)
+    np.testing.assert_allclose(col_mc[12 - 3], 0.72, atol=1e-6)
+    np.testing.assert_array_equal(col_mc[: 11 - 3], 0.0)
+    np.testing.assert_array_equal(col_mc[13 - 3:], 1.0)
