@@ -195,12 +195,18 @@ __device__ __forceinline__ void load6(const ExGeo& g, const uint8_t* __restrict_
   for (int k = 0; k < 6; ++k) out[k] = ok[k] ? out[k] : 0u;
 }
 
-// seedActiveFront (…extrapolation.wgsl.ts:148-170), written to both sides
+// seedActiveFront (…extrapolation.wgsl.ts:148-170).  Side A everywhere; side
+// B only where a pass can read it before writing it: the band list's cells
+// and their 6-neighbours, all within one cell of the last candidate
+// dilation (near: that mask; a pass writes every list cell of its
+// destination side, and reads its source side at list cells and their
+// neighbours only).
 __global__ __launch_bounds__(kThreads) void k_ex_seed(ExGeo g, const uint8_t* __restrict__ cls,
+                                                     const uint8_t* __restrict__ near,
                                                      const float* __restrict__ u, const float* __restrict__ v,
                                                      const float* __restrict__ w, ExState a, ExState b,
                                                      ExCounters* cnt, unsigned n) {
-  unsigned me[kCells], nb[kCells][6];
+  unsigned me[kCells], nb[kCells][6], nr[kCells];
   float in[kCells][3];
 #pragma unroll
   for (int j = 0; j < kCells; ++j) {
@@ -209,6 +215,9 @@ __global__ __launch_bounds__(kThreads) void k_ex_seed(ExGeo g, const uint8_t* __
     edecode(g, i, x, y, z);
     me[j] = cls[i];
     load6(g, cls, i, x, y, z, nb[j]);
+    unsigned m6[6];
+    load6(g, near, i, x, y, z, m6);
+    nr[j] = near[i] | m6[0] | m6[1] | m6[2] | m6[3] | m6[4] | m6[5];
     // the positive face of every cell exists in the face arrays
     in[j][0] = u[face_of(g, 0, x, y, z)];
     in[j][1] = v[face_of(g, 1, x, y, z)];
@@ -236,12 +245,14 @@ __global__ __launch_bounds__(kThreads) void k_ex_seed(ExGeo g, const uint8_t* __
         }
       }
       a.val[c][i] = val;
-      b.val[c][i] = val;
       a.dist[c][i] = dist;
-      b.dist[c][i] = dist;
+      if (nr[j]) {
+        b.val[c][i] = val;
+        b.dist[c][i] = dist;
+      }
     }
     a.flags[i] = (uint8_t)flags;
-    b.flags[i] = (uint8_t)flags;
+    if (nr[j]) b.flags[i] = (uint8_t)flags;
   }
   const int s = block_sum(active);
   if (threadIdx.x == 0 && s) atomicAdd(&cnt->active[0], s);
@@ -249,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_ex_seed(ExGeo g, const uint8_t* __
 
 // one 6-neighbour OR of the candidate bits; the last one appends the cells
 // with an open non-source candidate face to the band list (one atomic per
-// workgroup)
+// workgroup) and also stores its mask (the seed's near set)
 __global__ __launch_bounds__(kThreads) void k_ex_dilate(ExGeo g, const uint8_t* __restrict__ cls,
                                                        const uint8_t* __restrict__ in, uint8_t* __restrict__ out,
                                                        int last, int* __restrict__ list, ExCounters* cnt,
@@ -265,14 +276,12 @@ __global__ __launch_bounds__(kThreads) void k_ex_dilate(ExGeo g, const uint8_t* 
     m[j] = in[i] | nb[0] | nb[1] | nb[2] | nb[3] | nb[4] | nb[5];
     me[j] = last ? cls[i] : 0u;
   }
-  if (!last) {
 #pragma unroll
-    for (int j = 0; j < kCells; ++j) {
-      const unsigned i = cell_of(n, j);
-      if (i < n) out[i] = (uint8_t)m[j];
-    }
-    return;
+  for (int j = 0; j < kCells; ++j) {
+    const unsigned i = cell_of(n, j);
+    if (i < n) out[i] = (uint8_t)m[j];
   }
+  if (!last) return;
   unsigned takes = 0;
 #pragma unroll
   for (int j = 0; j < kCells; ++j)
@@ -814,13 +823,14 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   HIPCHK(hipMemsetAsync(counters_, 0, sizeof(ExCounters), s));
   const unsigned un = (unsigned)n;
   hipLaunchKernelGGL(k_ex_classify, grid, block, 0, s, g, field, a->phi ? 1 : 0, cls_, cand_[0], un);
-  hipLaunchKernelGGL(k_ex_seed, grid, block, 0, s, g, (const uint8_t*)cls_, u, v, w, st_[0], st_[1], counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 0,
                      list_, counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], cand_[0], 0,
                      list_, counters_, un);
   hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 1,
                      list_, counters_, un);
+  hipLaunchKernelGGL(k_ex_seed, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], u, v, w,
+                     st_[0], st_[1], counters_, un);
   ExCounters host;
   HIPCHK(hipMemcpyAsync(&host, counters_, sizeof(ExCounters), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
