@@ -63,12 +63,23 @@ async function run() {
   const openTank = scene('dam-break', { container: { width_m: 0.8, height_m: 0.8, depth_m: 0.8, fillFraction: 0.22,
     top: 'open', fluidWallMode: 'no-slip' } });
   for (const [name, sc, values] of [['cm11aOpenTop', openTank, {}],
-    ['cm11aClosed', scene('dam-break'), { pressureSolver: 'cm11a' }]]) {
+    ['cm11aClosed', scene('dam-break'), { pressureSolver: 'cm11a' }],
+    ['maccormack', scene('dam-break'), { velocityTransport: 'maccormack' }]]) {
     const solver = await meth.createSolverAsync(null, sc, 'high', values);
     const stepped = [solver.advanceTo(0.016, []), solver.advanceTo(0.048, [])];
     const info = await solver.readStats();
     solver.destroy();
     out[name] = { stepped, info };
+  }
+  // the advection scheme switched at run time (applyRuntimeValues)
+  {
+    const solver = await meth.createSolverAsync(null, scene('dam-break'), 'high', {});
+    solver.advanceTo(0.016, []);
+    solver.applyRuntimeValues({ velocityTransport: 'maccormack' });
+    solver.advanceTo(0.032, []);
+    const info = await solver.readStats();
+    solver.destroy();
+    out.runtimeSwitch = { info };
   }
   // addon entry points: project, submit and projectBatch agree
   const dims = [24, 20, 16];

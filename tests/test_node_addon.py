@@ -88,6 +88,14 @@ def test_plugin_solver_runs_scene_steps_and_entry_points(fpl):
     t = out["tall-cell"]["info"]
     assert t["storedNy"] == 34 and t["ny"] == 64 and t["regularLayers"] == 32
     assert t["compressionRatio"] == pytest.approx(64 / 34)
+    # the uniform step advects on the GPU before each projection, either scheme
+    for name in ("maccormack", "runtimeSwitch"):
+        info = out[name]["info"]
+        assert info["steps"] == (3 if name == "maccormack" else 2), (name, info)
+        assert info["advectionDeviceMs"] > 0 and info["pressureFlags"] == 0, (name, info)
+        assert info["pressureSolveConverged"] is True, (name, info)
+        assert info["maxDivergenceAfter_s"] < 1e-3 * info["maxDivergenceBefore_s"], (name, info)
+    assert out["uniform"]["info"]["advectionDeviceMs"] > 0
     for name in ("cm11aOpenTop", "cm11aClosed"):
         o = out[name]
         info = o["info"]
