@@ -177,3 +177,31 @@ def test_full_size_translation_is_exact(fpl, cells, scheme):
     assert float((got - want[:, None, None]).abs().max()) <= 2e-6
     assert float((uo[lo:hi, 3:-3, 3:-3] - U).abs().max()) <= 2e-6
     assert float(wo[:, :, 1:-1].abs().max()) == 0.0
+
+
+def test_budget_refusal_leaves_the_grid_usable(fpl, oracle):
+    """A context whose HBM budget holds the grid but not the advection state:
+    fp_advect_velocity fails with FP_ERR_BUDGET and allocates nothing; the grid
+    still projects, and a context with room advects bit-exactly."""
+    s = scenes.dam_spheres_scene(32, seed=1)
+    # what the grid and one host-memory projection hold (state and staging)
+    probe = fpl.Context(0, 0)
+    g = fpl.UniformPressureProjection(s.phi.shape, s.h, context=probe)
+    g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho)
+    grid_bytes = probe.bytes_in_use()
+    g.destroy()
+    probe.destroy()
+    ctx = fpl.Context(0, grid_bytes + 4096)
+    g = fpl.UniformPressureProjection(s.phi.shape, s.h, context=ctx)
+    try:
+        assert g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho).report["flags"] == 0
+        with pytest.raises(fpl.FluidProjectionError) as e:
+            g.advect(s.u, s.v, s.w, 1 / 30, phi=s.phi)
+        assert e.value.code == fpl.ERR_BUDGET
+        assert ctx.bytes_in_use() == grid_bytes
+        assert g.project(s.phi, s.u, s.v, s.w, s.dt, s.rho).report["flags"] == 0
+    finally:
+        g.destroy()
+        ctx.destroy()
+    got = run(fpl, oracle, s, dt=1 / 30)
+    assert_same(got, oracle.advect_velocity(s.u, s.v, s.w, s.h, 1 / 30, phi=s.phi, gravity=-9.80665))
