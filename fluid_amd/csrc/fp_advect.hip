@@ -551,7 +551,8 @@ int Advector::run(hipStream_t s, Extrapolator* ex, const fp_advect_args* a, fp_a
     e.u_out = split[3][0];
     e.v_out = split[3][1];
     e.w_out = split[3][2];
-    int rc = ex->run(s, &e, nullptr);
+    // the same surface as the transport's extrapolation when this call made it
+    int rc = ex->run(s, &e, nullptr, !have_transport);
     if (rc) return rc;
     const Faces P{{split[1][0], split[1][1], split[1][2]}}, PE{{split[3][0], split[3][1], split[3][2]}};
     const Faces R{{split[2][0], split[2][1], split[2][2]}};

@@ -772,7 +772,7 @@ int Extrapolator::create(int nx, int ny, int nz, double h) {
   return FP_OK;
 }
 
-int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolate_report* rep) {
+int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolate_report* rep, bool same_band) {
   const auto t0 = std::chrono::steady_clock::now();
   const int nx = d_[0], ny = d_[1], nz = d_[2];
   const long long n = n_;
@@ -822,19 +822,25 @@ int Extrapolator::run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolat
   const dim3 grid(chunked_grid(n, kCells)), block(kThreads);
   HIPCHK(hipMemsetAsync(counters_, 0, sizeof(ExCounters), s));
   const unsigned un = (unsigned)n;
-  hipLaunchKernelGGL(k_ex_classify, grid, block, 0, s, g, field, a->phi ? 1 : 0, cls_, cand_[0], un);
-  hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 0,
-                     list_, counters_, un);
-  hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], cand_[0], 0,
-                     list_, counters_, un);
-  hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1], 1,
-                     list_, counters_, un);
+  same_band = same_band && band_count_ >= 0;
+  if (!same_band) {
+    hipLaunchKernelGGL(k_ex_classify, grid, block, 0, s, g, field, a->phi ? 1 : 0, cls_, cand_[0], un);
+    hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1],
+                       0, list_, counters_, un);
+    hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], cand_[0],
+                       0, list_, counters_, un);
+    hipLaunchKernelGGL(k_ex_dilate, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[0], cand_[1],
+                       1, list_, counters_, un);
+  }
   hipLaunchKernelGGL(k_ex_seed, grid, block, 0, s, g, (const uint8_t*)cls_, (const uint8_t*)cand_[1], u, v, w,
                      st_[0], st_[1], counters_, un);
   ExCounters host;
-  HIPCHK(hipMemcpyAsync(&host, counters_, sizeof(ExCounters), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const int count = host.list_count;
+  if (!same_band) {
+    HIPCHK(hipMemcpyAsync(&host, counters_, sizeof(ExCounters), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    band_count_ = host.list_count;
+  }
+  const int count = band_count_;
   if (count > 0) {
     for (int pass = 1; pass <= ceiling; ++pass)
       hipLaunchKernelGGL(k_ex_pass, dim3((count + kThreads - 1) / kThreads), block, 0, s, g, (const uint8_t*)cls_,

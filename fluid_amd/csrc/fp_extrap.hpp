@@ -51,7 +51,10 @@ class Extrapolator {
   int create(int nx, int ny, int nz, double h);
   // One extrapolation of a->phi / a->density and a->u, a->v, a->w into the
   // outputs; synchronous on s.
-  int run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolate_report* rep);
+  // same_band: the field (phi / density) and open_top are those of this
+  // object's previous run, whose classes, near mask and band list are reused
+  // (a second extrapolation over the same surface, as a MacCormack step makes)
+  int run(hipStream_t s, const fp_extrapolate_args* a, fp_extrapolate_report* rep, bool same_band = false);
   size_t bytes() const { return bytes_; }
 
  private:
@@ -65,6 +68,7 @@ class Extrapolator {
   ExState st_[2];
   uint8_t* cls_ = nullptr;
   uint8_t* cand_[2] = {nullptr, nullptr};
+  int band_count_ = -1;  // band list length of the previous run (-1: none)
   int* list_ = nullptr;
   float* in_ = nullptr;  // host-memory staging: phi or density, u, v, w
   ExCounters* counters_ = nullptr;
