@@ -772,7 +772,8 @@ def bench_uniform(args):
                   "max_div_before": rep["max_div_before"], "device_ms": round(rep["device_ms"], 3)},
         "fp64_oracle_error": err,
         "plain_fp32_mgpcg": plain or None,
-        "extrapolation": extrapolation,
+        "extrapolation": {k: v for k, v in extrapolation.items() if k != "advection"},
+        "advection": extrapolation.get("advection"),
         "roofline": {"bound": "hbm", "kernel": args.timed_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_bytes(("uniform", args.n, args.precision, args.timed_kernel, args.sweeps, pairs)),
